@@ -1,0 +1,162 @@
+// FP64 tiled GEMM kernels (64x64 tiles, 256 threads, 4x4 outputs per thread).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "common.hpp"
+#include "gemm.hpp"
+
+namespace bd {
+
+namespace {
+constexpr int TM = 64, TN = 64, TK = 16;
+
+// NN: C = alpha A B + beta C.
+__global__ __launch_bounds__(256) void k_gemm_nn(int64_t M, int64_t N, int64_t K, double alpha,
+                                                 const double* __restrict__ A, int64_t lda,
+                                                 const double* __restrict__ B, int64_t ldb,
+                                                 double beta, double* __restrict__ C, int64_t ldc) {
+  __shared__ double As[TK][TM + 1];
+  __shared__ double Bs[TK][TN + 1];
+  const int tid = threadIdx.x;
+  const int tx = tid % 16, ty = tid / 16;
+  const int64_t row0 = int64_t(blockIdx.x) * TM, col0 = int64_t(blockIdx.y) * TN;
+  double acc[4][4] = {};
+  for (int64_t k0 = 0; k0 < K; k0 += TK) {
+    // A tile: 64 rows x 16 k, coalesced along rows.
+    for (int e = tid; e < TM * TK; e += 256) {
+      int i = e % TM, k = e / TM;
+      int64_t gi = row0 + i, gk = k0 + k;
+      As[k][i] = (gi < M && gk < K) ? A[gi + gk * lda] : 0.0;
+    }
+    // B tile: 16 k x 64 cols, coalesced along k.
+    for (int e = tid; e < TK * TN; e += 256) {
+      int k = e % TK, j = e / TK;
+      int64_t gk = k0 + k, gj = col0 + j;
+      Bs[k][j] = (gk < K && gj < N) ? B[gk + gj * ldb] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < TK; ++k) {
+      double a[4], b[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) a[r] = As[k][tx + 16 * r];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) b[c] = Bs[k][ty + 16 * c];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] = fma(a[r], b[c], acc[r][c]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    int64_t gj = col0 + ty + 16 * c;
+    if (gj >= N) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int64_t gi = row0 + tx + 16 * r;
+      if (gi >= M) continue;
+      double* p = C + gi + gj * ldc;
+      *p = beta == 0.0 ? alpha * acc[r][c] : alpha * acc[r][c] + beta * *p;
+    }
+  }
+}
+
+// TN partial: part[s](MxN) = sum over k in chunk s of A[k,i] B[k,j].
+__global__ __launch_bounds__(256) void k_gemm_tn_part(int64_t M, int64_t N, int64_t K, int64_t chunk,
+                                                      const double* __restrict__ A, int64_t lda,
+                                                      const double* __restrict__ B, int64_t ldb,
+                                                      double* __restrict__ part) {
+  __shared__ double As[TK][TM + 1];
+  __shared__ double Bs[TK][TN + 1];
+  const int tid = threadIdx.x;
+  const int tx = tid % 16, ty = tid / 16;
+  const int64_t row0 = int64_t(blockIdx.x) * TM, col0 = int64_t(blockIdx.y) * TN;
+  const int64_t kbeg = int64_t(blockIdx.z) * chunk, kend = std::min(K, kbeg + chunk);
+  double acc[4][4] = {};
+  for (int64_t k0 = kbeg; k0 < kend; k0 += TK) {
+    for (int e = tid; e < TM * TK; e += 256) {
+      int k = e % TK, i = e / TK;
+      int64_t gi = row0 + i, gk = k0 + k;
+      As[k][i] = (gi < M && gk < kend) ? A[gk + gi * lda] : 0.0;
+    }
+    for (int e = tid; e < TK * TN; e += 256) {
+      int k = e % TK, j = e / TK;
+      int64_t gk = k0 + k, gj = col0 + j;
+      Bs[k][j] = (gk < kend && gj < N) ? B[gk + gj * ldb] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < TK; ++k) {
+      double a[4], b[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) a[r] = As[k][tx + 16 * r];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) b[c] = Bs[k][ty + 16 * c];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] = fma(a[r], b[c], acc[r][c]);
+    }
+    __syncthreads();
+  }
+  double* out = part + int64_t(blockIdx.z) * M * N;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    int64_t gj = col0 + ty + 16 * c;
+    if (gj >= N) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int64_t gi = row0 + tx + 16 * r;
+      if (gi < M) out[gi + gj * M] = acc[r][c];
+    }
+  }
+}
+
+__global__ void k_gemm_tn_reduce(int64_t M, int64_t N, int splits, double alpha,
+                                 const double* __restrict__ part, double beta,
+                                 double* __restrict__ C, int64_t ldc) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= M * N) return;
+  const int64_t i = e % M, j = e / M;
+  double s = 0.0;
+  for (int p = 0; p < splits; ++p) s += part[int64_t(p) * M * N + e];
+  double* c = C + i + j * ldc;
+  *c = beta == 0.0 ? alpha * s : alpha * s + beta * *c;
+}
+
+int tn_splits(int64_t M, int64_t N, int64_t K) {
+  const int64_t tiles = ((M + TM - 1) / TM) * ((N + TN - 1) / TN);
+  int64_t want = std::max<int64_t>(1, 1024 / std::max<int64_t>(tiles, 1));
+  int64_t maxs = std::max<int64_t>(1, K / 256);
+  return int(std::min<int64_t>({want, maxs, 256}));
+}
+}  // namespace
+
+void gemm_nn(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
+             int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc) {
+  if (M <= 0 || N <= 0) return;
+  dim3 grid(unsigned((M + TM - 1) / TM), unsigned((N + TN - 1) / TN));
+  hipLaunchKernelGGL(k_gemm_nn, grid, dim3(256), 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+int64_t gemm_tn_work(int64_t M, int64_t N, int64_t K) { return int64_t(tn_splits(M, N, K)) * M * N; }
+
+void gemm_tn(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
+             int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
+             double* work) {
+  if (M <= 0 || N <= 0) return;
+  const int splits = tn_splits(M, N, K);
+  int64_t chunk = (K + splits - 1) / splits;
+  chunk = ((chunk + TK - 1) / TK) * TK;
+  const int used = int((K + chunk - 1) / std::max<int64_t>(chunk, 1));
+  dim3 grid(unsigned((M + TM - 1) / TM), unsigned((N + TN - 1) / TN), unsigned(std::max(used, 1)));
+  hipLaunchKernelGGL(k_gemm_tn_part, grid, dim3(256), 0, s, M, N, K, chunk, A, lda, B, ldb, work);
+  const int64_t total = M * N;
+  hipLaunchKernelGGL(k_gemm_tn_reduce, dim3(unsigned((total + 255) / 256)), dim3(256), 0, s, M, N,
+                     std::max(used, 1), alpha, work, beta, C, ldc);
+}
+
+}  // namespace bd

@@ -1,0 +1,623 @@
+// Gamma H·X on MI355X: band-pair-batched pruned 3D FFT (sticks along x, LDS
+// y-z planes with the local-potential multiply), kinetic term and the
+// nonlocal P·D·PᵀX products.
+//
+// Data path per apply (m ORC columns, P = ceil(m/2) complex pairs):
+//   k_stick_inverse  X (ORC) -> pack two columns (distributed_real_gamma.rs:651-682)
+//                    -> scatter to sticks -> inverse x FFT -> stick buffer [pair][x][stick]
+//   k_plane          per (x, pair): sticks -> LDS plane (z-compact) -> inverse y FFT ->
+//                    per y-row chunk: inverse z FFT, * V(r), forward z FFT -> forward y FFT
+//                    -> sticks (distributed_fft.rs:956-1080 fused in one kernel)
+//   k_stick_forward  forward x FFT -> gather +G/-G (both in one tile) -> even/odd unpack
+//                    (distributed_real_gamma.rs:1316-1339), 1/N, + T_G x -> Y
+//   projections      Q = P^T X (split-K GEMM), W = D Q (per-atom blocks), Y += P W.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <set>
+#include <string>
+
+#include "common.hpp"
+#include "gemm.hpp"
+#include "op.hpp"
+
+namespace bd {
+
+namespace {
+constexpr int kThreads = 256;
+constexpr double kSqrt2 = 1.41421356237309504880168872420969808;
+constexpr double kInvSqrt2 = 0.70710678118654752440084436210484904;
+
+template <int R>
+bool pass_fits(int nlines, int n) {
+  return int64_t(nlines) * (n / R) <= int64_t(MaxB<R>::v) * kThreads;
+}
+bool fft_fits(int nlines, const RadixPlan& rp) {
+  for (int p = 0; p < rp.count; ++p) {
+    int R = rp.radix[p];
+    bool ok = R == 2 ? pass_fits<2>(nlines, rp.n)
+              : R == 3 ? pass_fits<3>(nlines, rp.n)
+              : R == 4 ? pass_fits<4>(nlines, rp.n)
+                       : pass_fits<5>(nlines, rp.n);
+    if (!ok) return false;
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void stage_roots(c2<T>* dst, const c2<double>* src, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = {T(src[i].x), T(src[i].y)};
+}
+
+// Pack ORC column pair t into its sticks and inverse-transform along x.
+__global__ __launch_bounds__(kThreads) void k_stick_inverse(
+    const double* __restrict__ x, int64_t ldx, int m, int n0, int T, int Spad,
+    const int* __restrict__ tile_off, const int4* __restrict__ entries,
+    const c2<double>* __restrict__ roots_x, RadixPlan rx, c2<double>* __restrict__ sticks) {
+  extern __shared__ char smem[];
+  c2<double>* L = reinterpret_cast<c2<double>*>(smem);
+  c2<double>* rt = L + T * n0;
+  const int tile = blockIdx.x, t = blockIdx.y;
+  const int ca = 2 * t, cb = 2 * t + 1;
+  const double* xa = x + int64_t(ca) * ldx;
+  const double* xb = cb < m ? x + int64_t(cb) * ldx : nullptr;
+  for (int i = threadIdx.x; i < T * n0; i += blockDim.x) L[i] = {0.0, 0.0};
+  stage_roots(rt, roots_x, n0);
+  __syncthreads();
+  const int e0 = tile_off[tile], e1 = tile_off[tile + 1];
+  for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+    const int4 en = entries[e];
+    const int i = en.x;
+    if (i == 0) {
+      L[en.y] = {xa[0], xb ? xb[0] : 0.0};
+    } else {
+      const double ar = xa[2 * i - 1] * kInvSqrt2, ai = xa[2 * i] * kInvSqrt2;
+      const double br = xb ? xb[2 * i - 1] * kInvSqrt2 : 0.0, bi = xb ? xb[2 * i] * kInvSqrt2 : 0.0;
+      L[en.y] = {ar - bi, ai + br};   // a + i b
+      L[en.z] = {ar + bi, br - ai};   // conj(a) + i conj(b)
+    }
+  }
+  __syncthreads();
+  lds_fft<true>(L, T, n0, 1, rx, rt);
+  c2<double>* out = sticks + int64_t(t) * n0 * Spad + int64_t(tile) * T;
+  for (int e = threadIdx.x; e < T * n0; e += blockDim.x) {
+    const int xi = e / T, slot = e - xi * T;
+    out[int64_t(xi) * Spad + slot] = L[slot * n0 + xi];
+  }
+}
+
+// Forward x FFT of a tile, gather +G/-G, unpack to ORC, add T_G x.
+__global__ __launch_bounds__(kThreads) void k_stick_forward(
+    const double* __restrict__ x, int64_t ldx, double* __restrict__ y, int64_t ldy, int m,
+    int n0, int T, int Spad, const int* __restrict__ tile_off, const int4* __restrict__ entries,
+    const c2<double>* __restrict__ roots_x, RadixPlan rx, const c2<double>* __restrict__ sticks,
+    const double* __restrict__ kinetic, double inv_n) {
+  extern __shared__ char smem[];
+  c2<double>* L = reinterpret_cast<c2<double>*>(smem);
+  c2<double>* rt = L + T * n0;
+  const int tile = blockIdx.x, t = blockIdx.y;
+  const c2<double>* in = sticks + int64_t(t) * n0 * Spad + int64_t(tile) * T;
+  for (int e = threadIdx.x; e < T * n0; e += blockDim.x) {
+    const int xi = e / T, slot = e - xi * T;
+    L[slot * n0 + xi] = in[int64_t(xi) * Spad + slot];
+  }
+  stage_roots(rt, roots_x, n0);
+  __syncthreads();
+  lds_fft<false>(L, T, n0, 1, rx, rt);
+  const int ca = 2 * t, cb = 2 * t + 1;
+  const bool hb = cb < m;
+  const double* xa = x + int64_t(ca) * ldx;
+  const double* xb = x + int64_t(hb ? cb : ca) * ldx;
+  double* ya = y + int64_t(ca) * ldy;
+  double* yb = y + int64_t(hb ? cb : ca) * ldy;
+  const int e0 = tile_off[tile], e1 = tile_off[tile + 1];
+  for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+    const int4 en = entries[e];
+    const int i = en.x;
+    const c2<double> hp = cscale(L[en.y], inv_n);
+    if (i == 0) {
+      const double k0 = kinetic[0];
+      ya[0] = hp.x + k0 * xa[0];
+      if (hb) yb[0] = hp.y + k0 * xb[0];
+    } else {
+      const c2<double> hm = cscale(L[en.z], inv_n);
+      // even = (hp + conj(hm))/2, odd = (hp - conj(hm))/2, rows x sqrt2.
+      const double er = 0.5 * (hp.x + hm.x), ei = 0.5 * (hp.y - hm.y);
+      const double orr = 0.5 * (hp.x - hm.x), oi = 0.5 * (hp.y + hm.y);
+      const int r1 = 2 * i - 1, r2 = 2 * i;
+      const double k = kinetic[r1];
+      ya[r1] = er * kSqrt2 + k * xa[r1];
+      ya[r2] = ei * kSqrt2 + k * xa[r2];
+      if (hb) {
+        yb[r1] = oi * kSqrt2 + k * xb[r1];
+        yb[r2] = -orr * kSqrt2 + k * xb[r2];
+      }
+    }
+  }
+}
+
+// Plane pass: inverse y, per-row-chunk inverse z * V * forward z, forward y.
+// mode 0: H apply (writes sticks back); mode 1: band density (accumulates
+// w_a psi_a^2 + w_b psi_b^2 over all pairs into rho of plane x).
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void k_plane(
+    c2<double>* __restrict__ sticks, int n0, int n1, int n2, int Spad, int Z, int chunk,
+    const int2* __restrict__ stick_yz, const int* __restrict__ z_of_zc,
+    const c2<double>* __restrict__ roots, RadixPlan ry, RadixPlan rz,
+    const double* __restrict__ potential, int pairs, const double* __restrict__ weights,
+    double inv_volume, double* __restrict__ rho) {
+  extern __shared__ char smem[];
+  c2<double>* P = reinterpret_cast<c2<double>*>(smem);  // [Z][n1]
+  c2<double>* Rb = P + Z * n1;                          // [chunk][n2]
+  c2<double>* rty = Rb + chunk * n2;
+  c2<double>* rtz = rty + n1;
+  const int xi = MODE == 0 ? blockIdx.y : blockIdx.x;
+  stage_roots(rty, roots + n0, n1);
+  stage_roots(rtz, roots + n0 + n1, n2);
+  const int tbeg = MODE == 0 ? blockIdx.x : 0;
+  const int tend = MODE == 0 ? tbeg + 1 : pairs;
+  for (int t = tbeg; t < tend; ++t) {
+    c2<double>* col = sticks + int64_t(t) * n0 * Spad + int64_t(xi) * Spad;
+    for (int i = threadIdx.x; i < Z * n1; i += blockDim.x) P[i] = {0.0, 0.0};
+    __syncthreads();
+    for (int s = threadIdx.x; s < Spad; s += blockDim.x) {
+      const int2 yz = stick_yz[s];
+      if (yz.x >= 0) P[yz.y * n1 + yz.x] = col[s];
+    }
+    __syncthreads();
+    lds_fft<true>(P, Z, n1, 1, ry, rty);
+    double wa = 0.0, wb = 0.0;
+    if (MODE == 1) {
+      wa = weights[2 * t] * inv_volume;
+      wb = (2 * t + 1 < 2 * pairs) ? weights[2 * t + 1] * inv_volume : 0.0;
+    }
+    for (int y0 = 0; y0 < n1; y0 += chunk) {
+      const int c = min(chunk, n1 - y0);
+      for (int i = threadIdx.x; i < c * n2; i += blockDim.x) Rb[i] = {0.0, 0.0};
+      __syncthreads();
+      for (int i = threadIdx.x; i < c * Z; i += blockDim.x) {
+        const int r = i / Z, zc = i - r * Z;
+        Rb[r * n2 + z_of_zc[zc]] = P[zc * n1 + y0 + r];
+      }
+      __syncthreads();
+      lds_fft<true>(Rb, c, n2, 1, rz, rtz);
+      if (MODE == 0) {
+        const double* v = potential + (int64_t(xi) * n1 + y0) * n2;
+        for (int i = threadIdx.x; i < c * n2; i += blockDim.x) Rb[i] = cscale(Rb[i], v[i]);
+        __syncthreads();
+        lds_fft<false>(Rb, c, n2, 1, rz, rtz);
+        for (int i = threadIdx.x; i < c * Z; i += blockDim.x) {
+          const int r = i / Z, zc = i - r * Z;
+          P[zc * n1 + y0 + r] = Rb[r * n2 + z_of_zc[zc]];
+        }
+        __syncthreads();
+      } else {
+        double* out = rho + (int64_t(xi) * n1 + y0) * n2;
+        for (int i = threadIdx.x; i < c * n2; i += blockDim.x) {
+          const double d = wa * Rb[i].x * Rb[i].x + wb * Rb[i].y * Rb[i].y;
+          out[i] = t == 0 ? d : out[i] + d;
+        }
+        __syncthreads();
+      }
+    }
+    if (MODE == 0) {
+      lds_fft<false>(P, Z, n1, 1, ry, rty);
+      for (int s = threadIdx.x; s < Spad; s += blockDim.x) {
+        const int2 yz = stick_yz[s];
+        if (yz.x >= 0) col[s] = P[yz.y * n1 + yz.x];
+      }
+    }
+  }
+}
+
+// W = D Q per atom block (distributed_real_gamma.rs:1183-1195).
+__global__ void k_coupling(const double* __restrict__ q, double* __restrict__ w, int64_t nproj,
+                           int m, int nc, const double* __restrict__ blockd) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= nproj * m) return;
+  const int64_t row = e % nproj, col = e / nproj;
+  const int64_t start = (row / nc) * nc;
+  const int a = int(row - start);
+  const double* qc = q + col * nproj + start;
+  double s = 0.0;
+  for (int b = 0; b < nc; ++b) s += blockd[a * nc + b] * qc[b];
+  w[e] = s;
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Plans
+// ---------------------------------------------------------------------------
+RadixPlan radix_plan(int n) {
+  RadixPlan rp{};
+  rp.n = n;
+  int r = n;
+  auto take = [&](int f) {
+    while (r % f == 0) {
+      if (rp.count >= 12) fail(BD_ERR_INVALID_PARAMETER, "FFT length has too many factors");
+      rp.radix[rp.count++] = f;
+      r /= f;
+    }
+  };
+  take(4);
+  take(2);
+  take(3);
+  take(5);
+  if (r != 1) fail(BD_ERR_INVALID_PARAMETER, "FFT length must be 2,3,5-smooth");
+  return rp;
+}
+
+StickPlan build_stick_plan(const System& s) {
+  StickPlan p{};
+  p.n0 = s.dims[0];
+  p.n1 = s.dims[1];
+  p.n2 = s.dims[2];
+  p.rx = radix_plan(p.n0);
+  p.ry = radix_plan(p.n1);
+  p.rz = radix_plan(p.n2);
+  auto split = [&](int64_t f, int& i0, int& i1, int& i2) {
+    i2 = int(f % p.n2);
+    i1 = int((f / p.n2) % p.n1);
+    i0 = int(f / (int64_t(p.n1) * p.n2));
+  };
+  // Occupied sticks keyed (z, y), as the reference keys sticks (distributed_fft.rs:296-312).
+  std::set<std::pair<int, int>> keys;
+  for (int64_t r = 0; r < s.nrep; ++r)
+    for (int64_t f : {s.plus_flat[r], s.minus_flat[r]}) {
+      int a, b, c;
+      split(f, a, b, c);
+      keys.insert({c, b});
+    }
+  std::vector<std::pair<int, int>> sticks(keys.begin(), keys.end());
+  p.S = int(sticks.size());
+  std::map<std::pair<int, int>, int> index;
+  for (int i = 0; i < p.S; ++i) index[sticks[i]] = i;
+  std::set<int> zs;
+  for (auto& k : sticks) zs.insert(k.first);
+  p.z_of_zc.assign(zs.begin(), zs.end());
+  p.Z = int(p.z_of_zc.size());
+  std::map<int, int> zc_of;
+  for (int i = 0; i < p.Z; ++i) zc_of[p.z_of_zc[i]] = i;
+  // Tile size: the largest power of two <= 32 whose x transform fits a workgroup.
+  p.T = 32;
+  while (p.T > 1 && !fft_fits(p.T, p.rx)) p.T /= 2;
+  if (!fft_fits(p.T, p.rx)) fail(BD_ERR_INVALID_PARAMETER, "x FFT does not fit a workgroup");
+  if (!fft_fits(p.Z, p.ry)) fail(BD_ERR_INVALID_PARAMETER, "y FFT plane does not fit a workgroup");
+  p.chunk = 16;
+  while (p.chunk > 1 && !fft_fits(p.chunk, p.rz)) p.chunk /= 2;
+  p.chunk = std::min(p.chunk, p.n1);
+  // Mirror units (stick + its inversion partner) packed whole into tiles, so
+  // +G and -G of every representative share a tile.
+  std::vector<int> tile_of(p.S, -1), slot_of(p.S, -1);
+  int tile = 0, used = 0;
+  for (int i = 0; i < p.S; ++i) {
+    if (tile_of[i] >= 0) continue;
+    auto [z, y] = sticks[i];
+    std::pair<int, int> mk{(p.n2 - z) % p.n2, (p.n1 - y) % p.n1};
+    auto it = index.find(mk);
+    if (it == index.end()) fail(BD_ERR_INVALID_PARAMETER, "stick set is not inversion symmetric");
+    int j = it->second;
+    int need = (j == i) ? 1 : 2;
+    if (need > p.T) fail(BD_ERR_INVALID_PARAMETER, "tile too small for a mirror unit");
+    if (used + need > p.T) {
+      ++tile;
+      used = 0;
+    }
+    tile_of[i] = tile;
+    slot_of[i] = used++;
+    if (j != i) {
+      tile_of[j] = tile;
+      slot_of[j] = used++;
+    }
+  }
+  p.ntiles = tile + 1;
+  p.Spad = p.ntiles * p.T;
+  p.stick_yz.assign(p.Spad, make_int2(-1, -1));
+  for (int i = 0; i < p.S; ++i)
+    p.stick_yz[tile_of[i] * p.T + slot_of[i]] = make_int2(sticks[i].second, zc_of[sticks[i].first]);
+  // Representative entries grouped by tile.
+  std::vector<std::vector<int4>> per(p.ntiles);
+  for (int64_t r = 0; r < s.nrep; ++r) {
+    int a0, b0, c0, a1, b1, c1;
+    split(s.plus_flat[r], a0, b0, c0);
+    split(s.minus_flat[r], a1, b1, c1);
+    int sp = index[{c0, b0}], sm = index[{c1, b1}];
+    if (tile_of[sp] != tile_of[sm]) fail(BD_ERR_APPLICATION, "mirror pair split across tiles");
+    per[tile_of[sp]].push_back(
+        make_int4(int(r), slot_of[sp] * p.n0 + a0, slot_of[sm] * p.n0 + a1, 0));
+  }
+  p.tile_off.assign(1, 0);
+  for (auto& v : per) {
+    p.entries.insert(p.entries.end(), v.begin(), v.end());
+    p.tile_off.push_back(int(p.entries.size()));
+  }
+  return p;
+}
+
+// ---------------------------------------------------------------------------
+// Operator
+// ---------------------------------------------------------------------------
+static size_t stick_lds(const StickPlan& p) { return size_t(p.T * p.n0 + p.n0) * sizeof(c2<double>); }
+static size_t plane_lds(const StickPlan& p) {
+  return size_t(p.Z * p.n1 + p.chunk * p.n2 + p.n1 + p.n2) * sizeof(c2<double>);
+}
+
+GammaOp::GammaOp(const System& s, int cap, int prec) : sys(&s) {
+  if (prec != BD_FP64) fail(BD_ERR_INVALID_PARAMETER, "only BD_FP64 is implemented in this build");
+  if (cap <= 0) fail(BD_ERR_INVALID_PARAMETER, "capacity must be positive");
+  int dev = 0, count = 0;
+  BD_HIP(hipGetDeviceCount(&count));
+  if (count <= 0) fail(BD_ERR_DEVICE, "no HIP device");
+  BD_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  BD_HIP(hipGetDeviceProperties(&prop, dev));
+  if (std::string(prop.gcnArchName).rfind("gfx950", 0) != 0)
+    fail(BD_ERR_DEVICE, std::string("this build targets gfx950, device is ") + prop.gcnArchName);
+  n = s.n;
+  nrep = s.nrep;
+  for (int i = 0; i < 3; ++i) dims[i] = s.dims[i];
+  N = int64_t(dims[0]) * dims[1] * dims[2];
+  capacity = cap;
+  precision = prec;
+  nproj = s.nproj;
+  plan = build_stick_plan(s);
+  size_t lds_max = size_t(prop.sharedMemPerBlock);
+  if (stick_lds(plan) > lds_max || plane_lds(plan) > lds_max)
+    fail(BD_ERR_INVALID_PARAMETER, "FFT workgroup LDS exceeds the device limit");
+  if (plane_lds(plan) > 65536) {
+    BD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane<0>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(plane_lds(plan))));
+    BD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane<1>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(plane_lds(plan))));
+  }
+  if (stick_lds(plan) > 65536) {
+    BD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_stick_inverse),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(stick_lds(plan))));
+    BD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_stick_forward),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(stick_lds(plan))));
+  }
+  BD_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  // Roots e^{-2 pi i k/n} per axis, in long double.
+  std::vector<c2<double>> roots;
+  for (int a = 0; a < 3; ++a)
+    for (int k = 0; k < dims[a]; ++k) {
+      long double ang = -2.0L * 3.14159265358979323846264338327950288L * k / dims[a];
+      roots.push_back({double(cosl(ang)), double(sinl(ang))});
+    }
+  d_roots.alloc(roots.size());
+  d_roots.upload(roots.data(), roots.size(), stream);
+  d_tile_off.alloc(plan.tile_off.size());
+  d_tile_off.upload(plan.tile_off.data(), plan.tile_off.size(), stream);
+  d_entries.alloc(plan.entries.size());
+  d_entries.upload(plan.entries.data(), plan.entries.size(), stream);
+  d_stick_yz.alloc(plan.stick_yz.size());
+  d_stick_yz.upload(plan.stick_yz.data(), plan.stick_yz.size(), stream);
+  d_zc.alloc(plan.z_of_zc.size());
+  d_zc.upload(plan.z_of_zc.data(), plan.z_of_zc.size(), stream);
+  d_kinetic.alloc(n);
+  d_kinetic.upload(s.kinetic_orc.data(), n, stream);
+  d_potential.alloc(N);
+  if (nproj > 0) {
+    d_proj.alloc(size_t(n) * nproj);
+    d_proj.upload(s.projectors_orc.data(), size_t(n) * nproj, stream);
+    nc = s.channel_count;
+    nblocks = int(s.blocks.size());
+    d_blockd.alloc(s.block_d.size());
+    d_blockd.upload(s.block_d.data(), s.block_d.size(), stream);
+    d_q.alloc(size_t(nproj) * cap);
+    d_w.alloc(size_t(nproj) * cap);
+    d_work.alloc(size_t(gemm_tn_work(nproj, cap, n)));
+  }
+  const int pairs = (cap + 1) / 2;
+  d_sticks.alloc(size_t(pairs) * plan.n0 * plan.Spad);
+  // Host-pointer entry points stage through these (no allocation after create).
+  d_x.alloc(size_t(n) * cap);
+  d_y.alloc(size_t(n) * cap);
+  d_weights.alloc(size_t(2 * pairs));
+  for (auto& e : ev) BD_HIP(hipEventCreate(&e));
+  BD_HIP(hipStreamSynchronize(stream));
+}
+
+GammaOp::~GammaOp() {
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+void GammaOp::set_potential_dev(const double* v, uint64_t ep) {
+  if (has_potential && ep <= epoch) fail(BD_ERR_INVALID_PARAMETER, "potential epoch must increase");
+  BD_HIP(hipMemcpyAsync(d_potential.p, v, size_t(N) * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  epoch = ep;
+  has_potential = true;
+}
+
+void GammaOp::projections(const double* x, int64_t ldx, int m) {
+  gemm_tn(stream, nproj, m, n, 1.0, d_proj.p, n, x, ldx, 0.0, d_q.p, nproj, d_work.p);
+}
+
+void GammaOp::apply(const double* x, int64_t ldx, int m, double* y, int64_t ldy) {
+  if (m < 0 || m > capacity) fail(BD_ERR_INVALID_PARAMETER, "column count exceeds capacity");
+  if (ldx < n || ldy < n) fail(BD_ERR_INVALID_PARAMETER, "leading dimension smaller than rows");
+  if (!has_potential) fail(BD_ERR_INVALID_PARAMETER, "no potential installed");
+  if (m == 0) return;
+  const StickPlan& p = plan;
+  const int pairs = (m + 1) / 2;
+  if (profiling) BD_HIP(hipEventRecord(ev[0], stream));
+  hipLaunchKernelGGL(k_stick_inverse, dim3(p.ntiles, pairs), dim3(kThreads), stick_lds(p), stream, x,
+                     ldx, m, p.n0, p.T, p.Spad, d_tile_off.p, d_entries.p, d_roots.p, p.rx, d_sticks.p);
+  if (profiling) BD_HIP(hipEventRecord(ev[1], stream));
+  hipLaunchKernelGGL(k_plane<0>, dim3(pairs, p.n0), dim3(kThreads), plane_lds(p), stream, d_sticks.p,
+                     p.n0, p.n1, p.n2, p.Spad, p.Z, p.chunk, d_stick_yz.p, d_zc.p, d_roots.p, p.ry,
+                     p.rz, d_potential.p, pairs, nullptr, 0.0, nullptr);
+  if (profiling) BD_HIP(hipEventRecord(ev[2], stream));
+  hipLaunchKernelGGL(k_stick_forward, dim3(p.ntiles, pairs), dim3(kThreads), stick_lds(p), stream, x,
+                     ldx, y, ldy, m, p.n0, p.T, p.Spad, d_tile_off.p, d_entries.p, d_roots.p, p.rx,
+                     d_sticks.p, d_kinetic.p, 1.0 / double(N));
+  if (profiling) BD_HIP(hipEventRecord(ev[3], stream));
+  if (nproj > 0) {
+    projections(x, ldx, m);
+    const int64_t tot = nproj * m;
+    hipLaunchKernelGGL(k_coupling, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, stream, d_q.p,
+                       d_w.p, nproj, m, nc, d_blockd.p);
+    if (profiling) BD_HIP(hipEventRecord(ev[4], stream));
+    gemm_nn(stream, n, m, nproj, 1.0, d_proj.p, n, d_w.p, nproj, 1.0, y, ldy);
+  } else if (profiling) {
+    BD_HIP(hipEventRecord(ev[4], stream));
+  }
+  if (profiling) {
+    BD_HIP(hipEventRecord(ev[5], stream));
+    BD_HIP(hipEventSynchronize(ev[5]));
+    for (int i = 0; i < 5; ++i) {
+      float ms = 0.f;
+      BD_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+      stage_ms[i] = ms;
+    }
+  }
+  BD_HIP(hipGetLastError());
+}
+
+void GammaOp::band_density(const double* x, int64_t ldx, int m, const double* w, double* rho) {
+  if (m <= 0 || m > capacity) fail(BD_ERR_INVALID_PARAMETER, "column count");
+  const StickPlan& p = plan;
+  const int pairs = (m + 1) / 2;
+  hipLaunchKernelGGL(k_stick_inverse, dim3(p.ntiles, pairs), dim3(kThreads), stick_lds(p), stream, x,
+                     ldx, m, p.n0, p.T, p.Spad, d_tile_off.p, d_entries.p, d_roots.p, p.rx, d_sticks.p);
+  hipLaunchKernelGGL(k_plane<1>, dim3(p.n0), dim3(kThreads), plane_lds(p), stream, d_sticks.p, p.n0,
+                     p.n1, p.n2, p.Spad, p.Z, p.chunk, d_stick_yz.p, d_zc.p, d_roots.p, p.ry, p.rz,
+                     nullptr, pairs, w, 1.0 / sys->volume, rho);
+  BD_HIP(hipGetLastError());
+}
+
+}  // namespace bd
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+struct bd_gamma_op {
+  bd::GammaOp op;
+  bd_gamma_op(const bd::System& s, int cap, int prec) : op(s, cap, prec) {}
+};
+struct bd_system;
+namespace bd {
+const System& system_of(const bd_system* s);
+}
+
+extern "C" bd_status bd_gamma_op_create(const bd_system* sys, int32_t capacity, int32_t precision,
+                                        bd_gamma_op** out) {
+  BD_GUARD({
+    if (!sys || !out) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    *out = new bd_gamma_op(bd::system_of(sys), capacity, precision);
+  });
+}
+
+extern "C" void bd_gamma_op_destroy(bd_gamma_op* op) { delete op; }
+
+extern "C" void* bd_gamma_op_stream(bd_gamma_op* op) { return op ? op->op.stream : nullptr; }
+
+extern "C" bd_status bd_gamma_op_set_potential_dev(bd_gamma_op* op, const double* v, uint64_t epoch) {
+  BD_GUARD({
+    if (!op || !v) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    op->op.set_potential_dev(v, epoch);
+  });
+}
+
+extern "C" bd_status bd_gamma_op_set_potential(bd_gamma_op* op, const double* v, uint64_t epoch) {
+  BD_GUARD({
+    if (!op || !v) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    auto& o = op->op;
+    if (o.has_potential && epoch <= o.epoch) bd::fail(BD_ERR_INVALID_PARAMETER, "potential epoch must increase");
+    BD_HIP(hipMemcpyAsync(o.d_potential.p, v, size_t(o.N) * sizeof(double), hipMemcpyHostToDevice, o.stream));
+    BD_HIP(hipStreamSynchronize(o.stream));
+    o.epoch = epoch;
+    o.has_potential = true;
+  });
+}
+
+extern "C" bd_status bd_gamma_op_apply_dev(bd_gamma_op* op, const double* x, int64_t ldx, int32_t m,
+                                           double* y, int64_t ldy) {
+  BD_GUARD({
+    if (!op || (m > 0 && (!x || !y))) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    op->op.apply(x, ldx, m, y, ldy);
+  });
+}
+
+extern "C" bd_status bd_gamma_op_apply(bd_gamma_op* op, const double* x, int64_t ldx, int32_t m,
+                                       double* y, int64_t ldy) {
+  BD_GUARD({
+    if (!op || (m > 0 && (!x || !y))) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    auto& o = op->op;
+    if (m < 0 || m > o.capacity) bd::fail(BD_ERR_INVALID_PARAMETER, "column count exceeds capacity");
+    if (ldx < o.n || ldy < o.n) bd::fail(BD_ERR_INVALID_PARAMETER, "leading dimension smaller than rows");
+    if (m == 0) return BD_OK;
+    for (int j = 0; j < m; ++j)
+      BD_HIP(hipMemcpyAsync(o.d_x.p + size_t(j) * o.n, x + size_t(j) * ldx, size_t(o.n) * 8,
+                            hipMemcpyHostToDevice, o.stream));
+    o.apply(o.d_x.p, o.n, m, o.d_y.p, o.n);
+    for (int j = 0; j < m; ++j)
+      BD_HIP(hipMemcpyAsync(y + size_t(j) * ldy, o.d_y.p + size_t(j) * o.n, size_t(o.n) * 8,
+                            hipMemcpyDeviceToHost, o.stream));
+    BD_HIP(hipStreamSynchronize(o.stream));
+  });
+}
+
+extern "C" bd_status bd_gamma_op_band_density_dev(bd_gamma_op* op, const double* x, int64_t ldx, int32_t m,
+                                                  const double* weights, double* rho) {
+  BD_GUARD({
+    if (!op || !x || !weights || !rho) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    auto& o = op->op;
+    if (m <= 0 || m > o.capacity) bd::fail(BD_ERR_INVALID_PARAMETER, "column count");
+    std::vector<double> hw(size_t(2 * ((m + 1) / 2)), 0.0);
+    std::copy(weights, weights + m, hw.begin());
+    o.d_weights.upload(hw.data(), hw.size(), o.stream);
+    o.band_density(x, ldx, m, o.d_weights.p, rho);
+    BD_HIP(hipStreamSynchronize(o.stream));
+  });
+}
+
+extern "C" bd_status bd_gamma_op_nonlocal_energies_dev(bd_gamma_op* op, const double* x, int64_t ldx,
+                                                       int32_t m, double* out) {
+  BD_GUARD({
+    if (!op || !x || !out) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    auto& o = op->op;
+    if (m <= 0 || m > o.capacity) bd::fail(BD_ERR_INVALID_PARAMETER, "column count");
+    std::fill(out, out + m, 0.0);
+    if (o.nproj == 0) return BD_OK;
+    o.projections(x, ldx, m);
+    std::vector<double> q(size_t(o.nproj) * m);
+    BD_HIP(hipMemcpyAsync(q.data(), o.d_q.p, q.size() * 8, hipMemcpyDeviceToHost, o.stream));
+    BD_HIP(hipStreamSynchronize(o.stream));
+    const auto& s = *o.sys;
+    for (int b = 0; b < m; ++b) {
+      double sum = 0.0;
+      for (auto& blk : s.blocks)
+        for (int j = 0; j < blk.count; ++j) {
+          double v = 0.0;
+          for (int k = 0; k < blk.count; ++k)
+            v += s.block_d[j * blk.count + k] * q[size_t(b) * o.nproj + blk.start + k];
+          sum += q[size_t(b) * o.nproj + blk.start + j] * v;
+        }
+      out[b] = sum;
+    }
+  });
+}
+
+extern "C" bd_status bd_gamma_op_set_profiling(bd_gamma_op* op, int32_t on) {
+  BD_GUARD({
+    if (!op) bd::fail(BD_ERR_INVALID_PARAMETER, "null op");
+    op->op.profiling = on != 0;
+  });
+}
+
+extern "C" bd_status bd_gamma_op_stage_times(bd_gamma_op* op, double* out5) {
+  BD_GUARD({
+    if (!op || !out5) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    for (int i = 0; i < 5; ++i) out5[i] = op->op.stage_ms[i];
+  });
+}

@@ -1,0 +1,91 @@
+// Gamma Hamiltonian operator: FFT plan, device tables, scratch, launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "fft_device.hpp"
+#include "system.hpp"
+
+namespace bd {
+
+#define BD_HIP(call)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      ::bd::fail(BD_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+// Device buffer owned by the op (allocated at construction only).
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    n = count;
+    if (count) BD_HIP(hipMalloc(&p, count * sizeof(T)));
+  }
+  void upload(const T* src, size_t count, hipStream_t s) {
+    BD_HIP(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+// Stick decomposition of the sphere (sticks along grid axis 0).
+struct StickPlan {
+  int n0, n1, n2;
+  int T;        // sticks per tile
+  int ntiles;
+  int S;        // occupied sticks
+  int Spad;     // ntiles * T
+  int Z;        // occupied z planes
+  int chunk;    // y rows per z-pass chunk in the plane kernel
+  RadixPlan rx, ry, rz;
+  std::vector<int> tile_off;       // ntiles+1 offsets into entries
+  std::vector<int4> entries;       // (rep, plus slot*n0+x, minus slot*n0+x, 0)
+  std::vector<int2> stick_yz;      // per padded stick: (i1, zc), (-1,-1) pad
+  std::vector<int> z_of_zc;        // Z
+};
+
+struct GammaOp {
+  const System* sys;
+  int64_t n, nrep, N;
+  int dims[3];
+  int capacity;
+  int precision;
+  int64_t nproj;
+  hipStream_t stream = nullptr;
+  StickPlan plan;
+  uint64_t epoch = 0;
+  bool has_potential = false;
+  // device tables
+  DevBuf<int> d_tile_off, d_zc;
+  DevBuf<int4> d_entries;
+  DevBuf<int2> d_stick_yz;
+  DevBuf<c2<double>> d_roots;  // [n0 | n1 | n2]
+  DevBuf<double> d_kinetic, d_potential, d_proj, d_blockd;
+  // scratch
+  DevBuf<c2<double>> d_sticks;  // pairs_cap * n0 * Spad
+  DevBuf<double> d_q, d_w, d_work, d_x, d_y, d_weights;
+  int nc = 0;  // channels per atom block
+  int nblocks = 0;
+  // profiling
+  bool profiling = false;
+  hipEvent_t ev[6] = {};
+  double stage_ms[5] = {};
+
+  GammaOp(const System& s, int capacity, int precision);
+  ~GammaOp();
+  void set_potential_dev(const double* v, uint64_t epoch);
+  void apply(const double* x, int64_t ldx, int m, double* y, int64_t ldy);
+  void band_density(const double* x, int64_t ldx, int m, const double* w, double* rho);
+  void projections(const double* x, int64_t ldx, int m);  // d_q = P^T X
+};
+
+StickPlan build_stick_plan(const System& s);
+RadixPlan radix_plan(int n);
+
+}  // namespace bd

@@ -1,0 +1,63 @@
+// Set-up products of one Gamma cell (host memory).
+#pragma once
+#include <array>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/baandi_amd.h"
+
+namespace bd {
+
+struct System {
+  // Cell: lattice columns [cartesian][vector], inverse, B = 2 pi A^{-T}.
+  double lat[3][3];
+  double inv[3][3];
+  double B[3][3];
+  double volume = 0.0;
+  std::vector<std::array<double, 3>> atoms;  // Cartesian, Bohr
+  double electrons = 0.0;
+  // Synthetic pseudopotential on the log mesh (Ha).
+  double z = 0.0, rc = 1.0;
+  std::vector<double> r, rab, local_ha;
+  struct Proj {
+    int l;
+    std::vector<double> rbeta;
+  };
+  std::vector<Proj> projs;
+  std::vector<double> dij_ha;
+  // Basis (sorted by |G|^2 then Miller) and grid.
+  int64_t n = 0, nrep = 0;
+  int dims[3] = {0, 0, 0};
+  std::vector<std::array<int32_t, 3>> miller;
+  std::vector<std::array<double, 3>> cart;
+  std::vector<double> g2;
+  // Half sphere: representative -> basis index of +G / -G, and flat FFT index.
+  std::vector<int64_t> plus, minus, plus_flat, minus_flat;
+  std::vector<double> kinetic_orc;
+  // Nonlocal: ORC projector rows (n x nproj col-major), block-diagonal D.
+  int64_t nproj = 0;
+  int channel_count = 0;
+  std::vector<double> projectors_orc, coupling, block_d;
+  struct Block {
+    int64_t start;
+    int count;
+  };
+  std::vector<Block> blocks;
+  std::vector<double> diagonal_static;
+  // Density sphere (4 Ecut) and V_ion(G) on it.
+  std::vector<std::array<int32_t, 3>> dens_miller;
+  std::vector<std::array<double, 3>> dens_cart;
+  std::vector<double> dens_g2;
+  std::vector<int64_t> dens_flat;
+  std::vector<double> vion_re, vion_im;
+
+  explicit System(const bd_system_desc& d);
+
+ private:
+  void build_pseudo(const bd_system_desc& d);
+  void build_half_sphere();
+  void build_projectors();
+  void build_ionic();
+};
+
+}  // namespace bd

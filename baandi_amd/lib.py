@@ -1,0 +1,129 @@
+"""ctypes binding of libbaandi_amd.so (the C ABI in include/baandi_amd.h).
+
+The library is loaded from this package directory only; a missing library is
+an error (there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbaandi_amd.so")
+
+BD_OK, BD_ERR_LINALG, BD_ERR_INVALID_PARAMETER, BD_ERR_APPLICATION, BD_NOT_CONVERGED, BD_ERR_DEVICE = range(6)
+BD_FP64, BD_MIXED = 0, 1
+BD_PRECOND_QE, BD_PRECOND_TPA = 0, 1
+
+# Every function include/baandi_amd.h declares.
+EXPORTS = [
+    "bd_last_error", "bd_system_create", "bd_system_destroy", "bd_system_get_info", "bd_system_get",
+    "bd_gamma_op_create", "bd_gamma_op_destroy", "bd_gamma_op_set_potential",
+    "bd_gamma_op_set_potential_dev", "bd_gamma_op_apply", "bd_gamma_op_apply_dev",
+    "bd_gamma_op_band_density_dev", "bd_gamma_op_nonlocal_energies_dev", "bd_gamma_op_stream",
+    "bd_gamma_op_set_profiling", "bd_gamma_op_stage_times", "bd_davidson_solve_dev",
+]
+
+
+class SystemDesc(C.Structure):
+    _fields_ = [
+        ("lattice_rows", (C.c_double * 3) * 3),
+        ("natoms", C.c_int32),
+        ("frac_positions", C.POINTER(C.c_double)),
+        ("z_valence", C.c_double),
+        ("rc", C.c_double),
+        ("mesh_points", C.c_int32),
+        ("nproj", C.c_int32),
+        ("proj_l", C.POINTER(C.c_int32)),
+        ("proj_width", C.POINTER(C.c_double)),
+        ("dij_ry", C.POINTER(C.c_double)),
+        ("ecut_ha", C.c_double),
+        ("grid_override", C.c_int32 * 3),
+    ]
+
+
+class SystemInfo(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64), ("nrep", C.c_int64), ("dims", C.c_int32 * 3), ("nproj", C.c_int64),
+        ("natoms", C.c_int64), ("ndensity", C.c_int64), ("volume", C.c_double), ("electrons", C.c_double),
+    ]
+
+
+class DavidsonOptions(C.Structure):
+    _fields_ = [
+        ("bands", C.c_int32), ("threshold", C.c_double), ("occupied", C.c_int32),
+        ("first_step", C.c_int32), ("capacity", C.c_int32), ("max_iterations", C.c_int32),
+        ("return_hx", C.c_int32), ("preconditioner", C.c_int32),
+    ]
+
+
+class DavidsonRecord(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32), ("h_calls", C.c_int32), ("h_columns", C.c_int32),
+        ("repairs", C.c_int32), ("restarts", C.c_int32), ("all_converged", C.c_int32),
+        ("seconds_total", C.c_double), ("seconds_apply", C.c_double), ("seconds_dense", C.c_double),
+    ]
+
+
+class BaandiError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"[{status}] {message}")
+        self.status = status
+
+
+_lib = None
+
+
+def load():
+    """Load the in-tree library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run `python -m baandi_amd.build` (no CPU fallback)")
+    lib = C.CDLL(LIB_PATH)
+    P, I32, I64, D, U64 = C.c_void_p, C.c_int32, C.c_int64, C.c_double, C.c_uint64
+    dp = C.POINTER(C.c_double)
+    lib.bd_last_error.restype = C.c_char_p
+    sig = {
+        "bd_system_create": [C.POINTER(SystemDesc), C.POINTER(P)],
+        "bd_system_get_info": [P, C.POINTER(SystemInfo)],
+        "bd_system_get": [P, C.c_char_p, P, I64],
+        "bd_gamma_op_create": [P, I32, I32, C.POINTER(P)],
+        "bd_gamma_op_set_potential": [P, P, U64],
+        "bd_gamma_op_set_potential_dev": [P, P, U64],
+        "bd_gamma_op_apply": [P, P, I64, I32, P, I64],
+        "bd_gamma_op_apply_dev": [P, P, I64, I32, P, I64],
+        "bd_gamma_op_band_density_dev": [P, P, I64, I32, P, P],
+        "bd_gamma_op_nonlocal_energies_dev": [P, P, I64, I32, P],
+        "bd_gamma_op_set_profiling": [P, I32],
+        "bd_gamma_op_stage_times": [P, P],
+        "bd_davidson_solve_dev": [P, P, I64, P, P, C.POINTER(DavidsonOptions), P, P, P, I64, P,
+                                  C.POINTER(DavidsonRecord)],
+    }
+    for name, args in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    lib.bd_system_destroy.argtypes = [P]
+    lib.bd_system_destroy.restype = None
+    lib.bd_gamma_op_destroy.argtypes = [P]
+    lib.bd_gamma_op_destroy.restype = None
+    lib.bd_gamma_op_stream.argtypes = [P]
+    lib.bd_gamma_op_stream.restype = P
+    _lib = lib
+    return lib
+
+
+def check(status, allow=()):
+    if status != BD_OK and status not in allow:
+        msg = load().bd_last_error().decode()
+        raise BaandiError(status, msg)
+    return status
+
+
+def ptr(a):
+    """Address of a numpy array or torch tensor as void*."""
+    if hasattr(a, "data_ptr"):
+        return C.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(C.c_void_p)

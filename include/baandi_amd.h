@@ -1,0 +1,182 @@
+/*
+ * baandi_amd — MI355X (gfx950) Davidson H·ψ path of baandi, C ABI.
+ *
+ * Plain pointers and sizes only. "_dev" entry points take device pointers
+ * (hipMalloc'd, on the op's device); the others take host pointers and copy.
+ * Matrices are column-major with the given leading dimension (RealMatrix
+ * convention of crates/dft-numerics/src/real.rs). Energies are Hartree.
+ * An op is not thread-safe (the reference operator is !Sync,
+ * crates/dft-physics/src/hamiltonian.rs:1277); calls on one op serialise on
+ * its HIP stream. Results are deterministic per precision mode.
+ */
+#ifndef BAANDI_AMD_H
+#define BAANDI_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: one per SolverError variant (crates/dft-solvers/src/lib.rs:41-57)
+ * plus NotConverged and a device error. bd_last_error() returns the message. */
+typedef enum {
+  BD_OK = 0,
+  BD_ERR_LINALG = 1,            /* SolverError::Linalg */
+  BD_ERR_INVALID_PARAMETER = 2, /* SolverError::InvalidParameter */
+  BD_ERR_APPLICATION = 3,       /* SolverError::Application */
+  BD_NOT_CONVERGED = 4,         /* SolverError::NotConverged (exhausted iterations) */
+  BD_ERR_DEVICE = 5             /* HIP runtime error or no gfx950 device */
+} bd_status;
+
+/* Arithmetic of the hot path. FP64: every stage in double (the reference
+ * path). MIXED is reserved for the FP32-FFT / BF16x3-GEMM tier. */
+typedef enum { BD_FP64 = 0, BD_MIXED = 1 } bd_precision;
+
+/* Last error message of the calling thread ("" when none). */
+const char* bd_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Set-up of one Gamma cell with one synthetic species
+ * Replaces refstart.rs:536-650 (crates/dft-engine/src/refstart.rs): plane-wave
+ * basis (crates/dft-basis/src/lib.rs:234-269), FFT grid (lib.rs:414-455),
+ * ORC half sphere (crates/dft-basis/src/half.rs:41-153), projector table
+ * (crates/dft-physics/src/hamiltonian.rs:885-1035) packed to ORC rows
+ * (crates/dft-physics/src/distributed_real_gamma.rs:363-402), ionic local
+ * potential on the density sphere (hamiltonian.rs:211-340) and the Davidson
+ * diagonal (refstart.rs:605-650). The pseudopotential is the reference's
+ * synthetic Gaussian family (crates/dft-engine/tests/common/mod.rs:146-209).
+ * ---------------------------------------------------------------------- */
+typedef struct {
+  double lattice_rows[3][3]; /* each row one lattice vector, Bohr */
+  int32_t natoms;
+  const double* frac_positions; /* 3*natoms */
+  double z_valence;
+  double rc;             /* local part -2 Z erf(r/rc)/r Ry */
+  int32_t mesh_points;   /* log mesh to 12 Bohr */
+  int32_t nproj;         /* 0 disables the nonlocal part */
+  const int32_t* proj_l; /* nproj */
+  const double* proj_width; /* beta = r^(l+1) exp(-r^2/width) */
+  const double* dij_ry;  /* nproj*nproj row-major, Ry */
+  double ecut_ha;
+  int32_t grid_override[3]; /* all zero: automatic 2,3,5-smooth grid */
+} bd_system_desc;
+
+typedef struct bd_system bd_system;
+
+typedef struct {
+  int64_t n;          /* ORC rows = plane waves */
+  int64_t nrep;       /* half-sphere representatives incl. G=0: (n+1)/2 */
+  int32_t dims[3];    /* FFT grid */
+  int64_t nproj;      /* projector columns (atoms x channels) */
+  int64_t natoms;
+  int64_t ndensity;   /* density-sphere size (4 Ecut) */
+  double volume;      /* Bohr^3 */
+  double electrons;
+} bd_system_info;
+
+bd_status bd_system_create(const bd_system_desc* desc, bd_system** out);
+void bd_system_destroy(bd_system* sys);
+bd_status bd_system_get_info(const bd_system* sys, bd_system_info* out);
+/* Copy one named set-up array into `out` (count elements; error if count
+ * differs). Names (type, length):
+ *   "miller"           int32, 3n   basis Miller indices, basis order
+ *   "g2"               double, n   |G|^2, basis order
+ *   "kinetic_orc"      double, n   T_G per ORC row (distributed_real_gamma.rs:352-361)
+ *   "diagonal_static"  double, n   T_G + sum D_ij Re(b_i b_j*) per ORC row
+ *   "projectors_orc"   double, n*nproj  ORC rows, column-major
+ *   "coupling"         double, nproj*nproj  block-diagonal D (Ha), column-major
+ *   "plus_index", "minus_index"  int64, nrep  flat FFT index of +G / -G of each rep
+ *   "density_miller"   int32, 3*ndensity; "density_g2" double, ndensity
+ *   "density_index"    int64, ndensity  flat FFT index
+ *   "ionic_density_re", "ionic_density_im" double, ndensity  V_ion(G), Ha
+ *   "atoms_cart"       double, 3*natoms
+ */
+bd_status bd_system_get(const bd_system* sys, const char* name, void* out, int64_t count);
+
+/* ------------------------------------------------------------------------
+ * Gamma Hamiltonian operator (H·X on ORC columns)
+ * Replaces DistributedRealHamiltonian::{new_owned, set_owned_potential,
+ * apply_block} (crates/dft-physics/src/distributed_real_gamma.rs:222-1264) and
+ * its DistributedRealOperator impl (:1581-1618, trait at
+ * crates/dft-solvers/src/distributed_davidson.rs:19-40). Y is overwritten
+ * (previous contents unspecified), NaN propagates (not scanned).
+ * ---------------------------------------------------------------------- */
+typedef struct bd_gamma_op bd_gamma_op;
+
+/* Build the operator of a set-up system: FFT plan, device tables, scratch
+ * for `capacity` columns. No device allocation happens after this call. */
+bd_status bd_gamma_op_create(const bd_system* sys, int32_t capacity, int32_t precision,
+                             bd_gamma_op** out);
+void bd_gamma_op_destroy(bd_gamma_op* op);
+/* V(r) on the full row-major n0*n1*n2 grid, Ha (set_owned_potential,
+ * distributed_real_gamma.rs:476-492); epoch must increase. */
+bd_status bd_gamma_op_set_potential(bd_gamma_op* op, const double* v_grid, uint64_t epoch);
+bd_status bd_gamma_op_set_potential_dev(bd_gamma_op* op, const double* v_grid_dev, uint64_t epoch);
+/* Y = H X for m <= capacity columns (apply_block, :1062-1264). */
+bd_status bd_gamma_op_apply(bd_gamma_op* op, const double* x, int64_t ldx, int32_t m,
+                            double* y, int64_t ldy);
+bd_status bd_gamma_op_apply_dev(bd_gamma_op* op, const double* x_dev, int64_t ldx, int32_t m,
+                                double* y_dev, int64_t ldy);
+/* rho(r) = sum_b w_b psi_b(r)^2 / volume of the first m columns on the full
+ * grid (band_density_sum, distributed_real_gamma.rs:741-769). */
+bd_status bd_gamma_op_band_density_dev(bd_gamma_op* op, const double* x_dev, int64_t ldx,
+                                       int32_t m, const double* weights, double* rho_dev);
+/* x_b^T P D P^T x_b per column (nonlocal_band_energies, :878-928). */
+bd_status bd_gamma_op_nonlocal_energies_dev(bd_gamma_op* op, const double* x_dev, int64_t ldx,
+                                            int32_t m, double* out_host);
+/* HIP stream the op runs on (hipStream_t as void*). */
+void* bd_gamma_op_stream(bd_gamma_op* op);
+/* Mean kernel time (ms) of the last apply per stage, measured with HIP events
+ * on the op's stream when profiling is on: [stick_inverse, plane, stick_forward,
+ * projections, back_product]. */
+bd_status bd_gamma_op_set_profiling(bd_gamma_op* op, int32_t on);
+bd_status bd_gamma_op_stage_times(bd_gamma_op* op, double* out5_ms);
+
+/* ------------------------------------------------------------------------
+ * Reference block Davidson on the op
+ * Replaces ref_davidson::solve_with_initial_hx
+ * (crates/dft-solvers/src/ref_davidson.rs:117-366): bordered inverse Cholesky
+ * (lean_davidson.rs:167-331), Ritz-change convergence, contraction at
+ * capacity, rank repair. Options mirror ref_davidson::Options (:38-78).
+ * ---------------------------------------------------------------------- */
+typedef enum { BD_PRECOND_QE = 0, BD_PRECOND_TPA = 1 } bd_preconditioner;
+
+typedef struct {
+  int32_t bands;
+  double threshold;      /* occupied Ritz-change threshold, Ha (strict) */
+  int32_t occupied;
+  int32_t first_step;    /* all bands use `threshold` */
+  int32_t capacity;      /* 0: min(2*bands, n) */
+  int32_t max_iterations;/* default 20 */
+  int32_t return_hx;
+  int32_t preconditioner;/* bd_preconditioner */
+} bd_davidson_options;
+
+typedef struct {
+  int32_t iterations;
+  int32_t h_calls;
+  int32_t h_columns;
+  int32_t repairs;
+  int32_t restarts;
+  int32_t all_converged;
+  double seconds_total;
+  double seconds_apply;
+  double seconds_dense;  /* host factor + reduced eigensolve */
+} bd_davidson_record;
+
+/* X0 (n x bands, device), optional HX0 (device or NULL), diagonal (n, device).
+ * Outputs: eigenvalues (host, bands), X and HX (device, n x bands; HX may be
+ * NULL unless return_hx), converged flags (host, bands). Returns
+ * BD_NOT_CONVERGED when the iteration limit was hit (outputs still valid). */
+bd_status bd_davidson_solve_dev(bd_gamma_op* op, const double* x0_dev, int64_t ldx0,
+                                const double* hx0_dev, const double* diag_dev,
+                                const bd_davidson_options* options, double* eigenvalues,
+                                double* x_dev, double* hx_dev, int64_t ldx, int32_t* converged,
+                                bd_davidson_record* record);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BAANDI_AMD_H */
