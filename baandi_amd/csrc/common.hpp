@@ -22,10 +22,10 @@ void clear_last_error();
 }  // namespace bd
 
 // Wrap a C-ABI body: exceptions become status codes plus the message.
-#define BD_GUARD(body)                                               \
+#define BD_GUARD(...)                                                \
   try {                                                              \
     bd::clear_last_error();                                          \
-    body;                                                            \
+    __VA_ARGS__;                                                     \
     return BD_OK;                                                    \
   } catch (const bd::Error& e) {                                     \
     bd::set_last_error(e.what());                                    \

@@ -426,6 +426,7 @@ GammaOp::GammaOp(const System& s, int cap, int prec) : sys(&s) {
 }
 
 GammaOp::~GammaOp() {
+  destroy_davidson_work(dav);
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
@@ -478,7 +479,11 @@ void GammaOp::apply(const double* x, int64_t ldx, int m, double* y, int64_t ldy)
       float ms = 0.f;
       BD_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
       stage_ms[i] = ms;
+      stage_total_ms[i] += ms;
     }
+    applies += 1.0;
+    pairs_total += pairs;
+    columns_total += m;
   }
   BD_HIP(hipGetLastError());
 }
@@ -500,10 +505,6 @@ void GammaOp::band_density(const double* x, int64_t ldx, int m, const double* w,
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
-struct bd_gamma_op {
-  bd::GammaOp op;
-  bd_gamma_op(const bd::System& s, int cap, int prec) : op(s, cap, prec) {}
-};
 struct bd_system;
 namespace bd {
 const System& system_of(const bd_system* s);
@@ -608,16 +609,32 @@ extern "C" bd_status bd_gamma_op_nonlocal_energies_dev(bd_gamma_op* op, const do
   });
 }
 
-extern "C" bd_status bd_gamma_op_set_profiling(bd_gamma_op* op, int32_t on) {
+extern "C" bd_status bd_gamma_op_plan_info(bd_gamma_op* op, int64_t* out8) {
   BD_GUARD({
-    if (!op) bd::fail(BD_ERR_INVALID_PARAMETER, "null op");
-    op->op.profiling = on != 0;
+    if (!op || !out8) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    const auto& p = op->op.plan;
+    const int64_t v[8] = {p.n0, p.n1, p.n2, p.S, p.Spad, p.Z, p.T, p.ntiles};
+    for (int i = 0; i < 8; ++i) out8[i] = v[i];
   });
 }
 
-extern "C" bd_status bd_gamma_op_stage_times(bd_gamma_op* op, double* out5) {
+extern "C" bd_status bd_gamma_op_set_profiling(bd_gamma_op* op, int32_t on) {
   BD_GUARD({
-    if (!op || !out5) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
-    for (int i = 0; i < 5; ++i) out5[i] = op->op.stage_ms[i];
+    if (!op) bd::fail(BD_ERR_INVALID_PARAMETER, "null op");
+    auto& o = op->op;
+    o.profiling = on != 0;
+    for (double& t : o.stage_total_ms) t = 0.0;
+    o.applies = o.pairs_total = o.columns_total = 0.0;
+  });
+}
+
+extern "C" bd_status bd_gamma_op_stage_times(bd_gamma_op* op, double* out8) {
+  BD_GUARD({
+    if (!op || !out8) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    const auto& o = op->op;
+    for (int i = 0; i < 5; ++i) out8[i] = o.stage_total_ms[i];
+    out8[5] = o.applies;
+    out8[6] = o.pairs_total;
+    out8[7] = o.columns_total;
   });
 }

@@ -50,8 +50,12 @@ struct StickPlan {
   std::vector<int> z_of_zc;        // Z
 };
 
+struct DavidsonWork;
+void destroy_davidson_work(DavidsonWork* w);
+
 struct GammaOp {
   const System* sys;
+  DavidsonWork* dav = nullptr;  // Davidson workspace cached per (n, bands, cap)
   int64_t n, nrep, N;
   int dims[3];
   int capacity;
@@ -76,6 +80,8 @@ struct GammaOp {
   bool profiling = false;
   hipEvent_t ev[6] = {};
   double stage_ms[5] = {};
+  double stage_total_ms[5] = {};
+  double applies = 0.0, pairs_total = 0.0, columns_total = 0.0;
 
   GammaOp(const System& s, int capacity, int precision);
   ~GammaOp();
@@ -89,3 +95,9 @@ StickPlan build_stick_plan(const System& s);
 RadixPlan radix_plan(int n);
 
 }  // namespace bd
+
+// The C ABI handle (include/baandi_amd.h) owns one operator.
+struct bd_gamma_op {
+  bd::GammaOp op;
+  bd_gamma_op(const bd::System& s, int cap, int prec) : op(s, cap, prec) {}
+};

@@ -234,6 +234,65 @@ System::System(const bd_system_desc& d) {
   build_half_sphere();
   build_projectors();
   build_ionic();
+  build_ewald();
+}
+
+// Ewald energy of point ions Z in a neutralising background, erfc(eta r)
+// splitting with eta = sqrt(pi)/Omega^(1/3) (crates/dft-physics/src/ewald.rs:4,380);
+// the total does not depend on eta.
+void System::build_ewald() {
+  const double eta = std::sqrt(kPi) / std::cbrt(volume);
+  const double tol = 1e-20;
+  double rcut = 1.0, gcut = 1.0;
+  while (std::erfc(eta * rcut) > tol) rcut *= 1.1;
+  while (std::exp(-gcut * gcut / (4.0 * eta * eta)) > tol) gcut *= 1.1;
+  int nr[3], ng[3];
+  for (int i = 0; i < 3; ++i) {
+    double blen = std::hypot(std::hypot(B[0][i], B[1][i]), B[2][i]);
+    double alen = std::hypot(std::hypot(lat[0][i], lat[1][i]), lat[2][i]);
+    nr[i] = int(std::ceil(rcut * blen / kTau)) + 1;
+    ng[i] = int(std::ceil(gcut * alen / kTau)) + 1;
+  }
+  const size_t na = atoms.size();
+  double real = 0.0;
+  for (int a = -nr[0]; a <= nr[0]; ++a)
+    for (int b = -nr[1]; b <= nr[1]; ++b)
+      for (int c = -nr[2]; c <= nr[2]; ++c) {
+        double L[3];
+        for (int i = 0; i < 3; ++i) L[i] = lat[i][0] * a + lat[i][1] * b + lat[i][2] * c;
+        for (size_t i = 0; i < na; ++i)
+          for (size_t j = 0; j < na; ++j) {
+            double d2 = 0.0;
+            for (int k = 0; k < 3; ++k) {
+              double dk = atoms[i][k] - atoms[j][k] + L[k];
+              d2 += dk * dk;
+            }
+            if (d2 == 0.0) continue;
+            double r = std::sqrt(d2);
+            if (r < rcut) real += 0.5 * z * z * std::erfc(eta * r) / r;
+          }
+      }
+  double recip = 0.0;
+  for (int a = -ng[0]; a <= ng[0]; ++a)
+    for (int b = -ng[1]; b <= ng[1]; ++b)
+      for (int c = -ng[2]; c <= ng[2]; ++c) {
+        if (a == 0 && b == 0 && c == 0) continue;
+        double G[3];
+        for (int i = 0; i < 3; ++i) G[i] = B[i][0] * a + B[i][1] * b + B[i][2] * c;
+        double g2v = G[0] * G[0] + G[1] * G[1] + G[2] * G[2];
+        if (g2v > gcut * gcut) continue;
+        double sr = 0.0, si = 0.0;
+        for (auto& t : atoms) {
+          double ph = G[0] * t[0] + G[1] * t[1] + G[2] * t[2];
+          sr += z * std::cos(ph);
+          si += z * std::sin(ph);
+        }
+        recip += 2.0 * kPi / volume * (sr * sr + si * si) * std::exp(-g2v / (4.0 * eta * eta)) / g2v;
+      }
+  const double ztot = z * double(na);
+  const double self_e = -eta / std::sqrt(kPi) * z * z * double(na);
+  const double background = -kPi * ztot * ztot / (2.0 * volume * eta * eta);
+  ewald = real + recip + self_e + background;
 }
 
 void System::build_pseudo(const bd_system_desc& d) {
@@ -508,6 +567,7 @@ extern "C" bd_status bd_system_get(const bd_system* sys, const char* name, void*
     else if (k == "ionic_density_re") put(s.vion_re.data(), s.vion_re.size(), 8);
     else if (k == "ionic_density_im") put(s.vion_im.data(), s.vion_im.size(), 8);
     else if (k == "atoms_cart") put(s.atoms.data(), s.atoms.size() * 3, 8);
+    else if (k == "ewald") put(&s.ewald, 1, 8);
     else bd::fail(BD_ERR_INVALID_PARAMETER, "unknown array " + k);
   });
 }

@@ -50,6 +50,7 @@ struct System {
   std::vector<double> dens_g2;
   std::vector<int64_t> dens_flat;
   std::vector<double> vion_re, vion_im;
+  double ewald = 0.0;  // Ha
 
   explicit System(const bd_system_desc& d);
 
@@ -58,6 +59,7 @@ struct System {
   void build_half_sphere();
   void build_projectors();
   void build_ionic();
+  void build_ewald();
 };
 
 }  // namespace bd

@@ -81,7 +81,7 @@ class GammaSystem:
             "density_miller": (3 * self.ndensity, np.int32), "density_g2": (self.ndensity, np.float64),
             "density_index": (self.ndensity, np.int64),
             "ionic_density_re": (self.ndensity, np.float64), "ionic_density_im": (self.ndensity, np.float64),
-            "atoms_cart": (3 * self.natoms, np.float64),
+            "atoms_cart": (3 * self.natoms, np.float64), "ewald": (1, np.float64),
         }
         count, dtype = sizes[name]
         out = np.empty(count, dtype=dtype)
@@ -157,10 +157,21 @@ class DistributedRealHamiltonian:
     def set_profiling(self, on=True):
         L.check(L.load().bd_gamma_op_set_profiling(self.handle, 1 if on else 0))
 
-    def stage_times_ms(self):
-        out = np.zeros(5)
+    def plan_info(self):
+        out = np.zeros(8, dtype=np.int64)
+        L.check(L.load().bd_gamma_op_plan_info(self.handle, L.ptr(out)))
+        keys = ("n0", "n1", "n2", "sticks", "sticks_padded", "z_planes", "tile", "tiles")
+        return {k: int(v) for k, v in zip(keys, out)}
+
+    def stage_totals(self):
+        """Cumulative event timing since set_profiling(True): dict of ms per
+        stage and the launch/pair/column counts."""
+        out = np.zeros(8)
         L.check(L.load().bd_gamma_op_stage_times(self.handle, L.ptr(out)))
-        return out
+        names = ("stick_inverse", "plane", "stick_forward", "projections", "back_product")
+        d = {k: float(v) for k, v in zip(names, out[:5])}
+        d.update(applies=int(out[5]), pairs=int(out[6]), columns=int(out[7]))
+        return d
 
     def __del__(self):
         h = getattr(self, "handle", None)

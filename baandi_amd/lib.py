@@ -21,7 +21,7 @@ EXPORTS = [
     "bd_gamma_op_create", "bd_gamma_op_destroy", "bd_gamma_op_set_potential",
     "bd_gamma_op_set_potential_dev", "bd_gamma_op_apply", "bd_gamma_op_apply_dev",
     "bd_gamma_op_band_density_dev", "bd_gamma_op_nonlocal_energies_dev", "bd_gamma_op_stream",
-    "bd_gamma_op_set_profiling", "bd_gamma_op_stage_times", "bd_davidson_solve_dev",
+    "bd_gamma_op_set_profiling", "bd_gamma_op_stage_times", "bd_gamma_op_plan_info", "bd_davidson_solve_dev",
 ]
 
 
@@ -98,6 +98,7 @@ def load():
         "bd_gamma_op_nonlocal_energies_dev": [P, P, I64, I32, P],
         "bd_gamma_op_set_profiling": [P, I32],
         "bd_gamma_op_stage_times": [P, P],
+        "bd_gamma_op_plan_info": [P, P],
         "bd_davidson_solve_dev": [P, P, I64, P, P, C.POINTER(DavidsonOptions), P, P, P, I64, P,
                                   C.POINTER(DavidsonRecord)],
     }

@@ -1,0 +1,272 @@
+"""Gamma SCF driver around the GPU H·X operator and Davidson (the caller of
+the hot path: crates/dft-engine/src/refscf/driver.rs:479-757 and map.rs:200-384).
+
+Per map: Hartree + LDA-PZ fields of the input density, V = V_ion + V_H + V_xc
+installed on the operator, the reference Davidson on the GPU with the ethr
+schedule (refscf/profile.rs:49-51) and the iteration-1 redo, the output
+density from the GPU band-density transform, the half-sphere Hartree metric
+dr2 (refscf/metric.rs) and difference-history Anderson mixing with beta 0.7,
+history 8 (refscf/broyden.rs:317-445). Stops when dr2 < threshold_ry.
+
+The density-sphere field glue (grid transforms of the density and the Hartree
+potential, pointwise XC, mixing) runs on the host in NumPy in this version.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import lib as L
+from .hamiltonian import DistributedRealHamiltonian, GammaSystem
+from . import ref_davidson
+
+
+def lda_pz(n):
+    """Slater exchange + PZ81 correlation per point (crates/dft-physics/src/xc.rs:48-73):
+    returns (eps_xc, v_xc); zero at or below 1e-10."""
+    eps = np.zeros_like(n)
+    vxc = np.zeros_like(n)
+    live = n > 1e-10
+    d = n[live]
+    cd = np.cbrt(d)
+    rs = np.cbrt(3.0 / (4.0 * math.pi)) / cd
+    ex = -0.75 * np.cbrt(3.0 / math.pi) * cd
+    ec = np.empty_like(d)
+    dec = np.empty_like(d)
+    low = rs >= 1.0
+    sr = np.sqrt(rs[low])
+    den = 1.0 + 1.0529 * sr + 0.3334 * rs[low]
+    ec[low] = -0.1423 / den
+    dec[low] = 0.1423 * (1.0529 / (2.0 * sr) + 0.3334) / (den * den)
+    rh = rs[~low]
+    lg = np.log(rh)
+    ec[~low] = 0.0311 * lg - 0.048 + 0.0020 * rh * lg - 0.0116 * rh
+    dec[~low] = 0.0311 / rh + 0.0020 * (lg + 1.0) - 0.0116
+    eps[live] = ex + ec
+    vxc[live] = 4.0 * ex / 3.0 + ec - rs * dec / 3.0
+    return eps, vxc
+
+
+@dataclass
+class ScfOptions:
+    bands: int
+    threshold_ry: float = 1e-8       # refscf/profile.rs:17-26
+    beta: float = 0.7
+    history: int = 8
+    max_iterations: int = 100
+    davidson_max_iterations: int = 20
+    preconditioner: int = L.BD_PRECOND_QE
+    seed: int = 1
+    precision: int = L.BD_FP64
+    # Optional fixed-point verification map (driver.rs:727-757): one more
+    # Davidson at the converged potential with every band at this threshold (Ha).
+    final_threshold: float = 0.0
+
+
+class DensityField:
+    """Density-sphere coefficients (4 Ecut) and their grid transforms."""
+
+    def __init__(self, system: GammaSystem):
+        self.dims = system.dims
+        self.size = int(np.prod(self.dims))
+        self.flat = system.get("density_index")
+        self.g2 = system.get("density_g2")
+        miller = system.get("density_miller")
+        self.zero = np.all(miller == 0, axis=1)
+        # Positive representatives carry the half-sphere Hartree weight
+        # 8 pi Omega / G^2 in Ry (refscf/metric.rs:1-5).
+        first = np.where(miller[:, 0] != 0, miller[:, 0], np.where(miller[:, 1] != 0, miller[:, 1], miller[:, 2]))
+        positive = first > 0
+        self.metric = np.zeros(len(self.g2))
+        self.metric[positive] = 8.0 * math.pi * system.volume / self.g2[positive]
+        self.coulomb = np.zeros(len(self.g2))
+        self.coulomb[~self.zero] = 4.0 * math.pi / self.g2[~self.zero]
+
+    def to_grid(self, coeffs):
+        g = np.zeros(self.size, dtype=np.complex128)
+        g[self.flat] = coeffs
+        return np.fft.ifftn(g.reshape(self.dims)) * self.size
+
+    def from_grid(self, field):
+        return (np.fft.fftn(field) / self.size).reshape(-1)[self.flat]
+
+    def dot(self, a, b):
+        return float(np.sum(self.metric * (a.real * b.real + a.imag * b.imag)))
+
+
+class Mixer:
+    """Difference-history Anderson update with the Hartree metric (broyden.rs)."""
+
+    def __init__(self, field: DensityField, beta, history):
+        self.field, self.beta, self.history = field, beta, history
+        self.pairs = []  # (dx, dF), newest last
+        self.prev = None
+
+    def update(self, x, f):
+        if self.prev is not None:
+            px, pf = self.prev
+            self.pairs.append((px - x, pf - f))
+            if len(self.pairs) > self.history:
+                self.pairs.pop(0)
+        self.prev = (x.copy(), f.copy())
+        xx, ff = x.copy(), f.copy()
+        while self.pairs:
+            p = len(self.pairs)
+            a = np.empty((p, p))
+            for i in range(p):
+                for j in range(i + 1):
+                    a[i, j] = a[j, i] = self.field.dot(self.pairs[i][1], self.pairs[j][1])
+            c = np.array([self.field.dot(df, f) for _, df in self.pairs])
+            lam, q = np.linalg.eigh(a)
+            if lam[-1] <= 0.0 or lam[0] <= math.sqrt(p * np.finfo(float).eps) * lam[-1]:
+                self.pairs.pop(0)  # rank-deficient history: drop the oldest pair
+                continue
+            gamma = q @ ((q.T @ c) / lam)
+            for g, (dx, df) in zip(gamma, self.pairs):
+                xx -= g * dx
+                ff -= g * df
+            break
+        return xx + self.beta * ff
+
+
+def next_ethr_ry(previous, dr2_ry, electrons):
+    """refscf/profile.rs:49-51."""
+    return max(min(previous, 0.1 * dr2_ry / max(electrons, 1.0)), 1e-13)
+
+
+def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, profile_kernels=False):
+    """SCF to dr2 < threshold_ry; returns a dict with energies (Ha), eigenvalues,
+    per-iteration history and timings. ``start``: optional (bands, n) CUDA tensor."""
+    import torch
+    dev = torch.device("cuda")
+    t_setup = time.perf_counter()
+    bands = options.bands
+    electrons = system.electrons
+    occupied = int(round(electrons)) // 2
+    occ = np.zeros(bands)
+    occ[:occupied] = 2.0
+    field = DensityField(system)
+    vion_g = system.get("ionic_density_re") + 1j * system.get("ionic_density_im")
+    vion = field.to_grid(vion_g).real
+    op = DistributedRealHamiltonian(system, capacity=bands, precision=options.precision)
+    diag_static = torch.from_numpy(system.get("diagonal_static")).to(dev)
+    kinetic = system.get("kinetic_orc")
+    ewald = float(system.get("ewald")[0])
+    # Start density: superposed Gaussian atomic charges Z (alpha/pi)^1.5 e^{-alpha r^2}, alpha = 1/rc^2.
+    atoms = system.get("atoms_cart")
+    rows = np.array(system.lattice_rows)
+    b = 2.0 * math.pi * np.linalg.inv(rows).T  # reciprocal vectors as rows
+    dm = system.get("density_miller").astype(np.float64)
+    dcart = dm @ b
+    sfac = np.exp(-1j * (dcart @ atoms.T)).sum(axis=1)
+    x = system.z_valence / system.volume * np.exp(-field.g2 * system.rc ** 2 / 4.0) * sfac
+    x[field.zero] = electrons / system.volume
+    mixer = Mixer(field, options.beta, options.history)
+    if start is None:
+        g = torch.Generator(device="cpu").manual_seed(options.seed)
+        start = torch.randn(bands, system.n, generator=g, dtype=torch.float64)
+        start = (start / start.norm(dim=1, keepdim=True)).to(dev)
+    orbitals = start.contiguous()
+    xout = torch.empty_like(orbitals)
+    hxout = torch.empty_like(orbitals)
+    rho_dev = torch.empty(field.size, dtype=torch.float64, device=dev)
+    if profile_kernels:
+        op.set_profiling(True)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t_setup
+
+    t0 = time.perf_counter()
+    ethr = 1e-2
+    last_dr2 = math.inf
+    history = []
+    totals = dict(h_columns=0, davidson_iterations=0, dense_s=0.0, apply_s=0.0, davidson_s=0.0, fields_s=0.0)
+    result = None
+    for iteration in range(1, options.max_iterations + 1):
+        if iteration > 1:
+            ethr = next_ethr_ry(1e-2 if iteration == 2 else ethr, last_dr2, electrons)
+        redo = False
+        while True:
+            tf = time.perf_counter()
+            vh_g = field.coulomb * x
+            both = field.to_grid(x + 1j * vh_g)  # rho_in + i V_H (both real fields)
+            rho_in, vh = both.real, both.imag
+            eps, vxc = lda_pz(np.abs(rho_in))
+            potential = vion + vh + vxc
+            pot_dev = torch.from_numpy(potential.reshape(-1)).to(dev)
+            op.set_owned_potential(pot_dev)
+            diag = diag_static + float(potential.mean())
+            totals["fields_s"] += time.perf_counter() - tf
+            td = time.perf_counter()
+            opts = ref_davidson.Options(bands=bands, threshold=ethr / 2.0, occupied=occupied,
+                                        first_step=(iteration == 1 and not redo),
+                                        max_iterations=options.davidson_max_iterations, return_hx=True,
+                                        preconditioner=options.preconditioner)
+            sol = ref_davidson.solve_with_initial_hx(op, orbitals, diag, opts, out=(xout, hxout))
+            totals["davidson_s"] += time.perf_counter() - td
+            totals["h_columns"] += sol.record["h_columns"]
+            totals["davidson_iterations"] += sol.record["iterations"]
+            totals["dense_s"] += sol.record["seconds_dense"]
+            totals["apply_s"] += sol.record["seconds_apply"]
+            tf = time.perf_counter()
+            op.band_density_sum(xout, occ, rho_dev)
+            rho_out = rho_dev.cpu().numpy().reshape(field.dims)
+            y = field.from_grid(rho_out)
+            y[field.zero] = electrons / system.volume
+            f = y - x
+            dr2 = field.dot(f, f)
+            totals["fields_s"] += time.perf_counter() - tf
+            if iteration == 1 and not redo and dr2 < ethr * max(electrons, 1.0):
+                ethr = next_ethr_ry(ethr, dr2, electrons)
+                redo = True
+                orbitals, xout = xout, orbitals
+                continue
+            break
+        orbitals, xout = xout, orbitals
+        last_dr2 = dr2
+        history.append(dict(iteration=iteration, dr2_ry=dr2, ethr_ry=ethr,
+                            davidson_iterations=sol.record["iterations"], h_columns=sol.record["h_columns"],
+                            converged=bool(sol.converged.all())))
+        if log:
+            log(f"scf {iteration:3d} dr2={dr2:.3e} ethr={ethr:.2e} dav_it={sol.record['iterations']} "
+                f"hcols={sol.record['h_columns']}")
+        if dr2 < options.threshold_ry:
+            result = (sol, rho_in, potential, rho_out)
+            break
+        x = mixer.update(x, f)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    if result is None:
+        raise RuntimeError(f"SCF did not reach dr2 < {options.threshold_ry} Ry in {options.max_iterations} maps")
+    sol, rho_in, potential, rho_out = result
+    if options.final_threshold > 0.0:
+        diag = diag_static + float(potential.mean())
+        opts = ref_davidson.Options(bands=bands, threshold=options.final_threshold, occupied=occupied,
+                                    first_step=True, max_iterations=200, return_hx=True,
+                                    preconditioner=options.preconditioner)
+        sol = ref_davidson.solve_with_initial_hx(op, orbitals, diag, opts, out=(xout, hxout))
+        orbitals, xout = xout, orbitals
+        op.band_density_sum(orbitals, occ, rho_dev)
+        rho_out = rho_dev.cpu().numpy().reshape(field.dims)
+    # Energy audit of the converged map (refscf/map.rs:466-540).
+    xo = orbitals.cpu().numpy()
+    kin = float(np.sum(occ[:, None] * xo ** 2 * kinetic[None, :]))
+    nonlocal_ = float(np.dot(op.nonlocal_band_energies(orbitals), occ))
+    dv = system.volume / field.size
+    yo = field.from_grid(rho_out)
+    vh_out = field.to_grid(field.coulomb * yo).real
+    hartree = 0.5 * dv * float(np.sum(rho_out * vh_out))
+    eps_o, _ = lda_pz(np.abs(rho_out))
+    xc = dv * float(np.sum(np.abs(rho_out) * eps_o))
+    local = dv * float(np.sum(rho_out * vion))
+    total = kin + local + nonlocal_ + hartree + xc + ewald
+    energy = {"kinetic": kin, "local": local, "nonlocal": nonlocal_, "hartree": hartree, "xc": xc,
+              "ewald": ewald, "total": total, "band_sum": float(np.dot(sol.eigenvalues, occ))}
+    out = dict(energy=energy, eigenvalues=sol.eigenvalues.copy(), history=history, wall_s=wall,
+               setup_s=t_setup, iterations=len(history), totals=totals, orbitals=orbitals, op=op,
+               plan=op.plan_info())
+    if profile_kernels:
+        out["kernel_totals"] = op.stage_totals()
+    return out

@@ -1,0 +1,177 @@
+#!/usr/bin/env python3
+"""Si64 Gamma SCF wall time on one MI355X (the Davidson H·ψ path end to end).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+A step is one full SCF of the Si64 cubic supercell (2x2x2 diamond cells,
+a = 20.52 Bohr, Gamma, 30 Ry, 160 bands, 128 occupied, synthetic Z=4
+pseudopotential, seeded N(0,1) start) to dr2 < 1e-8 Ry, with all inputs
+already resident on the GPU. Prints one JSON line.
+
+The roofline object covers the dominant H·X kernel (k_plane: y/z transforms
+with the V(r) multiply), timed with HIP events on the operator's stream over
+the timed region. The CPU baseline times the FP64 oracle's H·X (NumPy, one
+thread) on a bounded sample of the same Si64 columns.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Si64 SCF wall-time to 1e-8 Ha/atom: mixed+learned vs plain FP64, 1 MI355X"
+
+
+def plane_bytes_model(plan, pairs, c=16, r=8):
+    """Algorithmic bytes of the y and z stages of the 3-pass pencil model
+    (per pair 4 c (Z n1 n0 + N) + r N for V), for `pairs` pairs."""
+    n0, n1, n2, Z = plan["n0"], plan["n1"], plan["n2"], plan["z_planes"]
+    N = n0 * n1 * n2
+    return pairs * (4 * c * (Z * n1 * n0 + N) + r * N)
+
+
+def fused_bytes(plan, pairs, c=16, r=8):
+    """Compulsory HBM bytes of k_plane as built: sticks in and out, V once."""
+    n0, n1, n2 = plan["n0"], plan["n1"], plan["n2"]
+    return 2 * c * pairs * n0 * plan["sticks"] + r * n0 * n1 * n2
+
+
+def cpu_baseline(system, seconds=12.0):
+    """Oracle (NumPy FP64) H·X columns/s on the Si64 set-up, one BLAS thread."""
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+    from oracle import physics as P
+    from oracle.hamiltonian import apply_orc
+    from oracle.fields import reciprocal_g2  # noqa: F401  (oracle import check)
+
+    class View:  # the oracle's apply reads only these set-up arrays
+        pass
+
+    v = View()
+    v.dims = system.dims
+    miller = system.get("miller")
+    basis = P.Basis(miller=miller, cart=None, g2=system.get("g2"))
+    v.half = P.half_sphere(basis)
+    v.fft_index = P.flat_index(P.map_miller(miller, v.dims), v.dims)
+    v.kinetic_orc = system.get("kinetic_orc")
+    v.orc_p = system.get("projectors_orc")
+    cpl = system.get("coupling")
+    nc = 5
+    v.coupling = [(s, nc, cpl[s:s + nc, s:s + nc].copy()) for s in range(0, system.nproj, nc)]
+    rng = np.random.default_rng(0)
+    pot = rng.standard_normal(v.dims) * 0.1
+    cols = 8
+    x = rng.standard_normal((system.n, cols))
+    done, t0 = 0, time.perf_counter()
+    with threadpool_limits(1):
+        while time.perf_counter() - t0 < seconds:
+            apply_orc(v, x, pot)
+            done += cols
+    dt = time.perf_counter() - t0
+    return done / dt, done, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one MI355X only: this build has no multi-GPU path "
+              "(run with --gpus 1).", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from baandi_amd import workloads
+    from baandi_amd.scf import run_scf, ScfOptions
+
+    wl = workloads.si64()
+    t = time.perf_counter()
+    system = workloads.build(wl)
+    setup_s = time.perf_counter() - t
+    opts = ScfOptions(bands=wl["bands"], threshold_ry=1e-8)
+    print(f"[bench] {wl['name']}: n={system.n} grid={system.dims} proj={system.nproj} setup={setup_s:.2f}s",
+          file=sys.stderr, flush=True)
+    for w in range(args.warmup):
+        r = run_scf(system, opts)
+        print(f"[bench] warmup {w}: {r['wall_s']:.3f}s iterations={r['iterations']}", file=sys.stderr, flush=True)
+    walls, results = [], []
+    torch.cuda.synchronize()
+    t_all = time.perf_counter()
+    for k in range(args.steps):
+        r = run_scf(system, opts, profile_kernels=True)
+        walls.append(r["wall_s"])
+        results.append(r)
+        print(f"[bench] step {k}: {r['wall_s']:.3f}s iterations={r['iterations']} E={r['energy']['total']:.12f} "
+              f"totals={r['totals']}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_all
+    last = results[-1]
+    st = last["kernel_totals"]
+    plan = last["plan"]
+    launches = st["applies"]
+    plane_ms = st["plane"]
+    achieved = plane_bytes_model(plan, st["pairs"]) / (plane_ms * 1e-3) / 1e9 if plane_ms > 0 else 0.0
+    fused = fused_bytes(plan, st["pairs"]) / (plane_ms * 1e-3) / 1e9 if plane_ms > 0 else 0.0
+    per_launch_bytes = plane_bytes_model(plan, st["pairs"] / max(launches, 1))
+    ms_per_step = 1e3 * sum(walls) / len(walls)
+    out = {
+        "metric": METRIC,
+        "value": round(ms_per_step / 1e3, 4),
+        "unit": "s (FP64 SCF wall)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded N(0,1) start, synthetic Gaussian pseudopotential)",
+        "config": {"workload": "Si64 cubic 2x2x2, a=20.52 Bohr, Gamma, 30 Ry, 160 bands (128 occ), "
+                               "SCF to dr2<1e-8 Ry, plain FP64 path",
+                   "n_pw": int(system.n), "fft_grid": list(system.dims), "bands": wl["bands"],
+                   "scf_iterations": last["iterations"], "h_columns": last["totals"]["h_columns"],
+                   "mixed_learned_s": None, "speedup_mixed_vs_fp64": None},
+        "energy_ha": last["energy"]["total"],
+        "breakdown_s": {k: round(v, 4) for k, v in last["totals"].items() if k.endswith("_s")},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_plane",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "model": "3-pass pencil model, y+z stages: 4*16*(Z*n1*n0+N)+8*N bytes per pair",
+            "bytes_per_launch_avg": per_launch_bytes,
+            "avg_launch_ms": round(plane_ms / max(launches, 1), 4),
+            "launches": launches,
+            "fused_compulsory_GBs": round(fused, 1),
+        },
+    }
+    if not args.no_cpu:
+        cps, cols, dt = cpu_baseline(system, args.cpu_seconds)
+        gpu_cps = st["columns"] / (sum(st[k] for k in ("stick_inverse", "plane", "stick_forward", "projections",
+                                                        "back_product")) * 1e-3)
+        out["cpu_baseline"] = {
+            "value": round(cps, 2), "unit": "H·X columns/s (Si64 FP64)", "cores": 1, "kind": "port",
+            "sample": f"oracle apply_orc on {cols} Si64 ORC columns in {dt:.1f}s (NumPy FFT, 1 BLAS thread)",
+            "gpu_value": round(gpu_cps, 1),
+            "scf_hx_only_estimate_s": round(last["totals"]["h_columns"] / cps, 1),
+        }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

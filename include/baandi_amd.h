@@ -92,6 +92,7 @@ bd_status bd_system_get_info(const bd_system* sys, bd_system_info* out);
  *   "density_index"    int64, ndensity  flat FFT index
  *   "ionic_density_re", "ionic_density_im" double, ndensity  V_ion(G), Ha
  *   "atoms_cart"       double, 3*natoms
+ *   "ewald"            double, 1   Ewald energy of the ions (crates/dft-physics/src/ewald.rs)
  */
 bd_status bd_system_get(const bd_system* sys, const char* name, void* out, int64_t count);
 
@@ -128,11 +129,15 @@ bd_status bd_gamma_op_nonlocal_energies_dev(bd_gamma_op* op, const double* x_dev
                                             int32_t m, double* out_host);
 /* HIP stream the op runs on (hipStream_t as void*). */
 void* bd_gamma_op_stream(bd_gamma_op* op);
-/* Mean kernel time (ms) of the last apply per stage, measured with HIP events
- * on the op's stream when profiling is on: [stick_inverse, plane, stick_forward,
- * projections, back_product]. */
+/* Kernel timing with HIP events on the op's stream (adds one event wait per
+ * apply). set_profiling(on) also resets the totals. stage_times fills 8
+ * values: cumulative ms of [stick_inverse, plane, stick_forward, projections,
+ * back_product] since profiling was switched on, then applies, pairs, columns. */
 bd_status bd_gamma_op_set_profiling(bd_gamma_op* op, int32_t on);
-bd_status bd_gamma_op_stage_times(bd_gamma_op* op, double* out5_ms);
+/* FFT plan shape: n0, n1, n2, occupied sticks S, padded sticks, occupied z
+ * planes Z, sticks per tile, tiles. */
+bd_status bd_gamma_op_plan_info(bd_gamma_op* op, int64_t* out8);
+bd_status bd_gamma_op_stage_times(bd_gamma_op* op, double* out8);
 
 /* ------------------------------------------------------------------------
  * Reference block Davidson on the op

@@ -1,0 +1,83 @@
+"""CPU checks of the C ABI library: it loads, exports every symbol the header
+declares, and its native set-up (basis order, grid, ORC maps, projector rows,
+V_ion(G), Davidson diagonal, Ewald) matches the FP64 oracle. No GPU calls."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, fcc_rows
+from oracle import physics as P
+from oracle.scf import davidson_diagonal
+
+HEADER = os.path.join(ROOT, "include", "baandi_amd.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(bd_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    import ctypes
+    from baandi_amd import lib as L
+    lib = L.load()
+    names = declared_functions()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(L.EXPORTS)
+
+
+CELLS = {
+    "kat1": (fcc_rows(10.261212856717933), [[0, 0, 0], [0.25] * 3], 4.0, 5.0, 801),
+    "si2_20ry": (fcc_rows(10.26), [[0, 0, 0], [0.25] * 3], 4.0, 10.0, 801),
+    "kat2_skew": ([[5.0, 0, 0], [0.2, 5.0, 0], [0, 0.1, 5.0]], [[0.13, 0.21, 0.32]], 2.0, 1.0, 161),
+}
+
+
+@pytest.mark.parametrize("name", list(CELLS))
+def test_native_setup_matches_oracle(name):
+    from baandi_amd.hamiltonian import GammaSystem
+    rows, atoms, z, ecut, points = CELLS[name]
+    s = GammaSystem(rows, atoms, z, ecut, mesh_points=points)
+    o = P.build_gamma_system(rows, atoms, P.synthetic_pseudo(z, 1.0, points), ecut)
+    assert s.n == len(o.basis.g2) and s.dims == o.dims and s.nproj == o.orc_p.shape[1]
+    assert np.array_equal(s.get("miller"), o.basis.miller)
+    assert np.array_equal(s.get("density_miller"), o.density.miller)
+    assert np.abs(s.get("g2") - o.basis.g2).max() == 0.0
+    assert np.abs(s.get("kinetic_orc") - o.kinetic_orc).max() < 1e-14
+    assert np.abs(s.get("projectors_orc") - o.orc_p).max() < 1e-13
+    assert np.abs(s.get("diagonal_static") - davidson_diagonal(o)).max() < 1e-12
+    vi = s.get("ionic_density_re") + 1j * s.get("ionic_density_im")
+    assert np.abs(vi - P.ionic_coefficients(o.density, o.cell.volume, o.species)).max() < 1e-13
+    flat = P.flat_index(P.map_miller(o.basis.miller, o.dims), o.dims)
+    assert np.array_equal(s.get("plus_index"), flat[o.half.plus])
+    assert np.array_equal(s.get("minus_index"), flat[o.half.minus])
+    ew = P.ewald(o.cell, [z] * len(atoms), o.species[0].cart)["energy"]
+    assert abs(s.get("ewald")[0] - ew) < 1e-11
+
+
+def test_invalid_descriptors_are_errors():
+    from baandi_amd.hamiltonian import GammaSystem
+    from baandi_amd import lib as L
+    with pytest.raises(L.BaandiError) as e:
+        GammaSystem(fcc_rows(10.26), [[0, 0, 0]], 4.0, -1.0)
+    assert e.value.status == L.BD_ERR_INVALID_PARAMETER
+    with pytest.raises(L.BaandiError):
+        GammaSystem([[1, 0, 0], [2, 0, 0], [0, 0, 1]], [[0, 0, 0]], 4.0, 5.0)  # singular lattice
+
+
+def test_operator_needs_a_gfx950_device():
+    """Without a GPU the operator fails loudly (no CPU fallback)."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from baandi_amd.hamiltonian import GammaSystem, DistributedRealHamiltonian
+    from baandi_amd import lib as L
+    s = GammaSystem(*CELLS["kat2_skew"][:4], mesh_points=161)
+    with pytest.raises(L.BaandiError) as e:
+        DistributedRealHamiltonian(s, capacity=4)
+    assert e.value.status == L.BD_ERR_DEVICE

@@ -1,0 +1,146 @@
+"""GPU parity of the reference Davidson and the Gamma SCF against the FP64
+oracle and the reference goldens, through the C ABI.
+
+Bars (north star): eigenvalues <= 1e-10 Ha against a dense FP64 eigensolve;
+SCF total energy <= 1e-8 Ha/atom against the stored goldens (KAT-1 certified
+-4.684561184277879 Ha; KAT-2 -0.8646074276373095 Ha)."""
+import numpy as np
+import pytest
+
+from conftest import fcc_rows
+
+pytestmark = pytest.mark.gpu
+
+
+SI8 = ([[10.26, 0, 0], [0, 10.26, 0], [0, 0, 10.26]],
+       [[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0],
+        [.25, .25, .25], [.25, .75, .75], [.75, .25, .75], [.75, .75, .25]])
+
+
+def si2(ecut=10.0, a=10.26):
+    import torch
+    from baandi_amd.hamiltonian import GammaSystem
+    from oracle import physics as P
+    rows, atoms = fcc_rows(a), [[0, 0, 0], [0.25] * 3]
+    s = GammaSystem(rows, atoms, 4.0, ecut)
+    o = P.build_gamma_system(rows, atoms, P.synthetic_pseudo(4.0, 1.0, 801), ecut)
+    return torch, s, o
+
+
+def si8(ecut=6.0):
+    import torch
+    from baandi_amd.hamiltonian import GammaSystem
+    from oracle import physics as P
+    s = GammaSystem(*SI8, 4.0, ecut)
+    o = P.build_gamma_system(*SI8, P.synthetic_pseudo(4.0, 1.0, 801), ecut)
+    return torch, s, o
+
+
+@pytest.mark.parametrize("cell,bands,occupied,precond",
+                         [("si2", 8, 4, 0), ("si2", 8, 4, 1), ("si8", 24, 16, 0), ("si8", 24, 16, 1)])
+def test_davidson_matches_dense_eigensolve(cell, bands, occupied, precond):
+    """Same H, same seeded start: GPU Davidson vs dense LAPACK. Occupied
+    bands converge at tau = 1e-12 Ha and must match to <= 1e-10 Ha; empty
+    bands use the reference's max(5 tau, 5e-6) rule (ref_davidson.rs:70-77)."""
+    from baandi_amd.hamiltonian import DistributedRealHamiltonian
+    from baandi_amd import ref_davidson
+    from oracle.hamiltonian import dense_orc
+    torch, s, o = si2() if cell == "si2" else si8()
+    print(f"[davidson parity] cell={cell} n={s.n} bands={bands} precond={precond}", flush=True)
+    pot = o.ionic_grid + 0.05 * np.cos(np.arange(o.ionic_grid.size)).reshape(o.dims)
+    hd = dense_orc(o, pot)
+    w = np.linalg.eigvalsh(hd)
+    op = DistributedRealHamiltonian(s, capacity=bands)
+    op.set_owned_potential(pot.reshape(-1))
+    g = np.random.default_rng(1).standard_normal((bands, s.n))
+    x0 = torch.from_numpy(g / np.linalg.norm(g, axis=1, keepdims=True)).cuda()
+    diag = torch.from_numpy(s.get("diagonal_static") + pot.mean()).cuda()
+    opts = ref_davidson.Options(bands=bands, threshold=1e-12, occupied=occupied, max_iterations=300,
+                                return_hx=True, preconditioner=precond)
+    sol = ref_davidson.solve_with_initial_hx(op, x0, diag, opts)
+    print(f"[davidson parity] record={sol.record}", flush=True)
+    assert sol.converged.all()
+    assert np.abs(sol.eigenvalues[:occupied] - w[:occupied]).max() < 1e-10
+    assert np.abs(sol.eigenvalues - w[:bands]).max() < 1e-3  # empty bands: Ritz change < 5e-6 only
+    x = sol.eigenvectors.cpu().numpy()
+    hx = sol.hx.cpu().numpy()
+    assert np.abs(x @ x.T - np.eye(bands)).max() < 1e-10
+    assert np.abs(hx - (hd @ x.T).T).max() < 1e-10
+
+
+def test_davidson_rejects_nonfinite_start():
+    from baandi_amd.hamiltonian import DistributedRealHamiltonian
+    from baandi_amd import ref_davidson, lib as L
+    torch, s, o = si2()
+    op = DistributedRealHamiltonian(s, capacity=4)
+    pot = o.ionic_grid.copy()
+    pot[0, 0, 0] = np.nan  # a NaN potential makes H X non-finite
+    op.set_owned_potential(pot.reshape(-1))
+    x0 = torch.eye(4, s.n, dtype=torch.float64).cuda()
+    diag = torch.from_numpy(s.get("diagonal_static")).cuda()
+    with pytest.raises(L.BaandiError) as e:
+        ref_davidson.solve(op, x0, diag, ref_davidson.Options(bands=4, threshold=1e-8, occupied=2))
+    assert e.value.status == L.BD_ERR_APPLICATION
+
+
+def test_davidson_iteration_limit_is_reported():
+    from baandi_amd.hamiltonian import DistributedRealHamiltonian
+    from baandi_amd import ref_davidson
+    torch, s, o = si2()
+    op = DistributedRealHamiltonian(s, capacity=8)
+    op.set_owned_potential(o.ionic_grid.reshape(-1))
+    g = np.random.default_rng(2).standard_normal((8, s.n))
+    x0 = torch.from_numpy(g).cuda()
+    diag = torch.from_numpy(s.get("diagonal_static")).cuda()
+    sol = ref_davidson.solve(op, x0, diag, ref_davidson.Options(bands=8, threshold=1e-14, occupied=4,
+                                                                max_iterations=2))
+    assert sol.record["iterations"] == 2 and not sol.converged.all()
+    with pytest.raises(ref_davidson.NotConverged):
+        ref_davidson.solve(op, x0, diag, ref_davidson.Options(bands=8, threshold=1e-14, occupied=4,
+                                                              max_iterations=2), raise_not_converged=True)
+
+
+def test_scf_kat1_golden(kat1):
+    from baandi_amd.hamiltonian import GammaSystem
+    from baandi_amd.scf import run_scf, ScfOptions
+    s = GammaSystem(fcc_rows(kat1["cell_a_bohr"]), [[0, 0, 0], [0.25] * 3], 4.0, kat1["ecut_ha"])
+    r = run_scf(s, ScfOptions(bands=8, threshold_ry=1e-13, final_threshold=1e-13))
+    e = r["energy"]
+    print(f"[kat1] E={e['total']!r} iterations={r['iterations']}", flush=True)
+    assert abs(e["total"] - kat1["certified_energy_ha"]) / 2 < 1e-8  # per atom
+    assert abs(e["total"] - kat1["certified_energy_ha"]) < 1e-9
+    # The stored eigenvalues carry the reference's own certificate of 2.78e-8 Ha
+    # (summary.json certificate_occupied_eigenvalue_change_hartree).
+    assert np.abs(r["eigenvalues"] - np.array(kat1["eigenvalues_ha"])).max() < 5e-8
+    # Single terms are first order in the density error of the stored run
+    # (density residual 1.1e-8); the total above is variational. The formulas
+    # of each term are pinned to 1e-12 by the stored-orbital audit test.
+    for k in ("kinetic", "local", "nonlocal", "hartree", "xc", "ewald"):
+        assert abs(e[k] - kat1["energy_terms_ha"][k]) < 5e-7, k
+
+
+def test_scf_kat2_golden():
+    from baandi_amd.hamiltonian import GammaSystem
+    from baandi_amd.scf import run_scf, ScfOptions
+    s = GammaSystem([[5.0, 0, 0], [0.2, 5.0, 0], [0, 0.1, 5.0]], [[0.13, 0.21, 0.32]], 2.0, 1.0, mesh_points=161)
+    r = run_scf(s, ScfOptions(bands=4, threshold_ry=1e-13))
+    assert abs(r["energy"]["total"] - (-0.8646074276373095)) < 1e-9
+
+
+def test_scf_si8_matches_oracle_scf():
+    """Cubic Si8 at 6 Ha: GPU SCF vs oracle SCF from their own seeded starts."""
+    from baandi_amd.hamiltonian import GammaSystem
+    from baandi_amd.scf import run_scf, ScfOptions
+    from oracle import physics as P
+    from oracle.scf import scf as oracle_scf
+    rows, atoms = SI8
+    s = GammaSystem(rows, atoms, 4.0, 6.0)
+    r = run_scf(s, ScfOptions(bands=20, threshold_ry=1e-13, final_threshold=1e-13))
+    o = P.build_gamma_system(rows, atoms, P.synthetic_pseudo(4.0, 1.0, 801), 6.0)
+    ro = oracle_scf(o, 20, threshold_ry=1e-13, final_tight=1e-13)
+    print(f"[si8] gpu={r['energy']['total']!r} oracle={ro['energy']['total']!r}", flush=True)
+    assert abs(r["energy"]["total"] - ro["energy"]["total"]) / 8 < 1e-8
+    # Eigenvalues are first order in the residual density error of each run
+    # (degenerate clusters split by ~1e-8 at dr2 < 1e-13 Ry); the 1e-10 Ha
+    # eigenvalue bar is checked at a fixed H in test_davidson_matches_dense_eigensolve.
+    assert np.abs(r["eigenvalues"][:16] - ro["eigenvalues"][:16]).max() < 1e-6
