@@ -26,6 +26,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# Measured HBM bytes per pair of k_plane at Si64 FP64: rocprofv3 FETCH_SIZE (x2.000,
+# the 256 MiB copy calibration) + WRITE_SIZE (x1.000) of an 80-pair launch,
+# 205081.8 KiB / 80 pairs (profiles/k_plane_si64_fp64.txt).
+PLANE_TRAFFIC_BYTES_PER_PAIR = 205081.8 * 1024 / 80
 METRIC = "Si64 SCF wall-time to 1e-8 Ha/atom: mixed+learned vs plain FP64, 1 MI355X"
 
 
@@ -78,6 +82,34 @@ def cpu_baseline(system, seconds=12.0):
     return done / dt, done, dt
 
 
+def hx_profile(system, wl, count):
+    """Fixed-shape H·X launches for counter runs, then a plain copy kernel of
+    known size (the FETCH_SIZE/WRITE_SIZE calibration)."""
+    import numpy as np
+    import torch
+    from baandi_amd.hamiltonian import DistributedRealHamiltonian
+    bands = wl["bands"]
+    op = DistributedRealHamiltonian(system, capacity=bands)
+    rng = np.random.default_rng(0)
+    op.set_owned_potential(torch.from_numpy(rng.standard_normal(int(np.prod(system.dims))) * 0.1).cuda())
+    x = torch.from_numpy(rng.standard_normal((bands, system.n))).cuda()
+    y = torch.empty_like(x)
+    op.set_profiling(True)
+    for _ in range(count):
+        op.apply_block(x, y)
+    torch.cuda.synchronize()
+    st = op.stage_totals()
+    a = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device="cuda")  # 256 MiB
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    plan = op.plan_info()
+    print(json.dumps({"mode": "hx_only", "applies": count, "columns": bands, "plan": plan,
+                      "stage_ms_per_apply": {k: v / count for k, v in st.items()
+                                             if k not in ("applies", "pairs", "columns")},
+                      "copy_bytes_each_way": a.numel() * 8}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +117,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--hx-only", type=int, default=0,
+                    help="profile mode: N applies of H to 160 Si64 columns, then a 256 MiB copy")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X only: this build has no multi-GPU path "
@@ -99,6 +133,9 @@ def main():
     t = time.perf_counter()
     system = workloads.build(wl)
     setup_s = time.perf_counter() - t
+    if args.hx_only:
+        hx_profile(system, wl, args.hx_only)
+        return
     opts = ScfOptions(bands=wl["bands"], threshold_ry=1e-8)
     print(f"[bench] {wl['name']}: n={system.n} grid={system.dims} proj={system.nproj} setup={setup_s:.2f}s",
           file=sys.stderr, flush=True)
@@ -152,7 +189,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": round(PLANE_TRAFFIC_BYTES_PER_PAIR * st["pairs"] / max(launches, 1)),
             "model": "3-pass pencil model, y+z stages: 4*16*(Z*n1*n0+N)+8*N bytes per pair",
             "bytes_per_launch_avg": per_launch_bytes,
             "avg_launch_ms": round(plane_ms / max(launches, 1), 4),
