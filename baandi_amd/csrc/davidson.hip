@@ -145,7 +145,7 @@ struct DavidsonWork {
     small1.alloc(size_t(cap) * cap);
     small2.alloc(size_t(cap) * cap);
     small3.alloc(size_t(cap) * cap);
-    gwork.alloc(size_t(std::max(gemm_tn_work(cap, cap, n), int64_t(cap) * cap)));
+    gwork.alloc(size_t(std::max(Gemm::tn_work(cap, cap, n), int64_t(cap) * cap)));
     theta.alloc(nb);
     norm2.alloc(nb);
     scale.alloc(nb);
@@ -238,7 +238,7 @@ struct Solver {
     const int k = to - from;
     if (k <= 0) return;
     const double* B = (is_h ? HV : V) + int64_t(from) * n;
-    gemm_tn(s, to, k, n, 1.0, V, n, B, n, 0.0, w.small1.p, to, w.gwork.p);
+    op.gemm.tn(s, to, k, n, 1.0, V, n, B, n, 0.0, w.small1.p, to, w.gwork.p);
     std::vector<double> h(size_t(to) * k);
     down(h.data(), w.small1.p, h.size());
     std::vector<double>& M = is_h ? Hr : S;
@@ -302,8 +302,8 @@ struct Solver {
   // V <- V C (nb columns), HV <- HV C, measured refresh, C = I (workspace.rs:772-935).
   void contract(int m) {
     up(w.small1.p, C.data(), size_t(m) * nb);
-    gemm_nn(s, n, nb, m, 1.0, V, n, w.small1.p, m, 0.0, X, n);
-    gemm_nn(s, n, nb, m, 1.0, HV, n, w.small1.p, m, 0.0, HX, n);
+    op.gemm.nn(s, n, nb, m, 1.0, V, n, w.small1.p, m, 0.0, X, n);
+    op.gemm.nn(s, n, nb, m, 1.0, HV, n, w.small1.p, m, 0.0, HX, n);
     std::swap(V, X);
     std::swap(HV, HX);
     refresh(nb);
@@ -328,8 +328,8 @@ struct Solver {
       for (int j = 0; j < k; ++j)
         std::copy(&C[size_t(active[j]) * m], &C[size_t(active[j]) * m] + m, &ca[size_t(j) * m]);
       up(w.small1.p, ca.data(), ca.size());
-      gemm_nn(s, n, k, m, 1.0, V, n, w.small1.p, m, 0.0, X, n);
-      gemm_nn(s, n, k, m, 1.0, HV, n, w.small1.p, m, 0.0, HX, n);
+      op.gemm.nn(s, n, k, m, 1.0, V, n, w.small1.p, m, 0.0, X, n);
+      op.gemm.nn(s, n, k, m, 1.0, HV, n, w.small1.p, m, 0.0, HX, n);
     }
     hipLaunchKernelGGL(k_corrections, dim3(k), dim3(kT), 0, s, X, HX, n, w.theta.p, diag, op.d_kinetic.p,
                        o.preconditioner == BD_PRECOND_TPA ? 1 : 0, w.E.p, w.norm2.p);
@@ -374,16 +374,16 @@ struct Solver {
         for (int i = j; i < m; ++i) TSC[i + size_t(c) * m] += T[i + size_t(j) * cap] * v;
       }
     up_square(w.small1.p, T, m, true);  // T^T
-    gemm_nn(s, n, m, m, 1.0, V, n, w.small1.p, m, 0.0, X, n);
-    gemm_nn(s, n, m, m, 1.0, HV, n, w.small1.p, m, 0.0, HX, n);
+    op.gemm.nn(s, n, m, m, 1.0, V, n, w.small1.p, m, 0.0, X, n);
+    op.gemm.nn(s, n, m, m, 1.0, HV, n, w.small1.p, m, 0.0, HX, n);
     std::swap(V, X);
     std::swap(HV, HX);
     C = TSC;
     refresh(m);
     double* E = w.E2.p;
     for (int pass = 0; pass < 2; ++pass) {
-      gemm_tn(s, m, k, n, 1.0, V, n, E, n, 0.0, w.small1.p, m, w.gwork.p);
-      gemm_nn(s, n, k, m, -1.0, V, n, w.small1.p, m, 1.0, E, n);
+      op.gemm.tn(s, m, k, n, 1.0, V, n, E, n, 0.0, w.small1.p, m, w.gwork.p);
+      op.gemm.nn(s, n, k, m, -1.0, V, n, w.small1.p, m, 1.0, E, n);
     }
     int kept = 0;
     for (int j = 0; j < k; ++j) {
@@ -477,8 +477,8 @@ struct Solver {
     }
     // Return X = V C, HX = HV C (workspace.rs:936-980).
     up(w.small1.p, C.data(), size_t(m) * nb);
-    gemm_nn(s, n, nb, m, 1.0, V, n, w.small1.p, m, 0.0, xout, ldx);
-    if (hxout) gemm_nn(s, n, nb, m, 1.0, HV, n, w.small1.p, m, 0.0, hxout, ldx);
+    op.gemm.nn(s, n, nb, m, 1.0, V, n, w.small1.p, m, 0.0, xout, ldx);
+    if (hxout) op.gemm.nn(s, n, nb, m, 1.0, HV, n, w.small1.p, m, 0.0, hxout, ldx);
     sync();
     bool all = true;
     for (int i = 0; i < nb; ++i) {

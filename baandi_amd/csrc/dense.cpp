@@ -3,9 +3,12 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <limits>
 #include <numeric>
 #include <vector>
+
+#include <omp.h>
 
 namespace bd {
 
@@ -95,73 +98,94 @@ bool extend_inverse_cholesky(const double* S, int64_t ld, int from, int to, doub
 
 bool symmetric_eigen(const double* A, int64_t lda, int n, double* values, double* vectors) {
   if (n <= 0) return true;
-  // Row-major working copy a[i*n+j]; symmetric input.
-  std::vector<double> a(size_t(n) * n), d(n), e(n);
+  // Column-major working copy of the symmetric input.
+  std::vector<double> a(size_t(n) * n), d(n, 0.0), e(n, 0.0), tau(n, 0.0), p(n), w(n);
   for (int j = 0; j < n; ++j)
-    for (int i = 0; i < n; ++i) a[size_t(i) * n + j] = A[i + int64_t(j) * lda];
-  auto at = [&](int i, int j) -> double& { return a[size_t(i) * n + j]; };
-  // Householder reduction to tridiagonal form, accumulating the transform.
-  for (int i = n - 1; i > 0; --i) {
-    const int l = i - 1;
-    double h = 0.0;
-    if (l > 0) {
-      double scale = 0.0;
-      for (int k = 0; k <= l; ++k) scale += std::fabs(at(i, k));
-      if (scale == 0.0) {
-        e[i] = at(i, l);
-      } else {
-        for (int k = 0; k <= l; ++k) {
-          at(i, k) /= scale;
-          h += at(i, k) * at(i, k);
-        }
-        double f = at(i, l);
-        double g = f >= 0.0 ? -std::sqrt(h) : std::sqrt(h);
-        e[i] = scale * g;
-        h -= f * g;
-        at(i, l) = f - g;
-        f = 0.0;
-        for (int j = 0; j <= l; ++j) {
-          at(j, i) = at(i, j) / h;
-          g = 0.0;
-          for (int k = 0; k <= j; ++k) g += at(j, k) * at(i, k);
-          for (int k = j + 1; k <= l; ++k) g += at(k, j) * at(i, k);
-          e[j] = g / h;
-          f += e[j] * at(i, j);
-        }
-        const double hh = f / (h + h);
-        for (int j = 0; j <= l; ++j) {
-          f = at(i, j);
-          e[j] = g = e[j] - hh * f;
-          for (int k = 0; k <= j; ++k) at(j, k) -= (f * e[k] + g * at(i, k));
-        }
-      }
-    } else {
-      e[i] = at(i, l);
+    for (int i = 0; i < n; ++i) a[i + size_t(j) * n] = A[i + int64_t(j) * lda];
+  auto at = [&](int i, int j) -> double& { return a[i + size_t(j) * n]; };
+  // Householder tridiagonalisation: step k reflects A[k+1:, k] onto e_1;
+  // the reflector v is stored in place of that column, tau[k] beside it.
+  for (int k = 0; k + 2 < n; ++k) {
+    const int L = n - k - 1;
+    double* x = &at(k + 1, k);
+    double sigma = 0.0;
+    for (int i = 0; i < L; ++i) sigma += x[i] * x[i];
+    sigma = std::sqrt(sigma);
+    d[k] = at(k, k);
+    if (sigma == 0.0) {
+      e[k] = 0.0;
+      tau[k] = 0.0;
+      continue;
     }
-    d[i] = h;
-  }
-  d[0] = 0.0;
-  e[0] = 0.0;
-  for (int i = 0; i < n; ++i) {
-    const int l = i - 1;
-    if (d[i] != 0.0) {
-      for (int j = 0; j <= l; ++j) {
-        double g = 0.0;
-        for (int k = 0; k <= l; ++k) g += at(i, k) * at(k, j);
-        for (int k = 0; k <= l; ++k) at(k, j) -= g * at(k, i);
-      }
+    const double alpha = x[0] >= 0.0 ? -sigma : sigma;
+    x[0] -= alpha;
+    double vv = 0.0;
+    for (int i = 0; i < L; ++i) vv += x[i] * x[i];
+    const double t = 2.0 / vv;
+    tau[k] = t;
+    e[k] = alpha;
+    // p = t B v over the trailing block B = A[k+1:, k+1:].
+    const bool par = int64_t(L) * L >= 16384;
+#pragma omp parallel for schedule(static) if (par)
+    for (int i = 0; i < L; ++i) {
+      // B is symmetric: row i of B is column i (contiguous).
+      const double* col = &at(k + 1, k + 1 + i);
+      double s = 0.0;
+      for (int j = 0; j < L; ++j) s += col[j] * x[j];
+      p[i] = t * s;
     }
-    d[i] = at(i, i);
-    at(i, i) = 1.0;
-    for (int j = 0; j <= l; ++j) at(j, i) = at(i, j) = 0.0;
+    double pv = 0.0;
+    for (int i = 0; i < L; ++i) pv += p[i] * x[i];
+    const double K = 0.5 * t * pv;
+    for (int i = 0; i < L; ++i) w[i] = p[i] - K * x[i];
+    // B -= v w^T + w v^T (columns in parallel).
+#pragma omp parallel for schedule(static) if (par)
+    for (int j = 0; j < L; ++j) {
+      double* col = &at(k + 1, k + 1 + j);
+      const double vj = x[j], wj = w[j];
+      for (int i = 0; i < L; ++i) col[i] -= x[i] * wj + w[i] * vj;
+    }
   }
-  // z[i][k] = component k of vector i (rows contiguous for the rotations).
-  std::vector<double> z(size_t(n) * n);
-  for (int i = 0; i < n; ++i)
-    for (int k = 0; k < n; ++k) z[size_t(i) * n + k] = at(k, i);
-  // Implicit-shift QL on the tridiagonal (d, e).
-  for (int i = 1; i < n; ++i) e[i - 1] = e[i];
-  e[n - 1] = 0.0;
+  if (n >= 2) {
+    d[n - 2] = at(n - 2, n - 2);
+    e[n - 2] = at(n - 1, n - 2);
+  }
+  d[n - 1] = at(n - 1, n - 1);
+  // Q = H_0 H_1 ... accumulated backwards (column-major, contiguous columns).
+  // Columns are padded to whole 64-byte lines so that threads working on
+  // disjoint row blocks never share a cache line.
+  const int ldq = (n + 7) / 8 * 8;
+  double* qc = static_cast<double*>(std::aligned_alloc(64, size_t(ldq) * n * sizeof(double)));
+  if (!qc) return false;
+  struct Free {
+    double* p;
+    ~Free() { std::free(p); }
+  } qc_guard{qc};
+  std::fill(qc, qc + size_t(ldq) * n, 0.0);
+  for (int i = 0; i < n; ++i) qc[size_t(i) * ldq + i] = 1.0;
+  for (int k = n - 3; k >= 0; --k) {
+    if (tau[k] == 0.0) continue;
+    const int L = n - k - 1;
+    const double* v = &at(k + 1, k);
+    const double t = tau[k];
+#pragma omp parallel for schedule(static) if (int64_t(L) * L >= 16384)
+    for (int c = k + 1; c < n; ++c) {
+      double* col = &qc[size_t(c) * ldq + k + 1];
+      double s = 0.0;
+      for (int i = 0; i < L; ++i) s += v[i] * col[i];
+      s *= t;
+      for (int i = 0; i < L; ++i) col[i] -= s * v[i];
+    }
+  }
+  // Implicit-shift QL on (d, e): e[i] couples d[i], d[i+1]. Every rotation
+  // (column pair i, i+1; c, s) is recorded in order; rows of Q are then
+  // independent, so the whole sequence is applied in one parallel pass.
+  struct Rot {
+    int i;
+    double c, s;
+  };
+  std::vector<Rot> rots;
+  rots.reserve(size_t(n) * n);
   for (int l = 0; l < n; ++l) {
     int iter = 0, m;
     do {
@@ -174,45 +198,58 @@ bool symmetric_eigen(const double* A, int64_t lda, int n, double* values, double
         double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
         double r = std::hypot(g, 1.0);
         g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? std::fabs(r) : -std::fabs(r)));
-        double s = 1.0, c = 1.0, p = 0.0;
+        double s = 1.0, c = 1.0, pp = 0.0;
         int i;
         for (i = m - 1; i >= l; --i) {
           double f = s * e[i];
           const double b = c * e[i];
           e[i + 1] = (r = std::hypot(f, g));
           if (r == 0.0) {
-            d[i + 1] -= p;
+            d[i + 1] -= pp;
             e[m] = 0.0;
             break;
           }
           s = f / r;
           c = g / r;
-          g = d[i + 1] - p;
+          g = d[i + 1] - pp;
           r = (d[i] - g) * s + 2.0 * c * b;
-          d[i + 1] = g + (p = s * r);
+          d[i + 1] = g + (pp = s * r);
           g = c * r - b;
-          double* zi = &z[size_t(i) * n];
-          double* zj = &z[size_t(i + 1) * n];
-          for (int k = 0; k < n; ++k) {
-            f = zj[k];
-            zj[k] = s * zi[k] + c * f;
-            zi[k] = c * zi[k] - s * f;
-          }
+          rots.push_back({i, c, s});
         }
         if (r == 0.0 && i >= l) continue;
-        d[l] -= p;
+        d[l] -= pp;
         e[l] = g;
         e[m] = 0.0;
       }
     } while (m != l);
   }
+  {
+    const int threads = std::max(1, std::min(omp_get_max_threads(), 16));
+    const int nblk = (rots.size() * size_t(n) >= (1u << 16)) ? threads : 1;
+    const int kBlock = ((n + nblk - 1) / nblk + 7) / 8 * 8;
+#pragma omp parallel for schedule(static) num_threads(nblk) if (nblk > 1)
+    for (int bk = 0; bk < nblk; ++bk) {
+      const int r0 = std::min(n, bk * kBlock), r1 = std::min(n, r0 + kBlock);
+      for (const Rot& t : rots) {
+        double* zi = &qc[size_t(t.i) * ldq];
+        double* zj = &qc[size_t(t.i + 1) * ldq];
+        for (int row = r0; row < r1; ++row) {
+          const double f = zj[row];
+          zj[row] = t.s * zi[row] + t.c * f;
+          zi[row] = t.c * zi[row] - t.s * f;
+        }
+      }
+    }
+  }
   std::vector<int> order(n);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return d[x] < d[y]; });
+#pragma omp parallel for schedule(static)
   for (int c = 0; c < n; ++c) {
     values[c] = d[order[c]];
-    const double* src = &z[size_t(order[c]) * n];
-    std::copy(src, src + n, vectors + size_t(c) * n);
+    const int src = order[c];
+    std::copy(&qc[size_t(src) * ldq], &qc[size_t(src) * ldq] + n, vectors + size_t(c) * n);
   }
   return true;
 }

@@ -160,3 +160,33 @@ void gemm_tn(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const
 }
 
 }  // namespace bd
+
+// ---------------------------------------------------------------------------
+// C ABI: the GEMMs as stand-alone entry points (row a28 check and tooling).
+// ---------------------------------------------------------------------------
+extern "C" int64_t bd_gemm_workspace(int32_t mode, int32_t trans_a, int64_t M, int64_t N, int64_t K) {
+  if (!trans_a) return 0;
+  return mode == 1 ? bd::gemm_tn_bf16x3_work(M, N, K) : bd::gemm_tn_work(M, N, K);
+}
+
+extern "C" bd_status bd_gemm_dev(int32_t mode, int32_t trans_a, int64_t M, int64_t N, int64_t K, double alpha,
+                                 const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                                 double* C, int64_t ldc, double* work) {
+  BD_GUARD({
+    if (M < 0 || N < 0 || K < 0 || (mode != 0 && mode != 1)) bd::fail(BD_ERR_INVALID_PARAMETER, "gemm shape/mode");
+    if (trans_a && !work && M * N > 0) bd::fail(BD_ERR_INVALID_PARAMETER, "gemm workspace");
+    bd::Gemm g;
+    g.bf16x3 = mode == 1;
+    if (trans_a) g.tn(nullptr, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, work);
+    else g.nn(nullptr, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+    if (hipDeviceSynchronize() != hipSuccess || hipGetLastError() != hipSuccess)
+      bd::fail(BD_ERR_DEVICE, "gemm launch failed");
+  });
+}
+
+extern "C" bd_status bd_mfma_probe_dev(const uint16_t* a_bits, const uint16_t* b_bits, float* d) {
+  BD_GUARD({
+    bd::mfma_probe(nullptr, a_bits, b_bits, d);
+    if (hipDeviceSynchronize() != hipSuccess) bd::fail(BD_ERR_DEVICE, "probe failed");
+  });
+}

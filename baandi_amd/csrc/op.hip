@@ -20,6 +20,7 @@
 #include <numeric>
 #include <set>
 #include <string>
+#include <type_traits>
 
 #include "common.hpp"
 #include "gemm.hpp"
@@ -29,6 +30,7 @@ namespace bd {
 
 namespace {
 constexpr int kThreads = 256;
+constexpr int kDensityGroups = 8;  // band-density pair groups per x plane
 constexpr double kSqrt2 = 1.41421356237309504880168872420969808;
 constexpr double kInvSqrt2 = 0.70710678118654752440084436210484904;
 
@@ -56,19 +58,22 @@ __device__ __forceinline__ void stage_roots(c2<T>* dst, const c2<double>* src, i
   for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = {T(src[i].x), T(src[i].y)};
 }
 
-// Pack ORC column pair t into its sticks and inverse-transform along x.
+// Pack ORC column pair t into its sticks and inverse-transform along x. The
+// stick buffer and the transform use F (double, or float in the mixed tier);
+// the ORC columns are always double.
+template <typename F>
 __global__ __launch_bounds__(kThreads) void k_stick_inverse(
     const double* __restrict__ x, int64_t ldx, int m, int n0, int T, int Spad,
     const int* __restrict__ tile_off, const int4* __restrict__ entries,
-    const c2<double>* __restrict__ roots_x, RadixPlan rx, c2<double>* __restrict__ sticks) {
+    const c2<double>* __restrict__ roots_x, RadixPlan rx, c2<F>* __restrict__ sticks) {
   extern __shared__ char smem[];
-  c2<double>* L = reinterpret_cast<c2<double>*>(smem);
-  c2<double>* rt = L + T * n0;
+  c2<F>* L = reinterpret_cast<c2<F>*>(smem);
+  c2<F>* rt = L + T * n0;
   const int tile = blockIdx.x, t = blockIdx.y;
   const int ca = 2 * t, cb = 2 * t + 1;
   const double* xa = x + int64_t(ca) * ldx;
   const double* xb = cb < m ? x + int64_t(cb) * ldx : nullptr;
-  for (int i = threadIdx.x; i < T * n0; i += blockDim.x) L[i] = {0.0, 0.0};
+  for (int i = threadIdx.x; i < T * n0; i += blockDim.x) L[i] = {F(0), F(0)};
   stage_roots(rt, roots_x, n0);
   __syncthreads();
   const int e0 = tile_off[tile], e1 = tile_off[tile + 1];
@@ -76,34 +81,35 @@ __global__ __launch_bounds__(kThreads) void k_stick_inverse(
     const int4 en = entries[e];
     const int i = en.x;
     if (i == 0) {
-      L[en.y] = {xa[0], xb ? xb[0] : 0.0};
+      L[en.y] = {F(xa[0]), F(xb ? xb[0] : 0.0)};
     } else {
       const double ar = xa[2 * i - 1] * kInvSqrt2, ai = xa[2 * i] * kInvSqrt2;
       const double br = xb ? xb[2 * i - 1] * kInvSqrt2 : 0.0, bi = xb ? xb[2 * i] * kInvSqrt2 : 0.0;
-      L[en.y] = {ar - bi, ai + br};   // a + i b
-      L[en.z] = {ar + bi, br - ai};   // conj(a) + i conj(b)
+      L[en.y] = {F(ar - bi), F(ai + br)};   // a + i b
+      L[en.z] = {F(ar + bi), F(br - ai)};   // conj(a) + i conj(b)
     }
   }
   __syncthreads();
   lds_fft<true>(L, T, n0, 1, rx, rt);
-  c2<double>* out = sticks + int64_t(t) * n0 * Spad + int64_t(tile) * T;
+  c2<F>* out = sticks + int64_t(t) * n0 * Spad + int64_t(tile) * T;
   for (int e = threadIdx.x; e < T * n0; e += blockDim.x) {
     const int xi = e / T, slot = e - xi * T;
     out[int64_t(xi) * Spad + slot] = L[slot * n0 + xi];
   }
 }
 
-// Forward x FFT of a tile, gather +G/-G, unpack to ORC, add T_G x.
+// Forward x FFT of a tile, gather +G/-G, unpack to ORC (in double), add T_G x.
+template <typename F>
 __global__ __launch_bounds__(kThreads) void k_stick_forward(
     const double* __restrict__ x, int64_t ldx, double* __restrict__ y, int64_t ldy, int m,
     int n0, int T, int Spad, const int* __restrict__ tile_off, const int4* __restrict__ entries,
-    const c2<double>* __restrict__ roots_x, RadixPlan rx, const c2<double>* __restrict__ sticks,
+    const c2<double>* __restrict__ roots_x, RadixPlan rx, const c2<F>* __restrict__ sticks,
     const double* __restrict__ kinetic, double inv_n) {
   extern __shared__ char smem[];
-  c2<double>* L = reinterpret_cast<c2<double>*>(smem);
-  c2<double>* rt = L + T * n0;
+  c2<F>* L = reinterpret_cast<c2<F>*>(smem);
+  c2<F>* rt = L + T * n0;
   const int tile = blockIdx.x, t = blockIdx.y;
-  const c2<double>* in = sticks + int64_t(t) * n0 * Spad + int64_t(tile) * T;
+  const c2<F>* in = sticks + int64_t(t) * n0 * Spad + int64_t(tile) * T;
   for (int e = threadIdx.x; e < T * n0; e += blockDim.x) {
     const int xi = e / T, slot = e - xi * T;
     L[slot * n0 + xi] = in[int64_t(xi) * Spad + slot];
@@ -121,13 +127,15 @@ __global__ __launch_bounds__(kThreads) void k_stick_forward(
   for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
     const int4 en = entries[e];
     const int i = en.x;
-    const c2<double> hp = cscale(L[en.y], inv_n);
+    const c2<F> lp = L[en.y];
+    const c2<double> hp = {double(lp.x) * inv_n, double(lp.y) * inv_n};
     if (i == 0) {
       const double k0 = kinetic[0];
       ya[0] = hp.x + k0 * xa[0];
       if (hb) yb[0] = hp.y + k0 * xb[0];
     } else {
-      const c2<double> hm = cscale(L[en.z], inv_n);
+      const c2<F> lm = L[en.z];
+      const c2<double> hm = {double(lm.x) * inv_n, double(lm.y) * inv_n};
       // even = (hp + conj(hm))/2, odd = (hp - conj(hm))/2, rows x sqrt2.
       const double er = 0.5 * (hp.x + hm.x), ei = 0.5 * (hp.y - hm.y);
       const double orr = 0.5 * (hp.x - hm.x), oi = 0.5 * (hp.y + hm.y);
@@ -146,26 +154,30 @@ __global__ __launch_bounds__(kThreads) void k_stick_forward(
 // Plane pass: inverse y, per-row-chunk inverse z * V * forward z, forward y.
 // mode 0: H apply (writes sticks back); mode 1: band density (accumulates
 // w_a psi_a^2 + w_b psi_b^2 over all pairs into rho of plane x).
-template <int MODE>
+template <int MODE, typename F>
 __global__ __launch_bounds__(kThreads) void k_plane(
-    c2<double>* __restrict__ sticks, int n0, int n1, int n2, int Spad, int Z, int chunk,
+    c2<F>* __restrict__ sticks, int n0, int n1, int n2, int Spad, int Z, int chunk,
     const int2* __restrict__ stick_yz, const int* __restrict__ z_of_zc,
     const c2<double>* __restrict__ roots, RadixPlan ry, RadixPlan rz,
-    const double* __restrict__ potential, int pairs, const double* __restrict__ weights,
+    const F* __restrict__ potential, int pairs, const double* __restrict__ weights,
     double inv_volume, double* __restrict__ rho) {
   extern __shared__ char smem[];
-  c2<double>* P = reinterpret_cast<c2<double>*>(smem);  // [Z][n1]
-  c2<double>* Rb = P + Z * n1;                          // [chunk][n2]
-  c2<double>* rty = Rb + chunk * n2;
-  c2<double>* rtz = rty + n1;
+  c2<F>* P = reinterpret_cast<c2<F>*>(smem);  // [Z][n1]
+  c2<F>* Rb = P + Z * n1;                     // [chunk][n2]
+  c2<F>* rty = Rb + chunk * n2;
+  c2<F>* rtz = rty + n1;
   const int xi = MODE == 0 ? blockIdx.y : blockIdx.x;
   stage_roots(rty, roots + n0, n1);
   stage_roots(rtz, roots + n0 + n1, n2);
-  const int tbeg = MODE == 0 ? blockIdx.x : 0;
+  // MODE 1: group blockIdx.y accumulates pairs y, y+G, y+2G, ... into its own
+  // partial density rho + y*N (summed in fixed group order afterwards).
+  const int groups = MODE == 0 ? 1 : gridDim.y;
+  const int tbeg = MODE == 0 ? blockIdx.x : blockIdx.y;
   const int tend = MODE == 0 ? tbeg + 1 : pairs;
-  for (int t = tbeg; t < tend; ++t) {
-    c2<double>* col = sticks + int64_t(t) * n0 * Spad + int64_t(xi) * Spad;
-    for (int i = threadIdx.x; i < Z * n1; i += blockDim.x) P[i] = {0.0, 0.0};
+  if (MODE == 1) rho += int64_t(blockIdx.y) * n0 * n1 * n2;
+  for (int t = tbeg; t < tend; t += groups) {
+    c2<F>* col = sticks + int64_t(t) * n0 * Spad + int64_t(xi) * Spad;
+    for (int i = threadIdx.x; i < Z * n1; i += blockDim.x) P[i] = {F(0), F(0)};
     __syncthreads();
     for (int s = threadIdx.x; s < Spad; s += blockDim.x) {
       const int2 yz = stick_yz[s];
@@ -180,7 +192,7 @@ __global__ __launch_bounds__(kThreads) void k_plane(
     }
     for (int y0 = 0; y0 < n1; y0 += chunk) {
       const int c = min(chunk, n1 - y0);
-      for (int i = threadIdx.x; i < c * n2; i += blockDim.x) Rb[i] = {0.0, 0.0};
+      for (int i = threadIdx.x; i < c * n2; i += blockDim.x) Rb[i] = {F(0), F(0)};
       __syncthreads();
       for (int i = threadIdx.x; i < c * Z; i += blockDim.x) {
         const int r = i / Z, zc = i - r * Z;
@@ -189,7 +201,7 @@ __global__ __launch_bounds__(kThreads) void k_plane(
       __syncthreads();
       lds_fft<true>(Rb, c, n2, 1, rz, rtz);
       if (MODE == 0) {
-        const double* v = potential + (int64_t(xi) * n1 + y0) * n2;
+        const F* v = potential + (int64_t(xi) * n1 + y0) * n2;
         for (int i = threadIdx.x; i < c * n2; i += blockDim.x) Rb[i] = cscale(Rb[i], v[i]);
         __syncthreads();
         lds_fft<false>(Rb, c, n2, 1, rz, rtz);
@@ -201,8 +213,9 @@ __global__ __launch_bounds__(kThreads) void k_plane(
       } else {
         double* out = rho + (int64_t(xi) * n1 + y0) * n2;
         for (int i = threadIdx.x; i < c * n2; i += blockDim.x) {
-          const double d = wa * Rb[i].x * Rb[i].x + wb * Rb[i].y * Rb[i].y;
-          out[i] = t == 0 ? d : out[i] + d;
+          const double re = double(Rb[i].x), im = double(Rb[i].y);
+          const double d = wa * re * re + wb * im * im;
+          out[i] = t == tbeg ? d : out[i] + d;
         }
         __syncthreads();
       }
@@ -215,6 +228,20 @@ __global__ __launch_bounds__(kThreads) void k_plane(
       }
     }
   }
+}
+
+__global__ void k_to_float(const double* __restrict__ src, int64_t n, float* __restrict__ dst) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = float(src[i]);
+}
+
+// rho = sum over groups of the partial densities, in group order.
+__global__ void k_sum_groups(const double* __restrict__ part, int64_t N, int groups, double* __restrict__ rho) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double s = 0.0;
+  for (int g = 0; g < groups; ++g) s += part[int64_t(g) * N + i];
+  rho[i] = s;
 }
 
 // W = D Q per atom block (distributed_real_gamma.rs:1183-1195).
@@ -344,13 +371,30 @@ StickPlan build_stick_plan(const System& s) {
 // ---------------------------------------------------------------------------
 // Operator
 // ---------------------------------------------------------------------------
-static size_t stick_lds(const StickPlan& p) { return size_t(p.T * p.n0 + p.n0) * sizeof(c2<double>); }
-static size_t plane_lds(const StickPlan& p) {
-  return size_t(p.Z * p.n1 + p.chunk * p.n2 + p.n1 + p.n2) * sizeof(c2<double>);
+static size_t stick_lds(const StickPlan& p, size_t elem) { return size_t(p.T * p.n0 + p.n0) * elem; }
+static size_t plane_lds(const StickPlan& p, size_t elem) {
+  return size_t(p.Z * p.n1 + p.chunk * p.n2 + p.n1 + p.n2) * elem;
+}
+
+template <typename F>
+static void allow_lds(const StickPlan& plan) {
+  const size_t e = sizeof(c2<F>);
+  if (plane_lds(plan, e) > 65536) {
+    BD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane<0, F>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(plane_lds(plan, e))));
+    BD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane<1, F>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(plane_lds(plan, e))));
+  }
+  if (stick_lds(plan, e) > 65536) {
+    BD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_stick_inverse<F>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(stick_lds(plan, e))));
+    BD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_stick_forward<F>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(stick_lds(plan, e))));
+  }
 }
 
 GammaOp::GammaOp(const System& s, int cap, int prec) : sys(&s) {
-  if (prec != BD_FP64) fail(BD_ERR_INVALID_PARAMETER, "only BD_FP64 is implemented in this build");
+  if (prec != BD_FP64 && prec != BD_MIXED) fail(BD_ERR_INVALID_PARAMETER, "unknown precision mode");
   if (cap <= 0) fail(BD_ERR_INVALID_PARAMETER, "capacity must be positive");
   int dev = 0, count = 0;
   BD_HIP(hipGetDeviceCount(&count));
@@ -366,23 +410,15 @@ GammaOp::GammaOp(const System& s, int cap, int prec) : sys(&s) {
   N = int64_t(dims[0]) * dims[1] * dims[2];
   capacity = cap;
   precision = prec;
+  gemm.bf16x3 = prec == BD_MIXED;
   nproj = s.nproj;
   plan = build_stick_plan(s);
   size_t lds_max = size_t(prop.sharedMemPerBlock);
-  if (stick_lds(plan) > lds_max || plane_lds(plan) > lds_max)
+  const size_t elem = precision == BD_MIXED ? sizeof(c2<float>) : sizeof(c2<double>);
+  if (stick_lds(plan, elem) > lds_max || plane_lds(plan, elem) > lds_max)
     fail(BD_ERR_INVALID_PARAMETER, "FFT workgroup LDS exceeds the device limit");
-  if (plane_lds(plan) > 65536) {
-    BD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane<0>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, int(plane_lds(plan))));
-    BD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane<1>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, int(plane_lds(plan))));
-  }
-  if (stick_lds(plan) > 65536) {
-    BD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_stick_inverse),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, int(stick_lds(plan))));
-    BD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_stick_forward),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, int(stick_lds(plan))));
-  }
+  if (precision == BD_MIXED) allow_lds<float>(plan);
+  else allow_lds<double>(plan);
   BD_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   // Roots e^{-2 pi i k/n} per axis, in long double.
   std::vector<c2<double>> roots;
@@ -404,6 +440,7 @@ GammaOp::GammaOp(const System& s, int cap, int prec) : sys(&s) {
   d_kinetic.alloc(n);
   d_kinetic.upload(s.kinetic_orc.data(), n, stream);
   d_potential.alloc(N);
+  if (precision == BD_MIXED) d_potential32.alloc(N);
   if (nproj > 0) {
     d_proj.alloc(size_t(n) * nproj);
     d_proj.upload(s.projectors_orc.data(), size_t(n) * nproj, stream);
@@ -413,7 +450,7 @@ GammaOp::GammaOp(const System& s, int cap, int prec) : sys(&s) {
     d_blockd.upload(s.block_d.data(), s.block_d.size(), stream);
     d_q.alloc(size_t(nproj) * cap);
     d_w.alloc(size_t(nproj) * cap);
-    d_work.alloc(size_t(gemm_tn_work(nproj, cap, n)));
+    d_work.alloc(size_t(Gemm::tn_work(nproj, cap, n)));
   }
   const int pairs = (cap + 1) / 2;
   d_sticks.alloc(size_t(pairs) * plan.n0 * plan.Spad);
@@ -421,6 +458,7 @@ GammaOp::GammaOp(const System& s, int cap, int prec) : sys(&s) {
   d_x.alloc(size_t(n) * cap);
   d_y.alloc(size_t(n) * cap);
   d_weights.alloc(size_t(2 * pairs));
+  d_rho_part.alloc(size_t(std::min(pairs, kDensityGroups)) * N);
   for (auto& e : ev) BD_HIP(hipEventCreate(&e));
   BD_HIP(hipStreamSynchronize(stream));
 }
@@ -437,10 +475,57 @@ void GammaOp::set_potential_dev(const double* v, uint64_t ep) {
   BD_HIP(hipMemcpyAsync(d_potential.p, v, size_t(N) * sizeof(double), hipMemcpyDeviceToDevice, stream));
   epoch = ep;
   has_potential = true;
+  refresh_potential32();
+}
+
+void GammaOp::refresh_potential32() {
+  if (precision != BD_MIXED) return;
+  hipLaunchKernelGGL(k_to_float, dim3(unsigned((N + 255) / 256)), dim3(256), 0, stream, d_potential.p, N,
+                     d_potential32.p);
+  BD_HIP(hipGetLastError());
+}
+
+template <typename F>
+void GammaOp::local_apply(const double* x, int64_t ldx, int m, double* y, int64_t ldy) {
+  const StickPlan& p = plan;
+  const int pairs = (m + 1) / 2;
+  const size_t e = sizeof(c2<F>);
+  c2<F>* sticks = reinterpret_cast<c2<F>*>(d_sticks.p);
+  const F* pot;
+  if constexpr (std::is_same_v<F, float>) pot = d_potential32.p;
+  else pot = d_potential.p;
+  if (profiling) BD_HIP(hipEventRecord(ev[0], stream));
+  hipLaunchKernelGGL(k_stick_inverse<F>, dim3(p.ntiles, pairs), dim3(kThreads), stick_lds(p, e), stream, x,
+                     ldx, m, p.n0, p.T, p.Spad, d_tile_off.p, d_entries.p, d_roots.p, p.rx, sticks);
+  if (profiling) BD_HIP(hipEventRecord(ev[1], stream));
+  hipLaunchKernelGGL((k_plane<0, F>), dim3(pairs, p.n0), dim3(kThreads), plane_lds(p, e), stream, sticks,
+                     p.n0, p.n1, p.n2, p.Spad, p.Z, p.chunk, d_stick_yz.p, d_zc.p, d_roots.p, p.ry,
+                     p.rz, pot, pairs, nullptr, 0.0, nullptr);
+  if (profiling) BD_HIP(hipEventRecord(ev[2], stream));
+  hipLaunchKernelGGL(k_stick_forward<F>, dim3(p.ntiles, pairs), dim3(kThreads), stick_lds(p, e), stream, x,
+                     ldx, y, ldy, m, p.n0, p.T, p.Spad, d_tile_off.p, d_entries.p, d_roots.p, p.rx,
+                     sticks, d_kinetic.p, 1.0 / double(N));
+  if (profiling) BD_HIP(hipEventRecord(ev[3], stream));
+}
+
+template <typename F>
+void GammaOp::density(const double* x, int64_t ldx, int m, const double* w, double* rho) {
+  const StickPlan& p = plan;
+  const int pairs = (m + 1) / 2;
+  const size_t e = sizeof(c2<F>);
+  c2<F>* sticks = reinterpret_cast<c2<F>*>(d_sticks.p);
+  hipLaunchKernelGGL(k_stick_inverse<F>, dim3(p.ntiles, pairs), dim3(kThreads), stick_lds(p, e), stream, x,
+                     ldx, m, p.n0, p.T, p.Spad, d_tile_off.p, d_entries.p, d_roots.p, p.rx, sticks);
+  const int groups = std::min(pairs, kDensityGroups);
+  hipLaunchKernelGGL((k_plane<1, F>), dim3(p.n0, groups), dim3(kThreads), plane_lds(p, e), stream, sticks,
+                     p.n0, p.n1, p.n2, p.Spad, p.Z, p.chunk, d_stick_yz.p, d_zc.p, d_roots.p, p.ry, p.rz,
+                     nullptr, pairs, w, 1.0 / sys->volume, d_rho_part.p);
+  hipLaunchKernelGGL(k_sum_groups, dim3(unsigned((N + 255) / 256)), dim3(256), 0, stream, d_rho_part.p, N,
+                     groups, rho);
 }
 
 void GammaOp::projections(const double* x, int64_t ldx, int m) {
-  gemm_tn(stream, nproj, m, n, 1.0, d_proj.p, n, x, ldx, 0.0, d_q.p, nproj, d_work.p);
+  gemm.tn(stream, nproj, m, n, 1.0, d_proj.p, n, x, ldx, 0.0, d_q.p, nproj, d_work.p);
 }
 
 void GammaOp::apply(const double* x, int64_t ldx, int m, double* y, int64_t ldy) {
@@ -448,27 +533,16 @@ void GammaOp::apply(const double* x, int64_t ldx, int m, double* y, int64_t ldy)
   if (ldx < n || ldy < n) fail(BD_ERR_INVALID_PARAMETER, "leading dimension smaller than rows");
   if (!has_potential) fail(BD_ERR_INVALID_PARAMETER, "no potential installed");
   if (m == 0) return;
-  const StickPlan& p = plan;
   const int pairs = (m + 1) / 2;
-  if (profiling) BD_HIP(hipEventRecord(ev[0], stream));
-  hipLaunchKernelGGL(k_stick_inverse, dim3(p.ntiles, pairs), dim3(kThreads), stick_lds(p), stream, x,
-                     ldx, m, p.n0, p.T, p.Spad, d_tile_off.p, d_entries.p, d_roots.p, p.rx, d_sticks.p);
-  if (profiling) BD_HIP(hipEventRecord(ev[1], stream));
-  hipLaunchKernelGGL(k_plane<0>, dim3(pairs, p.n0), dim3(kThreads), plane_lds(p), stream, d_sticks.p,
-                     p.n0, p.n1, p.n2, p.Spad, p.Z, p.chunk, d_stick_yz.p, d_zc.p, d_roots.p, p.ry,
-                     p.rz, d_potential.p, pairs, nullptr, 0.0, nullptr);
-  if (profiling) BD_HIP(hipEventRecord(ev[2], stream));
-  hipLaunchKernelGGL(k_stick_forward, dim3(p.ntiles, pairs), dim3(kThreads), stick_lds(p), stream, x,
-                     ldx, y, ldy, m, p.n0, p.T, p.Spad, d_tile_off.p, d_entries.p, d_roots.p, p.rx,
-                     d_sticks.p, d_kinetic.p, 1.0 / double(N));
-  if (profiling) BD_HIP(hipEventRecord(ev[3], stream));
+  if (precision == BD_MIXED) local_apply<float>(x, ldx, m, y, ldy);
+  else local_apply<double>(x, ldx, m, y, ldy);
   if (nproj > 0) {
     projections(x, ldx, m);
     const int64_t tot = nproj * m;
     hipLaunchKernelGGL(k_coupling, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, stream, d_q.p,
                        d_w.p, nproj, m, nc, d_blockd.p);
     if (profiling) BD_HIP(hipEventRecord(ev[4], stream));
-    gemm_nn(stream, n, m, nproj, 1.0, d_proj.p, n, d_w.p, nproj, 1.0, y, ldy);
+    gemm.nn(stream, n, m, nproj, 1.0, d_proj.p, n, d_w.p, nproj, 1.0, y, ldy);
   } else if (profiling) {
     BD_HIP(hipEventRecord(ev[4], stream));
   }
@@ -490,13 +564,8 @@ void GammaOp::apply(const double* x, int64_t ldx, int m, double* y, int64_t ldy)
 
 void GammaOp::band_density(const double* x, int64_t ldx, int m, const double* w, double* rho) {
   if (m <= 0 || m > capacity) fail(BD_ERR_INVALID_PARAMETER, "column count");
-  const StickPlan& p = plan;
-  const int pairs = (m + 1) / 2;
-  hipLaunchKernelGGL(k_stick_inverse, dim3(p.ntiles, pairs), dim3(kThreads), stick_lds(p), stream, x,
-                     ldx, m, p.n0, p.T, p.Spad, d_tile_off.p, d_entries.p, d_roots.p, p.rx, d_sticks.p);
-  hipLaunchKernelGGL(k_plane<1>, dim3(p.n0), dim3(kThreads), plane_lds(p), stream, d_sticks.p, p.n0,
-                     p.n1, p.n2, p.Spad, p.Z, p.chunk, d_stick_yz.p, d_zc.p, d_roots.p, p.ry, p.rz,
-                     nullptr, pairs, w, 1.0 / sys->volume, rho);
+  if (precision == BD_MIXED) density<float>(x, ldx, m, w, rho);
+  else density<double>(x, ldx, m, w, rho);
   BD_HIP(hipGetLastError());
 }
 
@@ -535,6 +604,7 @@ extern "C" bd_status bd_gamma_op_set_potential(bd_gamma_op* op, const double* v,
     auto& o = op->op;
     if (o.has_potential && epoch <= o.epoch) bd::fail(BD_ERR_INVALID_PARAMETER, "potential epoch must increase");
     BD_HIP(hipMemcpyAsync(o.d_potential.p, v, size_t(o.N) * sizeof(double), hipMemcpyHostToDevice, o.stream));
+    o.refresh_potential32();
     BD_HIP(hipStreamSynchronize(o.stream));
     o.epoch = epoch;
     o.has_potential = true;

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "fft_device.hpp"
+#include "gemm.hpp"
 #include "system.hpp"
 
 namespace bd {
@@ -71,9 +72,11 @@ struct GammaOp {
   DevBuf<int2> d_stick_yz;
   DevBuf<c2<double>> d_roots;  // [n0 | n1 | n2]
   DevBuf<double> d_kinetic, d_potential, d_proj, d_blockd;
+  DevBuf<float> d_potential32;  // mixed tier: V(r) in float
   // scratch
   DevBuf<c2<double>> d_sticks;  // pairs_cap * n0 * Spad
-  DevBuf<double> d_q, d_w, d_work, d_x, d_y, d_weights;
+  DevBuf<double> d_q, d_w, d_work, d_x, d_y, d_weights, d_rho_part;
+  Gemm gemm;  // FP64, or BF16x3 in the mixed tier
   int nc = 0;  // channels per atom block
   int nblocks = 0;
   // profiling
@@ -89,6 +92,11 @@ struct GammaOp {
   void apply(const double* x, int64_t ldx, int m, double* y, int64_t ldy);
   void band_density(const double* x, int64_t ldx, int m, const double* w, double* rho);
   void projections(const double* x, int64_t ldx, int m);  // d_q = P^T X
+  void refresh_potential32();
+  template <typename F>
+  void local_apply(const double* x, int64_t ldx, int m, double* y, int64_t ldy);
+  template <typename F>
+  void density(const double* x, int64_t ldx, int m, const double* w, double* rho);
 };
 
 StickPlan build_stick_plan(const System& s);

@@ -22,6 +22,7 @@ EXPORTS = [
     "bd_gamma_op_set_potential_dev", "bd_gamma_op_apply", "bd_gamma_op_apply_dev",
     "bd_gamma_op_band_density_dev", "bd_gamma_op_nonlocal_energies_dev", "bd_gamma_op_stream",
     "bd_gamma_op_set_profiling", "bd_gamma_op_stage_times", "bd_gamma_op_plan_info", "bd_davidson_solve_dev",
+    "bd_gemm_workspace", "bd_gemm_dev", "bd_mfma_probe_dev",
 ]
 
 
@@ -101,6 +102,8 @@ def load():
         "bd_gamma_op_plan_info": [P, P],
         "bd_davidson_solve_dev": [P, P, I64, P, P, C.POINTER(DavidsonOptions), P, P, P, I64, P,
                                   C.POINTER(DavidsonRecord)],
+        "bd_gemm_dev": [I32, I32, I64, I64, I64, D, P, I64, P, I64, D, P, I64, P],
+        "bd_mfma_probe_dev": [P, P, P],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -112,6 +115,8 @@ def load():
     lib.bd_gamma_op_destroy.restype = None
     lib.bd_gamma_op_stream.argtypes = [P]
     lib.bd_gamma_op_stream.restype = P
+    lib.bd_gemm_workspace.argtypes = [I32, I32, I64, I64, I64]
+    lib.bd_gemm_workspace.restype = I64
     _lib = lib
     return lib
 

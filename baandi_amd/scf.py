@@ -18,8 +18,11 @@ import time
 from dataclasses import dataclass
 
 import numpy as np
+import scipy.fft as sfft
 
 from . import lib as L
+
+FFT_WORKERS = -1  # all host threads for the density-sphere grid transforms
 from .hamiltonian import DistributedRealHamiltonian, GammaSystem
 from . import ref_davidson
 
@@ -60,7 +63,12 @@ class ScfOptions:
     davidson_max_iterations: int = 20
     preconditioner: int = L.BD_PRECOND_QE
     seed: int = 1
+    # BD_FP64: every map in FP64. BD_MIXED: maps whose Davidson threshold is at
+    # least mixed_floor_ha run on the mixed-tier operator (FP32 FFT H·X, BF16x3
+    # GEMMs, FP64 factor/eigensolve); the later maps, and the final answer, run
+    # in FP64, so the fixed point is the FP64 one.
     precision: int = L.BD_FP64
+    mixed_floor_ha: float = 5e-5
     # Optional fixed-point verification map (driver.rs:727-757): one more
     # Davidson at the converged potential with every band at this threshold (Ha).
     final_threshold: float = 0.0
@@ -88,10 +96,10 @@ class DensityField:
     def to_grid(self, coeffs):
         g = np.zeros(self.size, dtype=np.complex128)
         g[self.flat] = coeffs
-        return np.fft.ifftn(g.reshape(self.dims)) * self.size
+        return sfft.ifftn(g.reshape(self.dims), workers=FFT_WORKERS) * self.size
 
     def from_grid(self, field):
-        return (np.fft.fftn(field) / self.size).reshape(-1)[self.flat]
+        return (sfft.fftn(field, workers=FFT_WORKERS) / self.size).reshape(-1)[self.flat]
 
     def dot(self, a, b):
         return float(np.sum(self.metric * (a.real * b.real + a.imag * b.imag)))
@@ -151,7 +159,9 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
     field = DensityField(system)
     vion_g = system.get("ionic_density_re") + 1j * system.get("ionic_density_im")
     vion = field.to_grid(vion_g).real
-    op = DistributedRealHamiltonian(system, capacity=bands, precision=options.precision)
+    op = DistributedRealHamiltonian(system, capacity=bands, precision=L.BD_FP64)
+    op_mixed = (DistributedRealHamiltonian(system, capacity=bands, precision=L.BD_MIXED)
+                if options.precision == L.BD_MIXED else None)
     diag_static = torch.from_numpy(system.get("diagonal_static")).to(dev)
     kinetic = system.get("kinetic_orc")
     ewald = float(system.get("ewald")[0])
@@ -175,6 +185,8 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
     rho_dev = torch.empty(field.size, dtype=torch.float64, device=dev)
     if profile_kernels:
         op.set_profiling(True)
+        if op_mixed is not None:
+            op_mixed.set_profiling(True)
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t_setup
 
@@ -182,7 +194,8 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
     ethr = 1e-2
     last_dr2 = math.inf
     history = []
-    totals = dict(h_columns=0, davidson_iterations=0, dense_s=0.0, apply_s=0.0, davidson_s=0.0, fields_s=0.0)
+    totals = dict(h_columns=0, davidson_iterations=0, dense_s=0.0, apply_s=0.0, davidson_s=0.0, fields_s=0.0,
+                  mixed_maps=0)
     result = None
     for iteration in range(1, options.max_iterations + 1):
         if iteration > 1:
@@ -196,7 +209,9 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
             eps, vxc = lda_pz(np.abs(rho_in))
             potential = vion + vh + vxc
             pot_dev = torch.from_numpy(potential.reshape(-1)).to(dev)
-            op.set_owned_potential(pot_dev)
+            use_mixed = op_mixed is not None and ethr / 2.0 >= options.mixed_floor_ha
+            cur = op_mixed if use_mixed else op
+            cur.set_owned_potential(pot_dev)
             diag = diag_static + float(potential.mean())
             totals["fields_s"] += time.perf_counter() - tf
             td = time.perf_counter()
@@ -204,14 +219,15 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
                                         first_step=(iteration == 1 and not redo),
                                         max_iterations=options.davidson_max_iterations, return_hx=True,
                                         preconditioner=options.preconditioner)
-            sol = ref_davidson.solve_with_initial_hx(op, orbitals, diag, opts, out=(xout, hxout))
+            sol = ref_davidson.solve_with_initial_hx(cur, orbitals, diag, opts, out=(xout, hxout))
             totals["davidson_s"] += time.perf_counter() - td
+            totals["mixed_maps"] += int(use_mixed)
             totals["h_columns"] += sol.record["h_columns"]
             totals["davidson_iterations"] += sol.record["iterations"]
             totals["dense_s"] += sol.record["seconds_dense"]
             totals["apply_s"] += sol.record["seconds_apply"]
             tf = time.perf_counter()
-            op.band_density_sum(xout, occ, rho_dev)
+            cur.band_density_sum(xout, occ, rho_dev)
             rho_out = rho_dev.cpu().numpy().reshape(field.dims)
             y = field.from_grid(rho_out)
             y[field.zero] = electrons / system.volume
@@ -226,7 +242,7 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
             break
         orbitals, xout = xout, orbitals
         last_dr2 = dr2
-        history.append(dict(iteration=iteration, dr2_ry=dr2, ethr_ry=ethr,
+        history.append(dict(iteration=iteration, dr2_ry=dr2, ethr_ry=ethr, mixed=use_mixed,
                             davidson_iterations=sol.record["iterations"], h_columns=sol.record["h_columns"],
                             converged=bool(sol.converged.all())))
         if log:
@@ -269,4 +285,6 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
                plan=op.plan_info())
     if profile_kernels:
         out["kernel_totals"] = op.stage_totals()
+        if op_mixed is not None:
+            out["kernel_totals_mixed"] = op_mixed.stage_totals()
     return out

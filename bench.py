@@ -117,6 +117,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--mixed-steps", type=int, default=1,
+                    help="SCFs timed with the mixed tier (loose maps in FP32/BF16x3, final maps in FP64)")
     ap.add_argument("--hx-only", type=int, default=0,
                     help="profile mode: N applies of H to 160 Si64 columns, then a 256 MiB copy")
     args = ap.parse_args()
@@ -154,6 +156,22 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_all
     last = results[-1]
+    mixed = None
+    if args.mixed_steps > 0:
+        from baandi_amd import lib as L
+        mopts = ScfOptions(bands=wl["bands"], threshold_ry=1e-8, precision=L.BD_MIXED)
+        mwalls, mres = [], None
+        for k in range(args.mixed_steps):
+            mres = run_scf(system, mopts)
+            mwalls.append(mres["wall_s"])
+            print(f"[bench] mixed step {k}: {mres['wall_s']:.3f}s iterations={mres['iterations']} "
+                  f"E={mres['energy']['total']:.12f} totals={mres['totals']}", file=sys.stderr, flush=True)
+        natoms = len(system.get("atoms_cart"))
+        occ = int(round(system.electrons)) // 2
+        mixed = {"wall_s": sum(mwalls) / len(mwalls), "iterations": mres["iterations"],
+                 "mixed_maps": mres["totals"]["mixed_maps"],
+                 "energy_diff_per_atom_ha": abs(mres["energy"]["total"] - last["energy"]["total"]) / natoms,
+                 "eigenvalue_max_diff_ha": float(np.abs(mres["eigenvalues"][:occ] - last["eigenvalues"][:occ]).max())}
     st = last["kernel_totals"]
     plan = last["plan"]
     launches = st["applies"]
@@ -179,7 +197,10 @@ def main():
                                "SCF to dr2<1e-8 Ry, plain FP64 path",
                    "n_pw": int(system.n), "fft_grid": list(system.dims), "bands": wl["bands"],
                    "scf_iterations": last["iterations"], "h_columns": last["totals"]["h_columns"],
-                   "mixed_learned_s": None, "speedup_mixed_vs_fp64": None},
+                   "mixed_s": None if mixed is None else round(mixed["wall_s"], 4),
+                   "speedup_mixed_vs_fp64": None if mixed is None else round(ms_per_step / 1e3 / mixed["wall_s"], 3),
+                   "mixed_learned_s": None},
+        "mixed": mixed,
         "energy_ha": last["energy"]["total"],
         "breakdown_s": {k: round(v, 4) for k, v in last["totals"].items() if k.endswith("_s")},
         "roofline": {

@@ -181,6 +181,23 @@ bd_status bd_davidson_solve_dev(bd_gamma_op* op, const double* x0_dev, int64_t l
                                 double* x_dev, double* hx_dev, int64_t ldx, int32_t* converged,
                                 bd_davidson_record* record);
 
+/* ------------------------------------------------------------------------
+ * Dense products of the path as stand-alone entry points
+ * C = alpha op(A) B + beta C, column-major, device pointers; op(A) = A^T when
+ * trans_a (then a split-K reduction needs `work`, bd_gemm_workspace doubles).
+ * mode 0: FP64; mode 1: BF16x3 split products on MFMA, FP32 accumulation.
+ * These are the products behind PᵀX / P·W (distributed_real_gamma.rs:957-1251)
+ * and the Davidson border Grams / rotations (ref_davidson/workspace.rs:300-381,
+ * 982-1012); no single reference function corresponds to the BF16x3 mode.
+ * ---------------------------------------------------------------------- */
+int64_t bd_gemm_workspace(int32_t mode, int32_t trans_a, int64_t M, int64_t N, int64_t K);
+bd_status bd_gemm_dev(int32_t mode, int32_t trans_a, int64_t M, int64_t N, int64_t K, double alpha,
+                      const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+                      int64_t ldc, double* work);
+/* One v_mfma_f32_16x16x16_bf16 of row-major 16x16 bf16 bit patterns (a self
+ * check of the MFMA operand layout the kernels assume). */
+bd_status bd_mfma_probe_dev(const uint16_t* a_bits, const uint16_t* b_bits, float* d);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,116 @@
+"""Mixed tier on MI355X: the MFMA operand layout, BF16x3 split GEMMs against
+FP64, and the FP32-FFT / BF16x3 H·X and Davidson against the FP64 oracle.
+Each test states the error it measured."""
+import numpy as np
+import pytest
+
+from conftest import fcc_rows
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16_bits(x):
+    """Round-to-nearest-even float32 -> bf16 bit patterns (test helper)."""
+    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16)
+
+
+def test_mfma_operand_layout():
+    """A 16x16x16 bf16 MFMA on small integers (exact in bf16 and in FP32)
+    equals the host product: pins the lane layout the BF16x3 kernels assume."""
+    import torch
+    from baandi_amd import lib as L
+    rng = np.random.default_rng(0)
+    a = rng.integers(-4, 5, (16, 16)).astype(np.float32)
+    b = rng.integers(-4, 5, (16, 16)).astype(np.float32)
+    ad = torch.from_numpy(bf16_bits(a).astype(np.int16)).cuda()
+    bd = torch.from_numpy(bf16_bits(b).astype(np.int16)).cuda()
+    d = torch.zeros(256, dtype=torch.float32, device="cuda")
+    L.check(L.load().bd_mfma_probe_dev(L.ptr(ad), L.ptr(bd), L.ptr(d)))
+    torch.cuda.synchronize()
+    assert np.array_equal(d.cpu().numpy().reshape(16, 16), a @ b)
+
+
+@pytest.mark.parametrize("trans_a,M,N,K", [(False, 23847, 160, 320), (True, 320, 160, 23847),
+                                           (True, 37, 5, 1000), (False, 100, 7, 33)])
+def test_bf16x3_gemm_error_bound(trans_a, M, N, K):
+    import torch
+    from baandi_amd import lib as L
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((K, M) if trans_a else (M, K))
+    B = rng.standard_normal((K, N))
+    ref = (A.T if trans_a else A) @ B
+    lib = L.load()
+    out = {}
+    for mode in (0, 1):
+        Ad = torch.from_numpy(np.asfortranarray(A).ravel(order="F")).cuda()
+        Bd = torch.from_numpy(np.asfortranarray(B).ravel(order="F")).cuda()
+        Cd = torch.zeros(M * N, dtype=torch.float64, device="cuda")
+        wsz = lib.bd_gemm_workspace(mode, int(trans_a), M, N, K)
+        W = torch.empty(max(wsz, 1), dtype=torch.float64, device="cuda")
+        lda = A.shape[0]
+        L.check(lib.bd_gemm_dev(mode, int(trans_a), M, N, K, 1.0, L.ptr(Ad), lda, L.ptr(Bd), K, 0.0, L.ptr(Cd), M,
+                                L.ptr(W)))
+        out[mode] = Cd.cpu().numpy().reshape(N, M).T
+    # FP64 tiles: rounding only.
+    assert np.abs(out[0] - ref).max() < 1e-12 * np.abs(ref).max()
+    # BF16x3: per-product relative error ~2^-16, FP32 accumulation; the bound
+    # scales with the sum of |a_ik b_kj| (random signs cancel in the sum).
+    scale = (np.abs(A.T if trans_a else A) @ np.abs(B)).max()
+    err = np.abs(out[1] - ref).max()
+    print(f"[bf16x3] trans_a={trans_a} {M}x{N}x{K}: max err {err:.3e}, relative to sum|ab| {err / scale:.3e}")
+    assert err < 2.0 ** -15 * scale
+
+
+def test_mixed_hx_against_fp64_oracle():
+    import torch
+    from baandi_amd.hamiltonian import GammaSystem, DistributedRealHamiltonian
+    from baandi_amd import lib as L
+    from oracle import physics as P
+    from oracle.hamiltonian import apply_orc
+    rows, atoms = fcc_rows(10.26), [[0, 0, 0], [0.25] * 3]
+    s = GammaSystem(rows, atoms, 4.0, 10.0)
+    o = P.build_gamma_system(rows, atoms, P.synthetic_pseudo(4.0, 1.0, 801), 10.0)
+    pot = o.ionic_grid
+    op = DistributedRealHamiltonian(s, capacity=9, precision=L.BD_MIXED)
+    op.set_owned_potential(pot.reshape(-1))
+    x = np.random.default_rng(3).standard_normal((9, s.n))
+    x /= np.linalg.norm(x, axis=1, keepdims=True)
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.empty_like(xd)
+    op.apply_block(xd, yd)
+    torch.cuda.synchronize()
+    ref = apply_orc(o, x.T, pot).T
+    err = np.abs(yd.cpu().numpy() - ref).max() / np.abs(ref).max()
+    print(f"[mixed H·X] max relative error vs FP64 oracle: {err:.3e}")
+    assert 1e-12 < err < 2e-5  # really reduced precision, and within the FP32/BF16x3 budget
+
+
+@pytest.mark.parametrize("precond", [0, 1])
+def test_mixed_davidson_eigenvalues_vs_fp64(precond):
+    """Mixed-tier Davidson (FP32 FFT H·X, BF16x3 Grams/rotations, FP64 factor
+    and eigensolve) against the dense FP64 eigenvalues at a loose threshold."""
+    import torch
+    from baandi_amd.hamiltonian import GammaSystem, DistributedRealHamiltonian
+    from baandi_amd import ref_davidson, lib as L
+    from oracle import physics as P
+    from oracle.hamiltonian import dense_orc
+    rows, atoms = fcc_rows(10.26), [[0, 0, 0], [0.25] * 3]
+    s = GammaSystem(rows, atoms, 4.0, 10.0)
+    o = P.build_gamma_system(rows, atoms, P.synthetic_pseudo(4.0, 1.0, 801), 10.0)
+    pot = o.ionic_grid
+    w = np.linalg.eigvalsh(dense_orc(o, pot))
+    op = DistributedRealHamiltonian(s, capacity=8, precision=L.BD_MIXED)
+    op.set_owned_potential(pot.reshape(-1))
+    g = np.random.default_rng(1).standard_normal((8, s.n))
+    x0 = torch.from_numpy(g / np.linalg.norm(g, axis=1, keepdims=True)).cuda()
+    diag = torch.from_numpy(s.get("diagonal_static") + pot.mean()).cuda()
+    sol = ref_davidson.solve(op, x0, diag, ref_davidson.Options(bands=8, threshold=1e-6, occupied=4,
+                                                                max_iterations=100, preconditioner=precond))
+    err = np.abs(sol.eigenvalues[:4] - w[:4]).max()
+    print(f"[mixed davidson] occupied eigenvalue error vs FP64 dense: {err:.3e} Ha, record={sol.record}")
+    assert sol.converged[:4].all()
+    # BF16x3 Grams carry ~2^-16 relative noise: the mixed tier's eigenvalue
+    # floor is ~2e-5 Ha, which is why the SCF finishes in FP64.
+    assert err < 5e-5
