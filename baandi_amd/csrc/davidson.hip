@@ -1,10 +1,11 @@
 // Reference block Davidson (crates/dft-solvers/src/ref_davidson.rs:117-366) on
-// the GPU operator. Control flow runs on the host; every n-row operation
-// (H·X, border Gram GEMMs, rotations, corrections, normalisation, rank
-// repair projections) runs on the op's stream. The small FP64 subspace steps
-// run on the host: the bordered inverse Cholesky (lean_davidson.rs:167-331)
-// and the reduced eigensolve; the m x m products around them (A = T H T^T,
-// C = T^T Y) run on the GPU.
+// the GPU operator. Control flow runs on the host; every matrix lives on the
+// GPU: the n-row operations (H·X, border Gram GEMMs, rotations, corrections,
+// normalisation, rank repair projections) and the FP64 subspace step (the
+// bordered inverse Cholesky of lean_davidson.rs:167-331, A = T H T^T, the
+// reduced eigensolve and C = T^T Y) run on the op's stream. Per iteration the
+// host reads back only the Ritz values, the correction norms and the factor
+// verdict.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -17,7 +18,8 @@
 #include <vector>
 
 #include "common.hpp"
-#include "dense.hpp"
+#include "eigen.hpp"
+#include "mlp.hpp"
 #include "gemm.hpp"
 #include "op.hpp"
 
@@ -132,10 +134,36 @@ struct Dev {
 struct DavidsonWork {
   int64_t n = 0;
   int nb = 0, cap = 0;
-  Dev V, HV, X, HX, E, E2, small1, small2, small3, gwork, theta, norm2, scale;
+  Dev V, HV, X, HX, E, E2, small1, small2, small3, gwork, theta, norm2, rnorm2, scale, mlpw;
+  // Subspace state (cap x cap, ld cap): overlap S, projected H, T = L^{-1}.
+  Dev S, H, T, Tt, A, C, Y, vals, eigw, fW, fG, fU, fgwork, fmin;
   int* idx = nullptr;
   int* flag = nullptr;
+  int* info = nullptr;  // [0] eigensolver, [1] factor verdict
+  FactorWork fw;
   DavidsonWork(int64_t n_, int nb_, int cap_) : n(n_), nb(nb_), cap(cap_) {
+    const size_t c2 = size_t(cap) * cap;
+    S.alloc(c2);
+    H.alloc(c2);
+    T.alloc(c2);
+    Tt.alloc(c2);
+    A.alloc(c2);
+    C.alloc(size_t(cap) * nb);
+    Y.alloc(size_t(cap) * nb);
+    vals.alloc(nb);
+    eigw.alloc(size_t(sym_eigen_work(cap, nb)));
+    fW.alloc(c2);
+    fG.alloc(c2);
+    fU.alloc(c2);
+    fgwork.alloc(size_t(factor_gwork(cap)));
+    fmin.alloc(1);
+    BD_HIP(hipMalloc(&info, 2 * sizeof(int)));
+    fw.W = fW.p;
+    fw.G = fG.p;
+    fw.U = fU.p;
+    fw.gwork = fgwork.p;
+    fw.flag = info + 1;
+    fw.min_ratio = fmin.p;
     V.alloc(size_t(n) * cap);
     HV.alloc(size_t(n) * cap);
     X.alloc(size_t(n) * cap);
@@ -148,6 +176,8 @@ struct DavidsonWork {
     gwork.alloc(size_t(std::max(Gemm::tn_work(cap, cap, n), int64_t(cap) * cap)));
     theta.alloc(nb);
     norm2.alloc(nb);
+    rnorm2.alloc(nb);
+    mlpw.alloc(size_t(mlp::correct_work(n, nb)));
     scale.alloc(nb);
     BD_HIP(hipMalloc(&idx, sizeof(int) * std::max(nb, 1)));
     BD_HIP(hipMalloc(&flag, sizeof(int)));
@@ -155,6 +185,7 @@ struct DavidsonWork {
   ~DavidsonWork() {
     if (idx) (void)hipFree(idx);
     if (flag) (void)hipFree(flag);
+    if (info) (void)hipFree(info);
   }
 };
 
@@ -171,8 +202,14 @@ struct Solver {
   double* HV;
   double* X;
   double* HX;
-  // host subspace state (column-major, ld = cap)
-  std::vector<double> S, Hr, T, C, values, old;
+  // Ritz values of the current subspace (host copy of the device values).
+  std::vector<double> values, old;
+  // Learned-preconditioner state: switched off after a regression; last
+  // residual norm^2 per band (NaN before the first correction).
+  bool mlp_off = false;
+  int strikes = 0;
+  int step_now = 0;
+  std::vector<double> prev_r2;
   bd_davidson_record rec{};
   double t_apply = 0.0, t_dense = 0.0;
   hipEvent_t ev0, ev1;
@@ -183,10 +220,9 @@ struct Solver {
     HV = w.HV.p;
     X = w.X.p;
     HX = w.HX.p;
-    S.assign(size_t(cap) * cap, 0.0);
-    Hr.assign(size_t(cap) * cap, 0.0);
-    T.assign(size_t(cap) * cap, 0.0);
-    values.assign(cap, 0.0);
+    values.assign(nb, 0.0);
+    prev_r2.assign(nb, std::numeric_limits<double>::quiet_NaN());
+    rec.mlp_fallback = -1;
     old.assign(nb, 0.0);
     BD_HIP(hipEventCreate(&ev0));
     BD_HIP(hipEventCreate(&ev1));
@@ -238,59 +274,36 @@ struct Solver {
     const int k = to - from;
     if (k <= 0) return;
     const double* B = (is_h ? HV : V) + int64_t(from) * n;
-    op.gemm.tn(s, to, k, n, 1.0, V, n, B, n, 0.0, w.small1.p, to, w.gwork.p);
-    std::vector<double> h(size_t(to) * k);
-    down(h.data(), w.small1.p, h.size());
-    std::vector<double>& M = is_h ? Hr : S;
-    for (int jj = 0; jj < k; ++jj) {
-      const int j = from + jj;
-      for (int i = 0; i <= j; ++i) {
-        const double v = h[i + size_t(jj) * to];
-        M[i + size_t(j) * cap] = v;
-        M[j + size_t(i) * cap] = v;
-      }
-    }
+    double* M = is_h ? w.H.p : w.S.p;
+    op.gemm.tn(s, to, k, n, 1.0, V, n, B, n, 0.0, M + int64_t(from) * cap, cap, w.gwork.p);
+    mirror_border(s, M, cap, from, to);
   }
 
   bool factor(int from, int to) {
     auto t0 = std::chrono::steady_clock::now();
-    double ratio;
-    bool ok = extend_inverse_cholesky(S.data(), cap, from, to, T.data(), &ratio);
+    extend_inverse_cholesky_dev(s, w.S.p, w.T.p, cap, from, to, w.fw);
+    int verdict = 0;
+    BD_HIP(hipMemcpyAsync(&verdict, w.fw.flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    sync();
     t_dense += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    return ok;
+    return verdict == 0;
   }
 
-  // Host (m x m, ld cap) -> device compact (ld m).
-  void up_square(double* d, const std::vector<double>& h, int m, bool transpose) {
-    std::vector<double> c(size_t(m) * m);
-    for (int j = 0; j < m; ++j)
-      for (int i = 0; i < m; ++i) c[i + size_t(j) * m] = transpose ? h[j + size_t(i) * cap] : h[i + size_t(j) * cap];
-    up(d, c.data(), c.size());
-  }
-
-  // A = T H_r T^T, EVD, C = T^T Y[:, :nb] (workspace.rs:435-499).
+  // A = T H_r T^T, lowest nb eigenpairs, C = T^T Y (workspace.rs:435-499).
   void eigen(int m) {
-    up_square(w.small1.p, T, m, true);   // T^T
-    up_square(w.small2.p, Hr, m, false); // H_r
-    gemm_nn(s, m, m, m, 1.0, w.small2.p, m, w.small1.p, m, 0.0, w.small3.p, m);  // H_r T^T
-    up_square(w.small2.p, T, m, false);  // T
-    gemm_nn(s, m, m, m, 1.0, w.small2.p, m, w.small3.p, m, 0.0, w.small1.p, m);  // A
-    std::vector<double> A(size_t(m) * m), Y(size_t(m) * m);
-    down(A.data(), w.small1.p, A.size());
-    for (int j = 0; j < m; ++j)
-      for (int i = 0; i < j; ++i) {
-        const double v = 0.5 * (A[i + size_t(j) * m] + A[j + size_t(i) * m]);
-        A[i + size_t(j) * m] = A[j + size_t(i) * m] = v;
-      }
     auto t0 = std::chrono::steady_clock::now();
-    if (!symmetric_eigen(A.data(), m, m, values.data(), Y.data()))
-      fail(BD_ERR_APPLICATION, "reference reduced eigensolve failed");
+    transpose(s, m, m, w.T.p, cap, w.Tt.p, m);
+    gemm_nn(s, m, m, m, 1.0, w.H.p, cap, w.Tt.p, m, 0.0, w.small3.p, m);  // H_r T^T
+    gemm_nn(s, m, m, m, 1.0, w.T.p, cap, w.small3.p, m, 0.0, w.A.p, m);   // A
+    symmetrize(s, m, w.A.p, m);
+    sym_eigen_lowest(s, m, nb, w.A.p, m, w.vals.p, w.Y.p, m, w.eigw.p, w.info);
+    gemm_nn(s, m, nb, m, 1.0, w.Tt.p, m, w.Y.p, m, 0.0, w.C.p, m);  // C
+    int bad = 0;
+    BD_HIP(hipMemcpyAsync(values.data(), w.vals.p, size_t(nb) * 8, hipMemcpyDeviceToHost, s));
+    BD_HIP(hipMemcpyAsync(&bad, w.info, sizeof(int), hipMemcpyDeviceToHost, s));
+    sync();
     t_dense += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    up_square(w.small2.p, T, m, true);  // T^T
-    up(w.small3.p, Y.data(), size_t(m) * nb);
-    gemm_nn(s, m, nb, m, 1.0, w.small2.p, m, w.small3.p, m, 0.0, w.small1.p, m);  // C
-    C.assign(size_t(m) * nb, 0.0);
-    down(C.data(), w.small1.p, C.size());
+    if (bad) fail(BD_ERR_APPLICATION, "reference reduced eigensolve failed");
   }
 
   void refresh(int m) {
@@ -301,14 +314,12 @@ struct Solver {
 
   // V <- V C (nb columns), HV <- HV C, measured refresh, C = I (workspace.rs:772-935).
   void contract(int m) {
-    up(w.small1.p, C.data(), size_t(m) * nb);
-    op.gemm.nn(s, n, nb, m, 1.0, V, n, w.small1.p, m, 0.0, X, n);
-    op.gemm.nn(s, n, nb, m, 1.0, HV, n, w.small1.p, m, 0.0, HX, n);
+    op.gemm.nn(s, n, nb, m, 1.0, V, n, w.C.p, m, 0.0, X, n);
+    op.gemm.nn(s, n, nb, m, 1.0, HV, n, w.C.p, m, 0.0, HX, n);
     std::swap(V, X);
     std::swap(HV, HX);
     refresh(nb);
-    C.assign(size_t(nb) * nb, 0.0);
-    for (int i = 0; i < nb; ++i) C[i + size_t(i) * nb] = 1.0;
+    set_identity(s, nb, nb, w.C.p, nb);
     rec.restarts += 1;
   }
 
@@ -318,24 +329,58 @@ struct Solver {
     std::vector<double> th(k);
     for (int j = 0; j < k; ++j) th[j] = values[active[j]];
     up(w.theta.p, th.data(), k);
+    up_bytes(w.idx, active.data(), sizeof(int) * k);
     if (contracted) {
-      std::vector<int> idx(active);
-      up_bytes(w.idx, idx.data(), sizeof(int) * k);
       hipLaunchKernelGGL(k_gather_scale, dim3(64, k), dim3(256), 0, s, V, n, w.idx, nullptr, X);
       hipLaunchKernelGGL(k_gather_scale, dim3(64, k), dim3(256), 0, s, HV, n, w.idx, nullptr, HX);
     } else {
-      std::vector<double> ca(size_t(m) * k);
-      for (int j = 0; j < k; ++j)
-        std::copy(&C[size_t(active[j]) * m], &C[size_t(active[j]) * m] + m, &ca[size_t(j) * m]);
-      up(w.small1.p, ca.data(), ca.size());
+      hipLaunchKernelGGL(k_gather_scale, dim3(1, k), dim3(256), 0, s, w.C.p, int64_t(m), w.idx, nullptr, w.small1.p);
       op.gemm.nn(s, n, k, m, 1.0, V, n, w.small1.p, m, 0.0, X, n);
       op.gemm.nn(s, n, k, m, 1.0, HV, n, w.small1.p, m, 0.0, HX, n);
     }
-    hipLaunchKernelGGL(k_corrections, dim3(k), dim3(kT), 0, s, X, HX, n, w.theta.p, diag, op.d_kinetic.p,
-                       o.preconditioner == BD_PRECOND_TPA ? 1 : 0, w.E.p, w.norm2.p);
     std::vector<double> nrm(k);
+    if (o.preconditioner == BD_PRECOND_MLP && !mlp_off) {
+      mlp::correct(s, n, k, X, HX, w.theta.p, op.d_kinetic.p, o.mlp_vbar, o.mlp_blob, w.E.p, w.norm2.p,
+                   w.rnorm2.p, w.mlpw.p);
+      std::vector<double> rn(k);
+      BD_HIP(hipMemcpyAsync(rn.data(), w.rnorm2.p, size_t(k) * 8, hipMemcpyDeviceToHost, s));
+      down(nrm.data(), w.norm2.p, k);
+      bool finite = true;
+      for (double v : nrm) finite = finite && std::isfinite(v);
+      if (!finite || mlp_regressed(active, rn)) {
+        // TPA fallback for the rest of this solve; a non-finite network
+        // correction is replaced in this iteration already.
+        mlp_off = true;
+        rec.mlp_fallback = step_now;
+        if (finite) return nrm;
+      } else {
+        return nrm;
+      }
+    }
+    const int mode = (o.preconditioner == BD_PRECOND_QE) ? 0 : 1;
+    hipLaunchKernelGGL(k_corrections, dim3(k), dim3(kT), 0, s, X, HX, n, w.theta.p, diag, op.d_kinetic.p, mode,
+                       w.E.p, w.norm2.p);
     down(nrm.data(), w.norm2.p, k);
     return nrm;
+  }
+
+  // The fallback rule: the residual-reduction ratio over the columns active
+  // in this and the previous correction step, sqrt(sum |r_new|^2 / sum
+  // |r_old|^2), at or above 1 (or not finite) in two consecutive steps.
+  bool mlp_regressed(const std::vector<int>& active, const std::vector<double>& rn) {
+    double num = 0.0, den = 0.0;
+    for (size_t j = 0; j < active.size(); ++j) {
+      const int b = active[j];
+      if (std::isfinite(prev_r2[b])) {
+        num += rn[j];
+        den += prev_r2[b];
+      }
+      prev_r2[b] = rn[j];
+    }
+    if (den <= 0.0) return false;
+    const double ratio = std::sqrt(num / den);
+    strikes = (!std::isfinite(ratio) || ratio >= 1.0) ? strikes + 1 : 0;
+    return strikes >= 2;
   }
 
   // Drop |e| <= 32 eps, scale the rest into dst (stream.rs:81-143); returns kept.
@@ -362,23 +407,13 @@ struct Solver {
   int repair(int m, int k) {
     rec.repairs += 1;
     // Coordinates of the Ritz block in the whitened basis: C <- T S C.
-    std::vector<double> SC(size_t(m) * nb, 0.0), TSC(size_t(m) * nb, 0.0);
-    for (int c = 0; c < nb; ++c)
-      for (int j = 0; j < m; ++j) {
-        const double v = C[j + size_t(c) * m];
-        for (int i = 0; i < m; ++i) SC[i + size_t(c) * m] += S[i + size_t(j) * cap] * v;
-      }
-    for (int c = 0; c < nb; ++c)
-      for (int j = 0; j < m; ++j) {
-        const double v = SC[j + size_t(c) * m];
-        for (int i = j; i < m; ++i) TSC[i + size_t(c) * m] += T[i + size_t(j) * cap] * v;
-      }
-    up_square(w.small1.p, T, m, true);  // T^T
+    gemm_nn(s, m, nb, m, 1.0, w.S.p, cap, w.C.p, m, 0.0, w.small2.p, m);
+    gemm_nn(s, m, nb, m, 1.0, w.T.p, cap, w.small2.p, m, 0.0, w.C.p, m);
+    transpose(s, m, m, w.T.p, cap, w.small1.p, m);  // T^T
     op.gemm.nn(s, n, m, m, 1.0, V, n, w.small1.p, m, 0.0, X, n);
     op.gemm.nn(s, n, m, m, 1.0, HV, n, w.small1.p, m, 0.0, HX, n);
     std::swap(V, X);
     std::swap(HV, HX);
-    C = TSC;
     refresh(m);
     double* E = w.E2.p;
     for (int pass = 0; pass < 2; ++pass) {
@@ -417,6 +452,7 @@ struct Solver {
   bd_status run(const double* x0, int64_t ldx0, const double* hx0, double* eig, double* xout, double* hxout,
                 int64_t ldx, int32_t* conv) {
     auto t_start = std::chrono::steady_clock::now();
+    zero_doubles(s, w.T.p, int64_t(cap) * cap);  // T stays lower triangular from here on
     BD_HIP(hipMemcpy2DAsync(V, n * 8, x0, ldx0 * 8, n * 8, nb, hipMemcpyDeviceToDevice, s));
     if (hx0) BD_HIP(hipMemcpy2DAsync(HV, n * 8, hx0, ldx0 * 8, n * 8, nb, hipMemcpyDeviceToDevice, s));
     else apply(V, nb, HV);
@@ -436,6 +472,7 @@ struct Solver {
         m = nb;
         contracted = true;
       }
+      step_now = step;
       std::vector<double> nrm = corrections(m, active, contracted);
       contracted = false;
       int k = 0;
@@ -476,9 +513,8 @@ struct Solver {
       }
     }
     // Return X = V C, HX = HV C (workspace.rs:936-980).
-    up(w.small1.p, C.data(), size_t(m) * nb);
-    op.gemm.nn(s, n, nb, m, 1.0, V, n, w.small1.p, m, 0.0, xout, ldx);
-    if (hxout) op.gemm.nn(s, n, nb, m, 1.0, HV, n, w.small1.p, m, 0.0, hxout, ldx);
+    op.gemm.nn(s, n, nb, m, 1.0, V, n, w.C.p, m, 0.0, xout, ldx);
+    if (hxout) op.gemm.nn(s, n, nb, m, 1.0, HV, n, w.C.p, m, 0.0, hxout, ldx);
     sync();
     bool all = true;
     for (int i = 0; i < nb; ++i) {
@@ -528,6 +564,9 @@ extern "C" bd_status bd_davidson_solve_dev(bd_gamma_op* h, const double* x0, int
         ldx < n || (o->return_hx && !hxout))
       bd::fail(BD_ERR_INVALID_PARAMETER, "reference Davidson dimensions/threshold/start");
     if (nb > op.capacity) bd::fail(BD_ERR_INVALID_PARAMETER, "bands exceed the operator capacity");
+    if (o->preconditioner < BD_PRECOND_QE || o->preconditioner > BD_PRECOND_MLP ||
+        (o->preconditioner == BD_PRECOND_MLP && (!o->mlp_blob || !std::isfinite(o->mlp_vbar))))
+      bd::fail(BD_ERR_INVALID_PARAMETER, "reference Davidson preconditioner");
     if (!op.has_potential) bd::fail(BD_ERR_INVALID_PARAMETER, "no potential installed");
     bd::DavidsonWork* w = bd::davidson_work(op, n, nb, cap);
     bd::Solver solver(op, *w, *o, diag);

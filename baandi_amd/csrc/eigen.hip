@@ -1,0 +1,561 @@
+// FP64 symmetric eigensolver for the Rayleigh-Ritz step (the role of
+// RealEighWorkspace::solve_into, crates/dft-numerics/src/real.rs:2286-2330,
+// called from ref_davidson/workspace.rs:435-499). Only the lowest `want`
+// pairs are formed, which is all the Davidson step reads.
+//
+//  1. Householder tridiagonalisation (Golub & Van Loan, Matrix Computations,
+//     4th ed., Alg. 8.3.1) in one 1024-thread workgroup: the matrix stays in
+//     the L2 of one XCD, and each step updates the trailing block and
+//     multiplies it by the next reflector in a single pass.
+//  2. Sturm-sequence bisection (Barth, Martin & Wilkinson 1967), one wave per
+//     eigenvalue with 64-point multisection per step.
+//  3. Inverse iteration with a partially pivoted tridiagonal LU (the scheme
+//     of LAPACK dlagtf/dlagts/dstein), one lane per eigenvector.
+//  4. Two passes of classical Gram-Schmidt inside clusters of close
+//     eigenvalues (gap <= 1e-3 ||T||, the dstein criterion), in ascending
+//     order, with a fresh inverse-iteration solve when a vector loses most of
+//     its norm.
+//  5. Back-transformation by the stored reflectors, one wave per vector.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <string>
+
+#include "common.hpp"
+#include "eigen.hpp"
+
+namespace bd {
+
+namespace {
+constexpr int kTT = 1024;      // tridiagonalisation / Gram-Schmidt workgroup
+constexpr int kTW = kTT / 64;  // its waves
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Sum over a kTT-thread block, returned to every thread (fixed order).
+// `red` holds kTW doubles of LDS; ends with a barrier so it can be reused.
+__device__ double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int i = 0; i < kTW; ++i) t += red[i];
+  __syncthreads();
+  return t;
+}
+
+// Deterministic start vector entries in (-1, 1).
+__device__ __forceinline__ double hash_unit(uint32_t i, uint32_t j, uint32_t salt) {
+  uint32_t h = i * 0x9E3779B1u ^ (j * 0x85EBCA77u + salt * 0xC2B2AE3Du + 0x27D4EB2Fu);
+  h ^= h >> 16;
+  h *= 0x7FEB352Du;
+  h ^= h >> 15;
+  h *= 0x846CA68Bu;
+  h ^= h >> 16;
+  return (double(h) + 0.5) * (2.0 / 4294967296.0) - 1.0;
+}
+
+// ---------------------------------------------------------------------------
+// 1. Tridiagonalisation. Reflector k is H_k = I - tau_k x x^T with x stored in
+// A[k+1:, k] (x[0] = a - alpha, unnormalised); d, e the tridiagonal
+// (e[k] couples d[k] and d[k+1]).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kTT) void k_tridiag(int m, double* __restrict__ A, int64_t lda, double* __restrict__ d,
+                                                 double* __restrict__ e, double* __restrict__ tau) {
+  extern __shared__ double sh[];
+  double* xa = sh;
+  double* xb = sh + m;
+  double* w = sh + 2 * m;
+  double* q = sh + 3 * m;
+  double* red = sh + 4 * m;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  auto a = [&](int i, int j) -> double& { return A[i + int64_t(j) * lda]; };
+  if (m < 3) {
+    if (tid == 0) {
+      d[0] = a(0, 0);
+      tau[0] = 0.0;
+      if (m == 2) {
+        e[0] = a(1, 0);
+        d[1] = a(1, 1);
+        tau[1] = 0.0;
+      }
+    }
+    return;
+  }
+  // Reflector of column k from x (its rows k+1.., already in LDS) and
+  // sum(x^2); returns tau_k (0 when the column is already zero).
+  auto reflect = [&](double ss, double* xv, int k) -> double {
+    const double sigma = sqrt(ss);
+    const double x0 = xv[0];
+    double alpha = 0.0, tk = 0.0;
+    if (sigma > 0.0) {
+      alpha = x0 >= 0.0 ? -sigma : sigma;
+      tk = 1.0 / (sigma * (sigma + fabs(x0)));  // 2 / |x - alpha e1|^2
+    }
+    __syncthreads();
+    if (tid == 0) {
+      if (sigma > 0.0) {
+        xv[0] = x0 - alpha;
+        a(k + 1, k) = x0 - alpha;
+      }
+      e[k] = alpha;
+      tau[k] = tk;
+    }
+    __syncthreads();
+    return tk;
+  };
+
+  double* x = xa;
+  double* xn = xb;
+  int L = m - 1;
+  double ss = 0.0;
+  for (int r = tid; r < L; r += kTT) {
+    const double v = a(1 + r, 0);
+    x[r] = v;
+    ss += v * v;
+  }
+  if (tid == 0) d[0] = a(0, 0);
+  ss = block_sum(ss, red);
+  double t = reflect(ss, x, 0);
+  // q = B_0 x, B_0 = A[1:, 1:] (one wave per column, lanes along rows).
+  for (int j = wave; j < L; j += kTW) {
+    const double* col = &a(1, 1 + j);
+    double acc = 0.0;
+    for (int r = lane; r < L; r += 64) acc += col[r] * x[r];
+    acc = wave_sum(acc);
+    if (lane == 0) q[j] = acc;
+  }
+  __syncthreads();
+
+  for (int k = 0; k + 2 < m; ++k) {
+    L = m - k - 1;
+    const int c0 = k + 1;  // global index of the trailing block's first row/column
+    // p = t q, K = t/2 p.x, w = p - K x.
+    double part = 0.0;
+    for (int r = tid; r < L; r += kTT) part += q[r] * x[r];
+    const double K = 0.5 * t * t * block_sum(part, red);
+    for (int r = tid; r < L; r += kTT) w[r] = t * q[r] - K * x[r];
+    __syncthreads();
+    if (L == 2) {
+      if (tid == 0) {
+        const double b00 = a(c0, c0) - 2.0 * x[0] * w[0];
+        const double b10 = a(c0 + 1, c0) - (x[1] * w[0] + w[1] * x[0]);
+        const double b11 = a(c0 + 1, c0 + 1) - 2.0 * x[1] * w[1];
+        d[m - 2] = b00;
+        e[m - 2] = b10;
+        d[m - 1] = b11;
+        tau[m - 2] = 0.0;
+        tau[m - 1] = 0.0;
+      }
+      break;
+    }
+    // First column of B_k: rows 1.. become the next reflector's vector.
+    const double x0 = x[0], w0 = w[0];
+    ss = 0.0;
+    for (int r = tid; r < L; r += kTT) {
+      const double b = a(c0 + r, c0) - (x[r] * w0 + w[r] * x0);
+      if (r == 0) {
+        d[c0] = b;
+      } else {
+        xn[r - 1] = b;
+        a(c0 + r, c0) = b;
+        ss += b * b;
+      }
+    }
+    ss = block_sum(ss, red);
+    const double tn = reflect(ss, xn, c0);
+    // Rest of B_k: B -= x w^T + w x^T, and q' = B_{k+1} x' in the same pass.
+    for (int j = 1 + wave; j < L; j += kTW) {
+      const double xj = x[j], wj = w[j];
+      double* col = &a(c0, c0 + j);
+      double acc = 0.0;
+      for (int r = 1 + lane; r < L; r += 64) {
+        const double b = col[r] - (x[r] * wj + w[r] * xj);
+        col[r] = b;
+        acc += b * xn[r - 1];
+      }
+      acc = wave_sum(acc);
+      if (lane == 0) q[j - 1] = acc;
+    }
+    __syncthreads();
+    double* tmp = x;
+    x = xn;
+    xn = tmp;
+    t = tn;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 2. Bisection: one wave per wanted eigenvalue index j (ascending).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int sturm_count(int m, const double* ds, const double* e2, double x, double pivmin) {
+  double q = ds[0] - x;
+  if (fabs(q) <= pivmin) q = -pivmin;
+  int c = q < 0.0 ? 1 : 0;
+  for (int i = 1; i < m; ++i) {
+    q = ds[i] - x - e2[i - 1] / q;
+    if (fabs(q) <= pivmin) q = -pivmin;
+    c += q < 0.0 ? 1 : 0;
+  }
+  return c;
+}
+
+__global__ __launch_bounds__(256) void k_bisect(int m, int want, const double* __restrict__ d,
+                                                const double* __restrict__ e, double* __restrict__ values,
+                                                double* __restrict__ tnorm_out) {
+  extern __shared__ double sh[];
+  double* ds = sh;
+  double* e2 = sh + m;
+  double* ea = sh + 2 * m;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    ds[i] = d[i];
+    const double v = i + 1 < m ? e[i] : 0.0;
+    e2[i] = v * v;
+    ea[i] = fabs(v);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (j >= want) return;  // whole waves leave together; no barrier follows
+  double gl = INFINITY, gu = -INFINITY, emax2 = 0.0;
+  for (int i = lane; i < m; i += 64) {
+    const double r = (i > 0 ? ea[i - 1] : 0.0) + ea[i];
+    gl = fmin(gl, ds[i] - r);
+    gu = fmax(gu, ds[i] + r);
+    emax2 = fmax(emax2, e2[i]);
+  }
+  gl = wave_min(gl);
+  gu = wave_max(gu);
+  emax2 = wave_max(emax2);
+  const double tnorm = fmax(fabs(gl), fabs(gu));
+  const double pivmin = DBL_MIN * fmax(1.0, emax2);
+  const double pad = 2.1 * tnorm * DBL_EPSILON * m + 4.2 * pivmin;
+  double lo = gl - pad, hi = gu + pad;
+  // Invariant: count(lo) <= j < count(hi). 64 points per step shrink the
+  // bracket 65x; the cap bounds the loop.
+  for (int it = 0; it < 48; ++it) {
+    const double width = hi - lo;
+    if (width <= 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + pivmin) break;
+    const double xl = lo + width * (double(lane + 1) / 65.0);
+    const int c = sturm_count(m, ds, e2, xl, pivmin);
+    const unsigned long long above = __ballot(c >= j + 1);
+    if (above == 0ull) {
+      lo = __shfl(xl, 63, 64);
+    } else {
+      const int f = __ffsll(static_cast<long long>(above)) - 1;
+      const double nhi = __shfl(xl, f, 64);
+      const double nlo = __shfl(xl, f > 0 ? f - 1 : 0, 64);
+      hi = nhi;
+      if (f > 0) lo = nlo;
+    }
+  }
+  if (lane == 0) {
+    values[j] = 0.5 * (lo + hi);
+    if (j == 0) tnorm_out[0] = tnorm;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3. Inverse iteration. Per eigenvector j the LU of (T - lambda_j I) with
+// partial pivoting: U has the reciprocal diagonal U0i and two superdiagonals
+// U1, U2; Lm the multipliers, piv the row swaps. Arrays are laid out
+// [i * want + j] so lanes (consecutive j) access consecutive addresses.
+// ---------------------------------------------------------------------------
+struct LU {
+  double* U0i;
+  double* U1;
+  double* U2;
+  double* Lm;
+  unsigned char* piv;
+};
+
+// Solve (T - lambda_j I) y = z in place (z[i * zs]); returns max |y|.
+__device__ double lu_solve(int m, int want, int j, const LU& f, double* z, int64_t zs) {
+  auto at = [&](int i) { return int64_t(i) * want + j; };
+  double cur = z[0];
+  for (int i = 0; i + 1 < m; ++i) {
+    double nxt = z[int64_t(i + 1) * zs];
+    if (f.piv[at(i)]) {
+      const double tmp = cur;
+      cur = nxt;
+      nxt = tmp;
+    }
+    nxt -= f.Lm[at(i)] * cur;
+    z[int64_t(i) * zs] = cur;
+    cur = nxt;
+  }
+  z[int64_t(m - 1) * zs] = cur;
+  double y1 = 0.0, y2 = 0.0, big = 0.0;
+  for (int i = m - 1; i >= 0; --i) {
+    const double y = (z[int64_t(i) * zs] - f.U1[at(i)] * y1 - f.U2[at(i)] * y2) * f.U0i[at(i)];
+    z[int64_t(i) * zs] = y;
+    big = fmax(big, fabs(y));
+    y2 = y1;
+    y1 = y;
+  }
+  return big;
+}
+
+__global__ __launch_bounds__(64) void k_invit(int m, int want, const double* __restrict__ d,
+                                              const double* __restrict__ e, const double* __restrict__ values,
+                                              const double* __restrict__ tnorm_in, LU f, double* __restrict__ Z) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= want) return;
+  auto at = [&](int i) { return int64_t(i) * want + j; };
+  const double lam = values[j];
+  const double tiny = DBL_EPSILON * fmax(tnorm_in[0], DBL_MIN);
+  double a = d[0] - lam, b = m > 1 ? e[0] : 0.0;
+  for (int i = 0; i + 1 < m; ++i) {
+    const double c = e[i];
+    const double dn = d[i + 1] - lam;
+    const double en = i + 2 < m ? e[i + 1] : 0.0;
+    if (fabs(a) >= fabs(c)) {
+      double p = a;
+      if (fabs(p) < tiny) p = p >= 0.0 ? tiny : -tiny;
+      const double r = 1.0 / p;
+      f.U0i[at(i)] = r;
+      f.U1[at(i)] = b;
+      f.U2[at(i)] = 0.0;
+      f.Lm[at(i)] = c * r;
+      f.piv[at(i)] = 0;
+      a = dn - c * r * b;
+      b = en;
+    } else {
+      const double r = 1.0 / c;
+      const double l = a * r;
+      f.U0i[at(i)] = r;
+      f.U1[at(i)] = dn;
+      f.U2[at(i)] = en;
+      f.Lm[at(i)] = l;
+      f.piv[at(i)] = 1;
+      a = b - l * dn;
+      b = -l * en;
+    }
+  }
+  if (fabs(a) < tiny) a = a >= 0.0 ? tiny : -tiny;
+  f.U0i[at(m - 1)] = 1.0 / a;
+  double* z = Z + j;
+  for (int i = 0; i < m; ++i) z[int64_t(i) * want] = hash_unit(i, j, 0);
+  for (int it = 0; it < 3; ++it) {
+    const double big = lu_solve(m, want, j, f, z, want);
+    const double s = big > 0.0 ? 1.0 / big : 1.0;
+    for (int i = 0; i < m; ++i) z[int64_t(i) * want] *= s;
+  }
+  double nn = 0.0;
+  for (int i = 0; i < m; ++i) nn += z[int64_t(i) * want] * z[int64_t(i) * want];
+  const double s = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
+  for (int i = 0; i < m; ++i) z[int64_t(i) * want] *= s;
+}
+
+// ---------------------------------------------------------------------------
+// 4. Cluster Gram-Schmidt (one workgroup, vectors in ascending order). Reads
+// the inverse-iteration vectors from Z ([i * want + j]) and writes the
+// orthonormal ones to Y (column-major, ldy).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kTT) void k_cluster_gs(int m, int want, const double* __restrict__ values,
+                                                    const double* __restrict__ tnorm_in, LU f,
+                                                    const double* __restrict__ Z, double* __restrict__ Y,
+                                                    int64_t ldy, int* __restrict__ info) {
+  extern __shared__ double sh[];
+  double* v = sh;
+  double* c = sh + m;
+  double* red = sh + m + want;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const double ortol = 1e-3 * tnorm_in[0];
+  int cs = 0;
+  for (int j = 0; j < want; ++j) {
+    if (j > 0 && values[j] - values[j - 1] > ortol) cs = j;
+    for (int i = tid; i < m; i += kTT) v[i] = Z[int64_t(i) * want + j];
+    __syncthreads();
+    double scale = 0.0;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+      for (int pass = 0; pass < 2 && j > cs; ++pass) {
+        for (int l = cs + wave; l < j; l += kTW) {
+          const double* yl = Y + int64_t(l) * ldy;
+          double acc = 0.0;
+          for (int i = lane; i < m; i += 64) acc += yl[i] * v[i];
+          acc = wave_sum(acc);
+          if (lane == 0) c[l] = acc;
+        }
+        __syncthreads();
+        for (int i = tid; i < m; i += kTT) {
+          double s0 = 0.0, s1 = 0.0;
+          int l = cs;
+          for (; l + 1 < j; l += 2) {
+            s0 += Y[i + int64_t(l) * ldy] * c[l];
+            s1 += Y[i + int64_t(l + 1) * ldy] * c[l + 1];
+          }
+          if (l < j) s0 += Y[i + int64_t(l) * ldy] * c[l];
+          v[i] -= s0 + s1;
+        }
+        __syncthreads();
+      }
+      double nn = 0.0;
+      for (int i = tid; i < m; i += kTT) nn += v[i] * v[i];
+      nn = block_sum(nn, red);
+      if (nn > 1e-2) {
+        scale = 1.0 / sqrt(nn);
+        break;
+      }
+      if (attempt == 3) {
+        if (tid == 0) info[0] = 1 + j;
+        scale = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
+        break;
+      }
+      // Most of the vector lay in the span of its cluster: restart from the
+      // remainder plus a fresh direction, and apply (T - lambda_j I)^{-1}.
+      const double rs = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
+      for (int i = tid; i < m; i += kTT) v[i] = v[i] * rs + 0.1 * hash_unit(i, j, 1 + attempt);
+      __syncthreads();
+      if (tid == 0) {
+        for (int it = 0; it < 2; ++it) {
+          const double big = lu_solve(m, want, j, f, v, 1);
+          const double s = big > 0.0 ? 1.0 / big : 1.0;
+          for (int i = 0; i < m; ++i) v[i] *= s;
+        }
+      }
+      __syncthreads();
+      nn = 0.0;
+      for (int i = tid; i < m; i += kTT) nn += v[i] * v[i];
+      nn = block_sum(nn, red);
+      const double s = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
+      for (int i = tid; i < m; i += kTT) v[i] *= s;
+      __syncthreads();
+    }
+    for (int i = tid; i < m; i += kTT) Y[i + int64_t(j) * ldy] = v[i] * scale;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 5. Y <- H_0 H_1 ... H_{m-3} Y, one wave per column; each lane owns the rows
+// lane, lane + 64, ... of its column in LDS.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_back_transform(int m, int want, const double* __restrict__ A, int64_t lda,
+                                                        const double* __restrict__ tau, double* __restrict__ Y,
+                                                        int64_t ldy) {
+  extern __shared__ double sh[];
+  const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
+  const int col = blockIdx.x * (blockDim.x >> 6) + wl;
+  if (col >= want) return;
+  double* y = sh + int64_t(wl) * m;
+  double* yg = Y + int64_t(col) * ldy;
+  for (int i = lane; i < m; i += 64) y[i] = yg[i];
+  for (int k = m - 3; k >= 0; --k) {
+    const double t = tau[k];
+    if (t == 0.0) continue;
+    const double* v = A + int64_t(k) * lda;  // v[i] for rows i >= k+1
+    double acc = 0.0;
+    for (int i = k + 1 + ((lane - (k + 1)) & 63); i < m; i += 64) acc += v[i] * y[i];
+    const double s = t * wave_sum(acc);
+    for (int i = k + 1 + ((lane - (k + 1)) & 63); i < m; i += 64) y[i] -= s * v[i];
+  }
+  for (int i = lane; i < m; i += 64) yg[i] = y[i];
+}
+
+__global__ void k_zero_i(int* p, int count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = 0;
+}
+__global__ void k_zero_d(double* p, int64_t count) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = 0.0;
+}
+
+struct EigenLayout {
+  double* d;
+  double* e;
+  double* tau;
+  double* tnorm;
+  LU f;
+  double* Z;
+};
+
+EigenLayout eigen_layout(int m, int want, double* work) {
+  const int64_t mw = int64_t(m) * want;
+  EigenLayout L;
+  L.d = work;
+  L.e = L.d + m;
+  L.tau = L.e + m;
+  L.tnorm = L.tau + m;
+  L.f.U0i = L.tnorm + 8;
+  L.f.U1 = L.f.U0i + mw;
+  L.f.U2 = L.f.U1 + mw;
+  L.f.Lm = L.f.U2 + mw;
+  L.Z = L.f.Lm + mw;
+  L.f.piv = reinterpret_cast<unsigned char*>(L.Z + mw);
+  return L;
+}
+}  // namespace
+
+int64_t sym_eigen_work(int m, int want) {
+  const int64_t mw = int64_t(m) * want;
+  return 3 * int64_t(m) + 8 + 5 * mw + (mw + 7) / 8 + 8;
+}
+
+void sym_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, double* values, double* Y,
+                      int64_t ldy, double* work, int* info) {
+  if (m <= 0 || want <= 0 || want > m || lda < m || ldy < m)
+    fail(BD_ERR_INVALID_PARAMETER, "sym_eigen_lowest: shape");
+  if (size_t(4 * m + kTW) * 8 > 160 * 1024 || size_t(3 * m) * 8 > 64 * 1024 || size_t(4 * m) * 8 > 64 * 1024)
+    fail(BD_ERR_INVALID_PARAMETER, "sym_eigen_lowest: subspace too large for the LDS kernels");
+  const EigenLayout L = eigen_layout(m, want, work);
+  hipLaunchKernelGGL(k_zero_i, dim3(1), dim3(64), 0, s, info, 1);
+  const size_t tri_lds = size_t(4 * m + kTW) * 8;
+  if (tri_lds > 64 * 1024 &&
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tridiag), hipFuncAttributeMaxDynamicSharedMemorySize,
+                          int(tri_lds)) != hipSuccess)
+    fail(BD_ERR_DEVICE, "k_tridiag LDS attribute");
+  hipLaunchKernelGGL(k_tridiag, dim3(1), dim3(kTT), tri_lds, s, m, A, lda, L.d, L.e, L.tau);
+  hipLaunchKernelGGL(k_bisect, dim3((want + 3) / 4), dim3(256), size_t(3 * m) * 8, s, m, want, L.d, L.e, values,
+                     L.tnorm);
+  hipLaunchKernelGGL(k_invit, dim3((want + 63) / 64), dim3(64), 0, s, m, want, L.d, L.e, values, L.tnorm, L.f, L.Z);
+  hipLaunchKernelGGL(k_cluster_gs, dim3(1), dim3(kTT), size_t(m + want + kTW) * 8, s, m, want, values, L.tnorm, L.f,
+                     L.Z, Y, ldy, info);
+  hipLaunchKernelGGL(k_back_transform, dim3((want + 3) / 4), dim3(256), size_t(4 * m) * 8, s, m, want, A, lda, L.tau,
+                     Y, ldy);
+}
+
+void zero_ints(hipStream_t s, int* p, int count) {
+  if (count > 0) hipLaunchKernelGGL(k_zero_i, dim3((count + 255) / 256), dim3(256), 0, s, p, count);
+}
+void zero_doubles(hipStream_t s, double* p, int64_t count) {
+  if (count > 0) hipLaunchKernelGGL(k_zero_d, dim3(unsigned((count + 255) / 256)), dim3(256), 0, s, p, count);
+}
+
+}  // namespace bd
+
+extern "C" int64_t bd_sym_eigen_workspace(int32_t m, int32_t want) {
+  if (m <= 0 || want <= 0 || want > m) return 0;
+  return bd::sym_eigen_work(m, want) + 1;  // + the status word
+}
+
+extern "C" bd_status bd_sym_eigen_dev(int32_t m, int32_t want, double* A, int64_t lda, double* values, double* Y,
+                                      int64_t ldy, double* work) {
+  BD_GUARD({
+    if (!A || !values || !Y || !work) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    int* info = reinterpret_cast<int*>(work + bd::sym_eigen_work(m, want));
+    bd::sym_eigen_lowest(nullptr, m, want, A, lda, values, Y, ldy, work, info);
+    int bad = 0;
+    if (hipMemcpy(&bad, info, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+      bd::fail(BD_ERR_DEVICE, "eigensolver launch failed");
+    if (bad) bd::fail(BD_ERR_APPLICATION, "eigenvector " + std::to_string(bad - 1) + " lost orthogonality");
+  });
+}

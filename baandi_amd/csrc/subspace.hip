@@ -1,0 +1,224 @@
+// Bordered inverse Cholesky of the Davidson overlap and the small matrix
+// helpers of the subspace step, on the GPU in FP64.
+//
+// extend_inverse_cholesky_dev follows crates/dft-solvers/src/lean_davidson.rs:
+// 167-331: with T11 = L11^{-1} known for the leading `from` columns of S,
+//   W = T11 S12,  G = S22 - W^T W = L22 L22^T,  T22 = L22^{-1},
+//   T21 = -T22 W^T T11,
+// rejecting when a squared pivot over its overlap diagonal falls below
+// eps/1e-12 (or is NaN), or when a new row of T fails the growth test
+// eps * (sum_j |T_ij| sqrt(S_jj))^2 <= sqrt(eps).
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+
+#include "common.hpp"
+#include "eigen.hpp"
+#include "gemm.hpp"
+
+namespace bd {
+
+namespace {
+constexpr int kCT = 1024;
+constexpr double kPivotRatioMin = DBL_EPSILON / 1e-12;  // lean_davidson.rs:62-64
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// G (k x k, ld k) <- S[from:to, from:to].
+__global__ void k_copy_block(int k, const double* __restrict__ S, int64_t lds, double* __restrict__ G) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i < k) G[i + int64_t(j) * k] = S[i + int64_t(j) * lds];
+}
+
+// In-place right-looking Cholesky of G with the pivot-ratio test, then
+// T22 = L^{-1} into T (ld ldt), zero above the diagonal.
+__global__ __launch_bounds__(kCT) void k_chol_inv(int k, double* __restrict__ G, const double* __restrict__ sdiag,
+                                                  int64_t sstride, double* __restrict__ T22, int64_t ldt,
+                                                  int* __restrict__ flag, double* __restrict__ min_ratio) {
+  extern __shared__ double b[];  // kCT/64 waves x k doubles
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int nw = kCT / 64;
+  double mr = INFINITY;
+  for (int j = 0; j < k; ++j) {
+    const double dj = G[j + int64_t(j) * k];
+    const double ratio = dj / sdiag[int64_t(j) * sstride];
+    if (isnan(ratio) || ratio < kPivotRatioMin) {
+      if (tid == 0) {
+        flag[0] = 1;
+        min_ratio[0] = isnan(ratio) ? 0.0 : fmin(mr, ratio);
+      }
+      return;  // every thread read the same pivot: uniform exit
+    }
+    mr = fmin(mr, ratio);
+    const double inv = 1.0 / sqrt(dj);
+    __syncthreads();  // all threads hold dj before column j changes
+    for (int i = j + tid; i < k; i += kCT) G[i + int64_t(j) * k] = i == j ? sqrt(dj) : G[i + int64_t(j) * k] * inv;
+    __syncthreads();
+    for (int l = j + 1 + wave; l < k; l += nw) {
+      const double llj = G[l + int64_t(j) * k];
+      for (int i = l + lane; i < k; i += 64) G[i + int64_t(l) * k] -= G[i + int64_t(j) * k] * llj;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) min_ratio[0] = mr;
+  // Column c of L^{-1}: forward substitution L x = e_c, one wave per column;
+  // lane owns rows i = lane (mod 64) of its right-hand side in LDS.
+  double* rhs = b + int64_t(wave) * k;
+  for (int c = wave; c < k; c += nw) {
+    for (int i = lane; i < k; i += 64) rhs[i] = i == c ? 1.0 : 0.0;
+    for (int l = c; l < k; ++l) {
+      double xl = 0.0;
+      if ((l & 63) == lane) xl = rhs[l] / G[l + int64_t(l) * k];
+      xl = __shfl(xl, l & 63, 64);
+      if ((l & 63) == lane) rhs[l] = xl;
+      for (int i = l + 1 + ((lane - (l + 1)) & 63); i < k; i += 64) rhs[i] -= G[i + int64_t(l) * k] * xl;
+    }
+    for (int i = lane; i < k; i += 64) T22[i + int64_t(c) * ldt] = i >= c ? rhs[i] : 0.0;
+  }
+}
+
+// One wave per new row i of T: finiteness and the growth guard.
+__global__ __launch_bounds__(256) void k_growth_guard(const double* __restrict__ S, const double* __restrict__ T,
+                                                      int cap, int from, int to, int* __restrict__ flag) {
+  const int lane = threadIdx.x & 63;
+  const int i = from + blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= to) return;
+  double g = 0.0;
+  int bad = 0;
+  for (int j = lane; j <= i; j += 64) {
+    const double v = T[i + int64_t(j) * cap];
+    bad |= !isfinite(v);
+    g += fabs(v) * sqrt(S[j + int64_t(j) * cap]);
+  }
+  g = wave_sum(g);
+  if (lane == 0 && (bad || !isfinite(g) || DBL_EPSILON * g * g > sqrt(DBL_EPSILON))) flag[0] = 1;
+  if (bad) flag[0] = 1;
+}
+
+__global__ void k_mirror(double* __restrict__ M, int cap, int from, int to) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = from + blockIdx.y;
+  if (j < to && i < j) M[j + int64_t(i) * cap] = M[i + int64_t(j) * cap];
+}
+
+__global__ void k_transpose(int m, int n, const double* __restrict__ A, int64_t lda, double* __restrict__ B,
+                            int64_t ldb) {
+  __shared__ double tile[32][33];
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8 threads
+  for (int r = ty; r < 32; r += 8) {
+    const int i = bx + tx, j = by + r;
+    if (i < m && j < n) tile[r][tx] = A[i + int64_t(j) * lda];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int j = by + tx, i = bx + r;
+    if (i < m && j < n) B[j + int64_t(i) * ldb] = tile[tx][r];
+  }
+}
+
+__global__ void k_symmetrize(int m, double* __restrict__ A, int64_t lda) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i < j && j < m) {
+    const double v = 0.5 * (A[i + int64_t(j) * lda] + A[j + int64_t(i) * lda]);
+    A[i + int64_t(j) * lda] = v;
+    A[j + int64_t(i) * lda] = v;
+  }
+}
+
+__global__ void k_identity(int m, int n, double* __restrict__ A, int64_t lda) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i < m && j < n) A[i + int64_t(j) * lda] = i == j ? 1.0 : 0.0;
+}
+
+__global__ void k_zero_flag(int* flag, double* min_ratio) {
+  flag[0] = 0;
+  min_ratio[0] = INFINITY;
+}
+}  // namespace
+
+// The split-K GEMMs above use at most max(1, K/256) chunks of an M x N tile
+// with M, N, K <= cap.
+int64_t factor_gwork(int cap) { return int64_t(cap < 256 ? 1 : cap / 256) * cap * cap; }
+
+void extend_inverse_cholesky_dev(hipStream_t s, const double* S, double* T, int cap, int from, int to,
+                                 const FactorWork& fw) {
+  const int k = to - from;
+  hipLaunchKernelGGL(k_zero_flag, dim3(1), dim3(1), 0, s, fw.flag, fw.min_ratio);
+  if (k <= 0) return;
+  if (from > 0) gemm_nn(s, from, k, from, 1.0, T, cap, S + int64_t(from) * cap, cap, 0.0, fw.W, from);
+  hipLaunchKernelGGL(k_copy_block, dim3((k + 255) / 256, k), dim3(256), 0, s, k,
+                     S + from + int64_t(from) * cap, int64_t(cap), fw.G);
+  if (from > 0) gemm_tn(s, k, k, from, -1.0, fw.W, from, fw.W, from, 1.0, fw.G, k, fw.gwork);
+  const size_t lds = size_t(kCT / 64) * k * 8;
+  if (lds > 160 * 1024) fail(BD_ERR_INVALID_PARAMETER, "inverse Cholesky block too large");
+  if (lds > 64 * 1024 &&
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&k_chol_inv), hipFuncAttributeMaxDynamicSharedMemorySize,
+                          int(lds)) != hipSuccess)
+    fail(BD_ERR_DEVICE, "k_chol_inv LDS attribute");
+  double* T22 = T + from + int64_t(from) * cap;
+  hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(kCT), lds, s, k, fw.G, S + from + int64_t(from) * cap,
+                     int64_t(cap) + 1, T22, int64_t(cap), fw.flag, fw.min_ratio);
+  if (from > 0) {
+    gemm_tn(s, k, from, from, 1.0, fw.W, from, T, cap, 0.0, fw.U, k, fw.gwork);         // U = W^T T11
+    gemm_nn(s, k, from, k, -1.0, T22, cap, fw.U, k, 0.0, T + from, cap);                  // T21 = -T22 U
+  }
+  hipLaunchKernelGGL(k_growth_guard, dim3((k + 3) / 4), dim3(256), 0, s, S, T, cap, from, to, fw.flag);
+}
+
+void mirror_border(hipStream_t s, double* M, int cap, int from, int to) {
+  if (to <= from) return;
+  hipLaunchKernelGGL(k_mirror, dim3((to + 255) / 256, to - from), dim3(256), 0, s, M, cap, from, to);
+}
+
+void transpose(hipStream_t s, int m, int n, const double* A, int64_t lda, double* B, int64_t ldb) {
+  if (m <= 0 || n <= 0) return;
+  hipLaunchKernelGGL(k_transpose, dim3((m + 31) / 32, (n + 31) / 32), dim3(256), 0, s, m, n, A, lda, B, ldb);
+}
+
+void symmetrize(hipStream_t s, int m, double* A, int64_t lda) {
+  if (m <= 1) return;
+  hipLaunchKernelGGL(k_symmetrize, dim3((m + 255) / 256, m), dim3(256), 0, s, m, A, lda);
+}
+
+void set_identity(hipStream_t s, int m, int n, double* A, int64_t lda) {
+  if (m <= 0 || n <= 0) return;
+  hipLaunchKernelGGL(k_identity, dim3((m + 255) / 256, n), dim3(256), 0, s, m, n, A, lda);
+}
+
+}  // namespace bd
+
+extern "C" bd_status bd_inverse_cholesky_dev(const double* S, double* T, int32_t cap, int32_t from, int32_t to,
+                                             int32_t* accepted, double* min_ratio) {
+  struct Scratch {
+    double* p = nullptr;
+    ~Scratch() {
+      if (p) (void)hipFree(p);
+    }
+  } buf;
+  BD_GUARD({
+    if (!S || !T || !accepted || !min_ratio || cap <= 0 || from < 0 || to < from || to > cap)
+      bd::fail(BD_ERR_INVALID_PARAMETER, "inverse Cholesky arguments");
+    const int64_t c2 = int64_t(cap) * cap;
+    const int64_t total = 3 * c2 + bd::factor_gwork(cap) + 2;
+    if (hipMalloc(&buf.p, size_t(total) * sizeof(double)) != hipSuccess) bd::fail(BD_ERR_DEVICE, "scratch");
+    bd::FactorWork fw;
+    fw.W = buf.p;
+    fw.G = fw.W + c2;
+    fw.U = fw.G + c2;
+    fw.gwork = fw.U + c2;
+    fw.min_ratio = fw.gwork + bd::factor_gwork(cap);
+    fw.flag = reinterpret_cast<int*>(fw.min_ratio + 1);
+    bd::extend_inverse_cholesky_dev(nullptr, S, T, cap, from, to, fw);
+    int flag = 0;
+    if (hipMemcpy(&flag, fw.flag, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(min_ratio, fw.min_ratio, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      bd::fail(BD_ERR_DEVICE, "inverse Cholesky launch failed");
+    *accepted = flag == 0 ? 1 : 0;
+  });
+}

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "libbaandi_amd.so")
 
 BD_OK, BD_ERR_LINALG, BD_ERR_INVALID_PARAMETER, BD_ERR_APPLICATION, BD_NOT_CONVERGED, BD_ERR_DEVICE = range(6)
 BD_FP64, BD_MIXED = 0, 1
-BD_PRECOND_QE, BD_PRECOND_TPA = 0, 1
+BD_PRECOND_QE, BD_PRECOND_TPA, BD_PRECOND_MLP = 0, 1, 2
 
 # Every function include/baandi_amd.h declares.
 EXPORTS = [
@@ -23,6 +23,9 @@ EXPORTS = [
     "bd_gamma_op_band_density_dev", "bd_gamma_op_nonlocal_energies_dev", "bd_gamma_op_stream",
     "bd_gamma_op_set_profiling", "bd_gamma_op_stage_times", "bd_gamma_op_plan_info", "bd_davidson_solve_dev",
     "bd_gemm_workspace", "bd_gemm_dev", "bd_mfma_probe_dev",
+    "bd_sym_eigen_workspace", "bd_sym_eigen_dev", "bd_inverse_cholesky_dev",
+    "bd_mlp_param_count", "bd_mlp_blob_floats", "bd_mlp_quantize_dev", "bd_mlp_forward_dev",
+    "bd_mlp_train_workspace", "bd_mlp_loss_grad_dev", "bd_mlp_adam_dev", "bd_mlp_train_dev",
 ]
 
 
@@ -55,6 +58,7 @@ class DavidsonOptions(C.Structure):
         ("bands", C.c_int32), ("threshold", C.c_double), ("occupied", C.c_int32),
         ("first_step", C.c_int32), ("capacity", C.c_int32), ("max_iterations", C.c_int32),
         ("return_hx", C.c_int32), ("preconditioner", C.c_int32),
+        ("mlp_blob", C.c_void_p), ("mlp_vbar", C.c_double),
     ]
 
 
@@ -63,6 +67,7 @@ class DavidsonRecord(C.Structure):
         ("iterations", C.c_int32), ("h_calls", C.c_int32), ("h_columns", C.c_int32),
         ("repairs", C.c_int32), ("restarts", C.c_int32), ("all_converged", C.c_int32),
         ("seconds_total", C.c_double), ("seconds_apply", C.c_double), ("seconds_dense", C.c_double),
+        ("mlp_fallback", C.c_int32),
     ]
 
 
@@ -83,7 +88,7 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: run `python -m baandi_amd.build` (no CPU fallback)")
     lib = C.CDLL(LIB_PATH)
-    P, I32, I64, D, U64 = C.c_void_p, C.c_int32, C.c_int64, C.c_double, C.c_uint64
+    P, I32, I64, D, U64, F = C.c_void_p, C.c_int32, C.c_int64, C.c_double, C.c_uint64, C.c_float
     dp = C.POINTER(C.c_double)
     lib.bd_last_error.restype = C.c_char_p
     sig = {
@@ -104,6 +109,13 @@ def load():
                                   C.POINTER(DavidsonRecord)],
         "bd_gemm_dev": [I32, I32, I64, I64, I64, D, P, I64, P, I64, D, P, I64, P],
         "bd_mfma_probe_dev": [P, P, P],
+        "bd_sym_eigen_dev": [I32, I32, P, I64, P, P, I64, P],
+        "bd_inverse_cholesky_dev": [P, P, I32, I32, I32, P, P],
+        "bd_mlp_quantize_dev": [P, P],
+        "bd_mlp_forward_dev": [I64, P, F, F, F, P, P],
+        "bd_mlp_loss_grad_dev": [I64, I32, P, P, P, P, P, I32, P, P, P, P],
+        "bd_mlp_adam_dev": [P, P, P, P, I32, F],
+        "bd_mlp_train_dev": [I64, I32, P, P, P, P, P, P, P, I32, I32, F, U64, P, P, P, P, P],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -117,6 +129,14 @@ def load():
     lib.bd_gamma_op_stream.restype = P
     lib.bd_gemm_workspace.argtypes = [I32, I32, I64, I64, I64]
     lib.bd_gemm_workspace.restype = I64
+    lib.bd_sym_eigen_workspace.argtypes = [I32, I32]
+    lib.bd_sym_eigen_workspace.restype = I64
+    lib.bd_mlp_param_count.argtypes = []
+    lib.bd_mlp_param_count.restype = I32
+    lib.bd_mlp_blob_floats.argtypes = []
+    lib.bd_mlp_blob_floats.restype = I32
+    lib.bd_mlp_train_workspace.argtypes = [I64, I32]
+    lib.bd_mlp_train_workspace.restype = I64
     _lib = lib
     return lib
 

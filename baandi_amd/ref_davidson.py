@@ -24,7 +24,9 @@ class Options:
     capacity: int = 0         # 0: min(2*bands, n)
     max_iterations: int = 20
     return_hx: bool = False
-    preconditioner: int = L.BD_PRECOND_QE  # BD_PRECOND_QE or BD_PRECOND_TPA
+    preconditioner: int = L.BD_PRECOND_QE  # BD_PRECOND_QE, BD_PRECOND_TPA or BD_PRECOND_MLP
+    mlp: object = None        # BD_PRECOND_MLP: a baandi_amd.learned.Network
+    mlp_vbar: float = 0.0     # BD_PRECOND_MLP: mean local potential (Ha)
 
     def band_threshold(self, band):
         """ref_davidson.rs:70-77."""
@@ -68,7 +70,11 @@ def solve_with_initial_hx(operator, start, hamiltonian_diagonal, options: Option
     o = L.DavidsonOptions(bands=nb, threshold=float(options.threshold), occupied=int(options.occupied),
                           first_step=int(bool(options.first_step)), capacity=int(options.capacity),
                           max_iterations=int(options.max_iterations), return_hx=int(bool(options.return_hx)),
-                          preconditioner=int(options.preconditioner))
+                          preconditioner=int(options.preconditioner),
+                          mlp_blob=(L.ptr(options.mlp.blob) if options.mlp is not None else None),
+                          mlp_vbar=float(options.mlp_vbar))
+    if options.preconditioner == L.BD_PRECOND_MLP and options.mlp is None:
+        raise ValueError("BD_PRECOND_MLP needs a trained network (Options.mlp)")
     eig = np.zeros(nb)
     conv = np.zeros(nb, dtype=np.int32)
     rec = L.DavidsonRecord()

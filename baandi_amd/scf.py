@@ -69,6 +69,7 @@ class ScfOptions:
     # in FP64, so the fixed point is the FP64 one.
     precision: int = L.BD_FP64
     mixed_floor_ha: float = 5e-5
+    mlp: object = None  # a baandi_amd.learned.Network when preconditioner == BD_PRECOND_MLP
     # Optional fixed-point verification map (driver.rs:727-757): one more
     # Davidson at the converged potential with every band at this threshold (Ha).
     final_threshold: float = 0.0
@@ -195,7 +196,7 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
     last_dr2 = math.inf
     history = []
     totals = dict(h_columns=0, davidson_iterations=0, dense_s=0.0, apply_s=0.0, davidson_s=0.0, fields_s=0.0,
-                  mixed_maps=0)
+                  mixed_maps=0, mlp_fallbacks=0)
     result = None
     for iteration in range(1, options.max_iterations + 1):
         if iteration > 1:
@@ -218,10 +219,12 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
             opts = ref_davidson.Options(bands=bands, threshold=ethr / 2.0, occupied=occupied,
                                         first_step=(iteration == 1 and not redo),
                                         max_iterations=options.davidson_max_iterations, return_hx=True,
-                                        preconditioner=options.preconditioner)
+                                        preconditioner=options.preconditioner, mlp=options.mlp,
+                                        mlp_vbar=float(potential.mean()))
             sol = ref_davidson.solve_with_initial_hx(cur, orbitals, diag, opts, out=(xout, hxout))
             totals["davidson_s"] += time.perf_counter() - td
             totals["mixed_maps"] += int(use_mixed)
+            totals["mlp_fallbacks"] += int(sol.record.get("mlp_fallback", -1) >= 0)
             totals["h_columns"] += sol.record["h_columns"]
             totals["davidson_iterations"] += sol.record["iterations"]
             totals["dense_s"] += sol.record["seconds_dense"]
@@ -261,7 +264,8 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
         diag = diag_static + float(potential.mean())
         opts = ref_davidson.Options(bands=bands, threshold=options.final_threshold, occupied=occupied,
                                     first_step=True, max_iterations=200, return_hx=True,
-                                    preconditioner=options.preconditioner)
+                                    preconditioner=options.preconditioner, mlp=options.mlp,
+                                    mlp_vbar=float(potential.mean()))
         sol = ref_davidson.solve_with_initial_hx(op, orbitals, diag, opts, out=(xout, hxout))
         orbitals, xout = xout, orbitals
         op.band_density_sum(orbitals, occ, rho_dev)
@@ -282,7 +286,7 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
               "ewald": ewald, "total": total, "band_sum": float(np.dot(sol.eigenvalues, occ))}
     out = dict(energy=energy, eigenvalues=sol.eigenvalues.copy(), history=history, wall_s=wall,
                setup_s=t_setup, iterations=len(history), totals=totals, orbitals=orbitals, op=op,
-               plan=op.plan_info())
+               plan=op.plan_info(), potential=potential)
     if profile_kernels:
         out["kernel_totals"] = op.stage_totals()
         if op_mixed is not None:

@@ -146,7 +146,9 @@ bd_status bd_gamma_op_stage_times(bd_gamma_op* op, double* out8);
  * (lean_davidson.rs:167-331), Ritz-change convergence, contraction at
  * capacity, rank repair. Options mirror ref_davidson::Options (:38-78).
  * ---------------------------------------------------------------------- */
-typedef enum { BD_PRECOND_QE = 0, BD_PRECOND_TPA = 1 } bd_preconditioner;
+/* BD_PRECOND_MLP: the learned diagonal (no reference counterpart; see the
+ * bd_mlp_* entry points) with the TPA weight as the fallback. */
+typedef enum { BD_PRECOND_QE = 0, BD_PRECOND_TPA = 1, BD_PRECOND_MLP = 2 } bd_preconditioner;
 
 typedef struct {
   int32_t bands;
@@ -157,6 +159,8 @@ typedef struct {
   int32_t max_iterations;/* default 20 */
   int32_t return_hx;
   int32_t preconditioner;/* bd_preconditioner */
+  const float* mlp_blob; /* BD_PRECOND_MLP: device blob from bd_mlp_quantize_dev */
+  double mlp_vbar;       /* BD_PRECOND_MLP: mean local potential (Ha), a feature */
 } bd_davidson_options;
 
 typedef struct {
@@ -168,7 +172,8 @@ typedef struct {
   int32_t all_converged;
   double seconds_total;
   double seconds_apply;
-  double seconds_dense;  /* host factor + reduced eigensolve */
+  double seconds_dense;  /* subspace step: factor + reduced eigensolve (GPU, synchronised) */
+  int32_t mlp_fallback;  /* BD_PRECOND_MLP: iteration that fell back to TPA, -1 if none */
 } bd_davidson_record;
 
 /* X0 (n x bands, device), optional HX0 (device or NULL), diagonal (n, device).
@@ -197,6 +202,58 @@ bd_status bd_gemm_dev(int32_t mode, int32_t trans_a, int64_t M, int64_t N, int64
 /* One v_mfma_f32_16x16x16_bf16 of row-major 16x16 bf16 bit patterns (a self
  * check of the MFMA operand layout the kernels assume). */
 bd_status bd_mfma_probe_dev(const uint16_t* a_bits, const uint16_t* b_bits, float* d);
+
+/* ------------------------------------------------------------------------
+ * The FP64 subspace step of bd_davidson_solve_dev as stand-alone entry points
+ * (device pointers; synchronous on the null stream).
+ * bd_sym_eigen_dev: lowest `want` eigenpairs of the symmetric m x m matrix A
+ *   (column-major, lda; overwritten): values ascending, Y (m x want, ldy)
+ *   orthonormal. The role of RealEighWorkspace::solve_into
+ *   (crates/dft-numerics/src/real.rs:2286-2330) as called from
+ *   ref_davidson/workspace.rs:435-499, which reads only the lowest `bands`
+ *   pairs. `work` holds bd_sym_eigen_workspace(m, want) doubles.
+ * bd_inverse_cholesky_dev: extends T = L^{-1} of the overlap S (both cap x cap,
+ *   ld cap; T lower triangular with a zero upper triangle) from `from` to `to`
+ *   columns (crates/dft-solvers/src/lean_davidson.rs:167-331). *accepted is 0
+ *   when the pivot-ratio (eps/1e-12) or row-growth test rejects the border;
+ *   *min_ratio is the smallest squared pivot over its overlap diagonal.
+ * ---------------------------------------------------------------------- */
+int64_t bd_sym_eigen_workspace(int32_t m, int32_t want);
+bd_status bd_sym_eigen_dev(int32_t m, int32_t want, double* A, int64_t lda, double* values, double* Y,
+                           int64_t ldy, double* work);
+bd_status bd_inverse_cholesky_dev(const double* S, double* T, int32_t cap, int32_t from, int32_t to,
+                                  int32_t* accepted, double* min_ratio);
+
+/* ------------------------------------------------------------------------
+ * Learned Davidson preconditioner (no reference counterpart: it takes the
+ * place of the Teter-Payne-Allan weight of crates/dft-solvers/src/lib.rs:
+ * 176-280 in e = -K r). K = exp(MLP(u)), a 4 -> 64 -> 64 -> 1 tanh network of
+ * u = [log(1 + g/T), log T, (theta - vbar)/T, vbar] (g = |G|^2/2 of the row,
+ * T the column's kinetic scale, theta its Ritz value, vbar the mean local
+ * potential). Parameters are bd_mlp_param_count() FP32 values (W1 64x4, b1,
+ * W2 64x64, b2, w3, b3; row-major); the device blob (bd_mlp_blob_floats()
+ * floats) holds bf16 weights for the MFMA kernels.
+ * Training data: `bands` residuals R and ideal corrections Tg (float, bands x
+ * n, row-major per band), per-band (T, theta, vbar) in scal (bands x 3), the
+ * row energies g (n). Loss per band: 1 - <K r, t>^2 / (|K r|^2 |t|^2).
+ * ---------------------------------------------------------------------- */
+int32_t bd_mlp_param_count(void);
+int32_t bd_mlp_blob_floats(void);
+bd_status bd_mlp_quantize_dev(const float* theta, float* blob);
+bd_status bd_mlp_forward_dev(int64_t n, const float* g, float T, float ritz, float vbar, const float* blob,
+                             float* K);
+int64_t bd_mlp_train_workspace(int64_t n, int32_t batch);
+bd_status bd_mlp_loss_grad_dev(int64_t n, int32_t bands, const float* g, const float* R, const float* Tg,
+                               const float* scal, const int32_t* batch_dev, int32_t batch, const float* blob,
+                               float* grad, double* loss, double* work);
+bd_status bd_mlp_adam_dev(float* theta, const float* grad, float* m, float* v, int32_t step, float lr);
+/* `steps` Adam steps on batches of `batch` bands drawn by a fixed LCG from
+ * `seed`; losses (host, steps) receives each step's batch loss. batch_dev
+ * holds `batch` ints; theta, m, v are the FP32 masters and moments. */
+bd_status bd_mlp_train_dev(int64_t n, int32_t bands, const float* g, const float* R, const float* Tg,
+                           const float* scal, float* theta, float* m, float* v, int32_t steps, int32_t batch,
+                           float lr, uint64_t seed, double* losses, int32_t* batch_dev, float* blob, float* grad,
+                           double* work);
 
 #ifdef __cplusplus
 }

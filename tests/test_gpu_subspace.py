@@ -1,0 +1,116 @@
+"""The FP64 subspace step on MI355X, through the C ABI: the GPU eigensolver
+(tridiagonalisation, bisection, inverse iteration, cluster Gram-Schmidt,
+back-transformation) against LAPACK (numpy.linalg.eigh), and the bordered
+inverse Cholesky against the oracle's factor."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _eigen_dev(A, want):
+    import torch
+    from baandi_amd import lib as L
+    lib = L.load()
+    m = A.shape[0]
+    Ad = torch.from_numpy(np.asfortranarray(A).ravel(order="F").copy()).cuda()
+    vals = torch.zeros(want, dtype=torch.float64, device="cuda")
+    Y = torch.zeros(m * want, dtype=torch.float64, device="cuda")
+    work = torch.zeros(lib.bd_sym_eigen_workspace(m, want), dtype=torch.float64, device="cuda")
+    L.check(lib.bd_sym_eigen_dev(m, want, L.ptr(Ad), m, L.ptr(vals), L.ptr(Y), m, L.ptr(work)))
+    return vals.cpu().numpy(), Y.cpu().numpy().reshape(want, m).T
+
+
+def _check_pairs(A, want, label):
+    w, Y = _eigen_dev(A, want)
+    ref = np.linalg.eigvalsh(A)[:want]
+    scale = max(np.abs(A).sum(axis=0).max(), 1e-300)
+    verr = np.abs(w - ref).max() / scale
+    res = np.abs(A @ Y - Y * w[None, :]).max() / scale
+    orth = np.abs(Y.T @ Y - np.eye(want)).max()
+    print(f"[eigen] {label} m={A.shape[0]} want={want}: value {verr:.2e}, residual {res:.2e}, orthogonality {orth:.2e}")
+    # Backward-stable reduction and bisection: O(m eps ||A||) values.
+    assert verr < 1e-13 * max(A.shape[0], 10)
+    assert res < 1e-12 * max(A.shape[0], 10)
+    assert orth < 1e-12 * max(A.shape[0], 10)
+
+
+@pytest.mark.parametrize("m,want", [(1, 1), (2, 2), (3, 3), (5, 2), (64, 64), (160, 160), (320, 160), (321, 100)])
+def test_eigen_random_symmetric(m, want):
+    rng = np.random.default_rng(m * 7 + want)
+    B = rng.standard_normal((m, m))
+    _check_pairs(0.5 * (B + B.T), want, "random")
+
+
+@pytest.mark.parametrize("m", [12, 96, 240])
+def test_eigen_degenerate_and_clustered(m):
+    """Exact multiplicities (the triplets of a cubic crystal), a 1e-13 cluster
+    and a diagonal block structure (zero off-diagonals after reduction)."""
+    rng = np.random.default_rng(m)
+    vals = np.sort(rng.uniform(-1.0, 3.0, m))
+    vals[3:6] = vals[3]            # triple
+    vals[8:10] = vals[8]           # pair
+    vals[-4:-1] = vals[-4] + np.array([0.0, 1e-13, 2e-13])
+    vals = np.sort(vals)
+    Q, _ = np.linalg.qr(rng.standard_normal((m, m)))
+    A = (Q * vals[None, :]) @ Q.T
+    _check_pairs(0.5 * (A + A.T), m // 2 + 1, "degenerate")
+    D = np.diag(vals.copy())
+    _check_pairs(D, m // 2, "diagonal")
+
+
+def test_eigen_davidson_like_spectrum():
+    """T H T^T of a Davidson subspace: a low occupied block, a wide tail."""
+    rng = np.random.default_rng(5)
+    m = 320
+    vals = np.concatenate([np.sort(rng.uniform(-0.3, 0.2, 160)), np.sort(rng.uniform(0.2, 40.0, m - 160))])
+    Q, _ = np.linalg.qr(rng.standard_normal((m, m)))
+    A = (Q * vals[None, :]) @ Q.T
+    _check_pairs(0.5 * (A + A.T), 160, "davidson-like")
+
+
+def _factor_dev(S, T, cap, frm, to):
+    import ctypes as C
+    import torch
+    from baandi_amd import lib as L
+    lib = L.load()
+    Sd = torch.from_numpy(np.asfortranarray(S).ravel(order="F").copy()).cuda()
+    Td = torch.from_numpy(np.asfortranarray(T).ravel(order="F").copy()).cuda()
+    acc = C.c_int32(0)
+    ratio = C.c_double(0.0)
+    L.check(lib.bd_inverse_cholesky_dev(L.ptr(Sd), L.ptr(Td), cap, frm, to, C.byref(acc), C.byref(ratio)))
+    return bool(acc.value), ratio.value, Td.cpu().numpy().reshape(cap, cap).T
+
+
+def test_inverse_cholesky_border_against_oracle():
+    from oracle.davidson import Davidson
+    rng = np.random.default_rng(2)
+    cap, n = 200, 900
+    V = rng.standard_normal((n, cap)) * rng.uniform(0.5, 2.0, cap)[None, :]
+    S = V.T @ V
+    T = np.zeros((cap, cap))
+    for frm, to in [(0, 90), (90, 150), (150, 200)]:
+        ok, ratio, T = _factor_dev(S, T, cap, frm, to)
+        ok_ref, Tref = Davidson._factor(None, S, frm, to, None)
+        assert ok and ok_ref
+        err = np.abs(T[:to, :to] - Tref).max() / np.abs(Tref).max()
+        ident = np.abs(T[:to, :to] @ S[:to, :to] @ T[:to, :to].T - np.eye(to)).max()
+        print(f"[factor] {frm}->{to}: T vs oracle {err:.2e}, |T S T^T - I| {ident:.2e}, min ratio {ratio:.3e}")
+        assert err < 1e-11 and ident < 1e-11
+        assert np.all(T[:to, :to][np.triu_indices(to, 1)] == 0.0)
+
+
+def test_inverse_cholesky_rejects_dependent_border():
+    from oracle.davidson import Davidson
+    rng = np.random.default_rng(3)
+    cap, n = 64, 300
+    V = rng.standard_normal((n, cap))
+    V[:, 40] = V[:, 3] + 1e-9 * V[:, 41]   # numerically dependent on column 3
+    S = V.T @ V
+    T = np.zeros((cap, cap))
+    ok, _, T = _factor_dev(S, T, cap, 0, 32)
+    assert ok
+    ok, ratio, _ = _factor_dev(S, T, cap, 32, 64)
+    ok_ref, _ = Davidson._factor(None, S, 32, 64, None)
+    print(f"[factor] dependent border: accepted={ok}, min ratio {ratio:.3e}, oracle accepted={ok_ref}")
+    assert not ok and not ok_ref
