@@ -25,6 +25,7 @@
 
 #include "common.hpp"
 #include "eigen.hpp"
+#include "gemm.hpp"
 
 namespace bd {
 
@@ -79,12 +80,16 @@ __device__ __forceinline__ double hash_unit(uint32_t i, uint32_t j, uint32_t sal
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kTT) void k_tridiag(int m, double* __restrict__ A, int64_t lda, double* __restrict__ d,
                                                  double* __restrict__ e, double* __restrict__ tau) {
+  // LDS: two reflector vectors, w, q = B x, the next step's first column,
+  // and two reduction slots per wave.
   extern __shared__ double sh[];
   double* xa = sh;
   double* xb = sh + m;
   double* w = sh + 2 * m;
   double* q = sh + 3 * m;
-  double* red = sh + 4 * m;
+  double* nc = sh + 4 * m;
+  double* red = sh + 5 * m;
+  double* red2 = red + kTW;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   auto a = [&](int i, int j) -> double& { return A[i + int64_t(j) * lda]; };
   if (m < 3) {
@@ -99,8 +104,8 @@ __global__ __launch_bounds__(kTT) void k_tridiag(int m, double* __restrict__ A, 
     }
     return;
   }
-  // Reflector of column k from x (its rows k+1.., already in LDS) and
-  // sum(x^2); returns tau_k (0 when the column is already zero).
+  // Reflector of column k from x (its rows k+1.., in LDS) and sum(x^2);
+  // returns tau_k (0 when the column is already zero).
   auto reflect = [&](double ss, double* xv, int k) -> double {
     const double sigma = sqrt(ss);
     const double x0 = xv[0];
@@ -134,43 +139,62 @@ __global__ __launch_bounds__(kTT) void k_tridiag(int m, double* __restrict__ A, 
   if (tid == 0) d[0] = a(0, 0);
   ss = block_sum(ss, red);
   double t = reflect(ss, x, 0);
-  // q = B_0 x, B_0 = A[1:, 1:] (one wave per column, lanes along rows).
-  for (int j = wave; j < L; j += kTW) {
-    const double* col = &a(1, 1 + j);
-    double acc = 0.0;
-    for (int r = lane; r < L; r += 64) acc += col[r] * x[r];
-    acc = wave_sum(acc);
-    if (lane == 0) q[j] = acc;
+  // q = B_0 x (B_0 = A[1:, 1:], four columns per wave, lanes along rows);
+  // the first column of B_0 is kept in nc; q.x is summed on the way.
+  {
+    double qx = 0.0;
+    for (int j0 = 4 * wave; j0 < L; j0 += 4 * kTW) {
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+      const double* col[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) col[c] = &a(1, 1 + (j0 + c < L ? j0 + c : j0));
+      for (int r = lane; r < L; r += 64) {
+        const double xr = x[r];
+        double b[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) b[c] = col[c][r];
+        if (j0 == 0) nc[r] = b[0];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[c] += b[c] * xr;
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const double v = wave_sum(acc[c]);
+        if (j0 + c < L) {
+          if (lane == 0) q[j0 + c] = v;
+          qx += v * x[j0 + c];
+        }
+      }
+    }
+    if (lane == 0) red2[wave] = qx;
   }
   __syncthreads();
 
   for (int k = 0; k + 2 < m; ++k) {
     L = m - k - 1;
     const int c0 = k + 1;  // global index of the trailing block's first row/column
-    // p = t q, K = t/2 p.x, w = p - K x.
-    double part = 0.0;
-    for (int r = tid; r < L; r += kTT) part += q[r] * x[r];
-    const double K = 0.5 * t * t * block_sum(part, red);
+    double qx = 0.0;
+#pragma unroll
+    for (int i = 0; i < kTW; ++i) qx += red2[i];
+    // p = t q, K = t/2 p.x = t^2/2 q.x, w = p - K x.
+    const double K = 0.5 * t * t * qx;
     for (int r = tid; r < L; r += kTT) w[r] = t * q[r] - K * x[r];
     __syncthreads();
     if (L == 2) {
       if (tid == 0) {
-        const double b00 = a(c0, c0) - 2.0 * x[0] * w[0];
-        const double b10 = a(c0 + 1, c0) - (x[1] * w[0] + w[1] * x[0]);
-        const double b11 = a(c0 + 1, c0 + 1) - 2.0 * x[1] * w[1];
-        d[m - 2] = b00;
-        e[m - 2] = b10;
-        d[m - 1] = b11;
+        d[m - 2] = nc[0] - 2.0 * x[0] * w[0];
+        e[m - 2] = nc[1] - (x[1] * w[0] + w[1] * x[0]);
+        d[m - 1] = a(c0 + 1, c0 + 1) - 2.0 * x[1] * w[1];
         tau[m - 2] = 0.0;
         tau[m - 1] = 0.0;
       }
       break;
     }
-    // First column of B_k: rows 1.. become the next reflector's vector.
+    // First column of B_k (held in nc): rows 1.. become the next reflector.
     const double x0 = x[0], w0 = w[0];
     ss = 0.0;
     for (int r = tid; r < L; r += kTT) {
-      const double b = a(c0 + r, c0) - (x[r] * w0 + w[r] * x0);
+      const double b = nc[r] - (x[r] * w0 + w[r] * x0);
       if (r == 0) {
         d[c0] = b;
       } else {
@@ -181,19 +205,45 @@ __global__ __launch_bounds__(kTT) void k_tridiag(int m, double* __restrict__ A, 
     }
     ss = block_sum(ss, red);
     const double tn = reflect(ss, xn, c0);
-    // Rest of B_k: B -= x w^T + w x^T, and q' = B_{k+1} x' in the same pass.
-    for (int j = 1 + wave; j < L; j += kTW) {
-      const double xj = x[j], wj = w[j];
-      double* col = &a(c0, c0 + j);
-      double acc = 0.0;
-      for (int r = 1 + lane; r < L; r += 64) {
-        const double b = col[r] - (x[r] * wj + w[r] * xj);
-        col[r] = b;
-        acc += b * xn[r - 1];
+    // Rest of B_k: B -= x w^T + w x^T and q' = B_{k+1} x' in one pass, four
+    // columns per wave (four loads in flight per lane). Local column 1 is the
+    // next step's first column: it goes to nc as well.
+    qx = 0.0;
+    for (int j0 = 1 + 4 * wave; j0 < L; j0 += 4 * kTW) {
+      double acc[4] = {0.0, 0.0, 0.0, 0.0}, xj[4], wj[4];
+      double* col[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const bool ok = j0 + c < L;
+        xj[c] = ok ? x[j0 + c] : 0.0;
+        wj[c] = ok ? w[j0 + c] : 0.0;
+        col[c] = &a(c0, c0 + (ok ? j0 + c : j0));
       }
-      acc = wave_sum(acc);
-      if (lane == 0) q[j - 1] = acc;
+      for (int r = 1 + lane; r < L; r += 64) {
+        const double xr = x[r], wr = w[r], xnr = xn[r - 1];
+        double b[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) b[c] = col[c][r];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          if (j0 + c < L) {
+            b[c] -= xr * wj[c] + wr * xj[c];
+            col[c][r] = b[c];
+            acc[c] += b[c] * xnr;
+          }
+        }
+        if (j0 == 1) nc[r - 1] = b[0];
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const double v = wave_sum(acc[c]);
+        if (j0 + c < L) {
+          if (lane == 0) q[j0 + c - 1] = v;
+          qx += v * xn[j0 + c - 1];
+        }
+      }
     }
+    if (lane == 0) red2[wave] = qx;
     __syncthreads();
     double* tmp = x;
     x = xn;
@@ -353,7 +403,7 @@ __global__ __launch_bounds__(64) void k_invit(int m, int want, const double* __r
   f.U0i[at(m - 1)] = 1.0 / a;
   double* z = Z + j;
   for (int i = 0; i < m; ++i) z[int64_t(i) * want] = hash_unit(i, j, 0);
-  for (int it = 0; it < 3; ++it) {
+  for (int it = 0; it < 2; ++it) {
     const double big = lu_solve(m, want, j, f, z, want);
     const double s = big > 0.0 ? 1.0 / big : 1.0;
     for (int i = 0; i < m; ++i) z[int64_t(i) * want] *= s;
@@ -365,109 +415,126 @@ __global__ __launch_bounds__(64) void k_invit(int m, int want, const double* __r
 }
 
 // ---------------------------------------------------------------------------
-// 4. Cluster Gram-Schmidt (one workgroup, vectors in ascending order). Reads
-// the inverse-iteration vectors from Z ([i * want + j]) and writes the
-// orthonormal ones to Y (column-major, ldy).
+// 4. Orthogonality. Inverse-iteration vectors of eigenvalues a gap g apart
+// overlap by about eps ||T|| / g, so vectors whose eigenvalues lie within
+// kClusterRel ||T|| of a neighbour form a cluster and are made orthogonal by
+// two passes of Gram-Schmidt in ascending order, one wave per cluster (a
+// vector that loses most of its norm is re-iterated from the remainder plus
+// a fresh direction). The remaining overlaps, <= ~1e-9, are removed by one
+// symmetric (Loewdin) step Y <- Y (3 I - Y^T Y) / 2 in the host driver.
+// Reads the inverse-iteration vectors from Z ([i * want + j]); writes Y.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kTT) void k_cluster_gs(int m, int want, const double* __restrict__ values,
+constexpr double kClusterRel = 1e-6;
+
+__global__ __launch_bounds__(256) void k_cluster_gs(int m, int want, const double* __restrict__ values,
                                                     const double* __restrict__ tnorm_in, LU f,
                                                     const double* __restrict__ Z, double* __restrict__ Y,
                                                     int64_t ldy, int* __restrict__ info) {
-  extern __shared__ double sh[];
-  double* v = sh;
-  double* c = sh + m;
-  double* red = sh + m + want;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const double ortol = 1e-3 * tnorm_in[0];
-  int cs = 0;
-  for (int j = 0; j < want; ++j) {
-    if (j > 0 && values[j] - values[j - 1] > ortol) cs = j;
-    for (int i = tid; i < m; i += kTT) v[i] = Z[int64_t(i) * want + j];
-    __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int h = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (h >= want) return;  // whole waves; no barrier follows
+  const double tol = kClusterRel * tnorm_in[0];
+  if (h > 0 && values[h] - values[h - 1] <= tol) return;  // not the head of its cluster
+  int end = h + 1;
+  while (end < want && values[end] - values[end - 1] <= tol) ++end;
+  for (int j = h; j < end; ++j) {
+    double* y = Y + int64_t(j) * ldy;
+    for (int i = lane; i < m; i += 64) y[i] = Z[int64_t(i) * want + j];
     double scale = 0.0;
     for (int attempt = 0; attempt < 4; ++attempt) {
-      for (int pass = 0; pass < 2 && j > cs; ++pass) {
-        for (int l = cs + wave; l < j; l += kTW) {
+      for (int pass = 0; pass < 2 && j > h; ++pass)
+        for (int l = h; l < j; ++l) {
           const double* yl = Y + int64_t(l) * ldy;
           double acc = 0.0;
-          for (int i = lane; i < m; i += 64) acc += yl[i] * v[i];
-          acc = wave_sum(acc);
-          if (lane == 0) c[l] = acc;
+          for (int i = lane; i < m; i += 64) acc += yl[i] * y[i];
+          const double c = wave_sum(acc);
+          for (int i = lane; i < m; i += 64) y[i] -= c * yl[i];
         }
-        __syncthreads();
-        for (int i = tid; i < m; i += kTT) {
-          double s0 = 0.0, s1 = 0.0;
-          int l = cs;
-          for (; l + 1 < j; l += 2) {
-            s0 += Y[i + int64_t(l) * ldy] * c[l];
-            s1 += Y[i + int64_t(l + 1) * ldy] * c[l + 1];
-          }
-          if (l < j) s0 += Y[i + int64_t(l) * ldy] * c[l];
-          v[i] -= s0 + s1;
-        }
-        __syncthreads();
-      }
       double nn = 0.0;
-      for (int i = tid; i < m; i += kTT) nn += v[i] * v[i];
-      nn = block_sum(nn, red);
-      if (nn > 1e-2) {
-        scale = 1.0 / sqrt(nn);
-        break;
-      }
-      if (attempt == 3) {
-        if (tid == 0) info[0] = 1 + j;
+      for (int i = lane; i < m; i += 64) nn += y[i] * y[i];
+      nn = wave_sum(nn);
+      if (nn > 1e-2 || attempt == 3) {
+        if (!(nn > 1e-2) && lane == 0) info[0] = 1 + j;
         scale = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
         break;
       }
       // Most of the vector lay in the span of its cluster: restart from the
-      // remainder plus a fresh direction, and apply (T - lambda_j I)^{-1}.
+      // remainder plus a fresh direction and apply (T - lambda_j I)^{-1}.
       const double rs = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
-      for (int i = tid; i < m; i += kTT) v[i] = v[i] * rs + 0.1 * hash_unit(i, j, 1 + attempt);
-      __syncthreads();
-      if (tid == 0) {
+      for (int i = lane; i < m; i += 64) y[i] = y[i] * rs + 0.1 * hash_unit(i, j, 1 + attempt);
+      __threadfence_block();
+      if (lane == 0) {
         for (int it = 0; it < 2; ++it) {
-          const double big = lu_solve(m, want, j, f, v, 1);
-          const double s = big > 0.0 ? 1.0 / big : 1.0;
-          for (int i = 0; i < m; ++i) v[i] *= s;
+          const double big = lu_solve(m, want, j, f, y, 1);
+          const double sc = big > 0.0 ? 1.0 / big : 1.0;
+          for (int i = 0; i < m; ++i) y[i] *= sc;
         }
       }
-      __syncthreads();
-      nn = 0.0;
-      for (int i = tid; i < m; i += kTT) nn += v[i] * v[i];
-      nn = block_sum(nn, red);
-      const double s = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
-      for (int i = tid; i < m; i += kTT) v[i] *= s;
-      __syncthreads();
+      __threadfence_block();
     }
-    for (int i = tid; i < m; i += kTT) Y[i + int64_t(j) * ldy] = v[i] * scale;
-    __syncthreads();
+    for (int i = lane; i < m; i += 64) y[i] *= scale;
+    __threadfence_block();
   }
 }
 
+// M = (3 I - G) / 2 in place (n x n, ld n).
+__global__ void k_loewdin_factor(int n, double* __restrict__ G) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i < n) G[i + int64_t(j) * n] = (i == j ? 1.5 : 0.0) - 0.5 * G[i + int64_t(j) * n];
+}
+
 // ---------------------------------------------------------------------------
-// 5. Y <- H_0 H_1 ... H_{m-3} Y, one wave per column; each lane owns the rows
-// lane, lane + 64, ... of its column in LDS.
+// 5. Y <- H_0 H_1 ... H_{m-3} Y, one wave per column (its column in LDS,
+// lane-owned rows); the block stages kBtChunk reflectors at a time in LDS.
 // ---------------------------------------------------------------------------
+constexpr int kBtChunk = 8;
+
 __global__ __launch_bounds__(256) void k_back_transform(int m, int want, const double* __restrict__ A, int64_t lda,
                                                         const double* __restrict__ tau, double* __restrict__ Y,
                                                         int64_t ldy) {
   extern __shared__ double sh[];
   const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
   const int col = blockIdx.x * (blockDim.x >> 6) + wl;
-  if (col >= want) return;
+  const bool active = col < want;
   double* y = sh + int64_t(wl) * m;
-  double* yg = Y + int64_t(col) * ldy;
-  for (int i = lane; i < m; i += 64) y[i] = yg[i];
-  for (int k = m - 3; k >= 0; --k) {
-    const double t = tau[k];
-    if (t == 0.0) continue;
-    const double* v = A + int64_t(k) * lda;  // v[i] for rows i >= k+1
-    double acc = 0.0;
-    for (int i = k + 1 + ((lane - (k + 1)) & 63); i < m; i += 64) acc += v[i] * y[i];
-    const double s = t * wave_sum(acc);
-    for (int i = k + 1 + ((lane - (k + 1)) & 63); i < m; i += 64) y[i] -= s * v[i];
+  double* vs = sh + 4 * int64_t(m);
+  double* ts = vs + int64_t(kBtChunk) * m;
+  double* yg = Y + int64_t(active ? col : 0) * ldy;
+  if (active)
+    for (int i = lane; i < m; i += 64) y[i] = yg[i];
+  for (int k1 = m - 3; k1 >= 0; k1 -= kBtChunk) {
+    const int cnt = k1 + 1 < kBtChunk ? k1 + 1 : kBtChunk;  // reflectors k1, k1-1, ..., k1-cnt+1
+    __syncthreads();
+    for (int e = threadIdx.x; e < cnt * m; e += blockDim.x) {
+      const int c = e / m, i = e - c * m, k = k1 - c;
+      vs[e] = i > k ? A[i + int64_t(k) * lda] : 0.0;
+    }
+    if (int(threadIdx.x) < cnt) ts[threadIdx.x] = tau[k1 - int(threadIdx.x)];
+    __syncthreads();
+    if (!active) continue;
+    for (int c = 0; c < cnt; ++c) {
+      const double t = ts[c];
+      if (t == 0.0) continue;
+      const int k = k1 - c;
+      const double* v = vs + int64_t(c) * m;
+      double acc = 0.0;
+      for (int i = k + 1 + ((lane - (k + 1)) & 63); i < m; i += 64) acc += v[i] * y[i];
+      const double sc = t * wave_sum(acc);
+      for (int i = k + 1 + ((lane - (k + 1)) & 63); i < m; i += 64) y[i] -= sc * v[i];
+    }
   }
-  for (int i = lane; i < m; i += 64) yg[i] = y[i];
+  if (active)
+    for (int i = lane; i < m; i += 64) yg[i] = y[i];
+}
+
+__global__ void k_copy_cols(int m, const double* __restrict__ src, int64_t lds, double* __restrict__ dst,
+                            int64_t ldd) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i < m) dst[i + int64_t(j) * ldd] = src[i + int64_t(j) * lds];
+}
+
+void copy_columns(hipStream_t s, int m, int n, const double* src, int64_t lds, double* dst, int64_t ldd) {
+  hipLaunchKernelGGL(k_copy_cols, dim3((m + 255) / 256, n), dim3(256), 0, s, m, src, lds, dst, ldd);
 }
 
 __global__ void k_zero_i(int* p, int count) {
@@ -486,6 +553,9 @@ struct EigenLayout {
   double* tnorm;
   LU f;
   double* Z;
+  double* G;      // want x want (the Loewdin step)
+  double* Ytmp;   // m x want
+  double* gwork;  // split-K workspace of Y^T Y
 };
 
 EigenLayout eigen_layout(int m, int want, double* work) {
@@ -501,24 +571,28 @@ EigenLayout eigen_layout(int m, int want, double* work) {
   L.f.Lm = L.f.U2 + mw;
   L.Z = L.f.Lm + mw;
   L.f.piv = reinterpret_cast<unsigned char*>(L.Z + mw);
+  L.G = L.Z + mw + (mw + 7) / 8 + 8;
+  L.Ytmp = L.G + int64_t(want) * want;
+  L.gwork = L.Ytmp + mw;
   return L;
 }
 }  // namespace
 
 int64_t sym_eigen_work(int m, int want) {
   const int64_t mw = int64_t(m) * want;
-  return 3 * int64_t(m) + 8 + 5 * mw + (mw + 7) / 8 + 8;
+  return 3 * int64_t(m) + 8 + 5 * mw + (mw + 7) / 8 + 8 + int64_t(want) * want + mw + gemm_tn_work(want, want, m);
 }
 
 void sym_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, double* values, double* Y,
                       int64_t ldy, double* work, int* info) {
   if (m <= 0 || want <= 0 || want > m || lda < m || ldy < m)
     fail(BD_ERR_INVALID_PARAMETER, "sym_eigen_lowest: shape");
-  if (size_t(4 * m + kTW) * 8 > 160 * 1024 || size_t(3 * m) * 8 > 64 * 1024 || size_t(4 * m) * 8 > 64 * 1024)
+  if (size_t(5 * m + 2 * kTW) * 8 > 160 * 1024 || size_t(3 * m) * 8 > 64 * 1024 ||
+      size_t((4 + kBtChunk) * m + kBtChunk) * 8 > 64 * 1024)
     fail(BD_ERR_INVALID_PARAMETER, "sym_eigen_lowest: subspace too large for the LDS kernels");
   const EigenLayout L = eigen_layout(m, want, work);
   hipLaunchKernelGGL(k_zero_i, dim3(1), dim3(64), 0, s, info, 1);
-  const size_t tri_lds = size_t(4 * m + kTW) * 8;
+  const size_t tri_lds = size_t(5 * m + 2 * kTW) * 8;
   if (tri_lds > 64 * 1024 &&
       hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tridiag), hipFuncAttributeMaxDynamicSharedMemorySize,
                           int(tri_lds)) != hipSuccess)
@@ -527,10 +601,15 @@ void sym_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, do
   hipLaunchKernelGGL(k_bisect, dim3((want + 3) / 4), dim3(256), size_t(3 * m) * 8, s, m, want, L.d, L.e, values,
                      L.tnorm);
   hipLaunchKernelGGL(k_invit, dim3((want + 63) / 64), dim3(64), 0, s, m, want, L.d, L.e, values, L.tnorm, L.f, L.Z);
-  hipLaunchKernelGGL(k_cluster_gs, dim3(1), dim3(kTT), size_t(m + want + kTW) * 8, s, m, want, values, L.tnorm, L.f,
-                     L.Z, Y, ldy, info);
-  hipLaunchKernelGGL(k_back_transform, dim3((want + 3) / 4), dim3(256), size_t(4 * m) * 8, s, m, want, A, lda, L.tau,
-                     Y, ldy);
+  hipLaunchKernelGGL(k_cluster_gs, dim3((want + 3) / 4), dim3(256), 0, s, m, want, values, L.tnorm, L.f, L.Z, Y, ldy,
+                     info);
+  // Loewdin step: Y <- Y (3 I - Y^T Y) / 2.
+  gemm_tn(s, want, want, m, 1.0, Y, ldy, Y, ldy, 0.0, L.G, want, L.gwork);
+  hipLaunchKernelGGL(k_loewdin_factor, dim3((want + 255) / 256, want), dim3(256), 0, s, want, L.G);
+  gemm_nn(s, m, want, want, 1.0, Y, ldy, L.G, want, 0.0, L.Ytmp, m);
+  hipLaunchKernelGGL(k_back_transform, dim3((want + 3) / 4), dim3(256), size_t((4 + kBtChunk) * m + kBtChunk) * 8, s, m,
+                     want, A, lda, L.tau, L.Ytmp, int64_t(m));
+  copy_columns(s, m, want, L.Ytmp, m, Y, ldy);
 }
 
 void zero_ints(hipStream_t s, int* p, int count) {

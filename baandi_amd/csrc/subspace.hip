@@ -83,6 +83,66 @@ __global__ __launch_bounds__(kCT) void k_chol_inv(int k, double* __restrict__ G,
   }
 }
 
+// The same with the block in LDS (packed lower triangle, column-major):
+// an LDL^T elimination with one barrier per column (column j is only read
+// while the columns right of it are updated), then L = column j over
+// sqrt(d_j), then L^{-1} by forward substitution as above.
+__global__ __launch_bounds__(kCT) void k_chol_inv_lds(int k, const double* __restrict__ G,
+                                                      const double* __restrict__ sdiag, int64_t sstride,
+                                                      double* __restrict__ T22, int64_t ldt, int* __restrict__ flag,
+                                                      double* __restrict__ min_ratio) {
+  extern __shared__ double sh[];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int nw = kCT / 64;
+  double* P = sh;
+  double* rhsall = sh + int64_t(k) * (k + 1) / 2;
+  auto pk = [k](int i, int l) { return l * k - (l * (l - 1)) / 2 + (i - l); };
+  for (int l = wave; l < k; l += nw)
+    for (int i = l + lane; i < k; i += 64) P[pk(i, l)] = G[i + int64_t(l) * k];
+  __syncthreads();
+  double mr = INFINITY;
+  for (int j = 0; j < k; ++j) {
+    const double dj = P[pk(j, j)];
+    const double ratio = dj / sdiag[int64_t(j) * sstride];
+    if (isnan(ratio) || ratio < kPivotRatioMin) {
+      if (tid == 0) {
+        flag[0] = 1;
+        min_ratio[0] = isnan(ratio) ? 0.0 : fmin(mr, ratio);
+      }
+      return;  // every thread read the same pivot: uniform exit
+    }
+    mr = fmin(mr, ratio);
+    const double inv_d = 1.0 / dj;
+    for (int l = j + 1 + wave; l < k; l += nw) {
+      const double alj = P[pk(l, j)] * inv_d;
+      const int base_j = pk(0, j), base_l = pk(0, l);  // column starts (row index offsets follow)
+      for (int i = l + lane; i < k; i += 64) P[base_l + i] -= P[base_j + i] * alj;
+    }
+    __syncthreads();
+  }
+  for (int l = wave; l < k; l += nw) {
+    const double sd = sqrt(P[pk(l, l)]);
+    for (int i = l + 1 + lane; i < k; i += 64) P[pk(i, l)] /= sd;
+  }
+  __syncthreads();
+  for (int l = tid; l < k; l += kCT) P[pk(l, l)] = sqrt(P[pk(l, l)]);
+  __syncthreads();
+  if (tid == 0) min_ratio[0] = mr;
+  double* rhs = rhsall + int64_t(wave) * k;
+  for (int c = wave; c < k; c += nw) {
+    for (int i = lane; i < k; i += 64) rhs[i] = i == c ? 1.0 : 0.0;
+    for (int l = c; l < k; ++l) {
+      const int base_l = pk(0, l);
+      double xl = 0.0;
+      if ((l & 63) == lane) xl = rhs[l] / P[base_l + l];
+      xl = __shfl(xl, l & 63, 64);
+      if ((l & 63) == lane) rhs[l] = xl;
+      for (int i = l + 1 + ((lane - (l + 1)) & 63); i < k; i += 64) rhs[i] -= P[base_l + i] * xl;
+    }
+    for (int i = lane; i < k; i += 64) T22[i + int64_t(c) * ldt] = i >= c ? rhs[i] : 0.0;
+  }
+}
+
 // One wave per new row i of T: finiteness and the growth guard.
 __global__ __launch_bounds__(256) void k_growth_guard(const double* __restrict__ S, const double* __restrict__ T,
                                                       int cap, int from, int to, int* __restrict__ flag) {
@@ -155,15 +215,25 @@ void extend_inverse_cholesky_dev(hipStream_t s, const double* S, double* T, int 
   hipLaunchKernelGGL(k_copy_block, dim3((k + 255) / 256, k), dim3(256), 0, s, k,
                      S + from + int64_t(from) * cap, int64_t(cap), fw.G);
   if (from > 0) gemm_tn(s, k, k, from, -1.0, fw.W, from, fw.W, from, 1.0, fw.G, k, fw.gwork);
-  const size_t lds = size_t(kCT / 64) * k * 8;
-  if (lds > 160 * 1024) fail(BD_ERR_INVALID_PARAMETER, "inverse Cholesky block too large");
-  if (lds > 64 * 1024 &&
-      hipFuncSetAttribute(reinterpret_cast<const void*>(&k_chol_inv), hipFuncAttributeMaxDynamicSharedMemorySize,
-                          int(lds)) != hipSuccess)
-    fail(BD_ERR_DEVICE, "k_chol_inv LDS attribute");
   double* T22 = T + from + int64_t(from) * cap;
-  hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(kCT), lds, s, k, fw.G, S + from + int64_t(from) * cap,
-                     int64_t(cap) + 1, T22, int64_t(cap), fw.flag, fw.min_ratio);
+  const size_t lds_packed = (size_t(k) * (k + 1) / 2 + size_t(kCT / 64) * k) * 8;
+  if (lds_packed <= 160 * 1024) {
+    if (lds_packed > 64 * 1024 &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_chol_inv_lds),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_packed)) != hipSuccess)
+      fail(BD_ERR_DEVICE, "k_chol_inv_lds LDS attribute");
+    hipLaunchKernelGGL(k_chol_inv_lds, dim3(1), dim3(kCT), lds_packed, s, k, fw.G, S + from + int64_t(from) * cap,
+                       int64_t(cap) + 1, T22, int64_t(cap), fw.flag, fw.min_ratio);
+  } else {
+    const size_t lds = size_t(kCT / 64) * k * 8;
+    if (lds > 160 * 1024) fail(BD_ERR_INVALID_PARAMETER, "inverse Cholesky block too large");
+    if (lds > 64 * 1024 &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_chol_inv), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            int(lds)) != hipSuccess)
+      fail(BD_ERR_DEVICE, "k_chol_inv LDS attribute");
+    hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(kCT), lds, s, k, fw.G, S + from + int64_t(from) * cap,
+                       int64_t(cap) + 1, T22, int64_t(cap), fw.flag, fw.min_ratio);
+  }
   if (from > 0) {
     gemm_tn(s, k, from, from, 1.0, fw.W, from, T, cap, 0.0, fw.U, k, fw.gwork);         // U = W^T T11
     gemm_nn(s, k, from, k, -1.0, T22, cap, fw.U, k, 0.0, T + from, cap);                  // T21 = -T22 U

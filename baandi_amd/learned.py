@@ -114,8 +114,9 @@ def kinetic_scale(x, g):
     return ((w * g[None, :]).sum(dim=1) / w.sum(dim=1)).clamp_min(0.1)
 
 
-def generate_pairs(operator, diag, vbar, bands, occupied, seed, iterations=(1, 2, 3, 4, 6, 8, 11, 15),
-                   extra=8, cluster_ha=1e-6, min_overlap=0.5):
+def generate_pairs(operator, diag, vbar, bands, occupied, seed,
+                   iterations=(1, 2, 3, 4, 5, 6, 8, 10, 12, 15, 19, 24, 30, 37),
+                   extra=8, cluster_ha=1e-6, min_overlap=0.3):
     """Residual -> ideal-correction pairs from the FP64 Davidson on `operator`
     (potential installed), started from a seeded N(0,1) block."""
     import torch

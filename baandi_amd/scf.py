@@ -9,7 +9,10 @@ dr2 (refscf/metric.rs) and difference-history Anderson mixing with beta 0.7,
 history 8 (refscf/broyden.rs:317-445). Stops when dr2 < threshold_ry.
 
 The density-sphere field glue (grid transforms of the density and the Hartree
-potential, pointwise XC, mixing) runs on the host in NumPy in this version.
+potential, pointwise XC, the mixer's metric products) is the caller's side of
+the boundary; it runs on the GPU with torch tensors (torch.fft for the 4 Ecut
+density grid), so that nothing but scalars and the small mixing matrix
+crosses to the host per map.
 """
 from __future__ import annotations
 
@@ -18,11 +21,8 @@ import time
 from dataclasses import dataclass
 
 import numpy as np
-import scipy.fft as sfft
 
 from . import lib as L
-
-FFT_WORKERS = -1  # all host threads for the density-sphere grid transforms
 from .hamiltonian import DistributedRealHamiltonian, GammaSystem
 from . import ref_davidson
 
@@ -53,6 +53,27 @@ def lda_pz(n):
     return eps, vxc
 
 
+def lda_pz_torch(n):
+    """lda_pz on a CUDA tensor (same formulas, same 1e-10 cut)."""
+    import torch
+    live = n > 1e-10
+    d = torch.where(live, n, torch.ones_like(n))
+    cd = torch.pow(d, 1.0 / 3.0)
+    rs = (3.0 / (4.0 * math.pi)) ** (1.0 / 3.0) / cd
+    ex = -0.75 * (3.0 / math.pi) ** (1.0 / 3.0) * cd
+    low = rs >= 1.0
+    sr = torch.sqrt(rs)
+    den = 1.0 + 1.0529 * sr + 0.3334 * rs
+    lg = torch.log(rs)
+    ec = torch.where(low, -0.1423 / den, 0.0311 * lg - 0.048 + 0.0020 * rs * lg - 0.0116 * rs)
+    dec = torch.where(low, 0.1423 * (1.0529 / (2.0 * sr) + 0.3334) / (den * den),
+                      0.0311 / rs + 0.0020 * (lg + 1.0) - 0.0116)
+    zero = torch.zeros_like(n)
+    eps = torch.where(live, ex + ec, zero)
+    vxc = torch.where(live, 4.0 * ex / 3.0 + ec - rs * dec / 3.0, zero)
+    return eps, vxc
+
+
 @dataclass
 class ScfOptions:
     bands: int
@@ -70,40 +91,61 @@ class ScfOptions:
     precision: int = L.BD_FP64
     mixed_floor_ha: float = 5e-5
     mlp: object = None  # a baandi_amd.learned.Network when preconditioner == BD_PRECOND_MLP
+    allow_unconverged: bool = False  # return the last map instead of raising at max_iterations
     # Optional fixed-point verification map (driver.rs:727-757): one more
     # Davidson at the converged potential with every band at this threshold (Ha).
     final_threshold: float = 0.0
 
 
 class DensityField:
-    """Density-sphere coefficients (4 Ecut) and their grid transforms."""
+    """Density-sphere coefficients (4 Ecut) and their grid transforms, on the GPU."""
 
     def __init__(self, system: GammaSystem):
-        self.dims = system.dims
+        import torch
+        dev = torch.device("cuda")
+        self.dims = tuple(system.dims)
         self.size = int(np.prod(self.dims))
-        self.flat = system.get("density_index")
-        self.g2 = system.get("density_g2")
+        self.flat = torch.from_numpy(system.get("density_index").astype(np.int64)).to(dev)
+        g2 = system.get("density_g2")
         miller = system.get("density_miller")
-        self.zero = np.all(miller == 0, axis=1)
+        zero = np.all(miller == 0, axis=1)
         # Positive representatives carry the half-sphere Hartree weight
         # 8 pi Omega / G^2 in Ry (refscf/metric.rs:1-5).
         first = np.where(miller[:, 0] != 0, miller[:, 0], np.where(miller[:, 1] != 0, miller[:, 1], miller[:, 2]))
         positive = first > 0
-        self.metric = np.zeros(len(self.g2))
-        self.metric[positive] = 8.0 * math.pi * system.volume / self.g2[positive]
-        self.coulomb = np.zeros(len(self.g2))
-        self.coulomb[~self.zero] = 4.0 * math.pi / self.g2[~self.zero]
+        metric = np.zeros(len(g2))
+        metric[positive] = 8.0 * math.pi * system.volume / g2[positive]
+        coulomb = np.zeros(len(g2))
+        coulomb[~zero] = 4.0 * math.pi / g2[~zero]
+        self.g2 = g2
+        self.zero_np = zero
+        self.zero = torch.from_numpy(zero).to(dev)
+        self.metric = torch.from_numpy(metric).to(dev)
+        self.coulomb = torch.from_numpy(coulomb).to(dev)
 
     def to_grid(self, coeffs):
-        g = np.zeros(self.size, dtype=np.complex128)
+        import torch
+        g = torch.zeros(self.size, dtype=torch.complex128, device=coeffs.device)
         g[self.flat] = coeffs
-        return sfft.ifftn(g.reshape(self.dims), workers=FFT_WORKERS) * self.size
+        return torch.fft.ifftn(g.view(self.dims)) * self.size
 
     def from_grid(self, field):
-        return (sfft.fftn(field, workers=FFT_WORKERS) / self.size).reshape(-1)[self.flat]
+        import torch
+        return (torch.fft.fftn(field.view(self.dims).to(torch.complex128)) / self.size).reshape(-1)[self.flat]
 
     def dot(self, a, b):
-        return float(np.sum(self.metric * (a.real * b.real + a.imag * b.imag)))
+        return float((self.metric * (a.real * b.real + a.imag * b.imag)).sum())
+
+    def gram(self, rows, f):
+        """Metric products <r_i, r_j> and <r_i, f> of stacked coefficients
+        (host arrays). Elementwise products and row sums: a library GEMM with
+        an 8 x 8 result and a 50k inner dimension is far slower here."""
+        w = self.metric
+        re, im = rows.real, rows.imag
+        wre, wim = re * w, im * w
+        a = (wre[:, None, :] * re[None, :, :]).sum(dim=-1) + (wim[:, None, :] * im[None, :, :]).sum(dim=-1)
+        c = (wre * f.real).sum(dim=-1) + (wim * f.imag).sum(dim=-1)
+        return a.cpu().numpy(), c.cpu().numpy()
 
 
 class Mixer:
@@ -115,28 +157,27 @@ class Mixer:
         self.prev = None
 
     def update(self, x, f):
+        import torch
         if self.prev is not None:
             px, pf = self.prev
             self.pairs.append((px - x, pf - f))
             if len(self.pairs) > self.history:
                 self.pairs.pop(0)
-        self.prev = (x.copy(), f.copy())
-        xx, ff = x.copy(), f.copy()
+        self.prev = (x.clone(), f.clone())
+        xx, ff = x.clone(), f.clone()
         while self.pairs:
             p = len(self.pairs)
-            a = np.empty((p, p))
-            for i in range(p):
-                for j in range(i + 1):
-                    a[i, j] = a[j, i] = self.field.dot(self.pairs[i][1], self.pairs[j][1])
-            c = np.array([self.field.dot(df, f) for _, df in self.pairs])
+            a, c = self.field.gram(torch.stack([df for _, df in self.pairs]), f)
+            a = 0.5 * (a + a.T)
             lam, q = np.linalg.eigh(a)
             if lam[-1] <= 0.0 or lam[0] <= math.sqrt(p * np.finfo(float).eps) * lam[-1]:
                 self.pairs.pop(0)  # rank-deficient history: drop the oldest pair
                 continue
-            gamma = q @ ((q.T @ c) / lam)
-            for g, (dx, df) in zip(gamma, self.pairs):
-                xx -= g * dx
-                ff -= g * df
+            gamma = torch.from_numpy(q @ ((q.T @ c) / lam)).to(x.device)
+            dX = torch.stack([dx for dx, _ in self.pairs])
+            dF = torch.stack([df for _, df in self.pairs])
+            xx = xx - (gamma[:, None] * dX).sum(dim=0)
+            ff = ff - (gamma[:, None] * dF).sum(dim=0)
             break
         return xx + self.beta * ff
 
@@ -158,8 +199,8 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
     occ = np.zeros(bands)
     occ[:occupied] = 2.0
     field = DensityField(system)
-    vion_g = system.get("ionic_density_re") + 1j * system.get("ionic_density_im")
-    vion = field.to_grid(vion_g).real
+    vion_g = torch.from_numpy(system.get("ionic_density_re") + 1j * system.get("ionic_density_im")).to(dev)
+    vion = field.to_grid(vion_g).real.contiguous()
     op = DistributedRealHamiltonian(system, capacity=bands, precision=L.BD_FP64)
     op_mixed = (DistributedRealHamiltonian(system, capacity=bands, precision=L.BD_MIXED)
                 if options.precision == L.BD_MIXED else None)
@@ -174,7 +215,8 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
     dcart = dm @ b
     sfac = np.exp(-1j * (dcart @ atoms.T)).sum(axis=1)
     x = system.z_valence / system.volume * np.exp(-field.g2 * system.rc ** 2 / 4.0) * sfac
-    x[field.zero] = electrons / system.volume
+    x[field.zero_np] = electrons / system.volume
+    x = torch.from_numpy(x).to(dev)
     mixer = Mixer(field, options.beta, options.history)
     if start is None:
         g = torch.Generator(device="cpu").manual_seed(options.seed)
@@ -207,20 +249,21 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
             vh_g = field.coulomb * x
             both = field.to_grid(x + 1j * vh_g)  # rho_in + i V_H (both real fields)
             rho_in, vh = both.real, both.imag
-            eps, vxc = lda_pz(np.abs(rho_in))
+            eps, vxc = lda_pz_torch(rho_in.abs())
             potential = vion + vh + vxc
-            pot_dev = torch.from_numpy(potential.reshape(-1)).to(dev)
+            pot_dev = potential.reshape(-1).contiguous()
+            vmean = float(pot_dev.mean())
             use_mixed = op_mixed is not None and ethr / 2.0 >= options.mixed_floor_ha
             cur = op_mixed if use_mixed else op
             cur.set_owned_potential(pot_dev)
-            diag = diag_static + float(potential.mean())
+            diag = diag_static + vmean
             totals["fields_s"] += time.perf_counter() - tf
             td = time.perf_counter()
             opts = ref_davidson.Options(bands=bands, threshold=ethr / 2.0, occupied=occupied,
                                         first_step=(iteration == 1 and not redo),
                                         max_iterations=options.davidson_max_iterations, return_hx=True,
                                         preconditioner=options.preconditioner, mlp=options.mlp,
-                                        mlp_vbar=float(potential.mean()))
+                                        mlp_vbar=vmean)
             sol = ref_davidson.solve_with_initial_hx(cur, orbitals, diag, opts, out=(xout, hxout))
             totals["davidson_s"] += time.perf_counter() - td
             totals["mixed_maps"] += int(use_mixed)
@@ -231,7 +274,7 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
             totals["apply_s"] += sol.record["seconds_apply"]
             tf = time.perf_counter()
             cur.band_density_sum(xout, occ, rho_dev)
-            rho_out = rho_dev.cpu().numpy().reshape(field.dims)
+            rho_out = rho_dev.view(field.dims)
             y = field.from_grid(rho_out)
             y[field.zero] = electrons / system.volume
             f = y - x
@@ -251,7 +294,7 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
         if log:
             log(f"scf {iteration:3d} dr2={dr2:.3e} ethr={ethr:.2e} dav_it={sol.record['iterations']} "
                 f"hcols={sol.record['h_columns']}")
-        if dr2 < options.threshold_ry:
+        if dr2 < options.threshold_ry or (options.allow_unconverged and iteration == options.max_iterations):
             result = (sol, rho_in, potential, rho_out)
             break
         x = mixer.update(x, f)
@@ -261,15 +304,16 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
         raise RuntimeError(f"SCF did not reach dr2 < {options.threshold_ry} Ry in {options.max_iterations} maps")
     sol, rho_in, potential, rho_out = result
     if options.final_threshold > 0.0:
-        diag = diag_static + float(potential.mean())
+        vmean = float(potential.mean())
+        diag = diag_static + vmean
         opts = ref_davidson.Options(bands=bands, threshold=options.final_threshold, occupied=occupied,
                                     first_step=True, max_iterations=200, return_hx=True,
                                     preconditioner=options.preconditioner, mlp=options.mlp,
-                                    mlp_vbar=float(potential.mean()))
+                                    mlp_vbar=vmean)
         sol = ref_davidson.solve_with_initial_hx(op, orbitals, diag, opts, out=(xout, hxout))
         orbitals, xout = xout, orbitals
         op.band_density_sum(orbitals, occ, rho_dev)
-        rho_out = rho_dev.cpu().numpy().reshape(field.dims)
+        rho_out = rho_dev.view(field.dims)
     # Energy audit of the converged map (refscf/map.rs:466-540).
     xo = orbitals.cpu().numpy()
     kin = float(np.sum(occ[:, None] * xo ** 2 * kinetic[None, :]))
@@ -277,16 +321,16 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
     dv = system.volume / field.size
     yo = field.from_grid(rho_out)
     vh_out = field.to_grid(field.coulomb * yo).real
-    hartree = 0.5 * dv * float(np.sum(rho_out * vh_out))
-    eps_o, _ = lda_pz(np.abs(rho_out))
-    xc = dv * float(np.sum(np.abs(rho_out) * eps_o))
-    local = dv * float(np.sum(rho_out * vion))
+    hartree = 0.5 * dv * float((rho_out * vh_out).sum())
+    eps_o, _ = lda_pz_torch(rho_out.abs())
+    xc = dv * float((rho_out.abs() * eps_o).sum())
+    local = dv * float((rho_out * vion).sum())
     total = kin + local + nonlocal_ + hartree + xc + ewald
     energy = {"kinetic": kin, "local": local, "nonlocal": nonlocal_, "hartree": hartree, "xc": xc,
               "ewald": ewald, "total": total, "band_sum": float(np.dot(sol.eigenvalues, occ))}
     out = dict(energy=energy, eigenvalues=sol.eigenvalues.copy(), history=history, wall_s=wall,
                setup_s=t_setup, iterations=len(history), totals=totals, orbitals=orbitals, op=op,
-               plan=op.plan_info(), potential=potential)
+               plan=op.plan_info(), potential=potential.cpu().numpy())
     if profile_kernels:
         out["kernel_totals"] = op.stage_totals()
         if op_mixed is not None:

@@ -34,7 +34,7 @@ def displaced(workload, sigma_bohr, seed):
     return out
 
 
-def cell_pairs(workload, seed, maps=30, log=print):
+def cell_pairs(workload, seed, maps=12, log=print):
     import torch
     from . import workloads
     from .hamiltonian import DistributedRealHamiltonian
@@ -45,7 +45,8 @@ def cell_pairs(workload, seed, maps=30, log=print):
     occupied = int(round(system.electrons)) // 2
     # A screened potential: a few SCF maps (the training needs realistic
     # spectra, not a converged density).
-    potential = run_scf(system, ScfOptions(bands=bands, threshold_ry=1e-3, max_iterations=maps))["potential"]
+    potential = run_scf(system, ScfOptions(bands=bands, threshold_ry=1e-3, max_iterations=maps,
+                                           allow_unconverged=True))["potential"]
     op = DistributedRealHamiltonian(system, capacity=bands + 8)
     pot = torch.from_numpy(potential.reshape(-1)).cuda()
     op.set_owned_potential(pot)

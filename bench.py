@@ -159,19 +159,29 @@ def main():
     mixed = None
     if args.mixed_steps > 0:
         from baandi_amd import lib as L
-        mopts = ScfOptions(bands=wl["bands"], threshold_ry=1e-8, precision=L.BD_MIXED)
-        mwalls, mres = [], None
-        for k in range(args.mixed_steps):
-            mres = run_scf(system, mopts)
-            mwalls.append(mres["wall_s"])
-            print(f"[bench] mixed step {k}: {mres['wall_s']:.3f}s iterations={mres['iterations']} "
-                  f"E={mres['energy']['total']:.12f} totals={mres['totals']}", file=sys.stderr, flush=True)
+        from baandi_amd.learned import Network
+        from baandi_amd.train_preconditioner import DEFAULT_OUT
+        net = Network.load(DEFAULT_OUT)
         natoms = len(system.get("atoms_cart"))
         occ = int(round(system.electrons)) // 2
-        mixed = {"wall_s": sum(mwalls) / len(mwalls), "iterations": mres["iterations"],
-                 "mixed_maps": mres["totals"]["mixed_maps"],
-                 "energy_diff_per_atom_ha": abs(mres["energy"]["total"] - last["energy"]["total"]) / natoms,
-                 "eigenvalue_max_diff_ha": float(np.abs(mres["eigenvalues"][:occ] - last["eigenvalues"][:occ]).max())}
+        legs = {"mixed": ScfOptions(bands=wl["bands"], threshold_ry=1e-8, precision=L.BD_MIXED),
+                "learned": ScfOptions(bands=wl["bands"], threshold_ry=1e-8, preconditioner=L.BD_PRECOND_MLP, mlp=net),
+                "mixed_learned": ScfOptions(bands=wl["bands"], threshold_ry=1e-8, precision=L.BD_MIXED,
+                                            preconditioner=L.BD_PRECOND_MLP, mlp=net)}
+        mixed = {}
+        for name, mopts in legs.items():
+            mwalls, mres = [], None
+            for k in range(args.mixed_steps):
+                mres = run_scf(system, mopts)
+                mwalls.append(mres["wall_s"])
+                print(f"[bench] {name} step {k}: {mres['wall_s']:.3f}s iterations={mres['iterations']} "
+                      f"E={mres['energy']['total']:.12f} totals={mres['totals']}", file=sys.stderr, flush=True)
+            mixed[name] = {
+                "wall_s": round(sum(mwalls) / len(mwalls), 4), "scf_iterations": mres["iterations"],
+                "h_columns": mres["totals"]["h_columns"], "davidson_iterations": mres["totals"]["davidson_iterations"],
+                "mixed_maps": mres["totals"]["mixed_maps"], "mlp_fallbacks": mres["totals"]["mlp_fallbacks"],
+                "energy_diff_per_atom_ha": abs(mres["energy"]["total"] - last["energy"]["total"]) / natoms,
+                "eigenvalue_max_diff_ha": float(np.abs(mres["eigenvalues"][:occ] - last["eigenvalues"][:occ]).max())}
     st = last["kernel_totals"]
     plan = last["plan"]
     launches = st["applies"]
@@ -192,15 +202,17 @@ def main():
         "scaling": "strong",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded N(0,1) start, synthetic Gaussian pseudopotential)",
+        "data": "synthetic (seeded N(0,1) start, synthetic Gaussian pseudopotential); learned weights "
+                "trained on displaced Si8 cells by baandi_amd.train_preconditioner",
         "config": {"workload": "Si64 cubic 2x2x2, a=20.52 Bohr, Gamma, 30 Ry, 160 bands (128 occ), "
                                "SCF to dr2<1e-8 Ry, plain FP64 path",
                    "n_pw": int(system.n), "fft_grid": list(system.dims), "bands": wl["bands"],
                    "scf_iterations": last["iterations"], "h_columns": last["totals"]["h_columns"],
-                   "mixed_s": None if mixed is None else round(mixed["wall_s"], 4),
-                   "speedup_mixed_vs_fp64": None if mixed is None else round(ms_per_step / 1e3 / mixed["wall_s"], 3),
-                   "mixed_learned_s": None},
-        "mixed": mixed,
+                   "mixed_s": None if mixed is None else mixed["mixed"]["wall_s"],
+                   "mixed_learned_s": None if mixed is None else mixed["mixed_learned"]["wall_s"],
+                   "speedup_mixed_learned_vs_fp64": None if mixed is None else
+                   round(ms_per_step / 1e3 / mixed["mixed_learned"]["wall_s"], 3)},
+        "variants": mixed,
         "energy_ha": last["energy"]["total"],
         "breakdown_s": {k: round(v, 4) for k, v in last["totals"].items() if k.endswith("_s")},
         "roofline": {

@@ -141,3 +141,44 @@ def test_davidson_falls_back_to_tpa_on_nonfinite_network():
     assert np.array_equal(sol.eigenvalues, tpa.eigenvalues)
     assert sol.record["h_columns"] == tpa.record["h_columns"]
     assert np.abs(sol.eigenvalues[:4] - w[:4]).max() < 1e-10
+
+
+def test_trained_network_beats_tpa_on_held_out_cell():
+    """The committed network (trained on displaced Si8 cells 1000-1007) on a
+    held-out displaced cell: fewer H columns than TPA from the same start,
+    and the same converged eigenvalues (the fixed point does not move)."""
+    import os
+    import torch
+    from baandi_amd import ref_davidson, workloads, lib as L
+    from baandi_amd.hamiltonian import DistributedRealHamiltonian
+    from baandi_amd.learned import Network
+    from baandi_amd.scf import run_scf, ScfOptions
+    from baandi_amd.train_preconditioner import displaced, DEFAULT_OUT
+    assert os.path.exists(DEFAULT_OUT)
+    net = Network.load(DEFAULT_OUT)
+    wl = displaced(workloads.si8(), 0.05, 2024)
+    system = workloads.build(wl)
+    potential = run_scf(system, ScfOptions(bands=wl["bands"], threshold_ry=1e-3, max_iterations=12,
+                                           allow_unconverged=True))["potential"]
+    op = DistributedRealHamiltonian(system, capacity=wl["bands"])
+    op.set_owned_potential(torch.from_numpy(potential.reshape(-1)).cuda())
+    vbar = float(potential.mean())
+    diag = torch.from_numpy(system.get("diagonal_static") + vbar).cuda()
+    g = torch.Generator(device="cpu").manual_seed(7)
+    x0 = torch.randn(wl["bands"], system.n, generator=g, dtype=torch.float64)
+    x0 = (x0 / x0.norm(dim=1, keepdim=True)).cuda()
+    occ = int(round(system.electrons)) // 2
+    out = {}
+    for name, pre in [("tpa", L.BD_PRECOND_TPA), ("mlp", L.BD_PRECOND_MLP), ("qe", L.BD_PRECOND_QE)]:
+        sol = ref_davidson.solve(op, x0, diag, ref_davidson.Options(
+            bands=wl["bands"], threshold=1e-10, occupied=occ, max_iterations=600, preconditioner=pre,
+            mlp=net if pre == L.BD_PRECOND_MLP else None, mlp_vbar=vbar))
+        out[name] = sol
+        print(f"[mlp gain] {name}: iterations {sol.record['iterations']}, H columns {sol.record['h_columns']}, "
+              f"fallback {sol.record['mlp_fallback']}, converged {bool(sol.converged.all())}")
+    tpa, mlp = out["tpa"], out["mlp"]
+    assert tpa.converged[:occ].all() and mlp.converged[:occ].all()
+    assert np.abs(tpa.eigenvalues[:occ] - mlp.eigenvalues[:occ]).max() < 1e-9
+    gain = 1.0 - mlp.record["h_columns"] / tpa.record["h_columns"]
+    print(f"[mlp gain] H columns saved against TPA: {100 * gain:.1f}%")
+    assert gain > 0.0
