@@ -26,18 +26,12 @@
 #include "common.hpp"
 #include "eigen.hpp"
 #include "gemm.hpp"
+#include "wave.hpp"
 
 namespace bd {
 
 namespace {
-constexpr int kTT = 1024;      // tridiagonalisation / Gram-Schmidt workgroup
-constexpr int kTW = kTT / 64;  // its waves
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
 __device__ __forceinline__ double wave_min(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
@@ -48,6 +42,9 @@ __device__ __forceinline__ double wave_max(double v) {
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
   return v;
 }
+
+constexpr int kTT = 1024;      // tridiagonalisation workgroup
+constexpr int kTW = kTT / 64;  // its waves
 
 // Sum over a kTT-thread block, returned to every thread (fixed order).
 // `red` holds kTW doubles of LDS; ends with a barrier so it can be reused.
@@ -363,19 +360,87 @@ __device__ double lu_solve(int m, int want, int j, const LU& f, double* z, int64
   return big;
 }
 
+// (T - lambda I)^{-1} z in place for the inverse-iteration kernel: the LU
+// factors and z are fetched kInvChunk rows ahead into registers, so the
+// sequential sweep waits on memory once per chunk. Returns sum y^2.
+constexpr int kInvChunk = 8;
+
+__device__ double lu_solve_prefetch(int m, int want, int j, const LU& f, double* z) {
+  auto at = [&](int i) { return int64_t(i) * want + j; };
+  double cur = z[at(0)];
+  for (int i0 = 0; i0 + 1 < m; i0 += kInvChunk) {
+    const int cnt = min(kInvChunk, m - 1 - i0);
+    double zn[kInvChunk], lm[kInvChunk];
+    unsigned char pv[kInvChunk];
+#pragma unroll
+    for (int c = 0; c < kInvChunk; ++c)
+      if (c < cnt) {
+        zn[c] = z[at(i0 + c + 1)];
+        lm[c] = f.Lm[at(i0 + c)];
+        pv[c] = f.piv[at(i0 + c)];
+      }
+#pragma unroll
+    for (int c = 0; c < kInvChunk; ++c)
+      if (c < cnt) {
+        double nxt = zn[c];
+        if (pv[c]) {
+          const double tmp = cur;
+          cur = nxt;
+          nxt = tmp;
+        }
+        nxt -= lm[c] * cur;
+        z[at(i0 + c)] = cur;
+        cur = nxt;
+      }
+  }
+  z[at(m - 1)] = cur;
+  double y1 = 0.0, y2 = 0.0, nn = 0.0;
+  for (int i1 = m - 1; i1 >= 0; i1 -= kInvChunk) {
+    const int cnt = min(kInvChunk, i1 + 1);
+    double zz[kInvChunk], u0[kInvChunk], u1[kInvChunk], u2[kInvChunk];
+#pragma unroll
+    for (int c = 0; c < kInvChunk; ++c)
+      if (c < cnt) {
+        const int64_t ix = at(i1 - c);
+        zz[c] = z[ix];
+        u0[c] = f.U0i[ix];
+        u1[c] = f.U1[ix];
+        u2[c] = f.U2[ix];
+      }
+#pragma unroll
+    for (int c = 0; c < kInvChunk; ++c)
+      if (c < cnt) {
+        const double y = (zz[c] - u1[c] * y1 - u2[c] * y2) * u0[c];
+        z[at(i1 - c)] = y;
+        nn += y * y;
+        y2 = y1;
+        y1 = y;
+      }
+  }
+  return nn;
+}
+
 __global__ __launch_bounds__(64) void k_invit(int m, int want, const double* __restrict__ d,
                                               const double* __restrict__ e, const double* __restrict__ values,
                                               const double* __restrict__ tnorm_in, LU f, double* __restrict__ Z) {
+  extern __shared__ double sh[];  // d, e of the tridiagonal (every lane reads the same entries)
+  double* ds = sh;
+  double* es = sh + m;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    ds[i] = d[i];
+    es[i] = i + 1 < m ? e[i] : 0.0;
+  }
+  __syncthreads();
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= want) return;
   auto at = [&](int i) { return int64_t(i) * want + j; };
   const double lam = values[j];
   const double tiny = DBL_EPSILON * fmax(tnorm_in[0], DBL_MIN);
-  double a = d[0] - lam, b = m > 1 ? e[0] : 0.0;
+  double a = ds[0] - lam, b = es[0];
   for (int i = 0; i + 1 < m; ++i) {
-    const double c = e[i];
-    const double dn = d[i + 1] - lam;
-    const double en = i + 2 < m ? e[i + 1] : 0.0;
+    const double c = es[i];
+    const double dn = ds[i + 1] - lam;
+    const double en = es[i + 1];
     if (fabs(a) >= fabs(c)) {
       double p = a;
       if (fabs(p) < tiny) p = p >= 0.0 ? tiny : -tiny;
@@ -401,17 +466,13 @@ __global__ __launch_bounds__(64) void k_invit(int m, int want, const double* __r
   }
   if (fabs(a) < tiny) a = a >= 0.0 ? tiny : -tiny;
   f.U0i[at(m - 1)] = 1.0 / a;
-  double* z = Z + j;
-  for (int i = 0; i < m; ++i) z[int64_t(i) * want] = hash_unit(i, j, 0);
-  for (int it = 0; it < 2; ++it) {
-    const double big = lu_solve(m, want, j, f, z, want);
-    const double s = big > 0.0 ? 1.0 / big : 1.0;
-    for (int i = 0; i < m; ++i) z[int64_t(i) * want] *= s;
-  }
-  double nn = 0.0;
-  for (int i = 0; i < m; ++i) nn += z[int64_t(i) * want] * z[int64_t(i) * want];
-  const double s = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
-  for (int i = 0; i < m; ++i) z[int64_t(i) * want] *= s;
+  for (int i = 0; i < m; ++i) Z[at(i)] = hash_unit(i, j, 0);
+  // Two solves (the second from the unscaled first: growth per solve is at
+  // most ~1/(eps ||T||), far from overflow), then unit norm.
+  lu_solve_prefetch(m, want, j, f, Z);
+  const double nn = lu_solve_prefetch(m, want, j, f, Z);
+  const double sc = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
+  for (int i = 0; i < m; ++i) Z[at(i)] *= sc;
 }
 
 // ---------------------------------------------------------------------------
@@ -505,9 +566,20 @@ __global__ __launch_bounds__(256) void k_back_transform(int m, int want, const d
   for (int k1 = m - 3; k1 >= 0; k1 -= kBtChunk) {
     const int cnt = k1 + 1 < kBtChunk ? k1 + 1 : kBtChunk;  // reflectors k1, k1-1, ..., k1-cnt+1
     __syncthreads();
-    for (int e = threadIdx.x; e < cnt * m; e += blockDim.x) {
-      const int c = e / m, i = e - c * m, k = k1 - c;
-      vs[e] = i > k ? A[i + int64_t(k) * lda] : 0.0;
+    // Stage the chunk's reflectors: 16 independent loads per thread per batch.
+    for (int e0 = 0; e0 < cnt * m; e0 += 16 * int(blockDim.x)) {
+      double v[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int e = e0 + u * int(blockDim.x) + int(threadIdx.x);
+        const int c = e / m, i = e - c * m, k = k1 - c;
+        v[u] = (e < cnt * m && i > k) ? A[i + int64_t(k) * lda] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int e = e0 + u * int(blockDim.x) + int(threadIdx.x);
+        if (e < cnt * m) vs[e] = v[u];
+      }
     }
     if (int(threadIdx.x) < cnt) ts[threadIdx.x] = tau[k1 - int(threadIdx.x)];
     __syncthreads();
@@ -585,9 +657,9 @@ int64_t sym_eigen_work(int m, int want) {
 
 void sym_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, double* values, double* Y,
                       int64_t ldy, double* work, int* info) {
-  if (m <= 0 || want <= 0 || want > m || lda < m || ldy < m)
+  if (m <= 0 || want <= 0 || want > m || lda < m || ldy < m || lda * m >= (int64_t(1) << 31))
     fail(BD_ERR_INVALID_PARAMETER, "sym_eigen_lowest: shape");
-  if (size_t(5 * m + 2 * kTW) * 8 > 160 * 1024 || size_t(3 * m) * 8 > 64 * 1024 ||
+  if (size_t(5 * m + 2 * kTW) * 8 > 160 * 1024 || size_t(3 * m) * 8 > 64 * 1024 || m > 1025 ||
       size_t((4 + kBtChunk) * m + kBtChunk) * 8 > 64 * 1024)
     fail(BD_ERR_INVALID_PARAMETER, "sym_eigen_lowest: subspace too large for the LDS kernels");
   const EigenLayout L = eigen_layout(m, want, work);
@@ -600,7 +672,7 @@ void sym_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, do
   hipLaunchKernelGGL(k_tridiag, dim3(1), dim3(kTT), tri_lds, s, m, A, lda, L.d, L.e, L.tau);
   hipLaunchKernelGGL(k_bisect, dim3((want + 3) / 4), dim3(256), size_t(3 * m) * 8, s, m, want, L.d, L.e, values,
                      L.tnorm);
-  hipLaunchKernelGGL(k_invit, dim3((want + 63) / 64), dim3(64), 0, s, m, want, L.d, L.e, values, L.tnorm, L.f, L.Z);
+  hipLaunchKernelGGL(k_invit, dim3((want + 63) / 64), dim3(64), size_t(2 * m) * 8, s, m, want, L.d, L.e, values, L.tnorm, L.f, L.Z);
   hipLaunchKernelGGL(k_cluster_gs, dim3((want + 3) / 4), dim3(256), 0, s, m, want, values, L.tnorm, L.f, L.Z, Y, ldy,
                      info);
   // Loewdin step: Y <- Y (3 I - Y^T Y) / 2.
