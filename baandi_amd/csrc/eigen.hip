@@ -434,7 +434,13 @@ __global__ __launch_bounds__(64) void k_invit(int m, int want, const double* __r
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= want) return;
   auto at = [&](int i) { return int64_t(i) * want + j; };
-  const double lam = values[j];
+  // Shifts of (numerically) equal eigenvalues are spread 10 eps ||T|| apart,
+  // as LAPACK's dstein does: an exactly repeated eigenvalue (the realified
+  // complex problem has only such pairs) would otherwise give every vector of
+  // the pair one singular factorisation and one direction.
+  const double pertol = 10.0 * DBL_EPSILON * tnorm_in[0];
+  double lam = values[0];
+  for (int i = 1; i <= j; ++i) lam = fmax(values[i], lam + pertol);
   const double tiny = DBL_EPSILON * fmax(tnorm_in[0], DBL_MIN);
   double a = ds[0] - lam, b = es[0];
   for (int i = 0; i + 1 < m; ++i) {

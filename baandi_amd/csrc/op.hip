@@ -151,6 +151,68 @@ __global__ __launch_bounds__(kThreads) void k_stick_forward(
   }
 }
 
+// k point: scatter complex column t (rows 2g, 2g+1 = Re, Im of coefficient g)
+// into its sticks and inverse-transform along x; one complex FFT per column
+// (distributed_complex.rs:340-386).
+template <typename F>
+__global__ __launch_bounds__(kThreads) void k_kstick_inverse(
+    const double* __restrict__ x, int64_t ldx, int n0, int T, int Spad, const int* __restrict__ tile_off,
+    const int4* __restrict__ entries, const c2<double>* __restrict__ roots_x, RadixPlan rx,
+    c2<F>* __restrict__ sticks) {
+  extern __shared__ char smem[];
+  c2<F>* L = reinterpret_cast<c2<F>*>(smem);
+  c2<F>* rt = L + T * n0;
+  const int tile = blockIdx.x, t = blockIdx.y;
+  const double* xc = x + int64_t(t) * ldx;
+  for (int i = threadIdx.x; i < T * n0; i += blockDim.x) L[i] = {F(0), F(0)};
+  stage_roots(rt, roots_x, n0);
+  __syncthreads();
+  const int e0 = tile_off[tile], e1 = tile_off[tile + 1];
+  for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+    const int4 en = entries[e];
+    const double2 v = *reinterpret_cast<const double2*>(xc + 2 * int64_t(en.x));
+    L[en.y] = {F(v.x), F(v.y)};
+  }
+  __syncthreads();
+  lds_fft<true>(L, T, n0, 1, rx, rt);
+  c2<F>* out = sticks + int64_t(t) * n0 * Spad + int64_t(tile) * T;
+  for (int e = threadIdx.x; e < T * n0; e += blockDim.x) {
+    const int xi = e / T, slot = e - xi * T;
+    out[int64_t(xi) * Spad + slot] = L[slot * n0 + xi];
+  }
+}
+
+// k point: forward x FFT of a tile, gather the sphere, 1/N, + |k+G|^2/2 x.
+template <typename F>
+__global__ __launch_bounds__(kThreads) void k_kstick_forward(
+    const double* __restrict__ x, int64_t ldx, double* __restrict__ y, int64_t ldy, int n0, int T, int Spad,
+    const int* __restrict__ tile_off, const int4* __restrict__ entries, const c2<double>* __restrict__ roots_x,
+    RadixPlan rx, const c2<F>* __restrict__ sticks, const double* __restrict__ kinetic, double inv_n) {
+  extern __shared__ char smem[];
+  c2<F>* L = reinterpret_cast<c2<F>*>(smem);
+  c2<F>* rt = L + T * n0;
+  const int tile = blockIdx.x, t = blockIdx.y;
+  const c2<F>* in = sticks + int64_t(t) * n0 * Spad + int64_t(tile) * T;
+  for (int e = threadIdx.x; e < T * n0; e += blockDim.x) {
+    const int xi = e / T, slot = e - xi * T;
+    L[slot * n0 + xi] = in[int64_t(xi) * Spad + slot];
+  }
+  stage_roots(rt, roots_x, n0);
+  __syncthreads();
+  lds_fft<false>(L, T, n0, 1, rx, rt);
+  const double* xc = x + int64_t(t) * ldx;
+  double* yc = y + int64_t(t) * ldy;
+  const int e0 = tile_off[tile], e1 = tile_off[tile + 1];
+  for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+    const int4 en = entries[e];
+    const int64_t r = 2 * int64_t(en.x);
+    const c2<F> h = L[en.y];
+    const double2 v = *reinterpret_cast<const double2*>(xc + r);
+    const double k = kinetic[r];
+    *reinterpret_cast<double2*>(yc + r) = make_double2(double(h.x) * inv_n + k * v.x, double(h.y) * inv_n + k * v.y);
+  }
+}
+
 // Plane pass: inverse y, per-row-chunk inverse z * V * forward z, forward y.
 // mode 0: H apply (writes sticks back); mode 1: band density (accumulates
 // w_a psi_a^2 + w_b psi_b^2 over all pairs into rho of plane x).
@@ -368,9 +430,71 @@ StickPlan build_stick_plan(const System& s) {
   return p;
 }
 
+// The k sphere has no inversion partner to keep in a tile: occupied sticks
+// (keyed (z, y) as at Gamma) fill tiles of T in key order.
+StickPlan build_k_stick_plan(const System& s, const KBasis& kb) {
+  StickPlan p{};
+  p.n0 = s.dims[0];
+  p.n1 = s.dims[1];
+  p.n2 = s.dims[2];
+  p.rx = radix_plan(p.n0);
+  p.ry = radix_plan(p.n1);
+  p.rz = radix_plan(p.n2);
+  auto split = [&](int64_t f, int& i0, int& i1, int& i2) {
+    i2 = int(f % p.n2);
+    i1 = int((f / p.n2) % p.n1);
+    i0 = int(f / (int64_t(p.n1) * p.n2));
+  };
+  std::set<std::pair<int, int>> keys;
+  for (int64_t f : kb.flat) {
+    int a, b, c;
+    split(f, a, b, c);
+    keys.insert({c, b});
+  }
+  std::vector<std::pair<int, int>> sticks(keys.begin(), keys.end());
+  p.S = int(sticks.size());
+  std::map<std::pair<int, int>, int> index;
+  for (int i = 0; i < p.S; ++i) index[sticks[i]] = i;
+  std::set<int> zs;
+  for (auto& k : sticks) zs.insert(k.first);
+  p.z_of_zc.assign(zs.begin(), zs.end());
+  p.Z = int(p.z_of_zc.size());
+  std::map<int, int> zc_of;
+  for (int i = 0; i < p.Z; ++i) zc_of[p.z_of_zc[i]] = i;
+  p.T = 32;
+  while (p.T > 1 && !fft_fits(p.T, p.rx)) p.T /= 2;
+  if (!fft_fits(p.T, p.rx)) fail(BD_ERR_INVALID_PARAMETER, "x FFT does not fit a workgroup");
+  if (!fft_fits(p.Z, p.ry)) fail(BD_ERR_INVALID_PARAMETER, "y FFT plane does not fit a workgroup");
+  p.chunk = 16;
+  while (p.chunk > 1 && !fft_fits(p.chunk, p.rz)) p.chunk /= 2;
+  p.chunk = std::min(p.chunk, p.n1);
+  p.ntiles = (p.S + p.T - 1) / p.T;
+  p.Spad = p.ntiles * p.T;
+  p.stick_yz.assign(p.Spad, make_int2(-1, -1));
+  for (int i = 0; i < p.S; ++i) p.stick_yz[i] = make_int2(sticks[i].second, zc_of[sticks[i].first]);
+  std::vector<std::vector<int4>> per(p.ntiles);
+  for (int64_t g = 0; g < kb.nk; ++g) {
+    int a, b, c;
+    split(kb.flat[g], a, b, c);
+    const int st = index[{c, b}];
+    per[st / p.T].push_back(make_int4(int(g), (st % p.T) * p.n0 + a, -1, 0));
+  }
+  p.tile_off.assign(1, 0);
+  for (auto& v : per) {
+    p.entries.insert(p.entries.end(), v.begin(), v.end());
+    p.tile_off.push_back(int(p.entries.size()));
+  }
+  return p;
+}
+
 // ---------------------------------------------------------------------------
 // Operator
 // ---------------------------------------------------------------------------
+// The k kernels move a coefficient's (Re, Im) as one 16-byte access.
+static bool complex_aligned(const double* p, int64_t ld) {
+  return (reinterpret_cast<uintptr_t>(p) & 15) == 0 && (ld & 1) == 0;
+}
+
 static size_t stick_lds(const StickPlan& p, size_t elem) { return size_t(p.T * p.n0 + p.n0) * elem; }
 static size_t plane_lds(const StickPlan& p, size_t elem) {
   return size_t(p.Z * p.n1 + p.chunk * p.n2 + p.n1 + p.n2) * elem;
@@ -390,10 +514,14 @@ static void allow_lds(const StickPlan& plan) {
                                hipFuncAttributeMaxDynamicSharedMemorySize, int(stick_lds(plan, e))));
     BD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_stick_forward<F>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, int(stick_lds(plan, e))));
+    BD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_kstick_inverse<F>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(stick_lds(plan, e))));
+    BD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_kstick_forward<F>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(stick_lds(plan, e))));
   }
 }
 
-GammaOp::GammaOp(const System& s, int cap, int prec) : sys(&s) {
+GammaOp::GammaOp(const System& s, const KBasis* k, int cap, int prec) : sys(&s), kb(k) {
   if (prec != BD_FP64 && prec != BD_MIXED) fail(BD_ERR_INVALID_PARAMETER, "unknown precision mode");
   if (cap <= 0) fail(BD_ERR_INVALID_PARAMETER, "capacity must be positive");
   int dev = 0, count = 0;
@@ -404,15 +532,15 @@ GammaOp::GammaOp(const System& s, int cap, int prec) : sys(&s) {
   BD_HIP(hipGetDeviceProperties(&prop, dev));
   if (std::string(prop.gcnArchName).rfind("gfx950", 0) != 0)
     fail(BD_ERR_DEVICE, std::string("this build targets gfx950, device is ") + prop.gcnArchName);
-  n = s.n;
-  nrep = s.nrep;
+  n = kb ? 2 * kb->nk : s.n;
+  nrep = kb ? kb->nk : s.nrep;
   for (int i = 0; i < 3; ++i) dims[i] = s.dims[i];
   N = int64_t(dims[0]) * dims[1] * dims[2];
   capacity = cap;
   precision = prec;
   gemm.bf16x3 = prec == BD_MIXED;
-  nproj = s.nproj;
-  plan = build_stick_plan(s);
+  nproj = kb ? 2 * s.nproj : s.nproj;
+  plan = kb ? build_k_stick_plan(s, *kb) : build_stick_plan(s);
   size_t lds_max = size_t(prop.sharedMemPerBlock);
   const size_t elem = precision == BD_MIXED ? sizeof(c2<float>) : sizeof(c2<double>);
   if (stick_lds(plan, elem) > lds_max || plane_lds(plan, elem) > lds_max)
@@ -438,12 +566,12 @@ GammaOp::GammaOp(const System& s, int cap, int prec) : sys(&s) {
   d_zc.alloc(plan.z_of_zc.size());
   d_zc.upload(plan.z_of_zc.data(), plan.z_of_zc.size(), stream);
   d_kinetic.alloc(n);
-  d_kinetic.upload(s.kinetic_orc.data(), n, stream);
+  d_kinetic.upload(kb ? kb->kinetic2.data() : s.kinetic_orc.data(), n, stream);
   d_potential.alloc(N);
   if (precision == BD_MIXED) d_potential32.alloc(N);
   if (nproj > 0) {
     d_proj.alloc(size_t(n) * nproj);
-    d_proj.upload(s.projectors_orc.data(), size_t(n) * nproj, stream);
+    d_proj.upload(kb ? kb->proj2.data() : s.projectors_orc.data(), size_t(n) * nproj, stream);
     nc = s.channel_count;
     nblocks = int(s.blocks.size());
     d_blockd.alloc(s.block_d.size());
@@ -452,7 +580,7 @@ GammaOp::GammaOp(const System& s, int cap, int prec) : sys(&s) {
     d_w.alloc(size_t(nproj) * cap);
     d_work.alloc(size_t(Gemm::tn_work(nproj, cap, n)));
   }
-  const int pairs = (cap + 1) / 2;
+  const int pairs = slots(cap);
   d_sticks.alloc(size_t(pairs) * plan.n0 * plan.Spad);
   // Host-pointer entry points stage through these (no allocation after create).
   d_x.alloc(size_t(n) * cap);
@@ -488,34 +616,47 @@ void GammaOp::refresh_potential32() {
 template <typename F>
 void GammaOp::local_apply(const double* x, int64_t ldx, int m, double* y, int64_t ldy) {
   const StickPlan& p = plan;
-  const int pairs = (m + 1) / 2;
+  const int pairs = slots(m);
   const size_t e = sizeof(c2<F>);
   c2<F>* sticks = reinterpret_cast<c2<F>*>(d_sticks.p);
   const F* pot;
   if constexpr (std::is_same_v<F, float>) pot = d_potential32.p;
   else pot = d_potential.p;
   if (profiling) BD_HIP(hipEventRecord(ev[0], stream));
-  hipLaunchKernelGGL(k_stick_inverse<F>, dim3(p.ntiles, pairs), dim3(kThreads), stick_lds(p, e), stream, x,
-                     ldx, m, p.n0, p.T, p.Spad, d_tile_off.p, d_entries.p, d_roots.p, p.rx, sticks);
+  if (kb)
+    hipLaunchKernelGGL(k_kstick_inverse<F>, dim3(p.ntiles, pairs), dim3(kThreads), stick_lds(p, e), stream, x,
+                       ldx, p.n0, p.T, p.Spad, d_tile_off.p, d_entries.p, d_roots.p, p.rx, sticks);
+  else
+    hipLaunchKernelGGL(k_stick_inverse<F>, dim3(p.ntiles, pairs), dim3(kThreads), stick_lds(p, e), stream, x,
+                       ldx, m, p.n0, p.T, p.Spad, d_tile_off.p, d_entries.p, d_roots.p, p.rx, sticks);
   if (profiling) BD_HIP(hipEventRecord(ev[1], stream));
   hipLaunchKernelGGL((k_plane<0, F>), dim3(pairs, p.n0), dim3(kThreads), plane_lds(p, e), stream, sticks,
                      p.n0, p.n1, p.n2, p.Spad, p.Z, p.chunk, d_stick_yz.p, d_zc.p, d_roots.p, p.ry,
                      p.rz, pot, pairs, nullptr, 0.0, nullptr);
   if (profiling) BD_HIP(hipEventRecord(ev[2], stream));
-  hipLaunchKernelGGL(k_stick_forward<F>, dim3(p.ntiles, pairs), dim3(kThreads), stick_lds(p, e), stream, x,
-                     ldx, y, ldy, m, p.n0, p.T, p.Spad, d_tile_off.p, d_entries.p, d_roots.p, p.rx,
-                     sticks, d_kinetic.p, 1.0 / double(N));
+  if (kb)
+    hipLaunchKernelGGL(k_kstick_forward<F>, dim3(p.ntiles, pairs), dim3(kThreads), stick_lds(p, e), stream, x,
+                       ldx, y, ldy, p.n0, p.T, p.Spad, d_tile_off.p, d_entries.p, d_roots.p, p.rx, sticks,
+                       d_kinetic.p, 1.0 / double(N));
+  else
+    hipLaunchKernelGGL(k_stick_forward<F>, dim3(p.ntiles, pairs), dim3(kThreads), stick_lds(p, e), stream, x,
+                       ldx, y, ldy, m, p.n0, p.T, p.Spad, d_tile_off.p, d_entries.p, d_roots.p, p.rx,
+                       sticks, d_kinetic.p, 1.0 / double(N));
   if (profiling) BD_HIP(hipEventRecord(ev[3], stream));
 }
 
 template <typename F>
 void GammaOp::density(const double* x, int64_t ldx, int m, const double* w, double* rho) {
   const StickPlan& p = plan;
-  const int pairs = (m + 1) / 2;
+  const int pairs = slots(m);
   const size_t e = sizeof(c2<F>);
   c2<F>* sticks = reinterpret_cast<c2<F>*>(d_sticks.p);
-  hipLaunchKernelGGL(k_stick_inverse<F>, dim3(p.ntiles, pairs), dim3(kThreads), stick_lds(p, e), stream, x,
-                     ldx, m, p.n0, p.T, p.Spad, d_tile_off.p, d_entries.p, d_roots.p, p.rx, sticks);
+  if (kb)
+    hipLaunchKernelGGL(k_kstick_inverse<F>, dim3(p.ntiles, pairs), dim3(kThreads), stick_lds(p, e), stream, x,
+                       ldx, p.n0, p.T, p.Spad, d_tile_off.p, d_entries.p, d_roots.p, p.rx, sticks);
+  else
+    hipLaunchKernelGGL(k_stick_inverse<F>, dim3(p.ntiles, pairs), dim3(kThreads), stick_lds(p, e), stream, x,
+                       ldx, m, p.n0, p.T, p.Spad, d_tile_off.p, d_entries.p, d_roots.p, p.rx, sticks);
   const int groups = std::min(pairs, kDensityGroups);
   hipLaunchKernelGGL((k_plane<1, F>), dim3(p.n0, groups), dim3(kThreads), plane_lds(p, e), stream, sticks,
                      p.n0, p.n1, p.n2, p.Spad, p.Z, p.chunk, d_stick_yz.p, d_zc.p, d_roots.p, p.ry, p.rz,
@@ -532,8 +673,10 @@ void GammaOp::apply(const double* x, int64_t ldx, int m, double* y, int64_t ldy)
   if (m < 0 || m > capacity) fail(BD_ERR_INVALID_PARAMETER, "column count exceeds capacity");
   if (ldx < n || ldy < n) fail(BD_ERR_INVALID_PARAMETER, "leading dimension smaller than rows");
   if (!has_potential) fail(BD_ERR_INVALID_PARAMETER, "no potential installed");
+  if (kb && !complex_aligned(x, ldx)) fail(BD_ERR_INVALID_PARAMETER, "k-point X must be 16-byte aligned, even ldx");
+  if (kb && !complex_aligned(y, ldy)) fail(BD_ERR_INVALID_PARAMETER, "k-point Y must be 16-byte aligned, even ldy");
   if (m == 0) return;
-  const int pairs = (m + 1) / 2;
+  const int pairs = slots(m);
   if (precision == BD_MIXED) local_apply<float>(x, ldx, m, y, ldy);
   else local_apply<double>(x, ldx, m, y, ldy);
   if (nproj > 0) {
@@ -564,6 +707,8 @@ void GammaOp::apply(const double* x, int64_t ldx, int m, double* y, int64_t ldy)
 
 void GammaOp::band_density(const double* x, int64_t ldx, int m, const double* w, double* rho) {
   if (m <= 0 || m > capacity) fail(BD_ERR_INVALID_PARAMETER, "column count");
+  if (ldx < n) fail(BD_ERR_INVALID_PARAMETER, "leading dimension smaller than rows");
+  if (kb && !complex_aligned(x, ldx)) fail(BD_ERR_INVALID_PARAMETER, "k-point X must be 16-byte aligned, even ldx");
   if (precision == BD_MIXED) density<float>(x, ldx, m, w, rho);
   else density<double>(x, ldx, m, w, rho);
   BD_HIP(hipGetLastError());
@@ -584,6 +729,50 @@ extern "C" bd_status bd_gamma_op_create(const bd_system* sys, int32_t capacity, 
   BD_GUARD({
     if (!sys || !out) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
     *out = new bd_gamma_op(bd::system_of(sys), capacity, precision);
+  });
+}
+
+extern "C" bd_status bd_k_op_create(const bd_system* sys, const double* k_frac, int32_t capacity,
+                                    int32_t precision, bd_gamma_op** out) {
+  BD_GUARD({
+    if (!sys || !k_frac || !out) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    const bd::System& s = bd::system_of(sys);
+    auto kb = std::make_unique<bd::KBasis>(s, k_frac);
+    *out = new bd_gamma_op(s, std::move(kb), capacity, precision);
+  });
+}
+
+extern "C" bd_status bd_k_op_info(const bd_gamma_op* op, int64_t* nk, double* k_frac3, double* k_cart3) {
+  BD_GUARD({
+    if (!op || !nk) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    if (!op->kbasis) bd::fail(BD_ERR_INVALID_PARAMETER, "not a k-point operator");
+    const auto& kb = *op->kbasis;
+    *nk = kb.nk;
+    for (int i = 0; i < 3; ++i) {
+      if (k_frac3) k_frac3[i] = kb.k_frac[i];
+      if (k_cart3) k_cart3[i] = kb.k_cart[i];
+    }
+  });
+}
+
+extern "C" bd_status bd_k_op_get(const bd_gamma_op* op, const char* name, void* out, int64_t count) {
+  BD_GUARD({
+    if (!op || !name || !out) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    if (!op->kbasis) bd::fail(BD_ERR_INVALID_PARAMETER, "not a k-point operator");
+    const auto& kb = *op->kbasis;
+    std::string k(name);
+    auto put = [&](const void* src, size_t elems, size_t bytes_each) {
+      if (int64_t(elems) != count) bd::fail(BD_ERR_INVALID_PARAMETER, "count mismatch for " + k);
+      std::memcpy(out, src, elems * bytes_each);
+    };
+    if (k == "miller") put(kb.miller.data(), kb.miller.size() * 3, 4);
+    else if (k == "g2") put(kb.g2.data(), kb.g2.size(), 8);
+    else if (k == "q2") put(kb.q2.data(), kb.q2.size(), 8);
+    else if (k == "fft_index") put(kb.flat.data(), kb.flat.size(), 8);
+    else if (k == "kinetic") put(kb.kinetic2.data(), kb.kinetic2.size(), 8);
+    else if (k == "diagonal_static") put(kb.diagonal2.data(), kb.diagonal2.size(), 8);
+    else if (k == "projectors") put(kb.proj2.data(), kb.proj2.size(), 8);
+    else bd::fail(BD_ERR_INVALID_PARAMETER, "unknown array " + k);
   });
 }
 
@@ -644,8 +833,13 @@ extern "C" bd_status bd_gamma_op_band_density_dev(bd_gamma_op* op, const double*
     if (!op || !x || !weights || !rho) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
     auto& o = op->op;
     if (m <= 0 || m > o.capacity) bd::fail(BD_ERR_INVALID_PARAMETER, "column count");
-    std::vector<double> hw(size_t(2 * ((m + 1) / 2)), 0.0);
-    std::copy(weights, weights + m, hw.begin());
+    // Per FFT slot (w_a, w_b): a column pair at Gamma; at k one complex
+    // column, whose |psi|^2 = Re^2 + Im^2 takes its weight twice.
+    std::vector<double> hw(size_t(2 * o.slots(m)), 0.0);
+    if (o.kb)
+      for (int j = 0; j < m; ++j) hw[2 * j] = hw[2 * j + 1] = weights[j];
+    else
+      std::copy(weights, weights + m, hw.begin());
     o.d_weights.upload(hw.data(), hw.size(), o.stream);
     o.band_density(x, ldx, m, o.d_weights.p, rho);
     BD_HIP(hipStreamSynchronize(o.stream));
@@ -665,14 +859,19 @@ extern "C" bd_status bd_gamma_op_nonlocal_energies_dev(bd_gamma_op* op, const do
     BD_HIP(hipMemcpyAsync(q.data(), o.d_q.p, q.size() * 8, hipMemcpyDeviceToHost, o.stream));
     BD_HIP(hipStreamSynchronize(o.stream));
     const auto& s = *o.sys;
+    // At k, rows [0, p) of q hold Re(P^H x) and rows [p, 2p) Im(P^H x); D is
+    // real, so <x|P D P^H|x> = Re^T D Re + Im^T D Im.
+    const int halves = o.kb ? 2 : 1;
     for (int b = 0; b < m; ++b) {
       double sum = 0.0;
-      for (auto& blk : s.blocks)
-        for (int j = 0; j < blk.count; ++j) {
-          double v = 0.0;
-          for (int k = 0; k < blk.count; ++k)
-            v += s.block_d[j * blk.count + k] * q[size_t(b) * o.nproj + blk.start + k];
-          sum += q[size_t(b) * o.nproj + blk.start + j] * v;
+      for (int h = 0; h < halves; ++h)
+        for (auto& blk : s.blocks) {
+          const double* qb = &q[size_t(b) * o.nproj + size_t(h) * s.nproj + blk.start];
+          for (int j = 0; j < blk.count; ++j) {
+            double v = 0.0;
+            for (int k = 0; k < blk.count; ++k) v += s.block_d[j * blk.count + k] * qb[k];
+            sum += qb[j] * v;
+          }
         }
       out[b] = sum;
     }

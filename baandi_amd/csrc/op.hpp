@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <vector>
 
 #include "fft_device.hpp"
@@ -46,7 +47,8 @@ struct StickPlan {
   int chunk;    // y rows per z-pass chunk in the plane kernel
   RadixPlan rx, ry, rz;
   std::vector<int> tile_off;       // ntiles+1 offsets into entries
-  std::vector<int4> entries;       // (rep, plus slot*n0+x, minus slot*n0+x, 0)
+  std::vector<int4> entries;       // Gamma: (rep, plus slot*n0+x, minus slot*n0+x, 0)
+                                   // k: (basis index, slot*n0+x, -1, 0)
   std::vector<int2> stick_yz;      // per padded stick: (i1, zc), (-1,-1) pad
   std::vector<int> z_of_zc;        // Z
 };
@@ -54,8 +56,12 @@ struct StickPlan {
 struct DavidsonWork;
 void destroy_davidson_work(DavidsonWork* w);
 
+// The H·X operator of one Gamma cell (ORC rows), or of one k point of it
+// (kb != nullptr: rows are interleaved (Re, Im) of complex coefficients, one
+// complex FFT per column instead of one per column pair).
 struct GammaOp {
   const System* sys;
+  const KBasis* kb = nullptr;
   DavidsonWork* dav = nullptr;  // Davidson workspace cached per (n, bands, cap)
   int64_t n, nrep, N;
   int dims[3];
@@ -86,7 +92,9 @@ struct GammaOp {
   double stage_total_ms[5] = {};
   double applies = 0.0, pairs_total = 0.0, columns_total = 0.0;
 
-  GammaOp(const System& s, int capacity, int precision);
+  GammaOp(const System& s, const KBasis* k, int capacity, int precision);
+  // FFT slots that m columns occupy: column pairs at Gamma, columns at k.
+  int slots(int m) const { return kb ? m : (m + 1) / 2; }
   ~GammaOp();
   void set_potential_dev(const double* v, uint64_t epoch);
   void apply(const double* x, int64_t ldx, int m, double* y, int64_t ldy);
@@ -100,12 +108,16 @@ struct GammaOp {
 };
 
 StickPlan build_stick_plan(const System& s);
+StickPlan build_k_stick_plan(const System& s, const KBasis& kb);
 RadixPlan radix_plan(int n);
 
 }  // namespace bd
 
-// The C ABI handle (include/baandi_amd.h) owns one operator.
+// The C ABI handle (include/baandi_amd.h) owns one operator, and at k its basis.
 struct bd_gamma_op {
+  std::unique_ptr<bd::KBasis> kbasis;
   bd::GammaOp op;
-  bd_gamma_op(const bd::System& s, int cap, int prec) : op(s, cap, prec) {}
+  bd_gamma_op(const bd::System& s, int cap, int prec) : op(s, nullptr, cap, prec) {}
+  bd_gamma_op(const bd::System& s, std::unique_ptr<bd::KBasis> k, int cap, int prec)
+      : kbasis(std::move(k)), op(s, kbasis.get(), cap, prec) {}
 };

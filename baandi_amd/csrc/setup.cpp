@@ -201,6 +201,7 @@ System::System(const bd_system_desc& d) {
     atoms.push_back(c);
   }
   electrons = d.z_valence * d.natoms;
+  ecut = d.ecut_ha;
 
   build_pseudo(d);
   // Basis and grid.
@@ -370,62 +371,97 @@ void System::build_half_sphere() {
   for (int64_t i = 1; i < nrep; ++i) kinetic_orc[2 * i - 1] = kinetic_orc[2 * i] = 0.5 * g2[plus[i]];
 }
 
-void System::build_projectors() {
-  using C = std::complex<double>;
-  const int npj = int(projs.size());
-  std::vector<std::array<int, 3>> channels;  // (projector, l, m)
-  for (int i = 0; i < npj; ++i)
-    for (int m = -projs[i].l; m <= projs[i].l; ++m) channels.push_back({i, projs[i].l, m});
+namespace {
+using Cplx = std::complex<double>;
+
+// Channels (projector, l, m), ordered by projector then m = -l..l.
+std::vector<std::array<int, 3>> channel_list(const System& s) {
+  std::vector<std::array<int, 3>> channels;
+  for (int i = 0; i < int(s.projs.size()); ++i)
+    for (int m = -s.projs[i].l; m <= s.projs[i].l; ++m) channels.push_back({i, s.projs[i].l, m});
+  return channels;
+}
+
+// Complex projector columns at the momenta q (hamiltonian.rs:930-1000):
+// (-i)^l Y_lm(q^) beta_i(|q|)/sqrt(Omega) e^{-i q.tau}, atom-major, each
+// column n long. Radial transforms are cached per exact |q|^2 bit pattern.
+std::vector<Cplx> projector_columns(const System& s, const std::vector<std::array<double, 3>>& q,
+                                    const std::vector<double>& q2) {
+  const auto channels = channel_list(s);
+  const int npj = int(s.projs.size());
   const int nc = int(channels.size());
-  nproj = int64_t(nc) * int64_t(atoms.size());
-  // Radial transforms per unique |G| (exact-key cache, hamiltonian.rs:940-955).
+  const int64_t n = int64_t(q2.size());
   std::unordered_map<uint64_t, int> slot;
   std::vector<double> radial;
   std::vector<int> gslot(n);
-  std::vector<double> buf(r.size());
+  std::vector<double> buf(s.r.size());
   for (int64_t g = 0; g < n; ++g) {
     uint64_t k;
-    std::memcpy(&k, &g2[g], 8);
+    std::memcpy(&k, &q2[g], 8);
     auto it = slot.find(k);
     if (it == slot.end()) {
-      int s = int(slot.size());
-      slot.emplace(k, s);
-      double q = std::sqrt(g2[g]);
+      int sl = int(slot.size());
+      slot.emplace(k, sl);
+      double qn = std::sqrt(q2[g]);
       for (int i = 0; i < npj; ++i) {
-        const auto& p = projs[i];
-        for (size_t t = 0; t < r.size(); ++t) buf[t] = r[t] * p.rbeta[t] * sph_bessel(p.l, q * r[t]) * rab[t];
-        radial.push_back(4.0 * kPi * simpson(buf.data(), r.size()));
+        const auto& p = s.projs[i];
+        for (size_t t = 0; t < s.r.size(); ++t)
+          buf[t] = s.r[t] * p.rbeta[t] * sph_bessel(p.l, qn * s.r[t]) * s.rab[t];
+        radial.push_back(4.0 * kPi * simpson(buf.data(), s.r.size()));
       }
-      gslot[g] = s;
+      gslot[g] = sl;
     } else {
       gslot[g] = it->second;
     }
   }
-  // Species table (-i)^l Y_lm beta(|G|)/sqrt(Omega) (hamiltonian.rs:957-965).
-  std::vector<C> table(size_t(nc) * n);
-  const double sv = std::sqrt(volume);
+  std::vector<Cplx> table(size_t(nc) * n);
+  const double sv = std::sqrt(s.volume);
   for (int64_t g = 0; g < n; ++g) {
-    double dir[3] = {cart[g][0], cart[g][1], cart[g][2]};
-    if (g2[g] == 0.0) dir[0] = 0.0, dir[1] = 0.0, dir[2] = 1.0;
+    double dir[3] = {q[g][0], q[g][1], q[g][2]};
+    if (q2[g] == 0.0) dir[0] = 0.0, dir[1] = 0.0, dir[2] = 1.0;
     for (int c = 0; c < nc; ++c) {
       int i = channels[c][0], l = channels[c][1], m = channels[c][2];
-      C il = l == 0 ? C(1, 0) : l == 1 ? C(0, -1) : l == 2 ? C(-1, 0) : C(0, 1);
+      Cplx il = l == 0 ? Cplx(1, 0) : l == 1 ? Cplx(0, -1) : l == 2 ? Cplx(-1, 0) : Cplx(0, 1);
       table[size_t(c) * n + g] = il * (real_ylm(l, m, dir) * radial[size_t(gslot[g]) * npj + i] / sv);
     }
   }
-  // Per-atom columns times e^{-iG.tau}, packed to ORC rows (:363-402).
-  projectors_orc.assign(size_t(n) * nproj, 0.0);
-  std::vector<C> full(size_t(n) * nproj);
-  for (size_t a = 0; a < atoms.size(); ++a) {
+  std::vector<Cplx> full(size_t(n) * nc * s.atoms.size());
+  for (size_t a = 0; a < s.atoms.size(); ++a) {
     for (int64_t g = 0; g < n; ++g) {
-      double ang = cart[g][0] * atoms[a][0] + cart[g][1] * atoms[a][1] + cart[g][2] * atoms[a][2];
-      C ph = std::polar(1.0, -ang);
+      double ang = q[g][0] * s.atoms[a][0] + q[g][1] * s.atoms[a][1] + q[g][2] * s.atoms[a][2];
+      Cplx ph = std::polar(1.0, -ang);
       for (int c = 0; c < nc; ++c) full[(a * nc + c) * n + g] = table[size_t(c) * n + g] * ph;
     }
   }
+  return full;
+}
+
+// T + sum_ij D_ij Re(b_i b_j*) of row g (refstart.rs:605-650 at Gamma,
+// distributed_complex.rs:29-54 at k).
+double static_row(const System& s, const std::vector<Cplx>& full, int64_t n, int64_t g, double kinetic) {
+  const int nc = s.channel_count;
+  double v = kinetic;
+  for (size_t a = 0; a < s.atoms.size(); ++a)
+    for (int j = 0; j < nc; ++j)
+      for (int i = 0; i < nc; ++i) {
+        Cplx bi = full[(a * nc + i) * n + g], bj = full[(a * nc + j) * n + g];
+        v += s.block_d[i + nc * j] * (bi * std::conj(bj)).real();
+      }
+  return v;
+}
+}  // namespace
+
+void System::build_projectors() {
+  const auto channels = channel_list(*this);
+  const int nc = int(channels.size());
+  const int npj = int(projs.size());
+  nproj = int64_t(nc) * int64_t(atoms.size());
+  std::vector<Cplx> full = projector_columns(*this, cart, g2);
+  // Per-atom columns packed to ORC rows (distributed_real_gamma.rs:363-402).
+  projectors_orc.assign(size_t(n) * nproj, 0.0);
   const double s2 = std::sqrt(2.0);
   for (int64_t col = 0; col < nproj; ++col) {
-    const C* f = &full[size_t(col) * n];
+    const Cplx* f = &full[size_t(col) * n];
     double* o = &projectors_orc[size_t(col) * n];
     o[0] = f[0].real();
     for (int64_t i = 1; i < nrep; ++i) {
@@ -451,18 +487,9 @@ void System::build_projectors() {
   // Davidson diagonal T + sum_ij D_ij Re(b_i b_j*) per representative
   // (refstart.rs:605-650), shared by its two ORC rows.
   diagonal_static.resize(n);
-  auto rep_value = [&](int64_t g) {
-    double v = 0.5 * g2[g];
-    for (size_t a = 0; a < atoms.size(); ++a)
-      for (int j = 0; j < nc; ++j)
-        for (int i = 0; i < nc; ++i) {
-          C bi = full[(a * nc + i) * n + g], bj = full[(a * nc + j) * n + g];
-          v += block[i + nc * j] * (bi * std::conj(bj)).real();
-        }
-    return v;
-  };
-  diagonal_static[0] = rep_value(0);
-  for (int64_t i = 1; i < nrep; ++i) diagonal_static[2 * i - 1] = diagonal_static[2 * i] = rep_value(plus[i]);
+  diagonal_static[0] = static_row(*this, full, n, 0, 0.5 * g2[0]);
+  for (int64_t i = 1; i < nrep; ++i)
+    diagonal_static[2 * i - 1] = diagonal_static[2 * i] = static_row(*this, full, n, plus[i], 0.5 * g2[plus[i]]);
 }
 
 void System::build_ionic() {
@@ -503,6 +530,81 @@ void System::build_ionic() {
     }
     vion_re[g] = v * sr;
     vion_im[g] = v * si;
+  }
+}
+
+KBasis::KBasis(const System& s, const double k_fractional[3]) {
+  // k wrapped into [-1/2, 1/2) (crates/dft-geometry/src/kpoints.rs:21-32).
+  for (int i = 0; i < 3; ++i) {
+    double x = k_fractional[i];
+    if (!std::isfinite(x)) fail(BD_ERR_INVALID_PARAMETER, "k point must be finite");
+    k_frac[i] = (x >= -0.5 && x < 0.5) ? x : x - std::floor(x + 0.5);
+  }
+  for (int i = 0; i < 3; ++i)
+    k_cart[i] = ((s.B[i][0] * k_frac[0]) + s.B[i][1] * k_frac[1]) + s.B[i][2] * k_frac[2];
+  // PlaneWaveBasis::at_k (crates/dft-basis/src/lib.rs:234-269, inclusion
+  // test :99-120): |k+G|^2/2 <= Ecut (1 + 1e-12), order by |G|^2 then Miller.
+  const double ecut = s.ecut;
+  double radius = std::sqrt(2.0 * ecut * (1.0 + kCutoffTol));
+  int bounds[3];
+  for (int i = 0; i < 3; ++i) {
+    double len = std::hypot(std::hypot(s.lat[0][i], s.lat[1][i]), s.lat[2][i]);
+    bounds[i] = int(std::ceil(len * radius / kTau) + 1.0) + 1;
+  }
+  struct Entry {
+    std::array<int32_t, 3> m;
+    std::array<double, 3> c;
+    double g2;
+  };
+  std::vector<Entry> v;
+  const double limit = ecut * (1.0 + kCutoffTol);
+  for (int m0 = -bounds[0]; m0 <= bounds[0]; ++m0)
+    for (int m1 = -bounds[1]; m1 <= bounds[1]; ++m1)
+      for (int m2 = -bounds[2]; m2 <= bounds[2]; ++m2) {
+        std::array<double, 3> c;
+        for (int i = 0; i < 3; ++i)
+          c[i] = ((s.B[i][0] * double(m0)) + s.B[i][1] * double(m1)) + s.B[i][2] * double(m2);
+        double g2v = (c[0] * c[0] + c[1] * c[1]) + c[2] * c[2];
+        double a = c[0] + k_cart[0], b = c[1] + k_cart[1], d = c[2] + k_cart[2];
+        double q2v = (a * a + b * b) + d * d;
+        if (0.5 * q2v <= limit) v.push_back({{m0, m1, m2}, c, g2v});
+      }
+  std::sort(v.begin(), v.end(), [](const Entry& a, const Entry& b) {
+    if (a.g2 != b.g2) return a.g2 < b.g2;
+    return a.m < b.m;
+  });
+  nk = int64_t(v.size());
+  if (nk == 0) fail(BD_ERR_INVALID_PARAMETER, "empty basis at this k point");
+  std::vector<std::array<double, 3>> q(nk);
+  for (int64_t g = 0; g < nk; ++g) {
+    miller.push_back(v[g].m);
+    g2.push_back(v[g].g2);
+    for (int i = 0; i < 3; ++i) q[g][i] = v[g].c[i] + k_cart[i];
+    q2.push_back((q[g][0] * q[g][0] + q[g][1] * q[g][1]) + q[g][2] * q[g][2]);
+    flat.push_back(flat_of(v[g].m, s.dims));
+  }
+  kinetic2.resize(2 * nk);
+  for (int64_t g = 0; g < nk; ++g) kinetic2[2 * g] = kinetic2[2 * g + 1] = 0.5 * q2[g];
+  const int64_t p = s.nproj;
+  proj2.assign(size_t(2 * nk) * size_t(2 * p), 0.0);
+  diagonal2.resize(2 * nk);
+  if (p > 0) {
+    std::vector<Cplx> full = projector_columns(s, q, q2);
+    const int64_t rows = 2 * nk;
+    for (int64_t c = 0; c < p; ++c) {
+      const Cplx* f = &full[size_t(c) * nk];
+      double* a = &proj2[size_t(c) * rows];
+      double* b = &proj2[size_t(p + c) * rows];
+      for (int64_t g = 0; g < nk; ++g) {
+        a[2 * g] = f[g].real();
+        a[2 * g + 1] = f[g].imag();
+        b[2 * g] = -f[g].imag();
+        b[2 * g + 1] = f[g].real();
+      }
+    }
+    for (int64_t g = 0; g < nk; ++g) diagonal2[2 * g] = diagonal2[2 * g + 1] = static_row(s, full, nk, g, 0.5 * q2[g]);
+  } else {
+    for (int64_t g = 0; g < nk; ++g) diagonal2[2 * g] = diagonal2[2 * g + 1] = 0.5 * q2[g];
   }
 }
 

@@ -16,6 +16,7 @@ struct System {
   double volume = 0.0;
   std::vector<std::array<double, 3>> atoms;  // Cartesian, Bohr
   double electrons = 0.0;
+  double ecut = 0.0;  // Ha
   // Synthetic pseudopotential on the log mesh (Ha).
   double z = 0.0, rc = 1.0;
   std::vector<double> r, rab, local_ha;
@@ -60,6 +61,25 @@ struct System {
   void build_projectors();
   void build_ionic();
   void build_ewald();
+};
+
+// The Bloch basis of one k point of a System (k != 0 in general): the same
+// cell, grid, pseudopotential and D blocks; complex coefficients stored as
+// interleaved (Re, Im) real rows, 2 nk of them.
+struct KBasis {
+  double k_frac[3];  // wrapped into [-1/2, 1/2)
+  double k_cart[3];  // Bohr^-1
+  int64_t nk = 0;
+  std::vector<std::array<int32_t, 3>> miller;  // |G|^2 then Miller order
+  std::vector<double> g2, q2;                  // |G|^2 and |k+G|^2
+  std::vector<int64_t> flat;                   // FFT index of each G
+  std::vector<double> kinetic2;                // |k+G|^2/2 on both rows of a coefficient
+  // Projectors as a real 2nk x 2p block (col-major): column c holds
+  // (Re P_c, Im P_c) interleaved, column p + c holds (-Im P_c, Re P_c), so
+  // that P2^T x gives (Re, Im) of P^H x and P2 w applies P to a complex w.
+  std::vector<double> proj2;
+  std::vector<double> diagonal2;  // static Davidson diagonal on both rows
+  KBasis(const System& s, const double k_fractional[3]);
 };
 
 }  // namespace bd

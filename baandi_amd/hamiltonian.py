@@ -116,21 +116,45 @@ class DistributedRealHamiltonian:
     def global_len(self):
         return self.system.n
 
+    def torch_stream(self):
+        """The op's HIP stream as a torch stream object."""
+        import torch
+        if getattr(self, "_torch_stream", None) is None:
+            self._torch_stream = torch.cuda.ExternalStream(self.stream())
+        return self._torch_stream
+
+    def enter(self):
+        """Order the op's stream after the work torch has queued so far (the
+        tensors handed to the op are complete before its kernels read them)."""
+        import torch
+        self.torch_stream().wait_stream(torch.cuda.current_stream())
+
+    def leave(self):
+        """Order torch's current stream after the op's queued work."""
+        import torch
+        torch.cuda.current_stream().wait_stream(self.torch_stream())
+
     def set_owned_potential(self, values, epoch=None):
         """V(r) on the full row-major grid (Ha): a CUDA float64 tensor or numpy array."""
         self.epoch = self.epoch + 1 if epoch is None else int(epoch)
         lib = L.load()
         if hasattr(values, "is_cuda") and values.is_cuda:
-            L.check(lib.bd_gamma_op_set_potential_dev(self.handle, L.ptr(values.contiguous()), self.epoch))
+            v = values.contiguous()
+            self.enter()
+            L.check(lib.bd_gamma_op_set_potential_dev(self.handle, L.ptr(v), self.epoch))
+            self.leave()
         else:
             v = np.ascontiguousarray(np.asarray(values, dtype=np.float64).reshape(-1))
             L.check(lib.bd_gamma_op_set_potential(self.handle, L.ptr(v), self.epoch))
 
     def apply_block(self, x, y):
-        """y = H x for CUDA float64 tensors of shape (m, n); y is overwritten."""
+        """y = H x for CUDA float64 tensors of shape (m, n); y is overwritten.
+        Ordered with torch's current stream on both sides."""
         m, n = x.shape
-        assert n == self.system.n and y.shape == x.shape and x.is_contiguous() and y.is_contiguous()
+        assert n == self.global_len() and y.shape == x.shape and x.is_contiguous() and y.is_contiguous()
+        self.enter()
         L.check(L.load().bd_gamma_op_apply_dev(self.handle, L.ptr(x), n, m, L.ptr(y), n))
+        self.leave()
 
     def apply_block_host(self, x):
         """numpy (n, m) column-major convenience: returns H x."""
@@ -142,11 +166,14 @@ class DistributedRealHamiltonian:
     def band_density_sum(self, x, weights, out):
         """out(grid) = sum_b w_b psi_b(r)^2 / volume for the rows of x."""
         w = np.ascontiguousarray(np.asarray(weights, dtype=np.float64))
+        self.enter()
         L.check(L.load().bd_gamma_op_band_density_dev(self.handle, L.ptr(x), x.shape[1], x.shape[0],
                                                        L.ptr(w), L.ptr(out)))
+        self.leave()
 
     def nonlocal_band_energies(self, x):
         out = np.empty(x.shape[0])
+        self.enter()
         L.check(L.load().bd_gamma_op_nonlocal_energies_dev(self.handle, L.ptr(x), x.shape[1], x.shape[0],
                                                             L.ptr(out)))
         return out

@@ -1,0 +1,400 @@
+"""Host-side mirror of the reference k-point path.
+
+``KPointHamiltonian`` is DistributedComplexHamiltonian
+(crates/dft-physics/src/distributed_complex.rs:88-386) at one k point of a
+``GammaSystem`` cell, built natively (bd_k_op_create). Complex coefficients
+live as interleaved (Re, Im) float64 rows, 2 nk per column: a CUDA complex128
+tensor of shape (m, nk) is the same memory as ``torch.view_as_real(x)``
+reshaped to (m, 2 nk), which is what the kernels read.
+
+``solve`` is ref_davidson_complex::solve_with_initial_hx
+(crates/dft-solvers/src/ref_davidson_complex.rs:641-1113) run as the GPU
+block Davidson on the realified operator: the start block is the pairs
+(x_j, i x_j), so the real search space stays closed under multiplication by
+i and its Ritz values are the complex ones, each twice. The complex
+eigenvectors are read back from the Ritz pairs by a complex Gram-Schmidt in
+eigenvalue order.
+
+``monkhorst_pack`` and ``fermi_dirac`` follow crates/dft-geometry/src/kpoints.rs
+and crates/dft-physics/src/occupations.rs:57-232; ``run_scf_k`` is the
+sequential-k map (crates/dft-engine/src/refscf/k_map.rs) over the Gamma
+driver's density field and mixer.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import lib as L
+from . import ref_davidson
+from .hamiltonian import DistributedRealHamiltonian, GammaSystem
+
+
+def wrap_fractional(k):
+    """kpoints.rs:21-32: each component into [-1/2, 1/2)."""
+    out = []
+    for x in k:
+        x = float(x)
+        if not math.isfinite(x):
+            raise ValueError("k point must be finite")
+        out.append(x if -0.5 <= x < 0.5 else x - math.floor(x + 0.5))
+    return tuple(out)
+
+
+def monkhorst_pack(mesh, shift=(0.0, 0.0, 0.0), time_reversal=True):
+    """Unshifted Monkhorst-Pack points (i + 1/2 + shift)/n - 1/2 per axis,
+    wrapped, weights 1/(n0 n1 n2); k and -k merged when time_reversal (the
+    first of the two, in mesh order, keeps the summed weight)."""
+    pts = []
+    for i in range(mesh[0]):
+        for j in range(mesh[1]):
+            for l in range(mesh[2]):
+                pts.append(wrap_fractional([(c + s + 0.5) / n - 0.5 for c, s, n in zip((i, j, l), shift, mesh)]))
+    weight = 1.0 / len(pts)
+    merged = {}
+    for p in pts:
+        key = tuple(round(x * 1e12) for x in p)
+        if time_reversal:
+            neg = tuple(round(x * 1e12) for x in wrap_fractional([-x for x in p]))
+            if neg in merged:
+                merged[neg][1] += weight
+                continue
+        if key in merged:
+            merged[key][1] += weight
+        else:
+            merged[key] = [p, weight]
+    return [(tuple(v[0]), v[1]) for v in merged.values()]
+
+
+def _fraction_entropy(x):
+    """occupations.rs:57-66: fraction and entropy from x = (e - mu)/theta,
+    both tails through exp(-|x|)."""
+    a = np.abs(x)
+    t = np.exp(-a)
+    q = t / (1.0 + t)
+    p = np.where(x >= 0.0, q, 1.0 / (1.0 + t))
+    s = np.log1p(t) + np.where(np.isinf(a), 0.0, a * q)
+    return p, s
+
+
+def fermi_dirac(energies, weights, electrons, theta, capacity=2.0, tolerance=1e-13, max_iterations=256):
+    """Occupations (capacity per level), chemical potential and TS (Ha) of
+    energies (nk, nb) with k weights: bracket mu by doubling, bisect to a
+    relative electron-count tolerance (occupations.rs:71-232)."""
+    e = np.asarray(energies, dtype=np.float64)
+    w = np.asarray(weights, dtype=np.float64)[:, None]
+
+    def number(mu):
+        return float((capacity * w * _fraction_entropy((e - mu) / theta)[0]).sum())
+
+    width = theta
+    for _ in range(max_iterations):
+        lo, hi = float(e.min()) - width, float(e.max()) + width
+        if number(lo) <= electrons <= number(hi):
+            break
+        width *= 2.0
+    else:
+        raise L.BaandiError(L.BD_ERR_APPLICATION, "unable to bracket the chemical potential")
+    mu = None
+    for _ in range(max_iterations):
+        mid = 0.5 * lo + 0.5 * hi
+        count = number(mid)
+        if abs(count - electrons) <= tolerance * electrons:
+            mu = mid
+            break
+        if mid == lo or mid == hi:
+            break
+        if count < electrons:
+            lo = mid
+        else:
+            hi = mid
+    if mu is None:
+        raise L.BaandiError(L.BD_ERR_APPLICATION, "electron count did not converge")
+    p, s = _fraction_entropy((e - mu) / theta)
+    return capacity * p, mu, theta * float((capacity * w * s).sum())
+
+
+class KPointHamiltonian(DistributedRealHamiltonian):
+    """H_k on one GPU. Blocks are CUDA tensors of shape (m, 2 nk) float64
+    (realified) or (m, nk) complex128 through the *_complex helpers."""
+
+    def __init__(self, system: GammaSystem, k_frac, capacity, precision=L.BD_FP64):
+        lib = L.load()
+        self.system = system
+        self.capacity = int(capacity)
+        k = np.ascontiguousarray(np.asarray(k_frac, dtype=np.float64).reshape(3))
+        h = C.c_void_p()
+        L.check(lib.bd_k_op_create(system.handle, L.ptr(k), self.capacity, int(precision), C.byref(h)))
+        self.handle = h
+        self.epoch = 0
+        nk = C.c_int64(0)
+        kf, kc = np.zeros(3), np.zeros(3)
+        L.check(lib.bd_k_op_info(h, C.byref(nk), L.ptr(kf), L.ptr(kc)))
+        self.nk = int(nk.value)
+        self.k_frac = tuple(float(v) for v in kf)
+        self.k_cart = kc
+
+    def global_len(self):
+        return 2 * self.nk
+
+    def get(self, name):
+        p = self.system.nproj
+        sizes = {"miller": (3 * self.nk, np.int32), "g2": (self.nk, np.float64), "q2": (self.nk, np.float64),
+                 "fft_index": (self.nk, np.int64), "kinetic": (2 * self.nk, np.float64),
+                 "diagonal_static": (2 * self.nk, np.float64),
+                 "projectors": (4 * self.nk * p, np.float64)}
+        count, dtype = sizes[name]
+        out = np.empty(count, dtype=dtype)
+        L.check(L.load().bd_k_op_get(self.handle, name.encode(), out.ctypes.data_as(C.c_void_p), count))
+        if name == "miller":
+            out = out.reshape(-1, 3)
+        if name == "projectors":
+            out = out.reshape(2 * p, 2 * self.nk).T
+        return out
+
+    def projectors_complex(self):
+        """The complex n x p projector table P (columns at q = k+G)."""
+        p2 = self.get("projectors")
+        p = self.system.nproj
+        return p2[0::2, :p] + 1j * p2[1::2, :p]
+
+    def apply_complex(self, x):
+        """H x for a CUDA complex128 tensor (m, nk)."""
+        import torch
+        xr = torch.view_as_real(x.contiguous()).reshape(x.shape[0], -1)
+        y = torch.empty_like(xr)
+        self.apply_block(xr, y)
+        return torch.view_as_complex(y.view(x.shape[0], self.nk, 2))
+
+
+def as_real(x):
+    """(m, nk) complex128 -> (m, 2 nk) float64 view of the same memory."""
+    import torch
+    return torch.view_as_real(x.contiguous()).reshape(x.shape[0], -1)
+
+
+def as_complex(x):
+    import torch
+    return torch.view_as_complex(x.contiguous().view(x.shape[0], -1, 2))
+
+
+def realified_start(x):
+    """Rows (x_j, i x_j) of a complex start block (bands, nk): the real start of
+    the realified Davidson (2 bands, 2 nk)."""
+    import torch
+    b, nk = x.shape
+    pairs = torch.stack([x, 1j * x], dim=1).reshape(2 * b, nk)
+    return as_real(pairs).contiguous()
+
+
+def complex_ritz(values, vectors, bands, keep=0.5):
+    """Complex eigenpairs from the realified Ritz block: vectors (2 bands, 2 nk)
+    in ascending eigenvalue order are read as complex rows and orthonormalised
+    by a two-pass complex Gram-Schmidt; a row whose remainder has norm <= keep
+    lies in the complex span of the rows kept before it (its partner i x) and
+    is dropped. Returns (values (bands,), vectors (bands, nk) complex128)."""
+    import torch
+    z = as_complex(vectors)
+    kept, vals = [], []
+    for j in range(z.shape[0]):
+        v = z[j].clone()
+        if kept:
+            q = torch.stack(kept)
+            for _ in range(2):
+                v = v - (q.conj() @ v) @ q
+        nv = float(torch.linalg.vector_norm(v))
+        if nv <= keep:
+            continue
+        kept.append(v / nv)
+        vals.append(float(values[j]))
+        if len(kept) == bands:
+            break
+    if len(kept) != bands:
+        raise L.BaandiError(L.BD_ERR_APPLICATION,
+                            f"realified Ritz block spans {len(kept)} complex vectors, expected {bands}")
+    return np.array(vals), torch.stack(kept)
+
+
+@dataclass
+class KSolution:
+    eigenvalues: np.ndarray   # (bands,) complex-problem eigenvalues, Ha
+    eigenvectors: object      # (bands, nk) complex128 CUDA
+    realified: object         # ref_davidson.Solution of the realified problem (2 bands)
+    converged: np.ndarray
+    record: dict
+
+
+def solve(operator: KPointHamiltonian, start, hamiltonian_diagonal, options: ref_davidson.Options):
+    """Lowest ``options.bands`` eigenpairs of H_k from a complex start block
+    (bands, nk). ``hamiltonian_diagonal``: (2 nk,) per row. The options are the
+    complex problem's; the realified solve runs with twice the bands,
+    occupied count and capacity."""
+    b = options.bands
+    ro = ref_davidson.Options(bands=2 * b, threshold=options.threshold, occupied=2 * options.occupied,
+                              first_step=options.first_step, capacity=2 * options.capacity,
+                              max_iterations=options.max_iterations, return_hx=options.return_hx,
+                              preconditioner=options.preconditioner, mlp=options.mlp, mlp_vbar=options.mlp_vbar)
+    sol = ref_davidson.solve(operator, realified_start(start), hamiltonian_diagonal.contiguous(), ro)
+    vals, vecs = complex_ritz(sol.eigenvalues, sol.eigenvectors, b)
+    conv = np.logical_and(sol.converged[0::2], sol.converged[1::2])
+    return KSolution(vals, vecs, sol, conv, sol.record)
+
+
+@dataclass
+class KScfOptions:
+    bands: int                      # complex bands per k point
+    theta: float = 0.01             # Fermi-Dirac smearing, Ha
+    threshold_ry: float = 1e-8      # density residual (refscf/profile.rs:17-26)
+    beta: float = 0.7
+    history: int = 8
+    max_iterations: int = 100
+    davidson_max_iterations: int = 100
+    preconditioner: int = L.BD_PRECOND_TPA
+    precision: int = L.BD_FP64
+    seed: int = 1
+
+
+def _complex_start(nk, bands, seed):
+    """Complex N(0,1) columns, each normalised (seeded, on the host)."""
+    import torch
+    g = torch.Generator(device="cpu").manual_seed(int(seed))
+    x = torch.complex(torch.randn(bands, nk, generator=g, dtype=torch.float64),
+                      torch.randn(bands, nk, generator=g, dtype=torch.float64))
+    return x / torch.linalg.vector_norm(x, dim=1, keepdim=True)
+
+
+def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
+    """Sequential-k SCF (refscf/k_map.rs) to dr2 < threshold_ry with
+    Fermi-Dirac occupations; returns the Mermin free energy terms (Ha),
+    eigenvalues (nk, bands), occupations, mu and timings.
+
+    Each k point's Davidson is the realified one (2 bands real columns, the
+    pairs (x, i x) of the complex start). A real Ritz level holds one
+    electron (its pair partner the other), so the occupations, the density
+    weights and the entropy are taken over the real levels with capacity 1."""
+    import torch
+    from .scf import DensityField, Mixer, lda_pz_torch, next_ethr_ry
+    dev = torch.device("cuda")
+    t_setup = time.perf_counter()
+    b = int(options.bands)
+    electrons = system.electrons
+    weights = np.array([w for _, w in kpoints], dtype=np.float64)
+    field = DensityField(system)
+    vion_g = torch.from_numpy(system.get("ionic_density_re") + 1j * system.get("ionic_density_im")).to(dev)
+    vion = field.to_grid(vion_g).real.contiguous()
+    ops = [KPointHamiltonian(system, k, capacity=2 * b, precision=options.precision) for k, _ in kpoints]
+    diag_static = [torch.from_numpy(op.get("diagonal_static")).to(dev) for op in ops]
+    kinetic = [torch.from_numpy(op.get("kinetic")).to(dev) for op in ops]
+    ewald = float(system.get("ewald")[0])
+    atoms = system.get("atoms_cart")
+    rows = np.array(system.lattice_rows)
+    recip = 2.0 * math.pi * np.linalg.inv(rows).T
+    dcart = system.get("density_miller").astype(np.float64) @ recip
+    sfac = np.exp(-1j * (dcart @ atoms.T)).sum(axis=1)
+    x = system.z_valence / system.volume * np.exp(-field.g2 * system.rc ** 2 / 4.0) * sfac
+    x[field.zero_np] = electrons / system.volume
+    x = torch.from_numpy(x).to(dev)
+    mixer = Mixer(field, options.beta, options.history)
+    orbitals = [realified_start(_complex_start(op.nk, b, options.seed + i).to(dev)) for i, op in enumerate(ops)]
+    outs = [torch.empty_like(o) for o in orbitals]
+    rho_dev = torch.empty(field.size, dtype=torch.float64, device=dev)
+    rho_k = torch.empty_like(rho_dev)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t_setup
+
+    t0 = time.perf_counter()
+    ethr = 1e-2
+    last_dr2 = math.inf
+    history = []
+    totals = dict(h_columns=0, davidson_iterations=0, dense_s=0.0, apply_s=0.0, davidson_s=0.0, fields_s=0.0)
+    result = None
+    for iteration in range(1, options.max_iterations + 1):
+        if iteration > 1:
+            ethr = next_ethr_ry(1e-2 if iteration == 2 else ethr, last_dr2, electrons)
+        redo = False
+        while True:
+            tf = time.perf_counter()
+            vh_g = field.coulomb * x
+            both = field.to_grid(x + 1j * vh_g)
+            rho_in, vh = both.real, both.imag
+            _, vxc = lda_pz_torch(rho_in.abs())
+            potential = (vion + vh + vxc).reshape(-1).contiguous()
+            vmean = float(potential.mean())  # also orders torch's work before the ops' streams
+            for op in ops:
+                op.set_owned_potential(potential)
+            totals["fields_s"] += time.perf_counter() - tf
+            td = time.perf_counter()
+            sols = []
+            for i, op in enumerate(ops):
+                opts = ref_davidson.Options(bands=2 * b, threshold=ethr / 2.0, occupied=2 * b,
+                                            first_step=(iteration == 1 and not redo),
+                                            max_iterations=options.davidson_max_iterations,
+                                            preconditioner=options.preconditioner)
+                sol = ref_davidson.solve(op, orbitals[i], diag_static[i] + vmean, opts, out=(outs[i], None))
+                sols.append(sol)
+                totals["h_columns"] += sol.record["h_columns"]
+                totals["davidson_iterations"] += sol.record["iterations"]
+                totals["dense_s"] += sol.record["seconds_dense"]
+                totals["apply_s"] += sol.record["seconds_apply"]
+            totals["davidson_s"] += time.perf_counter() - td
+            tf = time.perf_counter()
+            levels = np.stack([s.eigenvalues for s in sols])
+            occ, mu, ts = fermi_dirac(levels, weights, electrons, options.theta, capacity=1.0)
+            rho_dev.zero_()
+            for i, op in enumerate(ops):
+                op.band_density_sum(outs[i], weights[i] * occ[i], rho_k)
+                rho_dev += rho_k
+            rho_out = rho_dev.view(field.dims)
+            y = field.from_grid(rho_out)
+            y[field.zero] = electrons / system.volume
+            f = y - x
+            dr2 = field.dot(f, f)
+            totals["fields_s"] += time.perf_counter() - tf
+            if iteration == 1 and not redo and dr2 < ethr * max(electrons, 1.0):
+                ethr = next_ethr_ry(ethr, dr2, electrons)
+                redo = True
+                orbitals, outs = outs, orbitals
+                continue
+            break
+        orbitals, outs = outs, orbitals
+        last_dr2 = dr2
+        unconverged = sum(int((~s.converged).sum()) for s in sols)
+        history.append(dict(iteration=iteration, dr2_ry=dr2, ethr_ry=ethr, mu=mu,
+                            davidson_iterations=sum(s.record["iterations"] for s in sols),
+                            h_columns=sum(s.record["h_columns"] for s in sols), unconverged_levels=unconverged))
+        if log:
+            log(f"kscf {iteration:3d} dr2={dr2:.3e} ethr={ethr:.2e} mu={mu:.8f} "
+                f"hcols={history[-1]['h_columns']} unconverged={unconverged}")
+        if dr2 < options.threshold_ry:
+            result = (levels, occ, mu, ts, rho_out)
+            break
+        x = mixer.update(x, f)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    if result is None:
+        raise RuntimeError(f"k SCF did not reach dr2 < {options.threshold_ry} Ry in {options.max_iterations} maps")
+    levels, occ, mu, ts, rho_out = result
+    kin = nl = 0.0
+    for i, op in enumerate(ops):
+        w = weights[i] * occ[i]
+        xo = orbitals[i]
+        kin += float(((xo * xo * kinetic[i][None, :]).sum(dim=1).cpu().numpy() * w).sum())
+        nl += float(np.dot(op.nonlocal_band_energies(xo), w))
+    dv = system.volume / field.size
+    yo = field.from_grid(rho_out)
+    vh_out = field.to_grid(field.coulomb * yo).real
+    hartree = 0.5 * dv * float((rho_out * vh_out).sum())
+    eps_o, _ = lda_pz_torch(rho_out.abs())
+    xc = dv * float((rho_out.abs() * eps_o).sum())
+    local = dv * float((rho_out * vion).sum())
+    total = kin + nl + local + hartree + xc + ewald - ts
+    energy = {"kinetic": kin, "nonlocal": nl, "local": local, "hartree": hartree, "xc": xc, "ewald": ewald,
+              "entropy_term": ts, "total": total}
+    eig = 0.5 * (levels[:, 0::2] + levels[:, 1::2])
+    return dict(energy=energy, eigenvalues=eig, levels=levels, occupations=occ[:, 0::2] + occ[:, 1::2], mu=mu,
+                history=history, iterations=len(history), wall_s=wall, setup_s=t_setup, totals=totals,
+                kpoints=kpoints, ops=ops, orbitals=orbitals)

@@ -22,6 +22,7 @@ EXPORTS = [
     "bd_gamma_op_set_potential_dev", "bd_gamma_op_apply", "bd_gamma_op_apply_dev",
     "bd_gamma_op_band_density_dev", "bd_gamma_op_nonlocal_energies_dev", "bd_gamma_op_stream",
     "bd_gamma_op_set_profiling", "bd_gamma_op_stage_times", "bd_gamma_op_plan_info", "bd_davidson_solve_dev",
+    "bd_k_op_create", "bd_k_op_info", "bd_k_op_get",
     "bd_gemm_workspace", "bd_gemm_dev", "bd_mfma_probe_dev",
     "bd_sym_eigen_workspace", "bd_sym_eigen_dev", "bd_inverse_cholesky_dev",
     "bd_mlp_param_count", "bd_mlp_blob_floats", "bd_mlp_quantize_dev", "bd_mlp_forward_dev",
@@ -87,6 +88,10 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: run `python -m baandi_amd.build` (no CPU fallback)")
+    # torch ships its own libamdhip64.so.7. Loading it first makes the library
+    # bind to that same runtime (one HIP runtime per process); loaded the other
+    # way round, torch finds no device behind the system runtime.
+    import torch  # noqa: F401
     lib = C.CDLL(LIB_PATH)
     P, I32, I64, D, U64, F = C.c_void_p, C.c_int32, C.c_int64, C.c_double, C.c_uint64, C.c_float
     dp = C.POINTER(C.c_double)
@@ -96,6 +101,9 @@ def load():
         "bd_system_get_info": [P, C.POINTER(SystemInfo)],
         "bd_system_get": [P, C.c_char_p, P, I64],
         "bd_gamma_op_create": [P, I32, I32, C.POINTER(P)],
+        "bd_k_op_create": [P, P, I32, I32, C.POINTER(P)],
+        "bd_k_op_info": [P, P, P, P],
+        "bd_k_op_get": [P, C.c_char_p, P, I64],
         "bd_gamma_op_set_potential": [P, P, U64],
         "bd_gamma_op_set_potential_dev": [P, P, U64],
         "bd_gamma_op_apply": [P, P, I64, I32, P, I64],

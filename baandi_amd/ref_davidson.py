@@ -78,10 +78,12 @@ def solve_with_initial_hx(operator, start, hamiltonian_diagonal, options: Option
     eig = np.zeros(nb)
     conv = np.zeros(nb, dtype=np.int32)
     rec = L.DavidsonRecord()
+    operator.enter()
     st = L.load().bd_davidson_solve_dev(
         operator.handle, L.ptr(start), n, L.ptr(initial_hx) if initial_hx is not None else None,
         L.ptr(diag), C.byref(o), L.ptr(eig), L.ptr(x), L.ptr(hx) if hx is not None else None, n,
         L.ptr(conv), C.byref(rec))
+    operator.leave()
     L.check(st, allow=(L.BD_NOT_CONVERGED,))
     record = {f: getattr(rec, f) for f, _ in L.DavidsonRecord._fields_}
     sol = Solution(eigenvalues=eig, eigenvectors=x, hx=hx, converged=conv.astype(bool),

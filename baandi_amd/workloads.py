@@ -37,6 +37,23 @@ def si8(a=10.26, ecut_ha=15.0):
     return dict(name="si8_gamma_30ry", rows=rows, atoms=CONVENTIONAL, z=4.0, ecut_ha=ecut_ha, bands=20)
 
 
+# The aluminium variant of the synthetic family: one l=0 and one l=1 Gaussian
+# beta (widths 1 and 2 Bohr^2) with a diagonal D in Ry. No reference file holds
+# an Al potential; these values are this project's choice.
+AL_BETAS = ((0, 1.0), (1, 2.0))
+AL_DIJ_RY = ((2.0, 0.0), (0.0, 3.0))
+FCC = [[0.0, 0.0, 0.0], [0.0, 0.5, 0.5], [0.5, 0.0, 0.5], [0.5, 0.5, 0.0]]
+
+
+def al4(a=7.65, ecut_ha=12.5, mesh=(6, 6, 6)):
+    """Al fcc 4-atom cubic cell (a=7.65 Bohr), Z=3, 25 Ry, 16 bands per k,
+    Monkhorst-Pack mesh, Fermi-Dirac 0.01 Ha."""
+    rows = [[a, 0.0, 0.0], [0.0, a, 0.0], [0.0, 0.0, a]]
+    return dict(name="al4_fcc_%dx%dx%d_25ry" % tuple(mesh), rows=rows, atoms=FCC, z=3.0, ecut_ha=ecut_ha,
+                bands=16, mesh=tuple(mesh), theta=0.01, betas=AL_BETAS, dij_ry=AL_DIJ_RY)
+
+
 def build(workload):
-    from .hamiltonian import GammaSystem
-    return GammaSystem(workload["rows"], workload["atoms"], workload["z"], workload["ecut_ha"])
+    from .hamiltonian import GammaSystem, SYNTHETIC_BETAS, SYNTHETIC_DIJ_RY
+    return GammaSystem(workload["rows"], workload["atoms"], workload["z"], workload["ecut_ha"],
+                       betas=workload.get("betas", SYNTHETIC_BETAS), dij_ry=workload.get("dij_ry", SYNTHETIC_DIJ_RY))

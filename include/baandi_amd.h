@@ -140,6 +140,33 @@ bd_status bd_gamma_op_plan_info(bd_gamma_op* op, int64_t* out8);
 bd_status bd_gamma_op_stage_times(bd_gamma_op* op, double* out8);
 
 /* ------------------------------------------------------------------------
+ * k-point (complex) operator
+ * Replaces DistributedComplexHamiltonian::{new_owned, set_owned_potential,
+ * apply_block, project} and static_diagonal
+ * (crates/dft-physics/src/distributed_complex.rs:29-54, 88-386) on the basis
+ * PlaneWaveBasis::at_k (crates/dft-basis/src/lib.rs:234-269) of the system's
+ * cell, grid and pseudopotential. The handle is a bd_gamma_op whose n = 2 nk
+ * rows are the interleaved (Re, Im) parts of nk complex coefficients: a
+ * column is a complex vector, and the real symmetric operator on those rows
+ * is [[Re H, -Im H], [Im H, Re H]] in interleaved order. Every bd_gamma_op_*
+ * entry point and bd_davidson_solve_dev accept it: apply runs one complex FFT
+ * per column; band_density sums w_b |psi_b(r)|^2 / volume; the Davidson on
+ * it, started from pairs (x, i x), is the complex Davidson of the reference
+ * (ref_davidson_complex.rs:641-1113) on the realified subspace.
+ * k_frac is wrapped into [-1/2, 1/2) (crates/dft-geometry/src/kpoints.rs:21-32).
+ * ---------------------------------------------------------------------- */
+bd_status bd_k_op_create(const bd_system* sys, const double* k_frac3, int32_t capacity, int32_t precision,
+                         bd_gamma_op** out);
+/* Basis size nk and the wrapped fractional / Cartesian (Bohr^-1) k. */
+bd_status bd_k_op_info(const bd_gamma_op* op, int64_t* nk, double* k_frac3, double* k_cart3);
+/* Copy one named array (count elements, error if count differs):
+ *   "miller" int32 3nk; "g2", "q2" double nk (|G|^2, |k+G|^2);
+ *   "fft_index" int64 nk; "kinetic", "diagonal_static" double 2nk (per row);
+ *   "projectors" double 2nk x 2p column-major: column c = (Re P_c, Im P_c)
+ *   interleaved, column p + c = (-Im P_c, Re P_c). */
+bd_status bd_k_op_get(const bd_gamma_op* op, const char* name, void* out, int64_t count);
+
+/* ------------------------------------------------------------------------
  * Reference block Davidson on the op
  * Replaces ref_davidson::solve_with_initial_hx
  * (crates/dft-solvers/src/ref_davidson.rs:117-366): bordered inverse Cholesky

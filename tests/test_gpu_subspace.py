@@ -42,6 +42,18 @@ def test_eigen_random_symmetric(m, want):
     _check_pairs(0.5 * (B + B.T), want, "random")
 
 
+@pytest.mark.parametrize("half", [4, 16, 32, 80])
+def test_eigen_realified_hermitian(half):
+    """The realified form [[Re A, -Im A], [Im A, Re A]] of a Hermitian A (the
+    k-point Davidson's reduced matrix): every eigenvalue exactly twice."""
+    rng = np.random.default_rng(half)
+    B = rng.standard_normal((half, half)) + 1j * rng.standard_normal((half, half))
+    A = 0.5 * (B + B.conj().T)
+    R = np.block([[A.real, -A.imag], [A.imag, A.real]])
+    _check_pairs(R, half, "realified")
+    _check_pairs(R, 2 * half, "realified")
+
+
 @pytest.mark.parametrize("m", [12, 96, 240])
 def test_eigen_degenerate_and_clustered(m):
     """Exact multiplicities (the triplets of a cubic crystal), a 1e-13 cluster
