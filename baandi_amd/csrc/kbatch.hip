@@ -1,0 +1,424 @@
+// k-point H·X with one workgroup per column: the whole 3D FFT of the column
+// stays in LDS (its sticks, then one y-z plane at a time), so a column costs
+// one read of X, one read of V (L2-resident, shared by every column) and one
+// write of Y. One launch covers every column of every k point of a batch
+// (uneven basis sizes included); a second launch adds the nonlocal term of
+// every k point. The per-line FFT arithmetic, the V multiply and the kinetic
+// add are those of the three-stage path (k_kstick_inverse, k_plane,
+// k_kstick_forward), so both paths give the same bits for the local part.
+//
+// Reference: DistributedComplexHamiltonian::apply_block
+// (crates/dft-physics/src/distributed_complex.rs:284-386); the reference
+// applies one k point at a time (crates/dft-engine/src/refscf/k_map.rs).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "common.hpp"
+#include "op.hpp"
+
+namespace bd {
+
+namespace {
+
+template <typename T>
+__device__ __forceinline__ void stage_roots_k(c2<T>* dst, const c2<double>* src, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = {T(src[i].x), T(src[i].y)};
+}
+
+// The k point of work item `item` (works sorted by `first`).
+__device__ __forceinline__ int find_work(const KWork* __restrict__ works, int count, int item) {
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (works[mid].first <= item) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Persistent: workgroup b takes items b, b + grid, ... ; every workgroup
+// leaves after its last item.
+template <typename F, int NT>
+__global__ __launch_bounds__(NT) void k_hpsi_fused(const KWork* __restrict__ works, KWork one, int count, int total,
+                                                   int n0,
+                                                   int n1, int n2, const c2<double>* __restrict__ roots,
+                                                   RadixPlan rx, RadixPlan ry, RadixPlan rz, int xlines, int zrows,
+                                                   double inv_n) {
+  extern __shared__ char smem[];
+  c2<F>* rt = reinterpret_cast<c2<F>*>(smem);  // roots x | y | z
+  c2<F>* rtx = rt;
+  c2<F>* rty = rt + n0;
+  c2<F>* rtz = rt + n0 + n1;
+  c2<F>* L = rt + n0 + n1 + n2;  // S sticks x n0
+  stage_roots_k(rt, roots, n0 + n1 + n2);
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    const KWork w = works ? works[find_work(works, count, item)] : one;
+    const int j = item - w.first;
+    const int S = w.S, Z = w.Z, nk = w.nk;
+    c2<F>* P = L + S * n0;  // Z x n1
+    c2<F>* R = P + Z * n1;  // zrows x n2
+    const double* xc = w.x + int64_t(j) * w.ldx;
+    double* yc = w.y + int64_t(j) * w.ldy;
+    const F* pot = reinterpret_cast<const F*>(w.potential);
+    for (int i = threadIdx.x; i < S * n0; i += NT) L[i] = {F(0), F(0)};
+    __syncthreads();
+    for (int g = threadIdx.x; g < nk; g += NT) {
+      const double2 v = *reinterpret_cast<const double2*>(xc + 2 * int64_t(g));
+      L[w.fpos[g]] = {F(v.x), F(v.y)};
+    }
+    __syncthreads();
+    for (int s0 = 0; s0 < S; s0 += xlines) lds_fft<true>(L + s0 * n0, min(xlines, S - s0), n0, 1, rx, rtx);
+    for (int xi = 0; xi < n0; ++xi) {
+      for (int i = threadIdx.x; i < Z * n1; i += NT) P[i] = {F(0), F(0)};
+      __syncthreads();
+      for (int s = threadIdx.x; s < S; s += NT) {
+        const int2 yz = w.yz[s];
+        P[yz.y * n1 + yz.x] = L[s * n0 + xi];
+      }
+      __syncthreads();
+      lds_fft<true>(P, Z, n1, 1, ry, rty);
+      for (int y0 = 0; y0 < n1; y0 += zrows) {
+        const int c = min(zrows, n1 - y0);
+        for (int i = threadIdx.x; i < c * n2; i += NT) R[i] = {F(0), F(0)};
+        __syncthreads();
+        for (int i = threadIdx.x; i < c * Z; i += NT) {
+          const int r = i / Z, zc = i - r * Z;
+          R[r * n2 + w.z_of_zc[zc]] = P[zc * n1 + y0 + r];
+        }
+        __syncthreads();
+        lds_fft<true>(R, c, n2, 1, rz, rtz);
+        const F* v = pot + (int64_t(xi) * n1 + y0) * n2;
+        for (int i = threadIdx.x; i < c * n2; i += NT) R[i] = cscale(R[i], v[i]);
+        __syncthreads();
+        lds_fft<false>(R, c, n2, 1, rz, rtz);
+        for (int i = threadIdx.x; i < c * Z; i += NT) {
+          const int r = i / Z, zc = i - r * Z;
+          P[zc * n1 + y0 + r] = R[r * n2 + w.z_of_zc[zc]];
+        }
+        __syncthreads();
+      }
+      lds_fft<false>(P, Z, n1, 1, ry, rty);
+      for (int s = threadIdx.x; s < S; s += NT) {
+        const int2 yz = w.yz[s];
+        L[s * n0 + xi] = P[yz.y * n1 + yz.x];
+      }
+      __syncthreads();
+    }
+    for (int s0 = 0; s0 < S; s0 += xlines) lds_fft<false>(L + s0 * n0, min(xlines, S - s0), n0, 1, rx, rtx);
+    for (int g = threadIdx.x; g < nk; g += NT) {
+      const int64_t r = 2 * int64_t(g);
+      const c2<F> h = L[w.fpos[g]];
+      const double2 v = *reinterpret_cast<const double2*>(xc + r);
+      const double k = w.kinetic[r];
+      *reinterpret_cast<double2*>(yc + r) =
+          make_double2(double(h.x) * inv_n + k * v.x, double(h.y) * inv_n + k * v.y);
+    }
+    __syncthreads();  // LDS is reused by the next item
+  }
+}
+
+// Y += P2 D P2^T X for every column of one k point per workgroup (FP64).
+// Q (np2 x m) and W = D Q live in LDS; each wave forms whole dot products.
+constexpr int kNlThreads = 256;
+constexpr int kNlMaxQ = 2048;  // np2 * m doubles in LDS (two arrays: 32 KB)
+
+__global__ __launch_bounds__(kNlThreads) void k_nonlocal_k(const KWork* __restrict__ works, KWork one) {
+  __shared__ double Q[kNlMaxQ];
+  __shared__ double W[kNlMaxQ];
+  const KWork w = works ? works[blockIdx.x] : one;
+  const int m = w.m, np2 = w.np2, nc = w.nc;
+  const int64_t n = 2 * int64_t(w.nk);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int e = wave; e < np2 * m; e += kNlThreads / 64) {
+    const int c = e % np2, j = e / np2;
+    const double* p = w.proj2 + int64_t(c) * n;
+    const double* x = w.x + int64_t(j) * w.ldx;
+    double acc = 0.0;
+    for (int64_t r = lane; r < n; r += 64) acc += p[r] * x[r];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) Q[e] = acc;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < np2 * m; e += kNlThreads) {
+    const int c = e % np2, j = e / np2;
+    const int start = (c / nc) * nc, a = c - start;
+    double s = 0.0;
+    for (int b = 0; b < nc; ++b) s += w.blockd[a * nc + b] * Q[j * np2 + start + b];
+    W[e] = s;
+  }
+  __syncthreads();
+  for (int64_t r = threadIdx.x; r < n; r += kNlThreads) {
+    for (int j = 0; j < m; ++j) {
+      double s = 0.0;
+      for (int c = 0; c < np2; ++c) s += w.proj2[int64_t(c) * n + r] * W[j * np2 + c];
+      w.y[int64_t(j) * w.ldy + r] += s;
+    }
+  }
+}
+
+template <typename F, int NT>
+void launch_fused_t(const GammaOp& op, const FusedShape& f, size_t lds, const KWork* works, const KWork& one,
+                    int count, int total, hipStream_t s) {
+  const StickPlan& p = op.plan;
+  // Resident workgroups for this LDS size, looked up once per size.
+  static bool attr = false;
+  static size_t cached_lds = 0;
+  static int slots = 0;
+  if (!attr) {
+    BD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_hpsi_fused<F, NT>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    attr = true;
+  }
+  if (cached_lds != lds) {
+    int dev = 0, cus = 0, per_cu = 0;
+    BD_HIP(hipGetDevice(&dev));
+    BD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    BD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(&k_hpsi_fused<F, NT>), NT, lds));
+    slots = cus * std::max(per_cu, 1);
+    cached_lds = lds;
+  }
+  const int grid = std::max(1, std::min(total, slots));
+  hipLaunchKernelGGL((k_hpsi_fused<F, NT>), dim3(grid), dim3(NT), lds, s, works, one, count, total, p.n0, p.n1, p.n2,
+                     op.d_roots.p, p.rx, p.ry, p.rz, f.xlines, f.zrows, 1.0 / double(op.N));
+  BD_HIP(hipGetLastError());
+}
+}  // namespace
+
+size_t GammaOp::fused_lds(const FusedShape& f) const {
+  const size_t e = precision == BD_MIXED ? sizeof(c2<float>) : sizeof(c2<double>);
+  const StickPlan& p = plan;
+  return e * size_t(p.n0 + p.n1 + p.n2 + p.S * p.n0 + p.Z * p.n1 + f.zrows * p.n2);
+}
+
+bool GammaOp::fused_valid(const FusedShape& f) const {
+  if (!kb) return false;
+  if (f.threads != 256 && f.threads != 512) return false;
+  if (f.xlines < 1 || f.zrows < 1) return false;
+  if (int64_t(nproj) * capacity > kNlMaxQ) return false;
+  return fft_fits_threads(f.xlines, plan.rx, f.threads) && fft_fits_threads(plan.Z, plan.ry, f.threads) &&
+         fft_fits_threads(f.zrows, plan.rz, f.threads) && fused_lds(f) <= 160 * 1024;
+}
+
+KWork GammaOp::kwork(const double* x, int64_t ldx, int m, double* y, int64_t ldy, int first) const {
+  KWork w{};
+  w.x = x;
+  w.y = y;
+  w.ldx = ldx;
+  w.ldy = ldy;
+  w.m = m;
+  w.first = first;
+  w.nk = int(kb->nk);
+  w.S = plan.S;
+  w.Z = plan.Z;
+  w.fpos = d_fpos.p;
+  w.yz = d_stick_yz.p;
+  w.z_of_zc = d_zc.p;
+  w.kinetic = d_kinetic.p;
+  w.potential = precision == BD_MIXED ? static_cast<const void*>(d_potential32.p)
+                                      : static_cast<const void*>(d_potential.p);
+  w.proj2 = d_proj.p;
+  w.blockd = d_blockd.p;
+  w.np2 = int(nproj);
+  w.nc = nc;
+  return w;
+}
+
+void fused_launch(const GammaOp& op, const FusedShape& f, size_t lds, const KWork* works, const KWork& one, int count,
+                  int total, hipStream_t s) {
+  if (total <= 0) return;
+  if (lds > 160 * 1024) fail(BD_ERR_INVALID_PARAMETER, "fused kernel LDS exceeds 160 KB");
+  if (op.precision == BD_MIXED) {
+    if (f.threads == 512) launch_fused_t<float, 512>(op, f, lds, works, one, count, total, s);
+    else launch_fused_t<float, 256>(op, f, lds, works, one, count, total, s);
+  } else {
+    if (f.threads == 512) launch_fused_t<double, 512>(op, f, lds, works, one, count, total, s);
+    else launch_fused_t<double, 256>(op, f, lds, works, one, count, total, s);
+  }
+}
+
+void nonlocal_launch(const GammaOp& op, const KWork* works, const KWork& one, int count, hipStream_t s) {
+  if (op.nproj == 0 || count <= 0) return;
+  hipLaunchKernelGGL(k_nonlocal_k, dim3(count), dim3(kNlThreads), 0, s, works, one);
+  BD_HIP(hipGetLastError());
+}
+
+// One k point: the work entry travels as a kernel argument.
+void GammaOp::fused_apply(const double* x, int64_t ldx, int m, double* y, int64_t ldy) {
+  const KWork w = kwork(x, ldx, m, y, ldy, 0);
+  fused_launch(*this, fshape, fused_lds(fshape), nullptr, w, 1, m, stream);
+  nonlocal_launch(*this, nullptr, w, 1, stream);
+}
+
+}  // namespace bd
+
+// ---------------------------------------------------------------------------
+// C ABI: batches of k points and the launch-shape autotuner
+// ---------------------------------------------------------------------------
+struct bd_k_batch {
+  std::vector<bd_gamma_op*> ops;
+  bd::DevBuf<bd::KWork> works;
+  std::vector<bd::KWork> host;
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  ~bd_k_batch() {
+    if (done) (void)hipEventDestroy(done);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+extern "C" bd_status bd_k_batch_create(bd_gamma_op* const* ops, int32_t count, bd_k_batch** out) {
+  BD_GUARD({
+    if (!ops || !out || count <= 0) bd::fail(BD_ERR_INVALID_PARAMETER, "empty k batch");
+    auto b = std::make_unique<bd_k_batch>();
+    const bd::GammaOp& o0 = ops[0]->op;
+    for (int i = 0; i < count; ++i) {
+      if (!ops[i]) bd::fail(BD_ERR_INVALID_PARAMETER, "null op in k batch");
+      const bd::GammaOp& o = ops[i]->op;
+      if (!o.kb || !o.fused) bd::fail(BD_ERR_INVALID_PARAMETER, "k batch needs k-point ops on the fused path");
+      if (o.sys != o0.sys || o.precision != o0.precision)
+        bd::fail(BD_ERR_INVALID_PARAMETER, "k batch ops must share the system and precision");
+      b->ops.push_back(ops[i]);
+    }
+    b->works.alloc(size_t(count));
+    b->host.resize(size_t(count));
+    BD_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    BD_HIP(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+    *out = b.release();
+  });
+}
+
+extern "C" void bd_k_batch_destroy(bd_k_batch* b) { delete b; }
+
+extern "C" void* bd_k_batch_stream(bd_k_batch* b) { return b ? b->stream : nullptr; }
+
+// Y_i = H_i X_i for every op i of the batch: one fused launch for all columns
+// of all k points, one nonlocal launch for all k points. Asynchronous on the
+// batch stream, ordered after every op's queued work.
+extern "C" bd_status bd_k_batch_apply_dev(bd_k_batch* b, const double* const* x, const int64_t* ldx,
+                                          const int32_t* m, double* const* y, const int64_t* ldy) {
+  BD_GUARD({
+    if (!b || !x || !ldx || !m || !y || !ldy) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    const int count = int(b->ops.size());
+    int total = 0;
+    size_t lds = 0;
+    bd::FusedShape f = b->ops[0]->op.fshape;
+    for (int i = 0; i < count; ++i) {
+      bd::GammaOp& o = b->ops[i]->op;
+      if (m[i] < 0 || m[i] > o.capacity) bd::fail(BD_ERR_INVALID_PARAMETER, "column count exceeds capacity");
+      if (m[i] > 0 && (!x[i] || !y[i])) bd::fail(BD_ERR_INVALID_PARAMETER, "null column block");
+      if (ldx[i] < o.n || ldy[i] < o.n || (ldx[i] & 1) || (ldy[i] & 1) ||
+          (reinterpret_cast<uintptr_t>(x[i]) & 15) || (reinterpret_cast<uintptr_t>(y[i]) & 15))
+        bd::fail(BD_ERR_INVALID_PARAMETER, "k-point blocks must be 16-byte aligned with even ld >= 2 nk");
+      if (!o.has_potential) bd::fail(BD_ERR_INVALID_PARAMETER, "no potential installed");
+      // One shape for the launch: the widest x batch and z batch every op accepts.
+      if (!o.fused_valid(f)) bd::fail(BD_ERR_INVALID_PARAMETER, "k batch ops need a common fused shape");
+      b->host[i] = o.kwork(x[i], ldx[i], m[i], y[i], ldy[i], total);
+      lds = std::max(lds, o.fused_lds(f));  // sized for the largest sphere
+      total += m[i];
+      BD_HIP(hipEventRecord(b->done, o.stream));
+      BD_HIP(hipStreamWaitEvent(b->stream, b->done, 0));
+    }
+    BD_HIP(hipMemcpyAsync(b->works.p, b->host.data(), sizeof(bd::KWork) * count, hipMemcpyHostToDevice, b->stream));
+    const bd::KWork none{};
+    bd::fused_launch(b->ops[0]->op, f, lds, b->works.p, none, count, total, b->stream);
+    bd::nonlocal_launch(b->ops[0]->op, b->works.p, none, count, b->stream);
+    // The host copy of the work table must outlive the transfer.
+    BD_HIP(hipStreamSynchronize(b->stream));
+  });
+}
+
+// Time every valid fused shape (threads 256/512, x batches, z batches) on
+// m columns of the op's own scratch, reps launches each, and keep the
+// fastest. out4 (optional): threads, xlines, zrows, microseconds per launch.
+extern "C" bd_status bd_gamma_op_autotune(bd_gamma_op* h, int32_t m, int32_t reps, double* out4) {
+  BD_GUARD({
+    if (!h) bd::fail(BD_ERR_INVALID_PARAMETER, "null op");
+    bd::GammaOp& o = h->op;
+    if (!o.kb || !o.fused) bd::fail(BD_ERR_INVALID_PARAMETER, "autotuning covers the fused k-point path");
+    if (m <= 0 || m > o.capacity || reps <= 0) bd::fail(BD_ERR_INVALID_PARAMETER, "autotune columns/reps");
+    if (!o.has_potential) bd::fail(BD_ERR_INVALID_PARAMETER, "no potential installed");
+    std::vector<bd::FusedShape> cands;
+    for (int t : {256, 512}) {
+      std::vector<int> xs, zs;
+      for (int v = o.plan.S; v >= 1; v = v > 1 ? (v + 1) / 2 : 0) {
+        xs.push_back(v);
+        if (v == 1) break;
+      }
+      for (int v = o.plan.n1; v >= 1; v = v > 1 ? (v + 1) / 2 : 0) {
+        zs.push_back(v);
+        if (v == 1) break;
+      }
+      for (int xl : xs)
+        for (int zr : zs) {
+          bd::FusedShape f{t, xl, zr};
+          if (o.fused_valid(f)) cands.push_back(f);
+          if (cands.size() > 48) break;
+        }
+    }
+    if (cands.empty()) bd::fail(BD_ERR_APPLICATION, "no valid fused shape");
+    // Inputs: the op's staging buffers (contents do not affect the timing).
+    BD_HIP(hipMemsetAsync(o.d_x.p, 0, size_t(o.n) * m * 8, o.stream));
+    const bd::KWork w = o.kwork(o.d_x.p, o.n, m, o.d_y.p, o.n, 0);
+    hipEvent_t a, z;
+    BD_HIP(hipEventCreate(&a));
+    BD_HIP(hipEventCreate(&z));
+    double best = 1e300;
+    bd::FusedShape pick = o.fshape;
+    for (const auto& f : cands) {
+      const size_t lds = o.fused_lds(f);
+      bd::fused_launch(o, f, lds, nullptr, w, 1, m, o.stream);  // warm-up
+      BD_HIP(hipEventRecord(a, o.stream));
+      for (int r = 0; r < reps; ++r) bd::fused_launch(o, f, lds, nullptr, w, 1, m, o.stream);
+      BD_HIP(hipEventRecord(z, o.stream));
+      BD_HIP(hipEventSynchronize(z));
+      float ms = 0.f;
+      BD_HIP(hipEventElapsedTime(&ms, a, z));
+      const double us = 1e3 * ms / reps;
+      if (us < best) {
+        best = us;
+        pick = f;
+      }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(z);
+    o.fshape = pick;
+    if (out4) {
+      out4[0] = pick.threads;
+      out4[1] = pick.xlines;
+      out4[2] = pick.zrows;
+      out4[3] = best;
+    }
+  });
+}
+
+// Read or set the fused shape (threads, xlines, zrows); set validates it.
+// Setting threads = 0 switches the op to the three-stage path; reading gives
+// threads = 0 on that path.
+extern "C" bd_status bd_gamma_op_fused_shape(bd_gamma_op* h, int32_t* inout3, int32_t set) {
+  BD_GUARD({
+    if (!h || !inout3) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    bd::GammaOp& o = h->op;
+    if (!o.kb) bd::fail(BD_ERR_INVALID_PARAMETER, "not a k-point op");
+    if (set && inout3[0] == 0) {
+      o.fused = false;
+    } else if (set) {
+      bd::FusedShape f{inout3[0], inout3[1], inout3[2]};
+      if (!o.fused_valid(f)) bd::fail(BD_ERR_INVALID_PARAMETER, "invalid fused shape");
+      o.fshape = f;
+      o.fused = true;
+    } else if (!o.fused) {
+      inout3[0] = inout3[1] = inout3[2] = 0;
+    } else {
+      inout3[0] = o.fshape.threads;
+      inout3[1] = o.fshape.xlines;
+      inout3[2] = o.fshape.zrows;
+    }
+  });
+}

@@ -38,6 +38,25 @@ template <int R>
 bool pass_fits(int nlines, int n) {
   return int64_t(nlines) * (n / R) <= int64_t(MaxB<R>::v) * kThreads;
 }
+template <int R>
+bool pass_fits_t(int nlines, int n, int threads) {
+  return int64_t(nlines) * (n / R) <= int64_t(MaxB<R>::v) * threads;
+}
+}  // namespace
+
+bool fft_fits_threads(int nlines, const RadixPlan& rp, int threads) {
+  for (int p = 0; p < rp.count; ++p) {
+    int R = rp.radix[p];
+    bool ok = R == 2 ? pass_fits_t<2>(nlines, rp.n, threads)
+              : R == 3 ? pass_fits_t<3>(nlines, rp.n, threads)
+              : R == 4 ? pass_fits_t<4>(nlines, rp.n, threads)
+                       : pass_fits_t<5>(nlines, rp.n, threads);
+    if (!ok) return false;
+  }
+  return true;
+}
+
+namespace {
 bool fft_fits(int nlines, const RadixPlan& rp) {
   for (int p = 0; p < rp.count; ++p) {
     int R = rp.radix[p];
@@ -478,6 +497,7 @@ StickPlan build_k_stick_plan(const System& s, const KBasis& kb) {
     split(kb.flat[g], a, b, c);
     const int st = index[{c, b}];
     per[st / p.T].push_back(make_int4(int(g), (st % p.T) * p.n0 + a, -1, 0));
+    p.fpos.push_back(st * p.n0 + a);
   }
   p.tile_off.assign(1, 0);
   for (auto& v : per) {
@@ -587,6 +607,19 @@ GammaOp::GammaOp(const System& s, const KBasis* k, int cap, int prec) : sys(&s),
   d_y.alloc(size_t(n) * cap);
   d_weights.alloc(size_t(2 * pairs));
   d_rho_part.alloc(size_t(std::min(pairs, kDensityGroups)) * N);
+  if (kb) {
+    d_fpos.alloc(plan.fpos.size());
+    d_fpos.upload(plan.fpos.data(), plan.fpos.size(), stream);
+    // Default shape: 256 threads, every stick in one x batch and the whole
+    // plane in one z batch when they fit, halved until they do.
+    FusedShape f;
+    f.xlines = plan.S;
+    while (f.xlines > 1 && !fft_fits_threads(f.xlines, plan.rx, f.threads)) f.xlines = (f.xlines + 1) / 2;
+    f.zrows = plan.n1;
+    while (f.zrows > 1 && !fft_fits_threads(f.zrows, plan.rz, f.threads)) f.zrows = (f.zrows + 1) / 2;
+    fused = fused_valid(f) && fused_lds(f) <= lds_max;
+    if (fused) fshape = f;
+  }
   for (auto& e : ev) BD_HIP(hipEventCreate(&e));
   BD_HIP(hipStreamSynchronize(stream));
 }
@@ -677,6 +710,23 @@ void GammaOp::apply(const double* x, int64_t ldx, int m, double* y, int64_t ldy)
   if (kb && !complex_aligned(y, ldy)) fail(BD_ERR_INVALID_PARAMETER, "k-point Y must be 16-byte aligned, even ldy");
   if (m == 0) return;
   const int pairs = slots(m);
+  if (fused) {
+    if (profiling) BD_HIP(hipEventRecord(ev[0], stream));
+    fused_apply(x, ldx, m, y, ldy);
+    if (profiling) {
+      BD_HIP(hipEventRecord(ev[5], stream));
+      BD_HIP(hipEventSynchronize(ev[5]));
+      float ms = 0.f;
+      BD_HIP(hipEventElapsedTime(&ms, ev[0], ev[5]));
+      stage_ms[1] = ms;
+      stage_total_ms[1] += ms;
+      applies += 1.0;
+      pairs_total += pairs;
+      columns_total += m;
+    }
+    BD_HIP(hipGetLastError());
+    return;
+  }
   if (precision == BD_MIXED) local_apply<float>(x, ldx, m, y, ldy);
   else local_apply<double>(x, ldx, m, y, ldy);
   if (nproj > 0) {

@@ -51,6 +51,33 @@ struct StickPlan {
                                    // k: (basis index, slot*n0+x, -1, 0)
   std::vector<int2> stick_yz;      // per padded stick: (i1, zc), (-1,-1) pad
   std::vector<int> z_of_zc;        // Z
+  std::vector<int> fpos;           // k only: stick * n0 + x per basis coefficient
+};
+
+// One k point's columns in the fused k-point kernels (a device array entry).
+struct KWork {
+  const double* x;
+  double* y;
+  int64_t ldx, ldy;
+  int m, first;           // columns; index of the first work item
+  int nk, S, Z;           // coefficients, occupied sticks, occupied z planes
+  const int* fpos;        // nk: stick * n0 + x, the coefficient's LDS slot
+  const int2* yz;         // S: (y, zc) of each stick
+  const int* z_of_zc;     // Z
+  const double* kinetic;  // 2nk
+  const void* potential;  // N values of F
+  const double* proj2;    // 2nk x np2, column-major (ld 2nk)
+  const double* blockd;   // nc x nc
+  int np2, nc;
+};
+
+// Launch shape of the fused kernel: threads per workgroup, sticks per x-FFT
+// batch and y rows per z-FFT batch (the per-line arithmetic does not depend
+// on them, so every valid shape gives the same bits).
+struct FusedShape {
+  int threads = 256;
+  int xlines = 0;
+  int zrows = 0;
 };
 
 struct DavidsonWork;
@@ -83,6 +110,15 @@ struct GammaOp {
   DevBuf<c2<double>> d_sticks;  // pairs_cap * n0 * Spad
   DevBuf<double> d_q, d_w, d_work, d_x, d_y, d_weights, d_rho_part;
   Gemm gemm;  // FP64, or BF16x3 in the mixed tier
+  // k point: the fused one-workgroup-per-column H·X (whole 3D FFT in LDS)
+  // when its LDS fits; the three-stage stick/plane path otherwise.
+  bool fused = false;
+  FusedShape fshape;
+  DevBuf<int> d_fpos;
+  KWork kwork(const double* x, int64_t ldx, int m, double* y, int64_t ldy, int first) const;
+  size_t fused_lds(const FusedShape& f) const;
+  bool fused_valid(const FusedShape& f) const;
+  void fused_apply(const double* x, int64_t ldx, int m, double* y, int64_t ldy);
   int nc = 0;  // channels per atom block
   int nblocks = 0;
   // profiling
@@ -106,6 +142,14 @@ struct GammaOp {
   template <typename F>
   void density(const double* x, int64_t ldx, int m, const double* w, double* rho);
 };
+
+// Launch the fused H·X for `total` work items over `count` k points (works on
+// the device); all share the grid, radix plans, roots and precision of `op`.
+// works_dev == nullptr: the single entry `one` is used (passed by value).
+void fused_launch(const GammaOp& op, const FusedShape& f, size_t lds, const KWork* works_dev, const KWork& one,
+                  int count, int total, hipStream_t s);
+void nonlocal_launch(const GammaOp& op, const KWork* works_dev, const KWork& one, int count, hipStream_t s);
+bool fft_fits_threads(int nlines, const RadixPlan& rp, int threads);
 
 StickPlan build_stick_plan(const System& s);
 StickPlan build_k_stick_plan(const System& s, const KBasis& kb);

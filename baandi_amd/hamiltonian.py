@@ -97,7 +97,10 @@ class GammaSystem:
     def __del__(self):
         h = getattr(self, "handle", None)
         if h:
-            L.load().bd_system_destroy(h)
+            try:
+                L.load().bd_system_destroy(h)
+            except (TypeError, AttributeError):  # interpreter shutdown
+                pass
             self.handle = None
 
 
@@ -203,5 +206,8 @@ class DistributedRealHamiltonian:
     def __del__(self):
         h = getattr(self, "handle", None)
         if h:
-            L.load().bd_gamma_op_destroy(h)
+            try:
+                L.load().bd_gamma_op_destroy(h)
+            except (TypeError, AttributeError):  # interpreter shutdown
+                pass
             self.handle = None

@@ -162,6 +162,26 @@ class KPointHamiltonian(DistributedRealHamiltonian):
         p = self.system.nproj
         return p2[0::2, :p] + 1j * p2[1::2, :p]
 
+    def fused_shape(self):
+        """(threads, xlines, zrows) of the fused one-workgroup-per-column
+        kernel, or (0, 0, 0) on the three-stage path."""
+        v = np.zeros(3, dtype=np.int32)
+        L.check(L.load().bd_gamma_op_fused_shape(self.handle, L.ptr(v), 0))
+        return tuple(int(a) for a in v)
+
+    def set_fused_shape(self, shape):
+        """Select a fused shape, or (0, 0, 0) for the three-stage path."""
+        v = np.asarray(shape, dtype=np.int32)
+        L.check(L.load().bd_gamma_op_fused_shape(self.handle, L.ptr(v), 1))
+
+    def autotune(self, columns, reps=5):
+        """Time every valid fused shape on `columns` columns and keep the
+        fastest; returns (threads, xlines, zrows, microseconds per launch)."""
+        out = np.zeros(4)
+        self.enter()
+        L.check(L.load().bd_gamma_op_autotune(self.handle, int(columns), int(reps), L.ptr(out)))
+        return int(out[0]), int(out[1]), int(out[2]), float(out[3])
+
     def apply_complex(self, x):
         """H x for a CUDA complex128 tensor (m, nk)."""
         import torch
@@ -169,6 +189,45 @@ class KPointHamiltonian(DistributedRealHamiltonian):
         y = torch.empty_like(xr)
         self.apply_block(xr, y)
         return torch.view_as_complex(y.view(x.shape[0], self.nk, 2))
+
+
+class KPointBatch:
+    """Every column of every k point of `ops` through one fused H·X launch
+    (plus one nonlocal launch): bd_k_batch_apply_dev."""
+
+    def __init__(self, ops):
+        lib = L.load()
+        self.ops = list(ops)
+        arr = (C.c_void_p * len(self.ops))(*[op.handle.value for op in self.ops])
+        h = C.c_void_p()
+        L.check(lib.bd_k_batch_create(arr, len(self.ops), C.byref(h)))
+        self.handle = h
+
+    def apply(self, xs, ys):
+        """ys[i] = H_i xs[i] for CUDA float64 blocks (m_i, 2 nk_i)."""
+        import torch
+        n = len(self.ops)
+        assert len(xs) == n and len(ys) == n
+        xp = (C.c_void_p * n)(*[x.data_ptr() for x in xs])
+        yp = (C.c_void_p * n)(*[y.data_ptr() for y in ys])
+        ldx = np.array([x.shape[1] for x in xs], dtype=np.int64)
+        ldy = np.array([y.shape[1] for y in ys], dtype=np.int64)
+        m = np.array([x.shape[0] for x in xs], dtype=np.int32)
+        for op, x, y in zip(self.ops, xs, ys):
+            assert x.is_contiguous() and y.is_contiguous() and x.shape == y.shape
+            assert x.shape[1] == op.global_len()
+            op.enter()
+        torch.cuda.current_stream().synchronize()
+        L.check(L.load().bd_k_batch_apply_dev(self.handle, xp, L.ptr(ldx), L.ptr(m), yp, L.ptr(ldy)))
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            try:
+                L.load().bd_k_batch_destroy(h)
+            except (TypeError, AttributeError):
+                pass
+            self.handle = None
 
 
 def as_real(x):

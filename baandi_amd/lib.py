@@ -22,7 +22,8 @@ EXPORTS = [
     "bd_gamma_op_set_potential_dev", "bd_gamma_op_apply", "bd_gamma_op_apply_dev",
     "bd_gamma_op_band_density_dev", "bd_gamma_op_nonlocal_energies_dev", "bd_gamma_op_stream",
     "bd_gamma_op_set_profiling", "bd_gamma_op_stage_times", "bd_gamma_op_plan_info", "bd_davidson_solve_dev",
-    "bd_k_op_create", "bd_k_op_info", "bd_k_op_get",
+    "bd_k_op_create", "bd_k_op_info", "bd_k_op_get", "bd_k_batch_create", "bd_k_batch_destroy",
+    "bd_k_batch_stream", "bd_k_batch_apply_dev", "bd_gamma_op_autotune", "bd_gamma_op_fused_shape",
     "bd_gemm_workspace", "bd_gemm_dev", "bd_mfma_probe_dev",
     "bd_sym_eigen_workspace", "bd_sym_eigen_dev", "bd_inverse_cholesky_dev",
     "bd_mlp_param_count", "bd_mlp_blob_floats", "bd_mlp_quantize_dev", "bd_mlp_forward_dev",
@@ -104,6 +105,10 @@ def load():
         "bd_k_op_create": [P, P, I32, I32, C.POINTER(P)],
         "bd_k_op_info": [P, P, P, P],
         "bd_k_op_get": [P, C.c_char_p, P, I64],
+        "bd_k_batch_create": [P, I32, C.POINTER(P)],
+        "bd_k_batch_apply_dev": [P, P, P, P, P, P],
+        "bd_gamma_op_autotune": [P, I32, I32, P],
+        "bd_gamma_op_fused_shape": [P, P, I32],
         "bd_gamma_op_set_potential": [P, P, U64],
         "bd_gamma_op_set_potential_dev": [P, P, U64],
         "bd_gamma_op_apply": [P, P, I64, I32, P, I64],
@@ -135,6 +140,10 @@ def load():
     lib.bd_gamma_op_destroy.restype = None
     lib.bd_gamma_op_stream.argtypes = [P]
     lib.bd_gamma_op_stream.restype = P
+    lib.bd_k_batch_destroy.argtypes = [P]
+    lib.bd_k_batch_destroy.restype = None
+    lib.bd_k_batch_stream.argtypes = [P]
+    lib.bd_k_batch_stream.restype = P
     lib.bd_gemm_workspace.argtypes = [I32, I32, I64, I64, I64]
     lib.bd_gemm_workspace.restype = I64
     lib.bd_sym_eigen_workspace.argtypes = [I32, I32]

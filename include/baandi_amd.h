@@ -167,6 +167,30 @@ bd_status bd_k_op_info(const bd_gamma_op* op, int64_t* nk, double* k_frac3, doub
 bd_status bd_k_op_get(const bd_gamma_op* op, const char* name, void* out, int64_t count);
 
 /* ------------------------------------------------------------------------
+ * Batched k points (no reference counterpart: the reference applies one k
+ * point at a time, crates/dft-engine/src/refscf/k_map.rs). A k-point op whose
+ * FFT fits one workgroup's LDS applies H with one workgroup per column (the
+ * whole 3D FFT of the column in LDS); its local part is bit-identical to the
+ * three-stage stick/plane path. A batch runs every column of every k point
+ * in one persistent launch, plus one launch for the nonlocal term of all k.
+ * ---------------------------------------------------------------------- */
+typedef struct bd_k_batch bd_k_batch;
+/* All ops: k-point ops of one system and precision on the fused path. */
+bd_status bd_k_batch_create(bd_gamma_op* const* ops, int32_t count, bd_k_batch** out);
+void bd_k_batch_destroy(bd_k_batch* batch);
+void* bd_k_batch_stream(bd_k_batch* batch);
+/* Y_i = H_i X_i for each op i (device blocks, 16-byte aligned, even ld >= 2 nk_i,
+ * m_i <= capacity_i). Ordered after each op's queued work; returns when done. */
+bd_status bd_k_batch_apply_dev(bd_k_batch* batch, const double* const* x_dev, const int64_t* ldx,
+                               const int32_t* m, double* const* y_dev, const int64_t* ldy);
+/* Launch-shape autotuner of the fused kernel: times every valid (threads,
+ * sticks per x-FFT batch, rows per z-FFT batch) on m columns, reps launches
+ * each, keeps the fastest. out4: threads, xlines, zrows, microseconds. */
+bd_status bd_gamma_op_autotune(bd_gamma_op* op, int32_t m, int32_t reps, double* out4);
+/* Read (set = 0) or set the fused shape; threads = 0 selects the three-stage path. */
+bd_status bd_gamma_op_fused_shape(bd_gamma_op* op, int32_t* inout3, int32_t set);
+
+/* ------------------------------------------------------------------------
  * Reference block Davidson on the op
  * Replaces ref_davidson::solve_with_initial_hx
  * (crates/dft-solvers/src/ref_davidson.rs:117-366): bordered inverse Cholesky

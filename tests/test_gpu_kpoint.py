@@ -180,3 +180,107 @@ def test_k_scf_matches_oracle(kat1):
         assert abs(r["energy"][key] - ref["energy"][key]) < 1e-5, key
     assert de < 1e-8 * len(ATOMS)
     assert deig < 1e-6
+
+
+def _al_ops(mesh=(2, 2, 2), betas=True, capacity=8, precision=None):
+    from baandi_amd import workloads as W, lib as L
+    from baandi_amd.hamiltonian import GammaSystem
+    from baandi_amd.kpoint import KPointHamiltonian, monkhorst_pack
+    wl = W.al4(mesh=mesh)
+    system = GammaSystem(wl["rows"], wl["atoms"], wl["z"], wl["ecut_ha"],
+                         betas=wl["betas"] if betas else (), dij_ry=wl["dij_ry"] if betas else ())
+    kp = monkhorst_pack(mesh)
+    prec = L.BD_FP64 if precision is None else precision
+    ops = [KPointHamiltonian(system, k, capacity=capacity, precision=prec) for k, _ in kp]
+    rng = np.random.default_rng(1)
+    pot = 0.3 * rng.standard_normal(system.dims) - 0.4
+    for op in ops:
+        op.set_owned_potential(pot.reshape(-1))
+    return system, ops
+
+
+def _random_blocks(ops, m, seed):
+    import torch
+    rng = np.random.default_rng(seed)
+    return [torch.from_numpy(rng.standard_normal((m, op.global_len()))).cuda() for op in ops]
+
+
+@pytest.mark.parametrize("betas", [False, True])
+def test_fused_matches_three_stage(betas):
+    """The fused one-workgroup-per-column kernel against the three-stage
+    stick/plane path on the Al cell (25^3 grid): bit for bit without
+    projectors (same per-line FFT arithmetic); with projectors the nonlocal
+    sums differ in order only."""
+    import torch
+    system, ops = _al_ops(betas=betas)
+    for op in ops[:2]:
+        shape = op.fused_shape()
+        assert shape[0] > 0, "the Al k point fits one workgroup"
+        x = _random_blocks([op], 7, 3)[0]
+        y1 = torch.empty_like(x)
+        op.apply_block(x, y1)
+        op.set_fused_shape((0, 0, 0))
+        y0 = torch.empty_like(x)
+        op.apply_block(x, y0)
+        op.set_fused_shape(shape)
+        torch.cuda.synchronize()
+        d = (y1 - y0).abs().max().item() / y0.abs().max().item()
+        print(f"[fused] betas={betas} nk={op.nk} shape={shape} max relative difference {d:.2e}", flush=True)
+        if betas:
+            assert d < 1e-14
+        else:
+            assert torch.equal(y1, y0)
+
+
+def test_batch_apply_bitwise_equals_per_k():
+    """All k points of a 2x2x2 mesh (uneven nk) in one batched launch against
+    the per-k fused apply: bit for bit."""
+    import torch
+    from baandi_amd.kpoint import KPointBatch
+    system, ops = _al_ops(mesh=(3, 3, 3))
+    nks = sorted({op.nk for op in ops})
+    assert len(nks) > 1, "uneven basis sizes"
+    ms = [1 + (i % 8) for i in range(len(ops))]
+    xs = [_random_blocks([op], m, 10 + i)[0] for i, (op, m) in enumerate(zip(ops, ms))]
+    ys = [torch.empty_like(x) for x in xs]
+    KPointBatch(ops).apply(xs, ys)
+    torch.cuda.synchronize()
+    same = True
+    for op, x, y in zip(ops, xs, ys):
+        r = torch.empty_like(x)
+        op.apply_block(x, r)
+        torch.cuda.synchronize()
+        same = same and torch.equal(r, y)
+    print(f"[batch] {len(ops)} k points, nk {nks[0]}..{nks[-1]}, columns {sum(ms)}: bitwise {same}", flush=True)
+    assert same
+
+
+def test_autotuned_shape_is_bitwise_neutral():
+    import torch
+    system, ops = _al_ops(capacity=16)
+    op = ops[0]
+    x = _random_blocks([op], 16, 5)[0]
+    y0 = torch.empty_like(x)
+    op.apply_block(x, y0)
+    t, xl, zr, us = op.autotune(16, reps=3)
+    print(f"[autotune] nk={op.nk} picked threads={t} xlines={xl} zrows={zr}: {us:.1f} us per 16 columns", flush=True)
+    assert op.fused_shape() == (t, xl, zr)
+    y1 = torch.empty_like(x)
+    op.apply_block(x, y1)
+    torch.cuda.synchronize()
+    assert torch.equal(y0, y1)
+
+
+def test_fused_mixed_matches_fp64():
+    import torch
+    from baandi_amd import lib as L
+    _, ops64 = _al_ops()
+    _, ops32 = _al_ops(precision=L.BD_MIXED)
+    x = _random_blocks([ops64[0]], 5, 8)[0]
+    y64, y32 = torch.empty_like(x), torch.empty_like(x)
+    ops64[0].apply_block(x, y64)
+    ops32[0].apply_block(x, y32)
+    torch.cuda.synchronize()
+    d = (y64 - y32).abs().max().item() / y64.abs().max().item()
+    print(f"[fused mixed] shape={ops32[0].fused_shape()} relative difference to FP64 {d:.2e}", flush=True)
+    assert 1e-12 < d < 2e-5
