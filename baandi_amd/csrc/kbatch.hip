@@ -48,8 +48,8 @@ template <typename F, int NT>
 __global__ __launch_bounds__(NT) void k_hpsi_fused(const KWork* __restrict__ works, KWork one, int count, int total,
                                                    int n0,
                                                    int n1, int n2, const c2<double>* __restrict__ roots,
-                                                   RadixPlan rx, RadixPlan ry, RadixPlan rz, int xlines, int zrows,
-                                                   double inv_n) {
+                                                   RadixPlan rx, RadixPlan ry, RadixPlan rz, int xlines, int xplanes,
+                                                   int zrows, double inv_n) {
   extern __shared__ char smem[];
   c2<F>* rt = reinterpret_cast<c2<F>*>(smem);  // roots x | y | z
   c2<F>* rtx = rt;
@@ -61,8 +61,8 @@ __global__ __launch_bounds__(NT) void k_hpsi_fused(const KWork* __restrict__ wor
     const KWork w = works ? works[find_work(works, count, item)] : one;
     const int j = item - w.first;
     const int S = w.S, Z = w.Z, nk = w.nk;
-    c2<F>* P = L + S * n0;  // Z x n1
-    c2<F>* R = P + Z * n1;  // zrows x n2
+    c2<F>* P = L + S * n0;            // xplanes x Z x n1
+    c2<F>* R = P + xplanes * Z * n1;  // xplanes x zrows x n2
     const double* xc = w.x + int64_t(j) * w.ldx;
     double* yc = w.y + int64_t(j) * w.ldy;
     const F* pot = reinterpret_cast<const F*>(w.potential);
@@ -74,39 +74,47 @@ __global__ __launch_bounds__(NT) void k_hpsi_fused(const KWork* __restrict__ wor
     }
     __syncthreads();
     for (int s0 = 0; s0 < S; s0 += xlines) lds_fft<true>(L + s0 * n0, min(xlines, S - s0), n0, 1, rx, rtx);
-    for (int xi = 0; xi < n0; ++xi) {
-      for (int i = threadIdx.x; i < Z * n1; i += NT) P[i] = {F(0), F(0)};
+    // y-z planes, `xplanes` at a time: P = [plane][zc][y], R = [plane][row][z].
+    for (int x0 = 0; x0 < n0; x0 += xplanes) {
+      const int np = min(xplanes, n0 - x0);
+      for (int i = threadIdx.x; i < np * Z * n1; i += NT) P[i] = {F(0), F(0)};
       __syncthreads();
-      for (int s = threadIdx.x; s < S; s += NT) {
+      for (int i = threadIdx.x; i < np * S; i += NT) {
+        const int pl = i / S, s = i - pl * S;
         const int2 yz = w.yz[s];
-        P[yz.y * n1 + yz.x] = L[s * n0 + xi];
+        P[(pl * Z + yz.y) * n1 + yz.x] = L[s * n0 + x0 + pl];
       }
       __syncthreads();
-      lds_fft<true>(P, Z, n1, 1, ry, rty);
+      lds_fft<true>(P, np * Z, n1, 1, ry, rty);
       for (int y0 = 0; y0 < n1; y0 += zrows) {
         const int c = min(zrows, n1 - y0);
-        for (int i = threadIdx.x; i < c * n2; i += NT) R[i] = {F(0), F(0)};
+        for (int i = threadIdx.x; i < np * c * n2; i += NT) R[i] = {F(0), F(0)};
         __syncthreads();
-        for (int i = threadIdx.x; i < c * Z; i += NT) {
-          const int r = i / Z, zc = i - r * Z;
-          R[r * n2 + w.z_of_zc[zc]] = P[zc * n1 + y0 + r];
+        for (int i = threadIdx.x; i < np * c * Z; i += NT) {
+          const int pl = i / (c * Z), rem = i - pl * (c * Z);
+          const int r = rem / Z, zc = rem - r * Z;
+          R[(pl * c + r) * n2 + w.z_of_zc[zc]] = P[(pl * Z + zc) * n1 + y0 + r];
         }
         __syncthreads();
-        lds_fft<true>(R, c, n2, 1, rz, rtz);
-        const F* v = pot + (int64_t(xi) * n1 + y0) * n2;
-        for (int i = threadIdx.x; i < c * n2; i += NT) R[i] = cscale(R[i], v[i]);
+        lds_fft<true>(R, np * c, n2, 1, rz, rtz);
+        for (int i = threadIdx.x; i < np * c * n2; i += NT) {
+          const int pl = i / (c * n2), rem = i - pl * (c * n2);
+          R[i] = cscale(R[i], pot[(int64_t(x0 + pl) * n1 + y0) * n2 + rem]);
+        }
         __syncthreads();
-        lds_fft<false>(R, c, n2, 1, rz, rtz);
-        for (int i = threadIdx.x; i < c * Z; i += NT) {
-          const int r = i / Z, zc = i - r * Z;
-          P[zc * n1 + y0 + r] = R[r * n2 + w.z_of_zc[zc]];
+        lds_fft<false>(R, np * c, n2, 1, rz, rtz);
+        for (int i = threadIdx.x; i < np * c * Z; i += NT) {
+          const int pl = i / (c * Z), rem = i - pl * (c * Z);
+          const int r = rem / Z, zc = rem - r * Z;
+          P[(pl * Z + zc) * n1 + y0 + r] = R[(pl * c + r) * n2 + w.z_of_zc[zc]];
         }
         __syncthreads();
       }
-      lds_fft<false>(P, Z, n1, 1, ry, rty);
-      for (int s = threadIdx.x; s < S; s += NT) {
+      lds_fft<false>(P, np * Z, n1, 1, ry, rty);
+      for (int i = threadIdx.x; i < np * S; i += NT) {
+        const int pl = i / S, s = i - pl * S;
         const int2 yz = w.yz[s];
-        L[s * n0 + xi] = P[yz.y * n1 + yz.x];
+        L[s * n0 + x0 + pl] = P[(pl * Z + yz.y) * n1 + yz.x];
       }
       __syncthreads();
     }
@@ -186,7 +194,7 @@ void launch_fused_t(const GammaOp& op, const FusedShape& f, size_t lds, const KW
   }
   const int grid = std::max(1, std::min(total, slots));
   hipLaunchKernelGGL((k_hpsi_fused<F, NT>), dim3(grid), dim3(NT), lds, s, works, one, count, total, p.n0, p.n1, p.n2,
-                     op.d_roots.p, p.rx, p.ry, p.rz, f.xlines, f.zrows, 1.0 / double(op.N));
+                     op.d_roots.p, p.rx, p.ry, p.rz, f.xlines, f.xplanes, f.zrows, 1.0 / double(op.N));
   BD_HIP(hipGetLastError());
 }
 }  // namespace
@@ -194,16 +202,16 @@ void launch_fused_t(const GammaOp& op, const FusedShape& f, size_t lds, const KW
 size_t GammaOp::fused_lds(const FusedShape& f) const {
   const size_t e = precision == BD_MIXED ? sizeof(c2<float>) : sizeof(c2<double>);
   const StickPlan& p = plan;
-  return e * size_t(p.n0 + p.n1 + p.n2 + p.S * p.n0 + p.Z * p.n1 + f.zrows * p.n2);
+  return e * size_t(p.n0 + p.n1 + p.n2 + p.S * p.n0 + size_t(f.xplanes) * (p.Z * p.n1 + f.zrows * p.n2));
 }
 
 bool GammaOp::fused_valid(const FusedShape& f) const {
   if (!kb) return false;
   if (f.threads != 256 && f.threads != 512) return false;
-  if (f.xlines < 1 || f.zrows < 1) return false;
+  if (f.xlines < 1 || f.zrows < 1 || f.xplanes < 1 || f.xplanes > plan.n0) return false;
   if (int64_t(nproj) * capacity > kNlMaxQ) return false;
-  return fft_fits_threads(f.xlines, plan.rx, f.threads) && fft_fits_threads(plan.Z, plan.ry, f.threads) &&
-         fft_fits_threads(f.zrows, plan.rz, f.threads) && fused_lds(f) <= 160 * 1024;
+  return fft_fits_threads(f.xlines, plan.rx, f.threads) && fft_fits_threads(f.xplanes * plan.Z, plan.ry, f.threads) &&
+         fft_fits_threads(f.xplanes * f.zrows, plan.rz, f.threads) && fused_lds(f) <= 160 * 1024;
 }
 
 KWork GammaOp::kwork(const double* x, int64_t ldx, int m, double* y, int64_t ldy, int first) const {
@@ -334,10 +342,11 @@ extern "C" bd_status bd_k_batch_apply_dev(bd_k_batch* b, const double* const* x,
   });
 }
 
-// Time every valid fused shape (threads 256/512, x batches, z batches) on
-// m columns of the op's own scratch, reps launches each, and keep the
-// fastest. out4 (optional): threads, xlines, zrows, microseconds per launch.
-extern "C" bd_status bd_gamma_op_autotune(bd_gamma_op* h, int32_t m, int32_t reps, double* out4) {
+// Time every valid fused shape (threads 256/512, x batches, z batches,
+// planes at a time) on m columns of the op's own scratch, reps launches each,
+// and keep the fastest. out5 (optional): threads, xlines, zrows, xplanes,
+// microseconds per launch.
+extern "C" bd_status bd_gamma_op_autotune(bd_gamma_op* h, int32_t m, int32_t reps, double* out5) {
   BD_GUARD({
     if (!h) bd::fail(BD_ERR_INVALID_PARAMETER, "null op");
     bd::GammaOp& o = h->op;
@@ -356,11 +365,11 @@ extern "C" bd_status bd_gamma_op_autotune(bd_gamma_op* h, int32_t m, int32_t rep
         if (v == 1) break;
       }
       for (int xl : xs)
-        for (int zr : zs) {
-          bd::FusedShape f{t, xl, zr};
-          if (o.fused_valid(f)) cands.push_back(f);
-          if (cands.size() > 48) break;
-        }
+        for (int zr : zs)
+          for (int xp : {1, 2, 3, 4, 5, 6, 8}) {
+            bd::FusedShape f{t, xl, zr, xp};
+            if (o.fused_valid(f) && cands.size() < 96) cands.push_back(f);
+          }
     }
     if (cands.empty()) bd::fail(BD_ERR_APPLICATION, "no valid fused shape");
     // Inputs: the op's staging buffers (contents do not affect the timing).
@@ -389,36 +398,38 @@ extern "C" bd_status bd_gamma_op_autotune(bd_gamma_op* h, int32_t m, int32_t rep
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(z);
     o.fshape = pick;
-    if (out4) {
-      out4[0] = pick.threads;
-      out4[1] = pick.xlines;
-      out4[2] = pick.zrows;
-      out4[3] = best;
+    if (out5) {
+      out5[0] = pick.threads;
+      out5[1] = pick.xlines;
+      out5[2] = pick.zrows;
+      out5[3] = pick.xplanes;
+      out5[4] = best;
     }
   });
 }
 
-// Read or set the fused shape (threads, xlines, zrows); set validates it.
+// Read or set the fused shape (threads, xlines, zrows, xplanes); set validates it.
 // Setting threads = 0 switches the op to the three-stage path; reading gives
 // threads = 0 on that path.
-extern "C" bd_status bd_gamma_op_fused_shape(bd_gamma_op* h, int32_t* inout3, int32_t set) {
+extern "C" bd_status bd_gamma_op_fused_shape(bd_gamma_op* h, int32_t* inout4, int32_t set) {
   BD_GUARD({
-    if (!h || !inout3) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    if (!h || !inout4) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
     bd::GammaOp& o = h->op;
     if (!o.kb) bd::fail(BD_ERR_INVALID_PARAMETER, "not a k-point op");
-    if (set && inout3[0] == 0) {
+    if (set && inout4[0] == 0) {
       o.fused = false;
     } else if (set) {
-      bd::FusedShape f{inout3[0], inout3[1], inout3[2]};
+      bd::FusedShape f{inout4[0], inout4[1], inout4[2], inout4[3]};
       if (!o.fused_valid(f)) bd::fail(BD_ERR_INVALID_PARAMETER, "invalid fused shape");
       o.fshape = f;
       o.fused = true;
     } else if (!o.fused) {
-      inout3[0] = inout3[1] = inout3[2] = 0;
+      inout4[0] = inout4[1] = inout4[2] = inout4[3] = 0;
     } else {
-      inout3[0] = o.fshape.threads;
-      inout3[1] = o.fshape.xlines;
-      inout3[2] = o.fshape.zrows;
+      inout4[0] = o.fshape.threads;
+      inout4[1] = o.fshape.xlines;
+      inout4[2] = o.fshape.zrows;
+      inout4[3] = o.fshape.xplanes;
     }
   });
 }

@@ -617,6 +617,12 @@ GammaOp::GammaOp(const System& s, const KBasis* k, int cap, int prec) : sys(&s),
     while (f.xlines > 1 && !fft_fits_threads(f.xlines, plan.rx, f.threads)) f.xlines = (f.xlines + 1) / 2;
     f.zrows = plan.n1;
     while (f.zrows > 1 && !fft_fits_threads(f.zrows, plan.rz, f.threads)) f.zrows = (f.zrows + 1) / 2;
+    // Planes per pass: as many as keep two workgroups per CU (80 KB each).
+    for (int xp = 8; xp >= 1; --xp) {
+      f.xplanes = xp;
+      if (fused_valid(f) && fused_lds(f) <= 80 * 1024) break;
+    }
+    if (!fused_valid(f)) f.xplanes = 1;
     fused = fused_valid(f) && fused_lds(f) <= lds_max;
     if (fused) fshape = f;
   }

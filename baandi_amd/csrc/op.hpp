@@ -72,12 +72,13 @@ struct KWork {
 };
 
 // Launch shape of the fused kernel: threads per workgroup, sticks per x-FFT
-// batch and y rows per z-FFT batch (the per-line arithmetic does not depend
-// on them, so every valid shape gives the same bits).
+// batch, y rows per z-FFT batch and y-z planes per pass (the per-line
+// arithmetic does not depend on them, so every valid shape gives the same bits).
 struct FusedShape {
   int threads = 256;
   int xlines = 0;
   int zrows = 0;
+  int xplanes = 1;  // y-z planes transformed together
 };
 
 struct DavidsonWork;

@@ -163,24 +163,25 @@ class KPointHamiltonian(DistributedRealHamiltonian):
         return p2[0::2, :p] + 1j * p2[1::2, :p]
 
     def fused_shape(self):
-        """(threads, xlines, zrows) of the fused one-workgroup-per-column
-        kernel, or (0, 0, 0) on the three-stage path."""
-        v = np.zeros(3, dtype=np.int32)
+        """(threads, xlines, zrows, xplanes) of the fused one-workgroup-per-column
+        kernel, or (0, 0, 0, 0) on the three-stage path."""
+        v = np.zeros(4, dtype=np.int32)
         L.check(L.load().bd_gamma_op_fused_shape(self.handle, L.ptr(v), 0))
         return tuple(int(a) for a in v)
 
     def set_fused_shape(self, shape):
-        """Select a fused shape, or (0, 0, 0) for the three-stage path."""
-        v = np.asarray(shape, dtype=np.int32)
+        """Select a fused shape, or (0, 0, 0, 0) for the three-stage path."""
+        v = np.zeros(4, dtype=np.int32)
+        v[:len(shape)] = shape
         L.check(L.load().bd_gamma_op_fused_shape(self.handle, L.ptr(v), 1))
 
     def autotune(self, columns, reps=5):
         """Time every valid fused shape on `columns` columns and keep the
-        fastest; returns (threads, xlines, zrows, microseconds per launch)."""
-        out = np.zeros(4)
+        fastest; returns (threads, xlines, zrows, xplanes, microseconds per launch)."""
+        out = np.zeros(5)
         self.enter()
         L.check(L.load().bd_gamma_op_autotune(self.handle, int(columns), int(reps), L.ptr(out)))
-        return int(out[0]), int(out[1]), int(out[2]), float(out[3])
+        return int(out[0]), int(out[1]), int(out[2]), int(out[3]), float(out[4])
 
     def apply_complex(self, x):
         """H x for a CUDA complex128 tensor (m, nk)."""

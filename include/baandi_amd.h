@@ -184,11 +184,13 @@ void* bd_k_batch_stream(bd_k_batch* batch);
 bd_status bd_k_batch_apply_dev(bd_k_batch* batch, const double* const* x_dev, const int64_t* ldx,
                                const int32_t* m, double* const* y_dev, const int64_t* ldy);
 /* Launch-shape autotuner of the fused kernel: times every valid (threads,
- * sticks per x-FFT batch, rows per z-FFT batch) on m columns, reps launches
- * each, keeps the fastest. out4: threads, xlines, zrows, microseconds. */
-bd_status bd_gamma_op_autotune(bd_gamma_op* op, int32_t m, int32_t reps, double* out4);
-/* Read (set = 0) or set the fused shape; threads = 0 selects the three-stage path. */
-bd_status bd_gamma_op_fused_shape(bd_gamma_op* op, int32_t* inout3, int32_t set);
+ * sticks per x-FFT batch, rows per z-FFT batch, y-z planes per pass) on m
+ * columns, reps launches each, keeps the fastest.
+ * out5: threads, xlines, zrows, xplanes, microseconds per launch. */
+bd_status bd_gamma_op_autotune(bd_gamma_op* op, int32_t m, int32_t reps, double* out5);
+/* Read (set = 0) or set the fused shape (threads, xlines, zrows, xplanes);
+ * threads = 0 selects the three-stage path. */
+bd_status bd_gamma_op_fused_shape(bd_gamma_op* op, int32_t* inout4, int32_t set);
 
 /* ------------------------------------------------------------------------
  * Reference block Davidson on the op

@@ -219,7 +219,7 @@ def test_fused_matches_three_stage(betas):
         x = _random_blocks([op], 7, 3)[0]
         y1 = torch.empty_like(x)
         op.apply_block(x, y1)
-        op.set_fused_shape((0, 0, 0))
+        op.set_fused_shape((0, 0, 0, 0))
         y0 = torch.empty_like(x)
         op.apply_block(x, y0)
         op.set_fused_shape(shape)
@@ -262,9 +262,10 @@ def test_autotuned_shape_is_bitwise_neutral():
     x = _random_blocks([op], 16, 5)[0]
     y0 = torch.empty_like(x)
     op.apply_block(x, y0)
-    t, xl, zr, us = op.autotune(16, reps=3)
-    print(f"[autotune] nk={op.nk} picked threads={t} xlines={xl} zrows={zr}: {us:.1f} us per 16 columns", flush=True)
-    assert op.fused_shape() == (t, xl, zr)
+    t, xl, zr, xp, us = op.autotune(16, reps=3)
+    print(f"[autotune] nk={op.nk} picked threads={t} xlines={xl} zrows={zr} xplanes={xp}: {us:.1f} us per 16 "
+          f"columns", flush=True)
+    assert op.fused_shape() == (t, xl, zr, xp)
     y1 = torch.empty_like(x)
     op.apply_block(x, y1)
     torch.cuda.synchronize()
