@@ -9,11 +9,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <limits>
 #include <numeric>
+#include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -548,11 +551,215 @@ DavidsonWork* davidson_work(GammaOp& op, int64_t n, int nb, int cap) {
 void destroy_davidson_work(DavidsonWork* w) { delete w; }
 }  // namespace bd
 
-extern "C" bd_status bd_davidson_solve_dev(bd_gamma_op* h, const double* x0, int64_t ldx0, const double* hx0,
-                                           const double* diag, const bd_davidson_options* o, double* eig,
-                                           double* xout, double* hxout, int64_t ldx, int32_t* conv,
-                                           bd_davidson_record* record) {
-  bd::clear_last_error();
+namespace bd {
+namespace {
+// dst = src, count doubles (a kernel, so that it may sit in a captured graph).
+__global__ void k_copy_d(const double* __restrict__ src, int64_t count, double* __restrict__ dst) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += int64_t(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+
+// |E[:, j]|^2 per column, one workgroup per column.
+__global__ __launch_bounds__(kT) void k_col_norm2(const double* __restrict__ E, int64_t n, double* __restrict__ out) {
+  __shared__ double red[16];
+  const double* e = E + int64_t(blockIdx.x) * n;
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) acc += e[i] * e[i];
+  const double v = block_sum(acc, red);
+  if (threadIdx.x == 0) out[blockIdx.x] = v;
+}
+
+// The correction of a converged band (|e| < 1e-9, whose rounding-level
+// residual carries no direction) or one lying in span(X) (|e_perp|^2 <=
+// 1e-20 |e|^2) is replaced by a fixed pseudo-random direction, so that
+// [X, W] stays independent with fixed shapes.
+__global__ __launch_bounds__(kT) void k_fix_weak(double* __restrict__ E, int64_t n, const double* __restrict__ before,
+                                                 const double* __restrict__ after) {
+  const int j = blockIdx.x;
+  if (before[j] >= 1e-18 && after[j] > 1e-20 * before[j]) return;
+  double* e = E + int64_t(j) * n;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    uint64_t h = (uint64_t(i) * 0x9E3779B97F4A7C15ull) ^ (uint64_t(j + 1) * 0xC2B2AE3D27D4EB4Full);
+    h ^= h >> 31;
+    h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 29;
+    e[i] = double(h >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+  }
+}
+
+// dst[:, j] = src[:, j] / |src[:, j]| with |.|^2 = norm2[j] (0 for a zero column).
+__global__ void k_unit_cols(const double* __restrict__ src, int64_t n, const double* __restrict__ norm2,
+                            double* __restrict__ dst) {
+  const int j = blockIdx.y;
+  const double v = norm2[j];
+  const double sc = v > 0.0 ? 1.0 / sqrt(v) : 0.0;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    dst[int64_t(j) * n + i] = src[int64_t(j) * n + i] * sc;
+}
+}  // namespace
+
+// Canonical orthonormalisation of a small Gram matrix: given G = U diag(l) U^T
+// (ascending l), Tc[:, i] = U[:, i] / sqrt(l_i) for l_i > kCanonRel l_max,
+// else 0 (the dropped direction); dropped[i] = 1 for those.
+constexpr double kCanonRel = 1e-10;
+__global__ void k_canon(const double* __restrict__ U, const double* __restrict__ l, int k, double* __restrict__ Tc,
+                        int* __restrict__ dropped) {
+  const int i = blockIdx.y;
+  const double lmax = l[k - 1];
+  const bool keep = l[i] > kCanonRel * lmax && lmax > 0.0;
+  const double sc = keep ? 1.0 / sqrt(l[i]) : 0.0;
+  for (int r = threadIdx.x; r < k; r += blockDim.x) Tc[int64_t(i) * k + r] = U[int64_t(i) * k + r] * sc;
+  if (threadIdx.x == 0) dropped[i] = keep ? 0 : 1;
+}
+
+// A[off + i, off + i] += big for every dropped direction i: a zero column of
+// the basis then sits far above the wanted eigenvalues.
+__global__ void k_lift_dropped(double* __restrict__ A, int64_t lda, int off, const int* __restrict__ dropped, int k,
+                               double big) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < k && dropped[i]) A[int64_t(off + i) * lda + off + i] += big;
+}
+
+// One fixed-shape block Davidson iteration: the nb Ritz vectors X (with HX
+// and their Ritz values) and their nb preconditioned residual directions W
+// span V = [X, W]. W is made orthogonal to X (two projections) and
+// orthonormalised canonically (directions of a near-singular Gram matrix are
+// dropped), so V is orthonormal; Rayleigh-Ritz on V (A = V^T HV, lowest nb
+// eigenpairs Y) gives X = V Y, HX = HV Y. Every step is device work on the
+// op's stream with fixed shapes and no host round trip, so the iteration is
+// captured once as a HIP graph and replayed. It is the reference iteration
+// (ref_davidson.rs:117-366) with every band active and a restart to the Ritz
+// block after each step.
+struct BlockGraph {
+  GammaOp& op;
+  int64_t n;
+  int nb, m2;
+  int mode;  // 0 QE, 1 TPA
+  Dev X, HX, V, HV, E, theta, norm2, norm2b, diag, G, lam, U, Tc, A, Y, small, eigw, gwork;
+  int* info = nullptr;     // [0] eigensolver of A, [1] eigensolver of G
+  int* dropped = nullptr;  // nb
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  BlockGraph(GammaOp& o, int nb_, int precond) : op(o), n(o.n), nb(nb_), m2(2 * nb_), mode(precond) {
+    const size_t c2 = size_t(m2) * m2;
+    X.alloc(size_t(n) * nb);
+    HX.alloc(size_t(n) * nb);
+    V.alloc(size_t(n) * m2);
+    HV.alloc(size_t(n) * m2);
+    E.alloc(size_t(n) * nb);
+    theta.alloc(nb);
+    norm2.alloc(nb);
+    norm2b.alloc(nb);
+    diag.alloc(size_t(n));
+    G.alloc(size_t(nb) * nb);
+    lam.alloc(nb);
+    U.alloc(size_t(nb) * nb);
+    Tc.alloc(size_t(nb) * nb);
+    A.alloc(c2);
+    Y.alloc(size_t(m2) * nb);
+    small.alloc(size_t(nb) * nb);
+    eigw.alloc(size_t(std::max(sym_eigen_work(m2, nb), sym_eigen_work(nb, nb))));
+    gwork.alloc(size_t(std::max(Gemm::tn_work(m2, m2, n), int64_t(c2))));
+    BD_HIP(hipMalloc(&info, 2 * sizeof(int)));
+    BD_HIP(hipMalloc(&dropped, sizeof(int) * nb));
+  }
+  ~BlockGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (info) (void)hipFree(info);
+    if (dropped) (void)hipFree(dropped);
+  }
+
+  // dst (n x nb) = src Tc with Tc from the canonical orthonormalisation of
+  // src^T src (src is overwritten by nothing; dst must differ from src).
+  void canonical(const double* src, double* dst) {
+    hipStream_t s = op.stream;
+    op.gemm.tn(s, nb, nb, n, 1.0, src, n, src, n, 0.0, G.p, nb, gwork.p);
+    symmetrize(s, nb, G.p, nb);
+    sym_eigen_lowest(s, nb, nb, G.p, nb, lam.p, U.p, nb, eigw.p, info + 1);
+    hipLaunchKernelGGL(k_canon, dim3(1, nb), dim3(64), 0, s, U.p, lam.p, nb, Tc.p, dropped);
+    op.gemm.nn(s, n, nb, nb, 1.0, src, n, Tc.p, nb, 0.0, dst, n);
+  }
+
+  // Rayleigh-Ritz on the first m columns of the orthonormal V (HV = H V).
+  void ritz(int m) {
+    hipStream_t s = op.stream;
+    op.gemm.tn(s, m, m, n, 1.0, V.p, n, HV.p, n, 0.0, A.p, m, gwork.p);
+    symmetrize(s, m, A.p, m);
+    hipLaunchKernelGGL(k_lift_dropped, dim3(1), dim3(256), 0, s, A.p, int64_t(m), m - nb, dropped, nb, 1e6);
+    sym_eigen_lowest(s, m, nb, A.p, m, theta.p, Y.p, m, eigw.p, info);
+    op.gemm.nn(s, n, nb, m, 1.0, V.p, n, Y.p, m, 0.0, X.p, n);
+    op.gemm.nn(s, n, nb, m, 1.0, HV.p, n, Y.p, m, 0.0, HX.p, n);
+  }
+
+  // The iteration's device work (captured once, then replayed).
+  void enqueue() {
+    hipStream_t s = op.stream;
+    const unsigned g = unsigned(std::min<int64_t>((n * nb + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_corrections, dim3(nb), dim3(kT), 0, s, X.p, HX.p, n, theta.p, diag.p, op.d_kinetic.p, mode,
+                       E.p, norm2.p);
+    // W: corrections with span(X) projected out (twice), weak ones replaced.
+    auto project = [&]() {
+      op.gemm.tn(s, nb, nb, n, 1.0, X.p, n, E.p, n, 0.0, small.p, nb, gwork.p);
+      op.gemm.nn(s, n, nb, nb, -1.0, X.p, n, small.p, nb, 1.0, E.p, n);
+    };
+    project();
+    hipLaunchKernelGGL(k_col_norm2, dim3(nb), dim3(kT), 0, s, E.p, n, norm2b.p);
+    hipLaunchKernelGGL(k_fix_weak, dim3(nb), dim3(kT), 0, s, E.p, n, norm2.p, norm2b.p);
+    project();
+    project();
+    hipLaunchKernelGGL(k_copy_d, dim3(g), dim3(256), 0, s, X.p, n * nb, V.p);
+    hipLaunchKernelGGL(k_copy_d, dim3(g), dim3(256), 0, s, HX.p, n * nb, HV.p);
+    canonical(E.p, V.p + n * nb);
+    op.apply(V.p + n * nb, n, nb, HV.p + n * nb, n);
+    ritz(m2);
+  }
+
+  void start(const double* x0, int64_t ldx0) {
+    hipStream_t s = op.stream;
+    BD_HIP(hipMemcpy2DAsync(E.p, n * 8, x0, ldx0 * 8, n * 8, nb, hipMemcpyDeviceToDevice, s));
+    canonical(E.p, V.p);
+    op.apply(V.p, n, nb, HV.p, n);
+    ritz(nb);
+    check();
+  }
+
+  void step(bool use_graph) {
+    if (!use_graph) {
+      enqueue();
+    } else {
+      if (!exec) {
+        if (op.profiling) fail(BD_ERR_INVALID_PARAMETER, "graph capture needs profiling off");
+        BD_HIP(hipStreamBeginCapture(op.stream, hipStreamCaptureModeThreadLocal));
+        enqueue();
+        BD_HIP(hipStreamEndCapture(op.stream, &graph));
+        BD_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      }
+      BD_HIP(hipGraphLaunch(exec, op.stream));
+    }
+  }
+
+  // Verdicts of the last iteration (synchronises).
+  void check() {
+    int v[2] = {0, 0};
+    BD_HIP(hipMemcpyAsync(v, info, sizeof(v), hipMemcpyDeviceToHost, op.stream));
+    BD_HIP(hipStreamSynchronize(op.stream));
+    if (v[0] || v[1]) fail(BD_ERR_APPLICATION, "block iteration: reduced eigensolve failed");
+  }
+};
+}  // namespace bd
+
+struct bd_block {
+  bd::BlockGraph g;
+  bd_block(bd::GammaOp& op, int nb, int mode) : g(op, nb, mode) {}
+};
+
+namespace {
+// One solve on one op (the body of bd_davidson_solve_dev); errors as status
+// plus message.
+bd_status solve_one(bd_gamma_op* h, const double* x0, int64_t ldx0, const double* hx0, const double* diag,
+                    const bd_davidson_options* o, double* eig, double* xout, double* hxout, int64_t ldx,
+                    int32_t* conv, bd_davidson_record* record, std::string* msg) {
   try {
     if (!h || !x0 || !diag || !o || !eig || !xout) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
     auto& op = h->op;
@@ -572,13 +779,155 @@ extern "C" bd_status bd_davidson_solve_dev(bd_gamma_op* h, const double* x0, int
     bd::Solver solver(op, *w, *o, diag);
     bd_status st = solver.run(x0, ldx0, hx0, eig, xout, o->return_hx ? hxout : nullptr, ldx, conv);
     if (record) *record = solver.rec;
-    if (st == BD_NOT_CONVERGED) bd::set_last_error("reference Davidson reached the iteration limit");
+    if (st == BD_NOT_CONVERGED) *msg = "reference Davidson reached the iteration limit";
     return st;
   } catch (const bd::Error& e) {
-    bd::set_last_error(e.what());
+    *msg = e.what();
     return e.status;
   } catch (const std::exception& e) {
-    bd::set_last_error(e.what());
+    *msg = e.what();
     return BD_ERR_APPLICATION;
   }
+}
+}  // namespace
+
+extern "C" bd_status bd_davidson_solve_dev(bd_gamma_op* h, const double* x0, int64_t ldx0, const double* hx0,
+                                           const double* diag, const bd_davidson_options* o, double* eig,
+                                           double* xout, double* hxout, int64_t ldx, int32_t* conv,
+                                           bd_davidson_record* record) {
+  bd::clear_last_error();
+  std::string msg;
+  const bd_status st = solve_one(h, x0, ldx0, hx0, diag, o, eig, xout, hxout, ldx, conv, record, &msg);
+  if (st != BD_OK) bd::set_last_error(msg);
+  return st;
+}
+
+// Independent solves on `count` distinct ops, `threads` host threads at a
+// time, each op on its own stream (the k points of an SCF map). statuses[i]
+// per solve; the return value is the first non-OK status other than
+// BD_NOT_CONVERGED (then its message is the last error), else BD_NOT_CONVERGED
+// if any solve hit its iteration limit, else BD_OK.
+extern "C" bd_status bd_davidson_solve_many_dev(int32_t count, bd_gamma_op* const* ops, const double* const* x0,
+                                                const int64_t* ldx0, const double* const* diag,
+                                                const bd_davidson_options* options, double* const* eig,
+                                                double* const* xout, double* const* hxout, const int64_t* ldx,
+                                                int32_t* const* conv, bd_davidson_record* records,
+                                                bd_status* statuses, int32_t threads) {
+  bd::clear_last_error();
+  if (count < 0 || !ops || !x0 || !ldx0 || !diag || !options || !eig || !xout || !ldx || !statuses) {
+    bd::set_last_error("null argument");
+    return BD_ERR_INVALID_PARAMETER;
+  }
+  for (int i = 0; i < count; ++i)
+    for (int j = 0; j < i; ++j)
+      if (ops[i] == ops[j]) {
+        bd::set_last_error("bd_davidson_solve_many_dev: an op appears twice");
+        return BD_ERR_INVALID_PARAMETER;
+      }
+  std::vector<std::string> msgs(count);
+  std::atomic<int> next{0};
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  auto worker = [&]() {
+    (void)hipSetDevice(dev);
+    for (int i = next.fetch_add(1); i < count; i = next.fetch_add(1))
+      statuses[i] = solve_one(ops[i], x0[i], ldx0[i], nullptr, diag[i], &options[i], eig[i], xout[i],
+                              hxout ? hxout[i] : nullptr, ldx[i], conv ? conv[i] : nullptr,
+                              records ? &records[i] : nullptr, &msgs[i]);
+  };
+  const int nt = std::max(1, std::min<int>(threads, count));
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
+  worker();
+  for (auto& t : pool) t.join();
+  bd_status result = BD_OK;
+  for (int i = 0; i < count; ++i) {
+    if (statuses[i] != BD_OK && statuses[i] != BD_NOT_CONVERGED) {
+      bd::set_last_error("k " + std::to_string(i) + ": " + msgs[i]);
+      return statuses[i];
+    }
+    if (statuses[i] == BD_NOT_CONVERGED) result = BD_NOT_CONVERGED;
+  }
+  return result;
+}
+
+// ---------------------------------------------------------------------------
+// Fixed-shape block iteration, eager or replayed from a captured HIP graph
+// ---------------------------------------------------------------------------
+extern "C" bd_status bd_block_create(bd_gamma_op* h, int32_t bands, int32_t preconditioner, bd_block** out) {
+  BD_GUARD({
+    if (!h || !out) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    auto& op = h->op;
+    if (bands <= 0 || 2 * int64_t(bands) > op.n || bands > op.capacity)
+      bd::fail(BD_ERR_INVALID_PARAMETER, "block iteration: bands vs rows and capacity");
+    if (preconditioner != BD_PRECOND_QE && preconditioner != BD_PRECOND_TPA)
+      bd::fail(BD_ERR_INVALID_PARAMETER, "block iteration: QE or TPA preconditioner");
+    *out = new bd_block(op, bands, preconditioner == BD_PRECOND_QE ? 0 : 1);
+  });
+}
+
+extern "C" void bd_block_destroy(bd_block* b) { delete b; }
+
+// The Davidson diagonal (device, n rows); read by every later step.
+extern "C" bd_status bd_block_set_diagonal_dev(bd_block* b, const double* diag) {
+  BD_GUARD({
+    if (!b || !diag) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    BD_HIP(hipMemcpyAsync(b->g.diag.p, diag, size_t(b->g.n) * 8, hipMemcpyDeviceToDevice, b->g.op.stream));
+    BD_HIP(hipStreamSynchronize(b->g.op.stream));
+  });
+}
+
+// Rayleigh-Ritz of the start block (n x bands, ldx0): X, HX and the values.
+extern "C" bd_status bd_block_start_dev(bd_block* b, const double* x0, int64_t ldx0, double* values) {
+  BD_GUARD({
+    if (!b || !x0 || !values) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    auto& g = b->g;
+    if (ldx0 < g.n) bd::fail(BD_ERR_INVALID_PARAMETER, "leading dimension smaller than rows");
+    if (!g.op.has_potential) bd::fail(BD_ERR_INVALID_PARAMETER, "no potential installed");
+    g.start(x0, ldx0);
+    BD_HIP(hipMemcpy(values, g.theta.p, size_t(g.nb) * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+// `steps` iterations (use_graph: captured on the first call, then replayed);
+// values (host, bands) of the last one. Synchronises once at the end.
+extern "C" bd_status bd_block_step(bd_block* b, int32_t steps, int32_t use_graph, double* values) {
+  BD_GUARD({
+    if (!b || !values || steps <= 0) bd::fail(BD_ERR_INVALID_PARAMETER, "block step arguments");
+    auto& g = b->g;
+    for (int i = 0; i < steps; ++i) g.step(use_graph != 0);
+    BD_HIP(hipMemcpyAsync(values, g.theta.p, size_t(g.nb) * 8, hipMemcpyDeviceToHost, g.op.stream));
+    g.check();
+  });
+}
+
+// Enqueue `steps` iterations without waiting (several ops' streams can then
+// run side by side); bd_block_finish waits and reads the values.
+extern "C" bd_status bd_block_step_async(bd_block* b, int32_t steps, int32_t use_graph) {
+  BD_GUARD({
+    if (!b || steps <= 0) bd::fail(BD_ERR_INVALID_PARAMETER, "block step arguments");
+    for (int i = 0; i < steps; ++i) b->g.step(use_graph != 0);
+  });
+}
+
+extern "C" bd_status bd_block_finish(bd_block* b, double* values) {
+  BD_GUARD({
+    if (!b || !values) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    auto& g = b->g;
+    BD_HIP(hipMemcpyAsync(values, g.theta.p, size_t(g.nb) * 8, hipMemcpyDeviceToHost, g.op.stream));
+    g.check();
+  });
+}
+
+// Copy X (and HX when hx is not NULL) out, n x bands with leading dimension ld.
+extern "C" bd_status bd_block_vectors_dev(bd_block* b, double* x, double* hx, int64_t ld) {
+  BD_GUARD({
+    if (!b || !x) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    auto& g = b->g;
+    if (ld < g.n) bd::fail(BD_ERR_INVALID_PARAMETER, "leading dimension smaller than rows");
+    BD_HIP(hipMemcpy2DAsync(x, ld * 8, g.X.p, g.n * 8, g.n * 8, g.nb, hipMemcpyDeviceToDevice, g.op.stream));
+    if (hx)
+      BD_HIP(hipMemcpy2DAsync(hx, ld * 8, g.HX.p, g.n * 8, g.n * 8, g.nb, hipMemcpyDeviceToDevice, g.op.stream));
+    BD_HIP(hipStreamSynchronize(g.op.stream));
+  });
 }

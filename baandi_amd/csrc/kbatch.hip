@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -174,23 +175,30 @@ template <typename F, int NT>
 void launch_fused_t(const GammaOp& op, const FusedShape& f, size_t lds, const KWork* works, const KWork& one,
                     int count, int total, hipStream_t s) {
   const StickPlan& p = op.plan;
-  // Resident workgroups for this LDS size, looked up once per size.
+  // Resident workgroups for this LDS size, looked up once per size (ops may
+  // launch from several host threads).
+  static std::mutex mu;
   static bool attr = false;
   static size_t cached_lds = 0;
-  static int slots = 0;
-  if (!attr) {
-    BD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_hpsi_fused<F, NT>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    attr = true;
-  }
-  if (cached_lds != lds) {
-    int dev = 0, cus = 0, per_cu = 0;
-    BD_HIP(hipGetDevice(&dev));
-    BD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    BD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void*>(&k_hpsi_fused<F, NT>), NT, lds));
-    slots = cus * std::max(per_cu, 1);
-    cached_lds = lds;
+  static int cached_slots = 0;
+  int slots = 0;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    if (!attr) {
+      BD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_hpsi_fused<F, NT>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+      attr = true;
+    }
+    if (cached_lds != lds) {
+      int dev = 0, cus = 0, per_cu = 0;
+      BD_HIP(hipGetDevice(&dev));
+      BD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      BD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, reinterpret_cast<const void*>(&k_hpsi_fused<F, NT>), NT, lds));
+      cached_slots = cus * std::max(per_cu, 1);
+      cached_lds = lds;
+    }
+    slots = cached_slots;
   }
   const int grid = std::max(1, std::min(total, slots));
   hipLaunchKernelGGL((k_hpsi_fused<F, NT>), dim3(grid), dim3(NT), lds, s, works, one, count, total, p.n0, p.n1, p.n2,

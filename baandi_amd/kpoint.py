@@ -316,6 +316,8 @@ class KScfOptions:
     preconditioner: int = L.BD_PRECOND_TPA
     precision: int = L.BD_FP64
     seed: int = 1
+    threads: int = 8                # k points solved concurrently (host threads, one stream each)
+    preconditioning: str = "plain"  # density mixing: "plain" or "thomas_fermi"
 
 
 def _complex_start(nk, bands, seed):
@@ -358,7 +360,7 @@ def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
     x = system.z_valence / system.volume * np.exp(-field.g2 * system.rc ** 2 / 4.0) * sfac
     x[field.zero_np] = electrons / system.volume
     x = torch.from_numpy(x).to(dev)
-    mixer = Mixer(field, options.beta, options.history)
+    mixer = Mixer(field, options.beta, options.history, options.preconditioning, system.volume, electrons)
     orbitals = [realified_start(_complex_start(op.nk, b, options.seed + i).to(dev)) for i, op in enumerate(ops)]
     outs = [torch.empty_like(o) for o in orbitals]
     rho_dev = torch.empty(field.size, dtype=torch.float64, device=dev)
@@ -388,14 +390,14 @@ def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
                 op.set_owned_potential(potential)
             totals["fields_s"] += time.perf_counter() - tf
             td = time.perf_counter()
-            sols = []
-            for i, op in enumerate(ops):
-                opts = ref_davidson.Options(bands=2 * b, threshold=ethr / 2.0, occupied=2 * b,
-                                            first_step=(iteration == 1 and not redo),
-                                            max_iterations=options.davidson_max_iterations,
-                                            preconditioner=options.preconditioner)
-                sol = ref_davidson.solve(op, orbitals[i], diag_static[i] + vmean, opts, out=(outs[i], None))
-                sols.append(sol)
+            opts = ref_davidson.Options(bands=2 * b, threshold=ethr / 2.0, occupied=2 * b,
+                                        first_step=(iteration == 1 and not redo),
+                                        max_iterations=options.davidson_max_iterations,
+                                        preconditioner=options.preconditioner)
+            diags = [d + vmean for d in diag_static]
+            sols = ref_davidson.solve_many(ops, orbitals, diags, [opts] * len(ops), [(o, None) for o in outs],
+                                           threads=options.threads)
+            for sol in sols:
                 totals["h_columns"] += sol.record["h_columns"]
                 totals["davidson_iterations"] += sol.record["iterations"]
                 totals["dense_s"] += sol.record["seconds_dense"]

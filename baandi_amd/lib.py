@@ -22,6 +22,8 @@ EXPORTS = [
     "bd_gamma_op_set_potential_dev", "bd_gamma_op_apply", "bd_gamma_op_apply_dev",
     "bd_gamma_op_band_density_dev", "bd_gamma_op_nonlocal_energies_dev", "bd_gamma_op_stream",
     "bd_gamma_op_set_profiling", "bd_gamma_op_stage_times", "bd_gamma_op_plan_info", "bd_davidson_solve_dev",
+    "bd_davidson_solve_many_dev", "bd_block_create", "bd_block_destroy", "bd_block_set_diagonal_dev",
+    "bd_block_start_dev", "bd_block_step", "bd_block_step_async", "bd_block_finish", "bd_block_vectors_dev",
     "bd_k_op_create", "bd_k_op_info", "bd_k_op_get", "bd_k_batch_create", "bd_k_batch_destroy",
     "bd_k_batch_stream", "bd_k_batch_apply_dev", "bd_gamma_op_autotune", "bd_gamma_op_fused_shape",
     "bd_gemm_workspace", "bd_gemm_dev", "bd_mfma_probe_dev",
@@ -120,6 +122,15 @@ def load():
         "bd_gamma_op_plan_info": [P, P],
         "bd_davidson_solve_dev": [P, P, I64, P, P, C.POINTER(DavidsonOptions), P, P, P, I64, P,
                                   C.POINTER(DavidsonRecord)],
+        "bd_davidson_solve_many_dev": [I32, P, P, P, P, C.POINTER(DavidsonOptions), P, P, P, P, P,
+                                       C.POINTER(DavidsonRecord), P, I32],
+        "bd_block_create": [P, I32, I32, C.POINTER(P)],
+        "bd_block_set_diagonal_dev": [P, P],
+        "bd_block_start_dev": [P, P, I64, P],
+        "bd_block_step": [P, I32, I32, P],
+        "bd_block_step_async": [P, I32, I32],
+        "bd_block_finish": [P, P],
+        "bd_block_vectors_dev": [P, P, P, I64],
         "bd_gemm_dev": [I32, I32, I64, I64, I64, D, P, I64, P, I64, D, P, I64, P],
         "bd_mfma_probe_dev": [P, P, P],
         "bd_sym_eigen_dev": [I32, I32, P, I64, P, P, I64, P],
@@ -140,6 +151,8 @@ def load():
     lib.bd_gamma_op_destroy.restype = None
     lib.bd_gamma_op_stream.argtypes = [P]
     lib.bd_gamma_op_stream.restype = P
+    lib.bd_block_destroy.argtypes = [P]
+    lib.bd_block_destroy.restype = None
     lib.bd_k_batch_destroy.argtypes = [P]
     lib.bd_k_batch_destroy.restype = None
     lib.bd_k_batch_stream.argtypes = [P]

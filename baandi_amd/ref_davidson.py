@@ -96,3 +96,131 @@ def solve_with_initial_hx(operator, start, hamiltonian_diagonal, options: Option
 def solve(operator, start, hamiltonian_diagonal, options: Options, **kw):
     """ref_davidson::solve (:107-116): always applies H to the start."""
     return solve_with_initial_hx(operator, start, hamiltonian_diagonal, options, None, **kw)
+
+
+def solve_many(operators, starts, diagonals, options, outs, threads=8):
+    """Independent solves on distinct operators (the k points of one map), up
+    to ``threads`` at a time on the GPU, each on its operator's stream
+    (bd_davidson_solve_many_dev). ``options``: one Options per solve;
+    ``outs``: (x, hx or None) per solve. Returns the Solutions."""
+    import torch
+    count = len(operators)
+    assert len(starts) == count and len(diagonals) == count and len(options) == count and len(outs) == count
+    arr = lambda vals: (C.c_void_p * count)(*vals)
+    ops = arr([op.handle.value for op in operators])
+    x0 = arr([s.data_ptr() for s in starts])
+    dg = arr([d.data_ptr() for d in diagonals])
+    xo = arr([o[0].data_ptr() for o in outs])
+    hx_any = any(o[1] is not None for o in outs)
+    hxo = arr([(o[1].data_ptr() if o[1] is not None else None) for o in outs]) if hx_any else None
+    ld0 = np.array([s.shape[1] for s in starts], dtype=np.int64)
+    ldx = np.array([o[0].shape[1] for o in outs], dtype=np.int64)
+    opts = (L.DavidsonOptions * count)()
+    eigs = [np.zeros(o.bands) for o in options]
+    convs = [np.zeros(o.bands, dtype=np.int32) for o in options]
+    for i, o in enumerate(options):
+        assert starts[i].shape[0] == o.bands and starts[i].is_contiguous() and diagonals[i].is_contiguous()
+        if o.preconditioner == L.BD_PRECOND_MLP and o.mlp is None:
+            raise ValueError("BD_PRECOND_MLP needs a trained network (Options.mlp)")
+        opts[i] = L.DavidsonOptions(bands=o.bands, threshold=float(o.threshold), occupied=int(o.occupied),
+                                    first_step=int(bool(o.first_step)), capacity=int(o.capacity),
+                                    max_iterations=int(o.max_iterations), return_hx=int(bool(o.return_hx)),
+                                    preconditioner=int(o.preconditioner),
+                                    mlp_blob=(L.ptr(o.mlp.blob).value if o.mlp is not None else None),
+                                    mlp_vbar=float(o.mlp_vbar))
+    ev = arr([e.ctypes.data for e in eigs])
+    cv = arr([c.ctypes.data for c in convs])
+    recs = (L.DavidsonRecord * count)()
+    statuses = np.zeros(count, dtype=np.int32)
+    for op in operators:
+        op.enter()
+    torch.cuda.current_stream().synchronize()
+    st = L.load().bd_davidson_solve_many_dev(count, ops, x0, L.ptr(ld0), dg, opts, ev, xo, hxo, L.ptr(ldx), cv,
+                                            recs, L.ptr(statuses), int(threads))
+    for op in operators:
+        op.leave()
+    L.check(st, allow=(L.BD_NOT_CONVERGED,))
+    sols = []
+    for i, o in enumerate(options):
+        record = {f: getattr(recs[i], f) for f, _ in L.DavidsonRecord._fields_}
+        sols.append(Solution(eigenvalues=eigs[i], eigenvectors=outs[i][0], hx=outs[i][1],
+                             converged=convs[i].astype(bool),
+                             band_thresholds_ha=[o.band_threshold(b) for b in range(o.bands)], record=record))
+    return sols
+
+
+class BlockIteration:
+    """Fixed-shape block iteration on one operator (bd_block_*): every band
+    active, Rayleigh-Ritz on [X, W] each step, restart to the Ritz block. With
+    ``graph`` the step is captured once as a HIP graph and replayed."""
+
+    def __init__(self, operator, bands, preconditioner=L.BD_PRECOND_TPA, graph=True):
+        self.operator, self.bands, self.graph = operator, int(bands), bool(graph)
+        h = C.c_void_p()
+        L.check(L.load().bd_block_create(operator.handle, self.bands, int(preconditioner), C.byref(h)))
+        self.handle = h
+        self.values = np.zeros(self.bands)
+
+    def set_diagonal(self, diag):
+        self.operator.enter()
+        L.check(L.load().bd_block_set_diagonal_dev(self.handle, L.ptr(diag.contiguous())))
+
+    def start(self, x0):
+        """Rayleigh-Ritz of the (bands, n) start block; returns the values."""
+        x0 = x0.contiguous()
+        self.operator.enter()
+        L.check(L.load().bd_block_start_dev(self.handle, L.ptr(x0), x0.shape[1], L.ptr(self.values)))
+        return self.values.copy()
+
+    def step(self, steps=1):
+        self.operator.enter()
+        L.check(L.load().bd_block_step(self.handle, int(steps), int(self.graph), L.ptr(self.values)))
+        return self.values.copy()
+
+    def step_async(self, steps=1):
+        self.operator.enter()
+        L.check(L.load().bd_block_step_async(self.handle, int(steps), int(self.graph)))
+
+    def finish(self):
+        L.check(L.load().bd_block_finish(self.handle, L.ptr(self.values)))
+        return self.values.copy()
+
+    def vectors(self, x, hx=None):
+        """Copy X (and HX) into (bands, n) CUDA tensors."""
+        L.check(L.load().bd_block_vectors_dev(self.handle, L.ptr(x), L.ptr(hx) if hx is not None else None,
+                                              x.shape[1]))
+        self.operator.leave()
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            try:
+                L.load().bd_block_destroy(h)
+            except (TypeError, AttributeError):
+                pass
+            self.handle = None
+
+
+def solve_block(operator, start, hamiltonian_diagonal, bands, threshold, max_steps=200,
+                preconditioner=L.BD_PRECOND_TPA, graph=True, check_every=1, converge_bands=None):
+    """Block iterations until each of the lowest ``converge_bands`` Ritz
+    values (all by default; the others are a buffer) changes by less than
+    ``threshold`` (Ha) in one step; returns (values, X, steps)."""
+    import torch
+    it = BlockIteration(operator, bands, preconditioner, graph)
+    it.set_diagonal(hamiltonian_diagonal)
+    old = it.start(start)
+    steps = 0
+    while steps < max_steps:
+        try:
+            new = it.step(check_every)
+        except L.BaandiError as e:
+            raise L.BaandiError(e.status, f"{e} (after {steps} steps, values {old})") from None
+        steps += check_every
+        k = bands if converge_bands is None else int(converge_bands)
+        if np.abs(new[:k] - old[:k]).max() < threshold:
+            break
+        old = new
+    x = torch.empty_like(start)
+    it.vectors(x)
+    return it.values.copy(), x, steps

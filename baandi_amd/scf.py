@@ -92,6 +92,7 @@ class ScfOptions:
     mixed_floor_ha: float = 5e-5
     mlp: object = None  # a baandi_amd.learned.Network when preconditioner == BD_PRECOND_MLP
     allow_unconverged: bool = False  # return the last map instead of raising at max_iterations
+    preconditioning: str = "plain"   # density mixing: "plain" or "thomas_fermi" (refscf/profile.rs:4-7)
     # Optional fixed-point verification map (driver.rs:727-757): one more
     # Davidson at the converged potential with every band at this threshold (Ha).
     final_threshold: float = 0.0
@@ -149,12 +150,26 @@ class DensityField:
 
 
 class Mixer:
-    """Difference-history Anderson update with the Hartree metric (broyden.rs)."""
+    """Difference-history Anderson update with the Hartree metric (broyden.rs).
 
-    def __init__(self, field: DensityField, beta, history):
+    ``preconditioning`` "thomas_fermi" scales the history-reduced residual by
+    |G|^2 / (|G|^2 + a), a = (12/pi)^(2/3) / r_s (broyden.rs:106-113, applied
+    to F~ only, :439); "plain" (the reference default, profile.rs:17-26)
+    leaves it unscaled."""
+
+    def __init__(self, field: DensityField, beta, history, preconditioning="plain", volume=None, electrons=None):
+        import torch
         self.field, self.beta, self.history = field, beta, history
         self.pairs = []  # (dx, dF), newest last
         self.prev = None
+        self.screening = None
+        if preconditioning == "thomas_fermi":
+            rs = (3.0 * volume / (4.0 * math.pi * electrons)) ** (1.0 / 3.0)
+            a = (12.0 / math.pi) ** (2.0 / 3.0) / rs
+            g2 = torch.from_numpy(field.g2).to(field.metric.device)
+            self.screening = g2 / (g2 + a)
+        elif preconditioning != "plain":
+            raise ValueError(f"unknown preconditioning {preconditioning!r}")
 
     def update(self, x, f):
         import torch
@@ -179,6 +194,8 @@ class Mixer:
             xx = xx - (gamma[:, None] * dX).sum(dim=0)
             ff = ff - (gamma[:, None] * dF).sum(dim=0)
             break
+        if self.screening is not None:
+            ff = ff * self.screening
         return xx + self.beta * ff
 
 
@@ -217,7 +234,7 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
     x = system.z_valence / system.volume * np.exp(-field.g2 * system.rc ** 2 / 4.0) * sfac
     x[field.zero_np] = electrons / system.volume
     x = torch.from_numpy(x).to(dev)
-    mixer = Mixer(field, options.beta, options.history)
+    mixer = Mixer(field, options.beta, options.history, options.preconditioning, system.volume, electrons)
     if start is None:
         g = torch.Generator(device="cpu").manual_seed(options.seed)
         start = torch.randn(bands, system.n, generator=g, dtype=torch.float64)

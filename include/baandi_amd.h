@@ -238,6 +238,44 @@ bd_status bd_davidson_solve_dev(bd_gamma_op* op, const double* x0_dev, int64_t l
                                 const bd_davidson_options* options, double* eigenvalues,
                                 double* x_dev, double* hx_dev, int64_t ldx, int32_t* converged,
                                 bd_davidson_record* record);
+/* Independent solves on `count` distinct ops (the k points of one SCF map;
+ * no reference counterpart, the reference solves them in sequence,
+ * crates/dft-engine/src/refscf/k_map.rs), `threads` host threads at a time,
+ * each op on its own stream. Per-solve arrays; hxout and conv may be NULL
+ * (or hold NULL entries). statuses[i] per solve; returns the first status
+ * other than OK / NOT_CONVERGED (its message as the last error), else
+ * BD_NOT_CONVERGED if any solve hit its limit, else BD_OK. */
+bd_status bd_davidson_solve_many_dev(int32_t count, bd_gamma_op* const* ops, const double* const* x0_dev,
+                                     const int64_t* ldx0, const double* const* diag_dev,
+                                     const bd_davidson_options* options, double* const* eigenvalues,
+                                     double* const* x_dev, double* const* hx_dev, const int64_t* ldx,
+                                     int32_t* const* converged, bd_davidson_record* records,
+                                     bd_status* statuses, int32_t threads);
+
+/* ------------------------------------------------------------------------
+ * Fixed-shape block iteration (no reference counterpart; the iteration of
+ * ref_davidson.rs:117-366 with every band active and a restart to the Ritz
+ * block after each step): V = [X, W] with W the preconditioned residual
+ * directions, Rayleigh-Ritz on V (bordered inverse Cholesky, FP64 GPU
+ * eigensolve), X <- V C. Each step is device work with fixed shapes and no
+ * host round trip; with use_graph the step is captured once as a HIP graph
+ * and replayed, bit-identical to the eager step.
+ * ---------------------------------------------------------------------- */
+typedef struct bd_block bd_block;
+/* bands: 2 bands <= n, bands <= the op's capacity; QE or TPA preconditioner. */
+bd_status bd_block_create(bd_gamma_op* op, int32_t bands, int32_t preconditioner, bd_block** out);
+void bd_block_destroy(bd_block* block);
+/* Davidson diagonal (n rows, device), copied in; read by every later step. */
+bd_status bd_block_set_diagonal_dev(bd_block* block, const double* diag_dev);
+/* Rayleigh-Ritz of a start block (n x bands, ldx0): X, HX, values (host). */
+bd_status bd_block_start_dev(bd_block* block, const double* x0_dev, int64_t ldx0, double* values);
+/* `steps` iterations; values (host, bands) after the last. Synchronises. */
+bd_status bd_block_step(bd_block* block, int32_t steps, int32_t use_graph, double* values);
+/* The same without waiting; bd_block_finish waits and reads the values. */
+bd_status bd_block_step_async(bd_block* block, int32_t steps, int32_t use_graph);
+bd_status bd_block_finish(bd_block* block, double* values);
+/* X (and HX unless NULL), n x bands, leading dimension ld. */
+bd_status bd_block_vectors_dev(bd_block* block, double* x_dev, double* hx_dev, int64_t ld);
 
 /* ------------------------------------------------------------------------
  * Dense products of the path as stand-alone entry points
