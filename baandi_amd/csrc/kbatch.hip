@@ -2,8 +2,8 @@
 // stays in LDS (its sticks, then one y-z plane at a time), so a column costs
 // one read of X, one read of V (L2-resident, shared by every column) and one
 // write of Y. One launch covers every column of every k point of a batch
-// (uneven basis sizes included); a second launch adds the nonlocal term of
-// every k point. The per-line FFT arithmetic, the V multiply and the kinetic
+// (uneven basis sizes included), nonlocal term included. The per-line FFT
+// arithmetic, the V multiply and the kinetic
 // add are those of the three-stage path (k_kstick_inverse, k_plane,
 // k_kstick_forward), so both paths give the same bits for the local part.
 //
@@ -26,6 +26,10 @@
 namespace bd {
 
 namespace {
+
+constexpr int kMaxNp2 = 256;  // projector columns (2 p) the fused kernel holds in LDS
+// Dynamic LDS of the fused kernel: the 160 KB of a workgroup less q and w.
+constexpr size_t kFusedDynMax = 160 * 1024 - 2 * kMaxNp2 * sizeof(double);
 
 template <typename T>
 __device__ __forceinline__ void stage_roots_k(c2<T>* dst, const c2<double>* src, int n) {
@@ -52,6 +56,7 @@ __global__ __launch_bounds__(NT) void k_hpsi_fused(const KWork* __restrict__ wor
                                                    RadixPlan rx, RadixPlan ry, RadixPlan rz, int xlines, int xplanes,
                                                    int zrows, double inv_n) {
   extern __shared__ char smem[];
+  __shared__ double nlq[kMaxNp2], nlw[kMaxNp2];
   c2<F>* rt = reinterpret_cast<c2<F>*>(smem);  // roots x | y | z
   c2<F>* rtx = rt;
   c2<F>* rty = rt + n0;
@@ -120,54 +125,47 @@ __global__ __launch_bounds__(NT) void k_hpsi_fused(const KWork* __restrict__ wor
       __syncthreads();
     }
     for (int s0 = 0; s0 < S; s0 += xlines) lds_fft<false>(L + s0 * n0, min(xlines, S - s0), n0, 1, rx, rtx);
+    // Nonlocal: q = P2^T x (one wave per projector column), w = D q per atom
+    // block, then y += P2 w in the gather below (FP64).
+    const int np2 = w.np2;
+    const int64_t nr = 2 * int64_t(nk);
+    if (np2 > 0) {
+      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+      for (int c = wave; c < np2; c += NT / 64) {
+        const double* p = w.proj2 + int64_t(c) * nr;
+        double acc = 0.0;
+        for (int64_t r = lane; r < nr; r += 64) acc += p[r] * xc[r];
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+        if (lane == 0) nlq[c] = acc;
+      }
+      __syncthreads();
+      for (int c = threadIdx.x; c < np2; c += NT) {
+        const int start = (c / w.nc) * w.nc, a = c - start;
+        double v = 0.0;
+        for (int b = 0; b < w.nc; ++b) v += w.blockd[a * w.nc + b] * nlq[start + b];
+        nlw[c] = v;
+      }
+      __syncthreads();
+    }
     for (int g = threadIdx.x; g < nk; g += NT) {
       const int64_t r = 2 * int64_t(g);
       const c2<F> h = L[w.fpos[g]];
       const double2 v = *reinterpret_cast<const double2*>(xc + r);
       const double k = w.kinetic[r];
-      *reinterpret_cast<double2*>(yc + r) =
-          make_double2(double(h.x) * inv_n + k * v.x, double(h.y) * inv_n + k * v.y);
+      double y0 = double(h.x) * inv_n + k * v.x, y1 = double(h.y) * inv_n + k * v.y;
+      if (np2 > 0) {
+        double a0 = 0.0, a1 = 0.0;
+        for (int c = 0; c < np2; ++c) {
+          const double* p = w.proj2 + int64_t(c) * nr + r;
+          a0 += p[0] * nlw[c];
+          a1 += p[1] * nlw[c];
+        }
+        y0 += a0;
+        y1 += a1;
+      }
+      *reinterpret_cast<double2*>(yc + r) = make_double2(y0, y1);
     }
     __syncthreads();  // LDS is reused by the next item
-  }
-}
-
-// Y += P2 D P2^T X for every column of one k point per workgroup (FP64).
-// Q (np2 x m) and W = D Q live in LDS; each wave forms whole dot products.
-constexpr int kNlThreads = 256;
-constexpr int kNlMaxQ = 2048;  // np2 * m doubles in LDS (two arrays: 32 KB)
-
-__global__ __launch_bounds__(kNlThreads) void k_nonlocal_k(const KWork* __restrict__ works, KWork one) {
-  __shared__ double Q[kNlMaxQ];
-  __shared__ double W[kNlMaxQ];
-  const KWork w = works ? works[blockIdx.x] : one;
-  const int m = w.m, np2 = w.np2, nc = w.nc;
-  const int64_t n = 2 * int64_t(w.nk);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int e = wave; e < np2 * m; e += kNlThreads / 64) {
-    const int c = e % np2, j = e / np2;
-    const double* p = w.proj2 + int64_t(c) * n;
-    const double* x = w.x + int64_t(j) * w.ldx;
-    double acc = 0.0;
-    for (int64_t r = lane; r < n; r += 64) acc += p[r] * x[r];
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-    if (lane == 0) Q[e] = acc;
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < np2 * m; e += kNlThreads) {
-    const int c = e % np2, j = e / np2;
-    const int start = (c / nc) * nc, a = c - start;
-    double s = 0.0;
-    for (int b = 0; b < nc; ++b) s += w.blockd[a * nc + b] * Q[j * np2 + start + b];
-    W[e] = s;
-  }
-  __syncthreads();
-  for (int64_t r = threadIdx.x; r < n; r += kNlThreads) {
-    for (int j = 0; j < m; ++j) {
-      double s = 0.0;
-      for (int c = 0; c < np2; ++c) s += w.proj2[int64_t(c) * n + r] * W[j * np2 + c];
-      w.y[int64_t(j) * w.ldy + r] += s;
-    }
   }
 }
 
@@ -186,7 +184,7 @@ void launch_fused_t(const GammaOp& op, const FusedShape& f, size_t lds, const KW
     std::lock_guard<std::mutex> lock(mu);
     if (!attr) {
       BD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_hpsi_fused<F, NT>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(kFusedDynMax)));
       attr = true;
     }
     if (cached_lds != lds) {
@@ -217,9 +215,9 @@ bool GammaOp::fused_valid(const FusedShape& f) const {
   if (!kb) return false;
   if (f.threads != 256 && f.threads != 512) return false;
   if (f.xlines < 1 || f.zrows < 1 || f.xplanes < 1 || f.xplanes > plan.n0) return false;
-  if (int64_t(nproj) * capacity > kNlMaxQ) return false;
+  if (nproj > kMaxNp2) return false;
   return fft_fits_threads(f.xlines, plan.rx, f.threads) && fft_fits_threads(f.xplanes * plan.Z, plan.ry, f.threads) &&
-         fft_fits_threads(f.xplanes * f.zrows, plan.rz, f.threads) && fused_lds(f) <= 160 * 1024;
+         fft_fits_threads(f.xplanes * f.zrows, plan.rz, f.threads) && fused_lds(f) <= kFusedDynMax;
 }
 
 KWork GammaOp::kwork(const double* x, int64_t ldx, int m, double* y, int64_t ldy, int first) const {
@@ -249,7 +247,7 @@ KWork GammaOp::kwork(const double* x, int64_t ldx, int m, double* y, int64_t ldy
 void fused_launch(const GammaOp& op, const FusedShape& f, size_t lds, const KWork* works, const KWork& one, int count,
                   int total, hipStream_t s) {
   if (total <= 0) return;
-  if (lds > 160 * 1024) fail(BD_ERR_INVALID_PARAMETER, "fused kernel LDS exceeds 160 KB");
+  if (lds > kFusedDynMax) fail(BD_ERR_INVALID_PARAMETER, "fused kernel LDS exceeds the workgroup limit");
   if (op.precision == BD_MIXED) {
     if (f.threads == 512) launch_fused_t<float, 512>(op, f, lds, works, one, count, total, s);
     else launch_fused_t<float, 256>(op, f, lds, works, one, count, total, s);
@@ -259,17 +257,10 @@ void fused_launch(const GammaOp& op, const FusedShape& f, size_t lds, const KWor
   }
 }
 
-void nonlocal_launch(const GammaOp& op, const KWork* works, const KWork& one, int count, hipStream_t s) {
-  if (op.nproj == 0 || count <= 0) return;
-  hipLaunchKernelGGL(k_nonlocal_k, dim3(count), dim3(kNlThreads), 0, s, works, one);
-  BD_HIP(hipGetLastError());
-}
-
 // One k point: the work entry travels as a kernel argument.
 void GammaOp::fused_apply(const double* x, int64_t ldx, int m, double* y, int64_t ldy) {
   const KWork w = kwork(x, ldx, m, y, ldy, 0);
   fused_launch(*this, fshape, fused_lds(fshape), nullptr, w, 1, m, stream);
-  nonlocal_launch(*this, nullptr, w, 1, stream);
 }
 
 }  // namespace bd
@@ -344,7 +335,6 @@ extern "C" bd_status bd_k_batch_apply_dev(bd_k_batch* b, const double* const* x,
     BD_HIP(hipMemcpyAsync(b->works.p, b->host.data(), sizeof(bd::KWork) * count, hipMemcpyHostToDevice, b->stream));
     const bd::KWork none{};
     bd::fused_launch(b->ops[0]->op, f, lds, b->works.p, none, count, total, b->stream);
-    bd::nonlocal_launch(b->ops[0]->op, b->works.p, none, count, b->stream);
     // The host copy of the work table must outlive the transfer.
     BD_HIP(hipStreamSynchronize(b->stream));
   });

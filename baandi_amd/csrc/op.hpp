@@ -149,7 +149,6 @@ struct GammaOp {
 // works_dev == nullptr: the single entry `one` is used (passed by value).
 void fused_launch(const GammaOp& op, const FusedShape& f, size_t lds, const KWork* works_dev, const KWork& one,
                   int count, int total, hipStream_t s);
-void nonlocal_launch(const GammaOp& op, const KWork* works_dev, const KWork& one, int count, hipStream_t s);
 bool fft_fits_threads(int nlines, const RadixPlan& rp, int threads);
 
 StickPlan build_stick_plan(const System& s);

@@ -93,6 +93,7 @@ class ScfOptions:
     mlp: object = None  # a baandi_amd.learned.Network when preconditioner == BD_PRECOND_MLP
     allow_unconverged: bool = False  # return the last map instead of raising at max_iterations
     preconditioning: str = "plain"   # density mixing: "plain" or "thomas_fermi" (refscf/profile.rs:4-7)
+    capacity: int = 0                # Davidson subspace capacity (0: 2 bands, ref_davidson.rs:38-78)
     # Optional fixed-point verification map (driver.rs:727-757): one more
     # Davidson at the converged potential with every band at this threshold (Ha).
     final_threshold: float = 0.0
@@ -280,7 +281,7 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
                                         first_step=(iteration == 1 and not redo),
                                         max_iterations=options.davidson_max_iterations, return_hx=True,
                                         preconditioner=options.preconditioner, mlp=options.mlp,
-                                        mlp_vbar=vmean)
+                                        mlp_vbar=vmean, capacity=options.capacity)
             sol = ref_davidson.solve_with_initial_hx(cur, orbitals, diag, opts, out=(xout, hxout))
             totals["davidson_s"] += time.perf_counter() - td
             totals["mixed_maps"] += int(use_mixed)

@@ -24,6 +24,9 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# The k-point leg runs 16 solves side by side, one HIP stream each: give the
+# process as many hardware queues (read once, when the HIP runtime starts).
+os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 HBM_PEAK_GBS = 8000.0
 # Measured HBM bytes per pair of k_plane at Si64 FP64: rocprofv3 FETCH_SIZE (x2.000,
@@ -110,6 +113,58 @@ def hx_profile(system, wl, count):
                       "copy_bytes_each_way": a.numel() * 8}))
 
 
+def kpoint_leg(steps, threads=16):
+    """Al fcc 4-atom cell, 6x6x6 Monkhorst-Pack (108 k points), 25 Ry, 16
+    bands, Fermi-Dirac 0.01 Ha: k-point SCF wall time with the k points
+    solved concurrently, and one fused launch of H over every column of every
+    k point (timed with HIP events on the batch stream)."""
+    import numpy as np
+    import torch
+    from baandi_amd import workloads, kpoint as K
+    wl = workloads.al4()
+    system = workloads.build(wl)
+    kp = K.monkhorst_pack(wl["mesh"])
+    opts = K.KScfOptions(bands=wl["bands"], theta=wl["theta"], threshold_ry=1e-8, threads=threads,
+                         preconditioning="thomas_fermi")
+    walls, r = [], None
+    for k in range(steps):
+        r = K.run_scf_k(system, kp, opts)
+        walls.append(r["wall_s"])
+        print(f"[bench] al4 k-SCF step {k}: {r['wall_s']:.3f}s maps={r['iterations']} "
+              f"E={r['energy']['total']:.12f}", file=sys.stderr, flush=True)
+    ops = r["ops"]
+    xs = [o.contiguous() for o in r["orbitals"]]
+    ys = [torch.empty_like(x) for x in xs]
+    batch = K.KPointBatch(ops)
+    batch.apply(xs, ys)  # warm-up
+    torch.cuda.synchronize()
+    stream = torch.cuda.ExternalStream(K.L.load().bd_k_batch_stream(batch.handle))
+    reps = 5
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(reps):
+        batch.apply(xs, ys)
+    b.record(stream)
+    b.synchronize()
+    ms = a.elapsed_time(b) / reps
+    cols = sum(int(x.shape[0]) for x in xs)
+    # Compulsory bytes of one batched launch: every column read and written
+    # once (16 B per coefficient each way) and V(r) once per launch.
+    nbytes = sum(int(x.numel()) * 8 * 2 for x in xs) + 8 * int(np.prod(system.dims))
+    return {
+        "workload": "Al fcc 4-atom cubic a=7.65 Bohr, 6x6x6 MP (108 k), 25 Ry, 16 bands, FD 0.01 Ha, "
+                    "Thomas-Fermi mixing, SCF to dr2<1e-8 Ry",
+        "k_points": len(kp), "nk_range": [min(o.nk for o in ops), max(o.nk for o in ops)],
+        "wall_s": round(sum(walls) / len(walls), 4), "scf_iterations": r["iterations"],
+        "setup_s": round(r["setup_s"], 3), "concurrent_solves": threads,
+        "energy_ha": r["energy"]["total"], "entropy_term_ha": r["energy"]["entropy_term"], "mu_ha": r["mu"],
+        "h_columns": r["totals"]["h_columns"], "fused_shape": list(ops[0].fused_shape()),
+        "batched_apply": {"columns": cols, "ms": round(ms, 4), "columns_per_s": round(cols / (ms * 1e-3), 1),
+                          "compulsory_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                          "frac_hbm": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +174,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--mixed-steps", type=int, default=1,
                     help="SCFs timed with the mixed tier (loose maps in FP32/BF16x3, final maps in FP64)")
+    ap.add_argument("--kpoint-steps", type=int, default=1,
+                    help="Al 6x6x6 k-point SCFs timed (0 skips the k-point leg)")
+    ap.add_argument("--kpoint-only", action="store_true",
+                    help="profile mode: only the Al k-point leg, its JSON object printed")
     ap.add_argument("--hx-only", type=int, default=0,
                     help="profile mode: N applies of H to 160 Si64 columns, then a 256 MiB copy")
     args = ap.parse_args()
@@ -130,6 +189,9 @@ def main():
     import torch
     from baandi_amd import workloads
     from baandi_amd.scf import run_scf, ScfOptions
+    if args.kpoint_only:
+        print(json.dumps(kpoint_leg(max(args.kpoint_steps, 1))))
+        return
 
     wl = workloads.si64()
     t = time.perf_counter()
@@ -138,7 +200,11 @@ def main():
     if args.hx_only:
         hx_profile(system, wl, args.hx_only)
         return
-    opts = ScfOptions(bands=wl["bands"], threshold_ry=1e-8)
+    # Thomas-Fermi-screened Anderson mixing for every leg (the reference's
+    # Preconditioning::ThomasFermi, refscf/profile.rs:4-7): the same fixed
+    # point in a third of the maps of plain mixing on this cell.
+    mix = "thomas_fermi"
+    opts = ScfOptions(bands=wl["bands"], threshold_ry=1e-8, preconditioning=mix)
     print(f"[bench] {wl['name']}: n={system.n} grid={system.dims} proj={system.nproj} setup={setup_s:.2f}s",
           file=sys.stderr, flush=True)
     for w in range(args.warmup):
@@ -164,10 +230,12 @@ def main():
         net = Network.load(DEFAULT_OUT)
         natoms = len(system.get("atoms_cart"))
         occ = int(round(system.electrons)) // 2
-        legs = {"mixed": ScfOptions(bands=wl["bands"], threshold_ry=1e-8, precision=L.BD_MIXED),
-                "learned": ScfOptions(bands=wl["bands"], threshold_ry=1e-8, preconditioner=L.BD_PRECOND_MLP, mlp=net),
+        legs = {"mixed": ScfOptions(bands=wl["bands"], threshold_ry=1e-8, precision=L.BD_MIXED,
+                                    preconditioning=mix),
+                "learned": ScfOptions(bands=wl["bands"], threshold_ry=1e-8, preconditioner=L.BD_PRECOND_MLP,
+                                      mlp=net, preconditioning=mix),
                 "mixed_learned": ScfOptions(bands=wl["bands"], threshold_ry=1e-8, precision=L.BD_MIXED,
-                                            preconditioner=L.BD_PRECOND_MLP, mlp=net)}
+                                            preconditioner=L.BD_PRECOND_MLP, mlp=net, preconditioning=mix)}
         mixed = {}
         for name, mopts in legs.items():
             mwalls, mres = [], None
@@ -205,7 +273,7 @@ def main():
         "data": "synthetic (seeded N(0,1) start, synthetic Gaussian pseudopotential); learned weights "
                 "trained on displaced Si8 cells by baandi_amd.train_preconditioner",
         "config": {"workload": "Si64 cubic 2x2x2, a=20.52 Bohr, Gamma, 30 Ry, 160 bands (128 occ), "
-                               "SCF to dr2<1e-8 Ry, plain FP64 path",
+                               "SCF to dr2<1e-8 Ry (Thomas-Fermi mixing), plain FP64 path",
                    "n_pw": int(system.n), "fft_grid": list(system.dims), "bands": wl["bands"],
                    "scf_iterations": last["iterations"], "h_columns": last["totals"]["h_columns"],
                    "mixed_s": None if mixed is None else mixed["mixed"]["wall_s"],
@@ -230,6 +298,8 @@ def main():
             "fused_compulsory_GBs": round(fused, 1),
         },
     }
+    if args.kpoint_steps > 0:
+        out["kpoint"] = kpoint_leg(args.kpoint_steps)
     if not args.no_cpu:
         cps, cols, dt = cpu_baseline(system, args.cpu_seconds)
         gpu_cps = st["columns"] / (sum(st[k] for k in ("stick_inverse", "plane", "stick_forward", "projections",

@@ -172,7 +172,7 @@ bd_status bd_k_op_get(const bd_gamma_op* op, const char* name, void* out, int64_
  * FFT fits one workgroup's LDS applies H with one workgroup per column (the
  * whole 3D FFT of the column in LDS); its local part is bit-identical to the
  * three-stage stick/plane path. A batch runs every column of every k point
- * in one persistent launch, plus one launch for the nonlocal term of all k.
+ * in one persistent launch, nonlocal term included.
  * ---------------------------------------------------------------------- */
 typedef struct bd_k_batch bd_k_batch;
 /* All ops: k-point ops of one system and precision on the fused path. */
