@@ -344,11 +344,65 @@ extern "C" bd_status bd_k_batch_apply_dev(bd_k_batch* b, const double* const* x,
 // planes at a time) on m columns of the op's own scratch, reps launches each,
 // and keep the fastest. out5 (optional): threads, xlines, zrows, xplanes,
 // microseconds per launch.
+//
+// On the stick/plane path (Gamma ops, or k ops whose column does not fit one
+// workgroup) it times k_plane's shapes instead: threads 256/512 and the y
+// rows per z-FFT chunk; out5 = threads, chunk, 0, 0, microseconds per apply.
 extern "C" bd_status bd_gamma_op_autotune(bd_gamma_op* h, int32_t m, int32_t reps, double* out5) {
   BD_GUARD({
     if (!h) bd::fail(BD_ERR_INVALID_PARAMETER, "null op");
     bd::GammaOp& o = h->op;
-    if (!o.kb || !o.fused) bd::fail(BD_ERR_INVALID_PARAMETER, "autotuning covers the fused k-point path");
+    if (!(o.kb && o.fused)) {
+      if (m <= 0 || m > o.capacity || reps <= 0) bd::fail(BD_ERR_INVALID_PARAMETER, "autotune columns/reps");
+      if (!o.has_potential) bd::fail(BD_ERR_INVALID_PARAMETER, "no potential installed");
+      const bd::StickPlan& p = o.plan;
+      std::vector<std::pair<int, int>> cands;
+      for (int t : {256, 512})
+        for (int c = 1; c <= p.n1; ++c) {
+          const bool useful = c == p.n1 || (c % 4 == 0) || c == 1;
+          if (!useful) continue;
+          if (!bd::fft_fits_threads(p.Z, p.ry, t) || !bd::fft_fits_threads(c, p.rz, t)) continue;
+          if (o.plane_lds_bytes(c) > 160 * 1024) continue;
+          cands.push_back({t, c});
+        }
+      if (cands.empty()) bd::fail(BD_ERR_APPLICATION, "no valid plane shape");
+      BD_HIP(hipMemsetAsync(o.d_x.p, 0, size_t(o.n) * m * 8, o.stream));
+      hipEvent_t a, z;
+      BD_HIP(hipEventCreate(&a));
+      BD_HIP(hipEventCreate(&z));
+      double best = 1e300;
+      std::pair<int, int> pick{o.plane_threads, p.chunk};
+      const bool prof = o.profiling;
+      o.profiling = false;
+      for (const auto& c : cands) {
+        o.plane_threads = c.first;
+        o.plan.chunk = c.second;
+        o.apply(o.d_x.p, o.n, m, o.d_y.p, o.n);  // warm-up
+        BD_HIP(hipEventRecord(a, o.stream));
+        for (int r = 0; r < reps; ++r) o.apply(o.d_x.p, o.n, m, o.d_y.p, o.n);
+        BD_HIP(hipEventRecord(z, o.stream));
+        BD_HIP(hipEventSynchronize(z));
+        float ms = 0.f;
+        BD_HIP(hipEventElapsedTime(&ms, a, z));
+        const double us = 1e3 * ms / reps;
+        if (us < best) {
+          best = us;
+          pick = c;
+        }
+      }
+      o.profiling = prof;
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(z);
+      o.plane_threads = pick.first;
+      o.plan.chunk = pick.second;
+      if (out5) {
+        out5[0] = pick.first;
+        out5[1] = pick.second;
+        out5[2] = out5[3] = 0;
+        out5[4] = best;
+      }
+      return BD_OK;
+    }
     if (m <= 0 || m > o.capacity || reps <= 0) bd::fail(BD_ERR_INVALID_PARAMETER, "autotune columns/reps");
     if (!o.has_potential) bd::fail(BD_ERR_INVALID_PARAMETER, "no potential installed");
     std::vector<bd::FusedShape> cands;

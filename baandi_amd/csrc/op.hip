@@ -235,8 +235,8 @@ __global__ __launch_bounds__(kThreads) void k_kstick_forward(
 // Plane pass: inverse y, per-row-chunk inverse z * V * forward z, forward y.
 // mode 0: H apply (writes sticks back); mode 1: band density (accumulates
 // w_a psi_a^2 + w_b psi_b^2 over all pairs into rho of plane x).
-template <int MODE, typename F>
-__global__ __launch_bounds__(kThreads) void k_plane(
+template <int MODE, typename F, int NT>
+__global__ __launch_bounds__(NT) void k_plane(
     c2<F>* __restrict__ sticks, int n0, int n1, int n2, int Spad, int Z, int chunk,
     const int2* __restrict__ stick_yz, const int* __restrict__ z_of_zc,
     const c2<double>* __restrict__ roots, RadixPlan ry, RadixPlan rz,
@@ -510,6 +510,11 @@ StickPlan build_k_stick_plan(const System& s, const KBasis& kb) {
 // ---------------------------------------------------------------------------
 // Operator
 // ---------------------------------------------------------------------------
+size_t GammaOp::plane_lds_bytes(int chunk) const {
+  const size_t e = precision == BD_MIXED ? sizeof(c2<float>) : sizeof(c2<double>);
+  return size_t(plan.Z * plan.n1 + chunk * plan.n2 + plan.n1 + plan.n2) * e;
+}
+
 // The k kernels move a coefficient's (Re, Im) as one 16-byte access.
 static bool complex_aligned(const double* p, int64_t ld) {
   return (reinterpret_cast<uintptr_t>(p) & 15) == 0 && (ld & 1) == 0;
@@ -523,12 +528,12 @@ static size_t plane_lds(const StickPlan& p, size_t elem) {
 template <typename F>
 static void allow_lds(const StickPlan& plan) {
   const size_t e = sizeof(c2<F>);
-  if (plane_lds(plan, e) > 65536) {
-    BD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane<0, F>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, int(plane_lds(plan, e))));
-    BD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane<1, F>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, int(plane_lds(plan, e))));
-  }
+  // Every plane instantiation may take up to the whole LDS (the chunk is tuned).
+  for (const void* f : {reinterpret_cast<const void*>(&k_plane<0, F, 256>),
+                        reinterpret_cast<const void*>(&k_plane<1, F, 256>),
+                        reinterpret_cast<const void*>(&k_plane<0, F, 512>),
+                        reinterpret_cast<const void*>(&k_plane<1, F, 512>)})
+    BD_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
   if (stick_lds(plan, e) > 65536) {
     BD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_stick_inverse<F>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, int(stick_lds(plan, e))));
@@ -669,9 +674,14 @@ void GammaOp::local_apply(const double* x, int64_t ldx, int m, double* y, int64_
     hipLaunchKernelGGL(k_stick_inverse<F>, dim3(p.ntiles, pairs), dim3(kThreads), stick_lds(p, e), stream, x,
                        ldx, m, p.n0, p.T, p.Spad, d_tile_off.p, d_entries.p, d_roots.p, p.rx, sticks);
   if (profiling) BD_HIP(hipEventRecord(ev[1], stream));
-  hipLaunchKernelGGL((k_plane<0, F>), dim3(pairs, p.n0), dim3(kThreads), plane_lds(p, e), stream, sticks,
-                     p.n0, p.n1, p.n2, p.Spad, p.Z, p.chunk, d_stick_yz.p, d_zc.p, d_roots.p, p.ry,
-                     p.rz, pot, pairs, nullptr, 0.0, nullptr);
+  if (plane_threads == 512)
+    hipLaunchKernelGGL((k_plane<0, F, 512>), dim3(pairs, p.n0), dim3(512), plane_lds(p, e), stream, sticks, p.n0,
+                       p.n1, p.n2, p.Spad, p.Z, p.chunk, d_stick_yz.p, d_zc.p, d_roots.p, p.ry, p.rz, pot, pairs,
+                       nullptr, 0.0, nullptr);
+  else
+    hipLaunchKernelGGL((k_plane<0, F, 256>), dim3(pairs, p.n0), dim3(256), plane_lds(p, e), stream, sticks, p.n0,
+                       p.n1, p.n2, p.Spad, p.Z, p.chunk, d_stick_yz.p, d_zc.p, d_roots.p, p.ry, p.rz, pot, pairs,
+                       nullptr, 0.0, nullptr);
   if (profiling) BD_HIP(hipEventRecord(ev[2], stream));
   if (kb)
     hipLaunchKernelGGL(k_kstick_forward<F>, dim3(p.ntiles, pairs), dim3(kThreads), stick_lds(p, e), stream, x,
@@ -697,9 +707,14 @@ void GammaOp::density(const double* x, int64_t ldx, int m, const double* w, doub
     hipLaunchKernelGGL(k_stick_inverse<F>, dim3(p.ntiles, pairs), dim3(kThreads), stick_lds(p, e), stream, x,
                        ldx, m, p.n0, p.T, p.Spad, d_tile_off.p, d_entries.p, d_roots.p, p.rx, sticks);
   const int groups = std::min(pairs, kDensityGroups);
-  hipLaunchKernelGGL((k_plane<1, F>), dim3(p.n0, groups), dim3(kThreads), plane_lds(p, e), stream, sticks,
-                     p.n0, p.n1, p.n2, p.Spad, p.Z, p.chunk, d_stick_yz.p, d_zc.p, d_roots.p, p.ry, p.rz,
-                     nullptr, pairs, w, 1.0 / sys->volume, d_rho_part.p);
+  if (plane_threads == 512)
+    hipLaunchKernelGGL((k_plane<1, F, 512>), dim3(p.n0, groups), dim3(512), plane_lds(p, e), stream, sticks, p.n0,
+                       p.n1, p.n2, p.Spad, p.Z, p.chunk, d_stick_yz.p, d_zc.p, d_roots.p, p.ry, p.rz, nullptr,
+                       pairs, w, 1.0 / sys->volume, d_rho_part.p);
+  else
+    hipLaunchKernelGGL((k_plane<1, F, 256>), dim3(p.n0, groups), dim3(256), plane_lds(p, e), stream, sticks, p.n0,
+                       p.n1, p.n2, p.Spad, p.Z, p.chunk, d_stick_yz.p, d_zc.p, d_roots.p, p.ry, p.rz, nullptr,
+                       pairs, w, 1.0 / sys->volume, d_rho_part.p);
   hipLaunchKernelGGL(k_sum_groups, dim3(unsigned((N + 255) / 256)), dim3(256), 0, stream, d_rho_part.p, N,
                      groups, rho);
 }

@@ -115,6 +115,7 @@ struct GammaOp {
   // when its LDS fits; the three-stage stick/plane path otherwise.
   bool fused = false;
   FusedShape fshape;
+  int plane_threads = 256;  // k_plane workgroup size (256 or 512; tuned with plan.chunk)
   DevBuf<int> d_fpos;
   KWork kwork(const double* x, int64_t ldx, int m, double* y, int64_t ldy, int first) const;
   size_t fused_lds(const FusedShape& f) const;
@@ -140,6 +141,7 @@ struct GammaOp {
   void refresh_potential32();
   template <typename F>
   void local_apply(const double* x, int64_t ldx, int m, double* y, int64_t ldy);
+  size_t plane_lds_bytes(int chunk) const;
   template <typename F>
   void density(const double* x, int64_t ldx, int m, const double* w, double* rho);
 };

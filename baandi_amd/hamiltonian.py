@@ -184,6 +184,15 @@ class DistributedRealHamiltonian:
     def stream(self):
         return L.load().bd_gamma_op_stream(self.handle)
 
+    def autotune(self, columns, reps=3):
+        """Time the valid launch shapes of the H·X kernels on `columns` columns
+        of the op's own scratch and keep the fastest (bd_gamma_op_autotune):
+        returns the 5 figures it reports. Every shape gives the same bits."""
+        out = np.zeros(5)
+        self.enter()
+        L.check(L.load().bd_gamma_op_autotune(self.handle, int(columns), int(reps), L.ptr(out)))
+        return tuple(out)
+
     def set_profiling(self, on=True):
         L.check(L.load().bd_gamma_op_set_profiling(self.handle, 1 if on else 0))
 

@@ -94,6 +94,7 @@ class ScfOptions:
     allow_unconverged: bool = False  # return the last map instead of raising at max_iterations
     preconditioning: str = "plain"   # density mixing: "plain" or "thomas_fermi" (refscf/profile.rs:4-7)
     capacity: int = 0                # Davidson subspace capacity (0: 2 bands, ref_davidson.rs:38-78)
+    autotune: bool = False           # tune the H·X launch shapes at set-up (bit-neutral)
     # Optional fixed-point verification map (driver.rs:727-757): one more
     # Davidson at the converged potential with every band at this threshold (Ha).
     final_threshold: float = 0.0
@@ -222,6 +223,11 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
     op = DistributedRealHamiltonian(system, capacity=bands, precision=L.BD_FP64)
     op_mixed = (DistributedRealHamiltonian(system, capacity=bands, precision=L.BD_MIXED)
                 if options.precision == L.BD_MIXED else None)
+    if options.autotune:  # on the ionic potential (the shapes do not depend on V)
+        for o in (op, op_mixed):
+            if o is not None:
+                o.set_owned_potential(vion.reshape(-1).contiguous())
+                o.autotune(bands)
     diag_static = torch.from_numpy(system.get("diagonal_static")).to(dev)
     kinetic = system.get("kinetic_orc")
     ewald = float(system.get("ewald")[0])

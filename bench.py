@@ -125,7 +125,7 @@ def kpoint_leg(steps, threads=16):
     system = workloads.build(wl)
     kp = K.monkhorst_pack(wl["mesh"])
     opts = K.KScfOptions(bands=wl["bands"], theta=wl["theta"], threshold_ry=1e-8, threads=threads,
-                         preconditioning="thomas_fermi")
+                         preconditioning="thomas_fermi", autotune=True)
     walls, r = [], None
     for k in range(steps):
         r = K.run_scf_k(system, kp, opts)
@@ -204,7 +204,8 @@ def main():
     # Preconditioning::ThomasFermi, refscf/profile.rs:4-7): the same fixed
     # point in a third of the maps of plain mixing on this cell.
     mix = "thomas_fermi"
-    opts = ScfOptions(bands=wl["bands"], threshold_ry=1e-8, preconditioning=mix)
+    # The H·X launch shapes are tuned once per op at set-up (bit-neutral).
+    opts = ScfOptions(bands=wl["bands"], threshold_ry=1e-8, preconditioning=mix, autotune=True)
     print(f"[bench] {wl['name']}: n={system.n} grid={system.dims} proj={system.nproj} setup={setup_s:.2f}s",
           file=sys.stderr, flush=True)
     for w in range(args.warmup):
@@ -231,11 +232,12 @@ def main():
         natoms = len(system.get("atoms_cart"))
         occ = int(round(system.electrons)) // 2
         legs = {"mixed": ScfOptions(bands=wl["bands"], threshold_ry=1e-8, precision=L.BD_MIXED,
-                                    preconditioning=mix),
+                                    preconditioning=mix, autotune=True),
                 "learned": ScfOptions(bands=wl["bands"], threshold_ry=1e-8, preconditioner=L.BD_PRECOND_MLP,
-                                      mlp=net, preconditioning=mix),
+                                      mlp=net, preconditioning=mix, autotune=True),
                 "mixed_learned": ScfOptions(bands=wl["bands"], threshold_ry=1e-8, precision=L.BD_MIXED,
-                                            preconditioner=L.BD_PRECOND_MLP, mlp=net, preconditioning=mix)}
+                                            preconditioner=L.BD_PRECOND_MLP, mlp=net, preconditioning=mix,
+                                            autotune=True)}
         mixed = {}
         for name, mopts in legs.items():
             mwalls, mres = [], None

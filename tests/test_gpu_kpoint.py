@@ -364,3 +364,26 @@ def test_graph_iteration_bitwise_equals_eager(kind):
           flush=True)
     assert same
     assert err < 1e-9
+
+
+def test_plane_autotune_is_bitwise_neutral():
+    """Gamma op: the tuned k_plane shape (threads, rows per z chunk) gives the
+    same H·X bits as the default, FP64 and mixed."""
+    import torch
+    from baandi_amd import lib as L
+    from baandi_amd.hamiltonian import GammaSystem, DistributedRealHamiltonian
+    system = GammaSystem(fcc_rows(10.26), ATOMS, 4.0, 10.0)
+    rng = np.random.default_rng(3)
+    pot = rng.standard_normal(int(np.prod(system.dims))) * 0.1
+    for prec in (L.BD_FP64, L.BD_MIXED):
+        op = DistributedRealHamiltonian(system, capacity=12, precision=prec)
+        op.set_owned_potential(pot)
+        x = torch.from_numpy(rng.standard_normal((12, system.n))).cuda()
+        y0, y1 = torch.empty_like(x), torch.empty_like(x)
+        op.apply_block(x, y0)
+        shape = op.autotune(12, reps=2)
+        op.apply_block(x, y1)
+        torch.cuda.synchronize()
+        print(f"[plane autotune] precision={prec} threads={int(shape[0])} chunk={int(shape[1])} "
+              f"{shape[4]:.1f} us per apply", flush=True)
+        assert torch.equal(y0, y1)
