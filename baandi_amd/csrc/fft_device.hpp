@@ -92,11 +92,15 @@ __device__ __forceinline__ void dft(c2<T>* v) {
 }
 
 // One in-place Stockham pass (radix R, Ns = product of earlier radices).
-// `roots[k]` = e^{-2 pi i k/n} (forward); the inverse conjugates.
-template <int R, bool INV, typename T>
+// `roots[k]` = e^{-2 pi i k/n} (forward); the inverse conjugates. A workgroup
+// of NT threads holds MaxB<R>::v * 256 / NT butterflies per thread, so the
+// lines a pass can hold (and fft_fits) are the same for 256 and 512 threads
+// while the wider workgroup needs half the registers.
+template <int R, bool INV, typename T, int NT>
 __device__ __forceinline__ void stockham_pass(c2<T>* lds, int nlines, int lstride, int estride,
                                               int n, int Ns, const c2<T>* roots) {
-  constexpr int MB = MaxB<R>::v;
+  static_assert(MaxB<R>::v * 256 % NT == 0, "butterflies per thread must be whole");
+  constexpr int MB = MaxB<R>::v * 256 / NT;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int m = n / R;
   const int total = nlines * m;
@@ -138,18 +142,18 @@ __device__ __forceinline__ void stockham_pass(c2<T>* lds, int nlines, int lstrid
   __syncthreads();
 }
 
-// Full 1D transform of every line. The caller synchronises before the call;
-// the call ends synchronised.
-template <bool INV, typename T>
+// Full 1D transform of every line by a workgroup of NT threads (blockDim.x
+// == NT). The caller synchronises before the call; the call ends synchronised.
+template <bool INV, typename T, int NT = 256>
 __device__ void lds_fft(c2<T>* lds, int nlines, int lstride, int estride, const RadixPlan& rp,
                         const c2<T>* roots) {
   int Ns = 1;
   for (int p = 0; p < rp.count; ++p) {
     const int R = rp.radix[p];
-    if (R == 4) stockham_pass<4, INV>(lds, nlines, lstride, estride, rp.n, Ns, roots);
-    else if (R == 2) stockham_pass<2, INV>(lds, nlines, lstride, estride, rp.n, Ns, roots);
-    else if (R == 3) stockham_pass<3, INV>(lds, nlines, lstride, estride, rp.n, Ns, roots);
-    else stockham_pass<5, INV>(lds, nlines, lstride, estride, rp.n, Ns, roots);
+    if (R == 4) stockham_pass<4, INV, T, NT>(lds, nlines, lstride, estride, rp.n, Ns, roots);
+    else if (R == 2) stockham_pass<2, INV, T, NT>(lds, nlines, lstride, estride, rp.n, Ns, roots);
+    else if (R == 3) stockham_pass<3, INV, T, NT>(lds, nlines, lstride, estride, rp.n, Ns, roots);
+    else stockham_pass<5, INV, T, NT>(lds, nlines, lstride, estride, rp.n, Ns, roots);
     Ns *= R;
   }
 }

@@ -79,7 +79,7 @@ __global__ __launch_bounds__(NT) void k_hpsi_fused(const KWork* __restrict__ wor
       L[w.fpos[g]] = {F(v.x), F(v.y)};
     }
     __syncthreads();
-    for (int s0 = 0; s0 < S; s0 += xlines) lds_fft<true>(L + s0 * n0, min(xlines, S - s0), n0, 1, rx, rtx);
+    for (int s0 = 0; s0 < S; s0 += xlines) lds_fft<true, F, NT>(L + s0 * n0, min(xlines, S - s0), n0, 1, rx, rtx);
     // y-z planes, `xplanes` at a time: P = [plane][zc][y], R = [plane][row][z].
     for (int x0 = 0; x0 < n0; x0 += xplanes) {
       const int np = min(xplanes, n0 - x0);
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(NT) void k_hpsi_fused(const KWork* __restrict__ wor
         P[(pl * Z + yz.y) * n1 + yz.x] = L[s * n0 + x0 + pl];
       }
       __syncthreads();
-      lds_fft<true>(P, np * Z, n1, 1, ry, rty);
+      lds_fft<true, F, NT>(P, np * Z, n1, 1, ry, rty);
       for (int y0 = 0; y0 < n1; y0 += zrows) {
         const int c = min(zrows, n1 - y0);
         for (int i = threadIdx.x; i < np * c * n2; i += NT) R[i] = {F(0), F(0)};
@@ -102,13 +102,13 @@ __global__ __launch_bounds__(NT) void k_hpsi_fused(const KWork* __restrict__ wor
           R[(pl * c + r) * n2 + w.z_of_zc[zc]] = P[(pl * Z + zc) * n1 + y0 + r];
         }
         __syncthreads();
-        lds_fft<true>(R, np * c, n2, 1, rz, rtz);
+        lds_fft<true, F, NT>(R, np * c, n2, 1, rz, rtz);
         for (int i = threadIdx.x; i < np * c * n2; i += NT) {
           const int pl = i / (c * n2), rem = i - pl * (c * n2);
           R[i] = cscale(R[i], pot[(int64_t(x0 + pl) * n1 + y0) * n2 + rem]);
         }
         __syncthreads();
-        lds_fft<false>(R, np * c, n2, 1, rz, rtz);
+        lds_fft<false, F, NT>(R, np * c, n2, 1, rz, rtz);
         for (int i = threadIdx.x; i < np * c * Z; i += NT) {
           const int pl = i / (c * Z), rem = i - pl * (c * Z);
           const int r = rem / Z, zc = rem - r * Z;
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(NT) void k_hpsi_fused(const KWork* __restrict__ wor
         }
         __syncthreads();
       }
-      lds_fft<false>(P, np * Z, n1, 1, ry, rty);
+      lds_fft<false, F, NT>(P, np * Z, n1, 1, ry, rty);
       for (int i = threadIdx.x; i < np * S; i += NT) {
         const int pl = i / S, s = i - pl * S;
         const int2 yz = w.yz[s];
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(NT) void k_hpsi_fused(const KWork* __restrict__ wor
       }
       __syncthreads();
     }
-    for (int s0 = 0; s0 < S; s0 += xlines) lds_fft<false>(L + s0 * n0, min(xlines, S - s0), n0, 1, rx, rtx);
+    for (int s0 = 0; s0 < S; s0 += xlines) lds_fft<false, F, NT>(L + s0 * n0, min(xlines, S - s0), n0, 1, rx, rtx);
     // Nonlocal: q = P2^T x (one wave per projector column), w = D q per atom
     // block, then y += P2 w in the gather below (FP64).
     const int np2 = w.np2;

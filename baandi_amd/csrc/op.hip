@@ -38,9 +38,13 @@ template <int R>
 bool pass_fits(int nlines, int n) {
   return int64_t(nlines) * (n / R) <= int64_t(MaxB<R>::v) * kThreads;
 }
+// A workgroup holds MaxB<R>::v * 256 butterflies of a pass whatever its
+// size (stockham_pass scales the butterflies per thread with the size), so
+// `threads` only has to be one of the sizes the kernels are built for.
 template <int R>
 bool pass_fits_t(int nlines, int n, int threads) {
-  return int64_t(nlines) * (n / R) <= int64_t(MaxB<R>::v) * threads;
+  if (threads != 256 && threads != 512) return false;
+  return int64_t(nlines) * (n / R) <= int64_t(MaxB<R>::v) * 256;
 }
 }  // namespace
 
@@ -265,7 +269,7 @@ __global__ __launch_bounds__(NT) void k_plane(
       if (yz.x >= 0) P[yz.y * n1 + yz.x] = col[s];
     }
     __syncthreads();
-    lds_fft<true>(P, Z, n1, 1, ry, rty);
+    lds_fft<true, F, NT>(P, Z, n1, 1, ry, rty);
     double wa = 0.0, wb = 0.0;
     if (MODE == 1) {
       wa = weights[2 * t] * inv_volume;
@@ -280,12 +284,12 @@ __global__ __launch_bounds__(NT) void k_plane(
         Rb[r * n2 + z_of_zc[zc]] = P[zc * n1 + y0 + r];
       }
       __syncthreads();
-      lds_fft<true>(Rb, c, n2, 1, rz, rtz);
+      lds_fft<true, F, NT>(Rb, c, n2, 1, rz, rtz);
       if (MODE == 0) {
         const F* v = potential + (int64_t(xi) * n1 + y0) * n2;
         for (int i = threadIdx.x; i < c * n2; i += blockDim.x) Rb[i] = cscale(Rb[i], v[i]);
         __syncthreads();
-        lds_fft<false>(Rb, c, n2, 1, rz, rtz);
+        lds_fft<false, F, NT>(Rb, c, n2, 1, rz, rtz);
         for (int i = threadIdx.x; i < c * Z; i += blockDim.x) {
           const int r = i / Z, zc = i - r * Z;
           P[zc * n1 + y0 + r] = Rb[r * n2 + z_of_zc[zc]];
@@ -302,7 +306,7 @@ __global__ __launch_bounds__(NT) void k_plane(
       }
     }
     if (MODE == 0) {
-      lds_fft<false>(P, Z, n1, 1, ry, rty);
+      lds_fft<false, F, NT>(P, Z, n1, 1, ry, rty);
       for (int s = threadIdx.x; s < Spad; s += blockDim.x) {
         const int2 yz = stick_yz[s];
         if (yz.x >= 0) col[s] = P[yz.y * n1 + yz.x];
