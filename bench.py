@@ -29,10 +29,10 @@ sys.path.insert(0, ROOT)
 os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 HBM_PEAK_GBS = 8000.0
-# Measured HBM bytes per pair of k_plane at Si64 FP64: rocprofv3 FETCH_SIZE (x2.000,
-# the 256 MiB copy calibration) + WRITE_SIZE (x1.000) of an 80-pair launch,
-# 205081.8 KiB / 80 pairs (profiles/k_plane_si64_fp64.txt).
-PLANE_TRAFFIC_BYTES_PER_PAIR = 205081.8 * 1024 / 80
+# Measured HBM bytes per pair of k_plane at Si64 FP64 (autotuned 512-thread shape):
+# rocprofv3 FETCH_SIZE (x2.000, the 256 MiB copy calibration) + WRITE_SIZE (x1.000)
+# of an 80-pair launch, 203917.2 KiB / 80 pairs (profiles/k_plane_si64_fp64.txt).
+PLANE_TRAFFIC_BYTES_PER_PAIR = 203917.2 * 1024 / 80
 METRIC = "Si64 SCF wall-time to 1e-8 Ha/atom: mixed+learned vs plain FP64, 1 MI355X"
 
 
@@ -95,6 +95,7 @@ def hx_profile(system, wl, count):
     op = DistributedRealHamiltonian(system, capacity=bands)
     rng = np.random.default_rng(0)
     op.set_owned_potential(torch.from_numpy(rng.standard_normal(int(np.prod(system.dims))) * 0.1).cuda())
+    op.autotune(bands)  # the launch shape the SCF legs use
     x = torch.from_numpy(rng.standard_normal((bands, system.n))).cuda()
     y = torch.empty_like(x)
     op.set_profiling(True)
