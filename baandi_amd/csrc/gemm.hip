@@ -1,4 +1,7 @@
-// FP64 tiled GEMM kernels (64x64 tiles, 256 threads, 4x4 outputs per thread).
+// FP64 tiled GEMM kernels on the matrix cores: 64x64 output tiles, 256
+// threads (4 waves, a 32x32 quadrant each), K staged through LDS 16 at a
+// time, products by v_mfma_f64_16x16x4f64 (lane l holds A[l%16][l/16],
+// B[l/16][l%16] and D[l/16 + 4v][l%16], v = 0..3; pinned by the probe below).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -11,6 +14,27 @@ namespace bd {
 namespace {
 constexpr int TM = 64, TN = 64, TK = 16;
 
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// acc[bi][bj] += the wave's 32x32 quadrant (rows wr*32.., cols wc*32..) of As^T Bs.
+__device__ __forceinline__ void mfma_tile(const double (*As)[TM + 1], const double (*Bs)[TN + 1], int lane, int wr,
+                                          int wc, f64x4 (&acc)[2][2]) {
+  const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int kk = 0; kk < TK; kk += 4) {
+    double a[2], b[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      a[t] = As[kk + lk][wr * 32 + t * 16 + li];
+      b[t] = Bs[kk + lk][wc * 32 + t * 16 + li];
+    }
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], b[bj], acc[bi][bj], 0, 0, 0);
+  }
+}
+
 // NN: C = alpha A B + beta C.
 __global__ __launch_bounds__(256) void k_gemm_nn(int64_t M, int64_t N, int64_t K, double alpha,
                                                  const double* __restrict__ A, int64_t lda,
@@ -19,9 +43,13 @@ __global__ __launch_bounds__(256) void k_gemm_nn(int64_t M, int64_t N, int64_t K
   __shared__ double As[TK][TM + 1];
   __shared__ double Bs[TK][TN + 1];
   const int tid = threadIdx.x;
-  const int tx = tid % 16, ty = tid / 16;
+  const int lane = tid & 63, wave = tid >> 6, wr = wave & 1, wc = wave >> 1;
   const int64_t row0 = int64_t(blockIdx.x) * TM, col0 = int64_t(blockIdx.y) * TN;
-  double acc[4][4] = {};
+  f64x4 acc[2][2];
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = f64x4{0.0, 0.0, 0.0, 0.0};
   for (int64_t k0 = 0; k0 < K; k0 += TK) {
     // A tile: 64 rows x 16 k, coalesced along rows.
     for (int e = tid; e < TM * TK; e += 256) {
@@ -36,32 +64,23 @@ __global__ __launch_bounds__(256) void k_gemm_nn(int64_t M, int64_t N, int64_t K
       Bs[k][j] = (gk < K && gj < N) ? B[gk + gj * ldb] : 0.0;
     }
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < TK; ++k) {
-      double a[4], b[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) a[r] = As[k][tx + 16 * r];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) b[c] = Bs[k][ty + 16 * c];
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) acc[r][c] = fma(a[r], b[c], acc[r][c]);
-    }
+    mfma_tile(As, Bs, lane, wr, wc, acc);
     __syncthreads();
   }
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    int64_t gj = col0 + ty + 16 * c;
-    if (gj >= N) continue;
+  for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      int64_t gi = row0 + tx + 16 * r;
-      if (gi >= M) continue;
-      double* p = C + gi + gj * ldc;
-      *p = beta == 0.0 ? alpha * acc[r][c] : alpha * acc[r][c] + beta * *p;
+    for (int bj = 0; bj < 2; ++bj) {
+      const int64_t gj = col0 + wc * 32 + bj * 16 + (lane & 15);
+      if (gj >= N) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t gi = row0 + wr * 32 + bi * 16 + (lane >> 4) + 4 * v;
+        if (gi >= M) continue;
+        double* p = C + gi + gj * ldc;
+        *p = beta == 0.0 ? alpha * acc[bi][bj][v] : alpha * acc[bi][bj][v] + beta * *p;
+      }
     }
-  }
 }
 
 // TN partial: part[s](MxN) = sum over k in chunk s of A[k,i] B[k,j].
@@ -72,10 +91,14 @@ __global__ __launch_bounds__(256) void k_gemm_tn_part(int64_t M, int64_t N, int6
   __shared__ double As[TK][TM + 1];
   __shared__ double Bs[TK][TN + 1];
   const int tid = threadIdx.x;
-  const int tx = tid % 16, ty = tid / 16;
+  const int lane = tid & 63, wave = tid >> 6, wr = wave & 1, wc = wave >> 1;
   const int64_t row0 = int64_t(blockIdx.x) * TM, col0 = int64_t(blockIdx.y) * TN;
   const int64_t kbeg = int64_t(blockIdx.z) * chunk, kend = std::min(K, kbeg + chunk);
-  double acc[4][4] = {};
+  f64x4 acc[2][2];
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = f64x4{0.0, 0.0, 0.0, 0.0};
   for (int64_t k0 = kbeg; k0 < kend; k0 += TK) {
     for (int e = tid; e < TM * TK; e += 256) {
       int k = e % TK, i = e / TK;
@@ -88,31 +111,33 @@ __global__ __launch_bounds__(256) void k_gemm_tn_part(int64_t M, int64_t N, int6
       Bs[k][j] = (gk < kend && gj < N) ? B[gk + gj * ldb] : 0.0;
     }
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < TK; ++k) {
-      double a[4], b[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) a[r] = As[k][tx + 16 * r];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) b[c] = Bs[k][ty + 16 * c];
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) acc[r][c] = fma(a[r], b[c], acc[r][c]);
-    }
+    mfma_tile(As, Bs, lane, wr, wc, acc);
     __syncthreads();
   }
   double* out = part + int64_t(blockIdx.z) * M * N;
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    int64_t gj = col0 + ty + 16 * c;
-    if (gj >= N) continue;
+  for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      int64_t gi = row0 + tx + 16 * r;
-      if (gi < M) out[gi + gj * M] = acc[r][c];
+    for (int bj = 0; bj < 2; ++bj) {
+      const int64_t gj = col0 + wc * 32 + bj * 16 + (lane & 15);
+      if (gj >= N) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t gi = row0 + wr * 32 + bi * 16 + (lane >> 4) + 4 * v;
+        if (gi < M) out[gi + gj * M] = acc[bi][bj][v];
+      }
     }
-  }
+}
+
+// Lane-layout probe: D (16x16, row-major) = A (16x4, row-major) B (4x16,
+// row-major) by one v_mfma_f64_16x16x4f64 with the operand layout above.
+__global__ void k_mfma_f64_probe(const double* __restrict__ A, const double* __restrict__ B, double* __restrict__ D) {
+  const int lane = threadIdx.x;
+  const double a = A[(lane & 15) * 4 + (lane >> 4)];
+  const double b = B[(lane >> 4) * 16 + (lane & 15)];
+  f64x4 d = {0.0, 0.0, 0.0, 0.0};
+  d = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, d, 0, 0, 0);
+  for (int v = 0; v < 4; ++v) D[((lane >> 4) + 4 * v) * 16 + (lane & 15)] = d[v];
 }
 
 __global__ void k_gemm_tn_reduce(int64_t M, int64_t N, int splits, double alpha,
@@ -181,6 +206,14 @@ extern "C" bd_status bd_gemm_dev(int32_t mode, int32_t trans_a, int64_t M, int64
     else g.nn(nullptr, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
     if (hipDeviceSynchronize() != hipSuccess || hipGetLastError() != hipSuccess)
       bd::fail(BD_ERR_DEVICE, "gemm launch failed");
+  });
+}
+
+extern "C" bd_status bd_mfma_f64_probe_dev(const double* a, const double* b, double* d) {
+  BD_GUARD({
+    if (!a || !b || !d) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    hipLaunchKernelGGL(bd::k_mfma_f64_probe, dim3(1), dim3(64), 0, nullptr, a, b, d);
+    if (hipDeviceSynchronize() != hipSuccess) bd::fail(BD_ERR_DEVICE, "probe failed");
   });
 }
 

@@ -26,7 +26,7 @@ EXPORTS = [
     "bd_block_start_dev", "bd_block_step", "bd_block_step_async", "bd_block_finish", "bd_block_vectors_dev",
     "bd_k_op_create", "bd_k_op_info", "bd_k_op_get", "bd_k_batch_create", "bd_k_batch_destroy",
     "bd_k_batch_stream", "bd_k_batch_apply_dev", "bd_gamma_op_autotune", "bd_gamma_op_fused_shape",
-    "bd_gemm_workspace", "bd_gemm_dev", "bd_mfma_probe_dev",
+    "bd_gemm_workspace", "bd_gemm_dev", "bd_mfma_probe_dev", "bd_mfma_f64_probe_dev",
     "bd_sym_eigen_workspace", "bd_sym_eigen_dev", "bd_inverse_cholesky_dev",
     "bd_mlp_param_count", "bd_mlp_blob_floats", "bd_mlp_quantize_dev", "bd_mlp_forward_dev",
     "bd_mlp_train_workspace", "bd_mlp_loss_grad_dev", "bd_mlp_adam_dev", "bd_mlp_train_dev",
@@ -133,6 +133,7 @@ def load():
         "bd_block_vectors_dev": [P, P, P, I64],
         "bd_gemm_dev": [I32, I32, I64, I64, I64, D, P, I64, P, I64, D, P, I64, P],
         "bd_mfma_probe_dev": [P, P, P],
+        "bd_mfma_f64_probe_dev": [P, P, P],
         "bd_sym_eigen_dev": [I32, I32, P, I64, P, P, I64, P],
         "bd_inverse_cholesky_dev": [P, P, I32, I32, I32, P, P],
         "bd_mlp_quantize_dev": [P, P],

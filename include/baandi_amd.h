@@ -292,6 +292,9 @@ bd_status bd_gemm_dev(int32_t mode, int32_t trans_a, int64_t M, int64_t N, int64
                       int64_t ldc, double* work);
 /* One v_mfma_f32_16x16x16_bf16 of row-major 16x16 bf16 bit patterns (a self
  * check of the MFMA operand layout the kernels assume). */
+/* Lane-layout probe of v_mfma_f64_16x16x4f64 (the FP64 GEMMs' product):
+ * D (16x16) = A (16x4) B (4x16), all row-major device doubles. */
+bd_status bd_mfma_f64_probe_dev(const double* a_dev, const double* b_dev, double* d_dev);
 bd_status bd_mfma_probe_dev(const uint16_t* a_bits, const uint16_t* b_bits, float* d);
 
 /* ------------------------------------------------------------------------

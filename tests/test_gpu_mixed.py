@@ -32,6 +32,48 @@ def test_mfma_operand_layout():
     assert np.array_equal(d.cpu().numpy().reshape(16, 16), a @ b)
 
 
+def test_mfma_f64_operand_layout():
+    """One v_mfma_f64_16x16x4f64 on random doubles against the host product
+    (4-term sums: equal to a few ulps): pins the lane layout of the FP64 GEMMs."""
+    import torch
+    from baandi_amd import lib as L
+    rng = np.random.default_rng(2)
+    a = rng.standard_normal((16, 4))
+    b = rng.standard_normal((4, 16))
+    ad, bd = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()  # held until the kernel has run
+    d = torch.zeros(256, dtype=torch.float64, device="cuda")
+    L.check(L.load().bd_mfma_f64_probe_dev(L.ptr(ad), L.ptr(bd), L.ptr(d)))
+    torch.cuda.synchronize()
+    assert np.abs(d.cpu().numpy().reshape(16, 16) - a @ b).max() < 1e-14
+
+
+@pytest.mark.parametrize("trans_a,M,N,K", [(False, 300, 47, 130), (True, 70, 33, 5000), (False, 1, 1, 1),
+                                           (True, 65, 64, 17)])
+def test_fp64_gemm_matches_numpy(trans_a, M, N, K):
+    """The FP64 MFMA GEMMs (NN, and TN with split-K) against NumPy: 1e-14 of
+    sum |a||b| (edge tiles, beta = 0.5 included)."""
+    import torch
+    from baandi_amd import lib as L
+    rng = np.random.default_rng(M + N + K)
+    A = rng.standard_normal((K, M) if trans_a else (M, K))
+    B = rng.standard_normal((K, N))
+    C0 = rng.standard_normal((M, N))
+    ref = 1.5 * ((A.T if trans_a else A) @ B) + 0.5 * C0
+    bound = 1e-14 * (np.abs(A.T if trans_a else A) @ np.abs(B)).max() * 1.5 + 1e-14
+    Ad = torch.from_numpy(np.asfortranarray(A).ravel(order="F").copy()).cuda()
+    Bd = torch.from_numpy(np.asfortranarray(B).ravel(order="F").copy()).cuda()
+    Cd = torch.from_numpy(np.asfortranarray(C0).ravel(order="F").copy()).cuda()
+    lib = L.load()
+    work = torch.empty(max(lib.bd_gemm_workspace(0, int(trans_a), M, N, K), 1), dtype=torch.float64, device="cuda")
+    L.check(lib.bd_gemm_dev(0, int(trans_a), M, N, K, 1.5, L.ptr(Ad), A.shape[0], L.ptr(Bd), K, 0.5, L.ptr(Cd), M,
+                            L.ptr(work)))
+    torch.cuda.synchronize()
+    out = Cd.cpu().numpy().reshape(N, M).T
+    err = np.abs(out - ref).max()
+    print(f"[fp64 gemm] trans_a={trans_a} {M}x{N}x{K}: max err {err:.2e} (bound {bound:.2e})")
+    assert err < bound
+
+
 @pytest.mark.parametrize("trans_a,M,N,K", [(False, 23847, 160, 320), (True, 320, 160, 23847),
                                            (True, 37, 5, 1000), (False, 100, 7, 33)])
 def test_bf16x3_gemm_error_bound(trans_a, M, N, K):
