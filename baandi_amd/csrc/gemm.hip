@@ -1,7 +1,8 @@
-// FP64 tiled GEMM kernels on the matrix cores: 64x64 output tiles, 256
-// threads (4 waves, a 32x32 quadrant each), K staged through LDS 16 at a
-// time, products by v_mfma_f64_16x16x4f64 (lane l holds A[l%16][l/16],
-// B[l/16][l%16] and D[l/16 + 4v][l%16], v = 0..3; pinned by the probe below).
+// FP64 tiled GEMM kernels on the matrix cores: 64 x BN output tiles (BN 64;
+// 128 is built but slower here), 256 threads (4 waves, a 32 x BN/2 quadrant each), K
+// staged through LDS 16 at a time, products by v_mfma_f64_16x16x4f64 (lane l
+// holds A[l%16][l/16], B[l/16][l%16] and D[l/16 + 4v][l%16], v = 0..3;
+// pinned by the probe below).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -12,44 +13,49 @@
 namespace bd {
 
 namespace {
-constexpr int TM = 64, TN = 64, TK = 16;
+constexpr int TM = 64, TK = 16;
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-// acc[bi][bj] += the wave's 32x32 quadrant (rows wr*32.., cols wc*32..) of As^T Bs.
-__device__ __forceinline__ void mfma_tile(const double (*As)[TM + 1], const double (*Bs)[TN + 1], int lane, int wr,
-                                          int wc, f64x4 (&acc)[2][2]) {
+// acc[bi][bj] += the wave's quadrant (rows wr*32 + [0, 32), cols wc*BN/2 +
+// [0, BN/2)) of As^T Bs for one TK slice.
+template <int BN>
+__device__ __forceinline__ void mfma_tile(const double (*As)[TM + 1], const double (*Bs)[BN + 1], int lane, int wr,
+                                          int wc, f64x4 (&acc)[2][BN / 32]) {
+  constexpr int NB = BN / 32;
   const int li = lane & 15, lk = lane >> 4;
 #pragma unroll
   for (int kk = 0; kk < TK; kk += 4) {
-    double a[2], b[2];
+    double a[2], b[NB];
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      a[t] = As[kk + lk][wr * 32 + t * 16 + li];
-      b[t] = Bs[kk + lk][wc * 32 + t * 16 + li];
-    }
+    for (int t = 0; t < 2; ++t) a[t] = As[kk + lk][wr * 32 + t * 16 + li];
+#pragma unroll
+    for (int t = 0; t < NB; ++t) b[t] = Bs[kk + lk][wc * (BN / 2) + t * 16 + li];
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-      for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], b[bj], acc[bi][bj], 0, 0, 0);
+      for (int bj = 0; bj < NB; ++bj)
+        acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], b[bj], acc[bi][bj], 0, 0, 0);
   }
 }
 
-// NN: C = alpha A B + beta C.
+// NN: C = alpha A B + beta C; 64 x BN output tile per workgroup.
+template <int BN>
 __global__ __launch_bounds__(256) void k_gemm_nn(int64_t M, int64_t N, int64_t K, double alpha,
                                                  const double* __restrict__ A, int64_t lda,
                                                  const double* __restrict__ B, int64_t ldb,
                                                  double beta, double* __restrict__ C, int64_t ldc) {
+  constexpr int NB = BN / 32;
   __shared__ double As[TK][TM + 1];
-  __shared__ double Bs[TK][TN + 1];
+  __shared__ double Bs[TK][BN + 1];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6, wr = wave & 1, wc = wave >> 1;
-  const int64_t row0 = int64_t(blockIdx.x) * TM, col0 = int64_t(blockIdx.y) * TN;
-  f64x4 acc[2][2];
+  const int64_t row0 = int64_t(blockIdx.x) * TM, col0 = int64_t(blockIdx.y) * BN;
+  f64x4 acc[2][NB];
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-    for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int bj = 0; bj < NB; ++bj) acc[bi][bj] = f64x4{0.0, 0.0, 0.0, 0.0};
   for (int64_t k0 = 0; k0 < K; k0 += TK) {
     // A tile: 64 rows x 16 k, coalesced along rows.
     for (int e = tid; e < TM * TK; e += 256) {
@@ -57,21 +63,21 @@ __global__ __launch_bounds__(256) void k_gemm_nn(int64_t M, int64_t N, int64_t K
       int64_t gi = row0 + i, gk = k0 + k;
       As[k][i] = (gi < M && gk < K) ? A[gi + gk * lda] : 0.0;
     }
-    // B tile: 16 k x 64 cols, coalesced along k.
-    for (int e = tid; e < TK * TN; e += 256) {
+    // B tile: 16 k x BN cols, coalesced along k.
+    for (int e = tid; e < TK * BN; e += 256) {
       int k = e % TK, j = e / TK;
       int64_t gk = k0 + k, gj = col0 + j;
       Bs[k][j] = (gk < K && gj < N) ? B[gk + gj * ldb] : 0.0;
     }
     __syncthreads();
-    mfma_tile(As, Bs, lane, wr, wc, acc);
+    mfma_tile<BN>(As, Bs, lane, wr, wc, acc);
     __syncthreads();
   }
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-    for (int bj = 0; bj < 2; ++bj) {
-      const int64_t gj = col0 + wc * 32 + bj * 16 + (lane & 15);
+    for (int bj = 0; bj < NB; ++bj) {
+      const int64_t gj = col0 + wc * (BN / 2) + bj * 16 + (lane & 15);
       if (gj >= N) continue;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -84,42 +90,44 @@ __global__ __launch_bounds__(256) void k_gemm_nn(int64_t M, int64_t N, int64_t K
 }
 
 // TN partial: part[s](MxN) = sum over k in chunk s of A[k,i] B[k,j].
+template <int BN>
 __global__ __launch_bounds__(256) void k_gemm_tn_part(int64_t M, int64_t N, int64_t K, int64_t chunk,
                                                       const double* __restrict__ A, int64_t lda,
                                                       const double* __restrict__ B, int64_t ldb,
                                                       double* __restrict__ part) {
+  constexpr int NB = BN / 32;
   __shared__ double As[TK][TM + 1];
-  __shared__ double Bs[TK][TN + 1];
+  __shared__ double Bs[TK][BN + 1];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6, wr = wave & 1, wc = wave >> 1;
-  const int64_t row0 = int64_t(blockIdx.x) * TM, col0 = int64_t(blockIdx.y) * TN;
+  const int64_t row0 = int64_t(blockIdx.x) * TM, col0 = int64_t(blockIdx.y) * BN;
   const int64_t kbeg = int64_t(blockIdx.z) * chunk, kend = std::min(K, kbeg + chunk);
-  f64x4 acc[2][2];
+  f64x4 acc[2][NB];
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-    for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int bj = 0; bj < NB; ++bj) acc[bi][bj] = f64x4{0.0, 0.0, 0.0, 0.0};
   for (int64_t k0 = kbeg; k0 < kend; k0 += TK) {
     for (int e = tid; e < TM * TK; e += 256) {
       int k = e % TK, i = e / TK;
       int64_t gi = row0 + i, gk = k0 + k;
       As[k][i] = (gi < M && gk < kend) ? A[gk + gi * lda] : 0.0;
     }
-    for (int e = tid; e < TK * TN; e += 256) {
+    for (int e = tid; e < TK * BN; e += 256) {
       int k = e % TK, j = e / TK;
       int64_t gk = k0 + k, gj = col0 + j;
       Bs[k][j] = (gk < kend && gj < N) ? B[gk + gj * ldb] : 0.0;
     }
     __syncthreads();
-    mfma_tile(As, Bs, lane, wr, wc, acc);
+    mfma_tile<BN>(As, Bs, lane, wr, wc, acc);
     __syncthreads();
   }
   double* out = part + int64_t(blockIdx.z) * M * N;
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-    for (int bj = 0; bj < 2; ++bj) {
-      const int64_t gj = col0 + wc * 32 + bj * 16 + (lane & 15);
+    for (int bj = 0; bj < NB; ++bj) {
+      const int64_t gj = col0 + wc * (BN / 2) + bj * 16 + (lane & 15);
       if (gj >= N) continue;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -152,8 +160,13 @@ __global__ void k_gemm_tn_reduce(int64_t M, int64_t N, int splits, double alpha,
   *c = beta == 0.0 ? alpha * s : alpha * s + beta * *c;
 }
 
+// Column tile width. 128-wide tiles read A half as often but were slower on
+// the Davidson shapes (fewer workgroups, longer B-tile loads): 64 is used.
+int col_tile(int64_t) { return 64; }
+
 int tn_splits(int64_t M, int64_t N, int64_t K) {
-  const int64_t tiles = ((M + TM - 1) / TM) * ((N + TN - 1) / TN);
+  const int bn = col_tile(N);
+  const int64_t tiles = ((M + TM - 1) / TM) * ((N + bn - 1) / bn);
   int64_t want = std::max<int64_t>(1, 1024 / std::max<int64_t>(tiles, 1));
   int64_t maxs = std::max<int64_t>(1, K / 256);
   return int(std::min<int64_t>({want, maxs, 256}));
@@ -163,8 +176,12 @@ int tn_splits(int64_t M, int64_t N, int64_t K) {
 void gemm_nn(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
              int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc) {
   if (M <= 0 || N <= 0) return;
-  dim3 grid(unsigned((M + TM - 1) / TM), unsigned((N + TN - 1) / TN));
-  hipLaunchKernelGGL(k_gemm_nn, grid, dim3(256), 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+  const int bn = col_tile(N);
+  dim3 grid(unsigned((M + TM - 1) / TM), unsigned((N + bn - 1) / bn));
+  if (bn == 128)
+    hipLaunchKernelGGL(k_gemm_nn<128>, grid, dim3(256), 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+  else
+    hipLaunchKernelGGL(k_gemm_nn<64>, grid, dim3(256), 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
 int64_t gemm_tn_work(int64_t M, int64_t N, int64_t K) { return int64_t(tn_splits(M, N, K)) * M * N; }
@@ -177,8 +194,12 @@ void gemm_tn(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const
   int64_t chunk = (K + splits - 1) / splits;
   chunk = ((chunk + TK - 1) / TK) * TK;
   const int used = int((K + chunk - 1) / std::max<int64_t>(chunk, 1));
-  dim3 grid(unsigned((M + TM - 1) / TM), unsigned((N + TN - 1) / TN), unsigned(std::max(used, 1)));
-  hipLaunchKernelGGL(k_gemm_tn_part, grid, dim3(256), 0, s, M, N, K, chunk, A, lda, B, ldb, work);
+  const int bn = col_tile(N);
+  dim3 grid(unsigned((M + TM - 1) / TM), unsigned((N + bn - 1) / bn), unsigned(std::max(used, 1)));
+  if (bn == 128)
+    hipLaunchKernelGGL(k_gemm_tn_part<128>, grid, dim3(256), 0, s, M, N, K, chunk, A, lda, B, ldb, work);
+  else
+    hipLaunchKernelGGL(k_gemm_tn_part<64>, grid, dim3(256), 0, s, M, N, K, chunk, A, lda, B, ldb, work);
   const int64_t total = M * N;
   hipLaunchKernelGGL(k_gemm_tn_reduce, dim3(unsigned((total + 255) / 256)), dim3(256), 0, s, M, N,
                      std::max(used, 1), alpha, work, beta, C, ldc);

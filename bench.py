@@ -114,6 +114,46 @@ def hx_profile(system, wl, count):
                       "copy_bytes_each_way": a.numel() * 8}))
 
 
+def gemm_leg(reps=10):
+    """The Si64 subspace products on the matrix cores: the border Gram
+    V^T E (320 x 160 x 23,847, split-K) and the rotation V C (23,847 x 160 x
+    320), FP64 MFMA and BF16x3 MFMA. Rates: 2MNK / t ("effective") and, for
+    BF16x3, the 3 bf16 products per term the MFMA units execute."""
+    import numpy as np
+    import torch
+    from baandi_amd import lib as L
+    lib = L.load()
+    rng = np.random.default_rng(0)
+    out = {}
+    for name, trans, M, N, K in (("border_gram_tn", 1, 320, 160, 23847), ("rotation_nn", 0, 23847, 160, 320)):
+        A = torch.from_numpy(rng.standard_normal(M * K)).cuda()
+        B = torch.from_numpy(rng.standard_normal(K * N)).cuda()
+        C = torch.zeros(M * N, dtype=torch.float64, device="cuda")
+        lda = K if trans else M
+        for mode, label in ((0, "fp64"), (1, "bf16x3")):
+            work = torch.empty(max(lib.bd_gemm_workspace(mode, trans, M, N, K), 1), dtype=torch.float64,
+                               device="cuda")
+            call = lambda: L.check(lib.bd_gemm_dev(mode, trans, M, N, K, 1.0, L.ptr(A), lda, L.ptr(B), K, 0.0,
+                                                   L.ptr(C), M, L.ptr(work)))
+            call()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(reps):
+                call()
+            b.record()
+            b.synchronize()
+            ms = a.elapsed_time(b) / reps
+            tf = 2.0 * M * N * K / (ms * 1e-3) / 1e12
+            rec = {"shape": [M, N, K], "ms": round(ms, 4), "effective_TFLOPs": round(tf, 2)}
+            if mode == 0:
+                rec["frac_fp64_peak"] = round(tf / 78.6, 4)
+            else:
+                rec["mfma_TFLOPs"] = round(3 * tf, 2)
+                rec["frac_bf16_peak"] = round(3 * tf / 2500.0, 4)
+            out[f"{name}_{label}"] = rec
+    return out
+
+
 def kpoint_leg(steps, threads=16):
     """Al fcc 4-atom cell, 6x6x6 Monkhorst-Pack (108 k points), 25 Ry, 16
     bands, Fermi-Dirac 0.01 Ha: k-point SCF wall time with the k points
@@ -301,6 +341,7 @@ def main():
             "fused_compulsory_GBs": round(fused, 1),
         },
     }
+    out["subspace_gemms"] = gemm_leg()
     if args.kpoint_steps > 0:
         out["kpoint"] = kpoint_leg(args.kpoint_steps)
     if not args.no_cpu:
