@@ -45,6 +45,7 @@ __device__ __forceinline__ double wave_max(double v) {
 
 constexpr int kTT = 1024;      // tridiagonalisation workgroup
 constexpr int kTW = kTT / 64;  // its waves
+constexpr int kLC = 2;         // columns per wave pass of the lower-triangle update
 
 // Sum over a kTT-thread block, returned to every thread (fixed order).
 // `red` holds kTW doubles of LDS; ends with a barrier so it can be reused.
@@ -55,6 +56,20 @@ __device__ double block_sum(double v, double* red) {
   double t = 0.0;
 #pragma unroll
   for (int i = 0; i < kTW; ++i) t += red[i];
+  __syncthreads();
+  return t;
+}
+
+// block_sum that also reads *peek into *peeked between its two barriers (after
+// every thread's writes before the call, before any write after it).
+__device__ double block_sum_peek(double v, double* red, const double* peek, double* peeked) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int i = 0; i < kTW; ++i) t += red[i];
+  *peeked = *peek;
   __syncthreads();
   return t;
 }
@@ -75,10 +90,15 @@ __device__ __forceinline__ double hash_unit(uint32_t i, uint32_t j, uint32_t sal
 // A[k+1:, k] (x[0] = a - alpha, unnormalised); d, e the tridiagonal
 // (e[k] couples d[k] and d[k+1]).
 // ---------------------------------------------------------------------------
+// LOWER: after the first matvec only the lower triangle of the trailing
+// block is read and written (half the L2 traffic); the matvec's row part
+// B[j, j'] x_j' (j' < j) is accumulated per wave in LDS and summed after the
+// pass. Needs kTW more vectors of m in LDS.
+template <bool LOWER>
 __global__ __launch_bounds__(kTT) void k_tridiag(int m, double* __restrict__ A, int64_t lda, double* __restrict__ d,
                                                  double* __restrict__ e, double* __restrict__ tau) {
   // LDS: two reflector vectors, w, q = B x, the next step's first column,
-  // and two reduction slots per wave.
+  // two reduction slots per wave and (LOWER) a row accumulator per wave.
   extern __shared__ double sh[];
   double* xa = sh;
   double* xb = sh + m;
@@ -87,6 +107,7 @@ __global__ __launch_bounds__(kTT) void k_tridiag(int m, double* __restrict__ A, 
   double* nc = sh + 4 * m;
   double* red = sh + 5 * m;
   double* red2 = red + kTW;
+  double* pw = red2 + kTW;  // LOWER: kTW x m
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   auto a = [&](int i, int j) -> double& { return A[i + int64_t(j) * lda]; };
   if (m < 3) {
@@ -124,9 +145,31 @@ __global__ __launch_bounds__(kTT) void k_tridiag(int m, double* __restrict__ A, 
     return tk;
   };
 
+  // The same with x0 = xv[0] already read by every thread (block_sum_peek),
+  // so only the write needs a barrier.
+  auto reflect_read = [&](double ss, double x0, double* xv, int k) -> double {
+    const double sigma = sqrt(ss);
+    double alpha = 0.0, tk = 0.0;
+    if (sigma > 0.0) {
+      alpha = x0 >= 0.0 ? -sigma : sigma;
+      tk = 1.0 / (sigma * (sigma + fabs(x0)));
+    }
+    if (tid == 0) {
+      if (sigma > 0.0) {
+        xv[0] = x0 - alpha;
+        a(k + 1, k) = x0 - alpha;
+      }
+      e[k] = alpha;
+      tau[k] = tk;
+    }
+    __syncthreads();
+    return tk;
+  };
+
   double* x = xa;
   double* xn = xb;
   int L = m - 1;
+  double qx_carry = 0.0;  // LOWER: q'.x' of the step before, in every thread
   double ss = 0.0;
   for (int r = tid; r < L; r += kTT) {
     const double v = a(1 + r, 0);
@@ -165,14 +208,20 @@ __global__ __launch_bounds__(kTT) void k_tridiag(int m, double* __restrict__ A, 
     }
     if (lane == 0) red2[wave] = qx;
   }
+  if (LOWER)
+    for (int i = tid; i < kTW * m; i += kTT) pw[i] = 0.0;
   __syncthreads();
 
   for (int k = 0; k + 2 < m; ++k) {
     L = m - k - 1;
     const int c0 = k + 1;  // global index of the trailing block's first row/column
     double qx = 0.0;
+    if (LOWER && k > 0) {
+      qx = qx_carry;
+    } else {
 #pragma unroll
-    for (int i = 0; i < kTW; ++i) qx += red2[i];
+      for (int i = 0; i < kTW; ++i) qx += red2[i];
+    }
     // p = t q, K = t/2 p.x = t^2/2 q.x, w = p - K x.
     const double K = 0.5 * t * t * qx;
     for (int r = tid; r < L; r += kTT) w[r] = t * q[r] - K * x[r];
@@ -200,48 +249,102 @@ __global__ __launch_bounds__(kTT) void k_tridiag(int m, double* __restrict__ A, 
         ss += b * b;
       }
     }
-    ss = block_sum(ss, red);
-    const double tn = reflect(ss, xn, c0);
-    // Rest of B_k: B -= x w^T + w x^T and q' = B_{k+1} x' in one pass, four
-    // columns per wave (four loads in flight per lane). Local column 1 is the
-    // next step's first column: it goes to nc as well.
-    qx = 0.0;
-    for (int j0 = 1 + 4 * wave; j0 < L; j0 += 4 * kTW) {
-      double acc[4] = {0.0, 0.0, 0.0, 0.0}, xj[4], wj[4];
-      double* col[4];
+    double xn0 = 0.0;
+    ss = block_sum_peek(ss, red, xn, &xn0);
+    const double tn = reflect_read(ss, xn0, xn, c0);
+    if constexpr (LOWER) {
+      // Rest of B_k, lower triangle (rows r >= j of column j): B -= x w^T +
+      // w x^T and q' = B_{k+1} x', the column part sum_{r>=j} b xn_{r-1} per
+      // column, the row part b xn_{j-1} into the wave's row accumulator.
+      double* pwv = pw + int64_t(wave) * m;
+      for (int j0 = 1 + kLC * wave; j0 < L; j0 += kLC * kTW) {
+        double acc[kLC], xj[kLC], wj[kLC], xnj[kLC];
+        double* col[kLC];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const bool ok = j0 + c < L;
-        xj[c] = ok ? x[j0 + c] : 0.0;
-        wj[c] = ok ? w[j0 + c] : 0.0;
-        col[c] = &a(c0, c0 + (ok ? j0 + c : j0));
+        for (int c = 0; c < kLC; ++c) {
+          acc[c] = 0.0;
+          const bool ok = j0 + c < L;
+          xj[c] = ok ? x[j0 + c] : 0.0;
+          wj[c] = ok ? w[j0 + c] : 0.0;
+          xnj[c] = ok ? xn[j0 + c - 1] : 0.0;
+          col[c] = &a(c0, c0 + (ok ? j0 + c : j0));
+        }
+        for (int r = j0 + lane; r < L; r += 64) {
+          const double xr = x[r], wr = w[r], xnr = xn[r - 1];
+          double pr = 0.0;
+#pragma unroll
+          for (int c = 0; c < kLC; ++c) {
+            if (j0 + c < L && r >= j0 + c) {
+              const double b = col[c][r] - (xr * wj[c] + wr * xj[c]);
+              col[c][r] = b;
+              acc[c] += b * xnr;
+              if (r > j0 + c) pr += b * xnj[c];
+              if (j0 + c == 1) nc[r - 1] = b;
+            }
+          }
+          pwv[r - 1] += pr;
+        }
+#pragma unroll
+        for (int c = 0; c < kLC; ++c) {
+          const double v = wave_sum(acc[c]);
+          if (j0 + c < L && lane == 0) q[j0 + c - 1] = v;
+        }
       }
-      for (int r = 1 + lane; r < L; r += 64) {
-        const double xr = x[r], wr = w[r], xnr = xn[r - 1];
-        double b[4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) b[c] = col[c][r];
+      __syncthreads();
+      // q'_i += the row parts of every wave (waves in fixed order); q'.x'.
+      double qxs = 0.0;
+      for (int i = tid; i < L - 1; i += kTT) {
+        double sacc = q[i];
+        for (int ww = 0; ww < kTW; ++ww) {
+          sacc += pw[int64_t(ww) * m + i];
+          pw[int64_t(ww) * m + i] = 0.0;
+        }
+        q[i] = sacc;
+        qxs += sacc * xn[i];
+      }
+      qx_carry = block_sum(qxs, red);
+    } else {
+      // Rest of B_k: B -= x w^T + w x^T and q' = B_{k+1} x' in one pass, four
+      // columns per wave (four loads in flight per lane). Local column 1 is the
+      // next step's first column: it goes to nc as well.
+      qx = 0.0;
+      for (int j0 = 1 + 4 * wave; j0 < L; j0 += 4 * kTW) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0}, xj[4], wj[4];
+        double* col[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
+          const bool ok = j0 + c < L;
+          xj[c] = ok ? x[j0 + c] : 0.0;
+          wj[c] = ok ? w[j0 + c] : 0.0;
+          col[c] = &a(c0, c0 + (ok ? j0 + c : j0));
+        }
+        for (int r = 1 + lane; r < L; r += 64) {
+          const double xr = x[r], wr = w[r], xnr = xn[r - 1];
+          double b[4];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) b[c] = col[c][r];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            if (j0 + c < L) {
+              b[c] -= xr * wj[c] + wr * xj[c];
+              col[c][r] = b[c];
+              acc[c] += b[c] * xnr;
+            }
+          }
+          if (j0 == 1) nc[r - 1] = b[0];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const double v = wave_sum(acc[c]);
           if (j0 + c < L) {
-            b[c] -= xr * wj[c] + wr * xj[c];
-            col[c][r] = b[c];
-            acc[c] += b[c] * xnr;
+            if (lane == 0) q[j0 + c - 1] = v;
+            qx += v * xn[j0 + c - 1];
           }
         }
-        if (j0 == 1) nc[r - 1] = b[0];
       }
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const double v = wave_sum(acc[c]);
-        if (j0 + c < L) {
-          if (lane == 0) q[j0 + c - 1] = v;
-          qx += v * xn[j0 + c - 1];
-        }
-      }
+      if (lane == 0) red2[wave] = qx;
+      __syncthreads();
     }
-    if (lane == 0) red2[wave] = qx;
-    __syncthreads();
     double* tmp = x;
     x = xn;
     xn = tmp;
@@ -670,12 +773,18 @@ void sym_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, do
     fail(BD_ERR_INVALID_PARAMETER, "sym_eigen_lowest: subspace too large for the LDS kernels");
   const EigenLayout L = eigen_layout(m, want, work);
   hipLaunchKernelGGL(k_zero_i, dim3(1), dim3(64), 0, s, info, 1);
-  const size_t tri_lds = size_t(5 * m + 2 * kTW) * 8;
+  const size_t tri_lds_lower = size_t(5 * m + 2 * kTW + kTW * m) * 8;
+  const bool lower = tri_lds_lower <= 160 * 1024;
+  const size_t tri_lds = lower ? tri_lds_lower : size_t(5 * m + 2 * kTW) * 8;
+  const void* tri_fn = lower ? reinterpret_cast<const void*>(&k_tridiag<true>)
+                             : reinterpret_cast<const void*>(&k_tridiag<false>);
   if (tri_lds > 64 * 1024 &&
-      hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tridiag), hipFuncAttributeMaxDynamicSharedMemorySize,
-                          int(tri_lds)) != hipSuccess)
+      hipFuncSetAttribute(tri_fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(tri_lds)) != hipSuccess)
     fail(BD_ERR_DEVICE, "k_tridiag LDS attribute");
-  hipLaunchKernelGGL(k_tridiag, dim3(1), dim3(kTT), tri_lds, s, m, A, lda, L.d, L.e, L.tau);
+  if (lower)
+    hipLaunchKernelGGL(k_tridiag<true>, dim3(1), dim3(kTT), tri_lds, s, m, A, lda, L.d, L.e, L.tau);
+  else
+    hipLaunchKernelGGL(k_tridiag<false>, dim3(1), dim3(kTT), tri_lds, s, m, A, lda, L.d, L.e, L.tau);
   hipLaunchKernelGGL(k_bisect, dim3((want + 3) / 4), dim3(256), size_t(3 * m) * 8, s, m, want, L.d, L.e, values,
                      L.tnorm);
   hipLaunchKernelGGL(k_invit, dim3((want + 63) / 64), dim3(64), size_t(2 * m) * 8, s, m, want, L.d, L.e, values, L.tnorm, L.f, L.Z);
