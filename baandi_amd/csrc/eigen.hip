@@ -523,6 +523,63 @@ __device__ double lu_solve_prefetch(int m, int want, int j, const LU& f, double*
   return nn;
 }
 
+// The per-vector work of inverse iteration: LU of (T - lambda' I) into g
+// (entries at i * stride + lane), a hashed start, two solves, unit norm.
+// Column j of the caller's output numbering seeds the start vector.
+__device__ void invit_vector(int m, int stride, int lane, int j, const double* ds, const double* es, double lam,
+                             double tiny, const LU& g, double* z) {
+  auto at = [&](int i) { return int64_t(i) * stride + lane; };
+  double a = ds[0] - lam, b = es[0];
+  for (int i = 0; i + 1 < m; ++i) {
+    const double c = es[i];
+    const double dn = ds[i + 1] - lam;
+    const double en = es[i + 1];
+    if (fabs(a) >= fabs(c)) {
+      double p = a;
+      if (fabs(p) < tiny) p = p >= 0.0 ? tiny : -tiny;
+      const double r = 1.0 / p;
+      g.U0i[at(i)] = r;
+      g.U1[at(i)] = b;
+      g.U2[at(i)] = 0.0;
+      g.Lm[at(i)] = c * r;
+      g.piv[at(i)] = 0;
+      a = dn - c * r * b;
+      b = en;
+    } else {
+      const double r = 1.0 / c;
+      const double l = a * r;
+      g.U0i[at(i)] = r;
+      g.U1[at(i)] = dn;
+      g.U2[at(i)] = en;
+      g.Lm[at(i)] = l;
+      g.piv[at(i)] = 1;
+      a = b - l * dn;
+      b = -l * en;
+    }
+  }
+  if (fabs(a) < tiny) a = a >= 0.0 ? tiny : -tiny;
+  g.U0i[at(m - 1)] = 1.0 / a;
+  for (int i = 0; i < m; ++i) z[at(i)] = hash_unit(i, j, 0);
+  // Two solves (the second from the unscaled first: growth per solve is at
+  // most ~1/(eps ||T||), far from overflow), then unit norm.
+  lu_solve_prefetch(m, stride, lane, g, z);
+  const double nn = lu_solve_prefetch(m, stride, lane, g, z);
+  const double sc = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
+  for (int i = 0; i < m; ++i) z[at(i)] *= sc;
+}
+
+// Shift of vector j: shifts of (numerically) equal eigenvalues are spread
+// 10 eps ||T|| apart, as LAPACK's dstein does: an exactly repeated eigenvalue
+// (the realified complex problem has only such pairs) would otherwise give
+// every vector of the pair one singular factorisation and one direction.
+__device__ double invit_shift(const double* values, int j, double tnorm) {
+  const double pertol = 10.0 * DBL_EPSILON * tnorm;
+  double lam = values[0];
+  for (int i = 1; i <= j; ++i) lam = fmax(values[i], lam + pertol);
+  return lam;
+}
+
+// Lane per vector, factors and vector in global memory (any m).
 __global__ __launch_bounds__(64) void k_invit(int m, int want, const double* __restrict__ d,
                                               const double* __restrict__ e, const double* __restrict__ values,
                                               const double* __restrict__ tnorm_in, LU f, double* __restrict__ Z) {
@@ -536,52 +593,49 @@ __global__ __launch_bounds__(64) void k_invit(int m, int want, const double* __r
   __syncthreads();
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= want) return;
-  auto at = [&](int i) { return int64_t(i) * want + j; };
-  // Shifts of (numerically) equal eigenvalues are spread 10 eps ||T|| apart,
-  // as LAPACK's dstein does: an exactly repeated eigenvalue (the realified
-  // complex problem has only such pairs) would otherwise give every vector of
-  // the pair one singular factorisation and one direction.
-  const double pertol = 10.0 * DBL_EPSILON * tnorm_in[0];
-  double lam = values[0];
-  for (int i = 1; i <= j; ++i) lam = fmax(values[i], lam + pertol);
   const double tiny = DBL_EPSILON * fmax(tnorm_in[0], DBL_MIN);
-  double a = ds[0] - lam, b = es[0];
-  for (int i = 0; i + 1 < m; ++i) {
-    const double c = es[i];
-    const double dn = ds[i + 1] - lam;
-    const double en = es[i + 1];
-    if (fabs(a) >= fabs(c)) {
-      double p = a;
-      if (fabs(p) < tiny) p = p >= 0.0 ? tiny : -tiny;
-      const double r = 1.0 / p;
-      f.U0i[at(i)] = r;
-      f.U1[at(i)] = b;
-      f.U2[at(i)] = 0.0;
-      f.Lm[at(i)] = c * r;
-      f.piv[at(i)] = 0;
-      a = dn - c * r * b;
-      b = en;
-    } else {
-      const double r = 1.0 / c;
-      const double l = a * r;
-      f.U0i[at(i)] = r;
-      f.U1[at(i)] = dn;
-      f.U2[at(i)] = en;
-      f.Lm[at(i)] = l;
-      f.piv[at(i)] = 1;
-      a = b - l * dn;
-      b = -l * en;
-    }
+  invit_vector(m, want, j, j, ds, es, invit_shift(values, j, tnorm_in[0]), tiny, f, Z);
+}
+
+// The same with kInvLds vectors per workgroup whose factors and vector stay in
+// LDS (a dependent chain of LDS accesses instead of L2 round trips), copied
+// to the global layout at the end (the cluster re-iteration reads them there).
+constexpr int kInvLds = 8;
+__global__ __launch_bounds__(kInvLds) void k_invit_lds(int m, int want, const double* __restrict__ d,
+                                                       const double* __restrict__ e,
+                                                       const double* __restrict__ values,
+                                                       const double* __restrict__ tnorm_in, LU f,
+                                                       double* __restrict__ Z) {
+  extern __shared__ double sh[];
+  double* ds = sh;
+  double* es = sh + m;
+  const int64_t mv = int64_t(m) * kInvLds;
+  LU g;
+  g.U0i = sh + 2 * m;
+  g.U1 = g.U0i + mv;
+  g.U2 = g.U1 + mv;
+  g.Lm = g.U2 + mv;
+  double* z = g.Lm + mv;
+  g.piv = reinterpret_cast<unsigned char*>(z + mv);
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    ds[i] = d[i];
+    es[i] = i + 1 < m ? e[i] : 0.0;
   }
-  if (fabs(a) < tiny) a = a >= 0.0 ? tiny : -tiny;
-  f.U0i[at(m - 1)] = 1.0 / a;
-  for (int i = 0; i < m; ++i) Z[at(i)] = hash_unit(i, j, 0);
-  // Two solves (the second from the unscaled first: growth per solve is at
-  // most ~1/(eps ||T||), far from overflow), then unit norm.
-  lu_solve_prefetch(m, want, j, f, Z);
-  const double nn = lu_solve_prefetch(m, want, j, f, Z);
-  const double sc = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
-  for (int i = 0; i < m; ++i) Z[at(i)] *= sc;
+  __syncthreads();
+  const int lane = threadIdx.x;
+  const int j = blockIdx.x * kInvLds + lane;
+  if (j >= want) return;
+  const double tiny = DBL_EPSILON * fmax(tnorm_in[0], DBL_MIN);
+  invit_vector(m, kInvLds, lane, j, ds, es, invit_shift(values, j, tnorm_in[0]), tiny, g, z);
+  for (int i = 0; i < m; ++i) {
+    const int64_t src = int64_t(i) * kInvLds + lane, dst = int64_t(i) * want + j;
+    Z[dst] = z[src];
+    f.U0i[dst] = g.U0i[src];
+    f.U1[dst] = g.U1[src];
+    f.U2[dst] = g.U2[src];
+    f.Lm[dst] = g.Lm[src];
+    f.piv[dst] = g.piv[src];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -787,7 +841,18 @@ void sym_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, do
     hipLaunchKernelGGL(k_tridiag<false>, dim3(1), dim3(kTT), tri_lds, s, m, A, lda, L.d, L.e, L.tau);
   hipLaunchKernelGGL(k_bisect, dim3((want + 3) / 4), dim3(256), size_t(3 * m) * 8, s, m, want, L.d, L.e, values,
                      L.tnorm);
-  hipLaunchKernelGGL(k_invit, dim3((want + 63) / 64), dim3(64), size_t(2 * m) * 8, s, m, want, L.d, L.e, values, L.tnorm, L.f, L.Z);
+  const size_t invit_lds = size_t(2 * m + 5 * int64_t(m) * kInvLds) * 8 + size_t(m) * kInvLds;
+  if (invit_lds <= 160 * 1024) {
+    if (invit_lds > 64 * 1024 &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_invit_lds), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            int(invit_lds)) != hipSuccess)
+      fail(BD_ERR_DEVICE, "k_invit_lds LDS attribute");
+    hipLaunchKernelGGL(k_invit_lds, dim3((want + kInvLds - 1) / kInvLds), dim3(kInvLds), invit_lds, s, m, want, L.d,
+                       L.e, values, L.tnorm, L.f, L.Z);
+  } else {
+    hipLaunchKernelGGL(k_invit, dim3((want + 63) / 64), dim3(64), size_t(2 * m) * 8, s, m, want, L.d, L.e, values,
+                       L.tnorm, L.f, L.Z);
+  }
   hipLaunchKernelGGL(k_cluster_gs, dim3((want + 3) / 4), dim3(256), 0, s, m, want, values, L.tnorm, L.f, L.Z, Y, ldy,
                      info);
   // Loewdin step: Y <- Y (3 I - Y^T Y) / 2.
