@@ -352,6 +352,238 @@ __global__ __launch_bounds__(kTT) void k_tridiag(int m, double* __restrict__ A, 
   }
 }
 
+// Blocked form (the panel scheme of LAPACK's dsytrd/dlatrd): reflectors are
+// generated kPNB at a time against the matrix as it stood at the panel's
+// start, A_cur = A - sum_{i<s} (v_i w_i^T + w_i v_i^T), with the panel's
+// v_i, w_i held in LDS row by row (VW[row][i] = v_i, VW[row][kPNB + i] = w_i).
+// Both triangles of the trailing block are kept, so the per-step product
+// q = A_cur x is a plain column sweep with lanes along rows (no cross-lane
+// sums); the global matrix is only read during a panel. The rank-2 kPNB
+// update of the trailing block and the reflectors' write-back run once per
+// panel; the block starts symmetric (lower triangle mirrored) and the
+// update keeps it so bit for bit. Same outputs as k_tridiag: reflector k in A[k+1:, k]
+// (x[0] = a - alpha), d, e, tau, with tau = 0 for the last two indices.
+constexpr int kPNB = 8;             // reflectors per panel
+constexpr int kPSW = 2 * kPNB + 2;  // VW row stride (doubles; 16-byte rows)
+constexpr int kPSB = 16;            // columns per task of the product
+constexpr int kPUC = 16;            // columns per tile of the panel update
+
+// Upper triangle <- lower triangle (the panel kernel reads full columns).
+__global__ void k_lower_to_upper(int m, double* __restrict__ A, int64_t lda) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i < j && j < m) A[i + int64_t(j) * lda] = A[j + int64_t(i) * lda];
+}
+
+size_t tridiag_panel_lds(int m) { return size_t((kPSW + 5 + kTW) * m + 2 * kTW + 2 * kPNB) * 8; }
+
+// Sum over the workgroup with one barrier: `slot` (kTW doubles) must not be
+// written again before every thread has passed a later barrier (callers
+// alternate two slots).
+__device__ double block_sum1(double v, double* slot) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) slot[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int i = 0; i < kTW; ++i) t += slot[i];
+  return t;
+}
+
+__global__ __launch_bounds__(kTT) void k_tridiag_panel(int m, double* __restrict__ A, int64_t lda,
+                                                       double* __restrict__ d, double* __restrict__ e,
+                                                       double* __restrict__ tau) {
+  extern __shared__ __align__(16) double sh[];
+  double* VW = sh;               // m rows of kPSW
+  double* q = VW + kPSW * m;     // A_cur x of the current step
+  double* dl = q + m;            // d, e, tau, written out at the end
+  double* el = dl + m;
+  double* tl = el + m;
+  double* cb = tl + m;           // the current column below the diagonal
+  double* pw = cb + m;           // kTW x m: per-wave row sums of the product
+  double* red = pw + kTW * m;    // two reduction slots of kTW
+  double* coef = red + 2 * kTW;  // 2 kPNB: w_i.x at i, v_i.x at kPNB + i
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wv = __builtin_amdgcn_readfirstlane(wave);
+  auto a = [&](int i, int j) -> double& { return A[i + int64_t(j) * lda]; };
+  if (m < 3) {
+    if (tid == 0) {
+      d[0] = a(0, 0);
+      tau[0] = 0.0;
+      if (m == 2) {
+        e[0] = a(1, 0);
+        d[1] = a(1, 1);
+        tau[1] = 0.0;
+      }
+    }
+    return;
+  }
+  for (int i = tid; i < kTW * m; i += kTT) pw[i] = 0.0;
+  __syncthreads();
+  const int last = m - 2;  // reflector steps k = 0 .. m-3
+  double* pwv = pw + int64_t(wv) * m;
+  int slot = 0;
+  for (int p0 = 0;;) {
+    const int se = min(kPNB, last - p0);
+    for (int s = 0; s < se; ++s) {
+      const int k = p0 + s, c0 = k + 1, L = m - k - 1;
+      // Column k of A_cur, rows k..m-1: d[k] and the raw reflector in cb.
+      double ss = 0.0;
+      const double* vk = VW + int64_t(k) * kPSW;
+      for (int r = tid; r <= L; r += kTT) {
+        double v = a(k + r, k);
+        const double* vr = VW + int64_t(k + r) * kPSW;
+        // Unrolled and branch-free (columns i >= s selected out), so the LDS
+        // reads are issued together.
+#pragma unroll
+        for (int i = 0; i < kPNB; ++i) {
+          const double t = vr[i] * vk[kPNB + i] + vr[kPNB + i] * vk[i];
+          v -= i < s ? t : 0.0;
+        }
+        if (r == 0) {
+          dl[k] = v;
+        } else {
+          cb[r - 1] = v;
+          ss += v * v;
+        }
+      }
+      ss = block_sum1(ss, red + slot * kTW);
+      slot ^= 1;
+      const double x0 = cb[0];
+      const double sigma = sqrt(ss);
+      double alpha = 0.0, tk = 0.0;
+      if (sigma > 0.0) {
+        alpha = x0 >= 0.0 ? -sigma : sigma;
+        tk = 1.0 / (sigma * (sigma + fabs(x0)));  // 2 / |x - alpha e1|^2
+      }
+      const double x0m = sigma > 0.0 ? x0 - alpha : x0;  // x[0]; x[r] = cb[r] for r > 0
+      if (tid == 0) {
+        el[k] = alpha;
+        tl[k] = tk;
+      }
+      // Row sums of A[c0:, c0:] x: tasks of 64 rows x kPSB columns dealt to
+      // the waves in turn; a task's loads are issued together.
+      const int nrc = (L + 63) / 64, ntask = nrc * ((L + kPSB - 1) / kPSB);
+      for (int t = wv; t < ntask; t += kTW) {
+        const int rc = t % nrc, j0 = (t / nrc) * kPSB;
+        const int r = 64 * rc + lane, rr = min(r, L - 1);
+        const double* base = A + c0 + rr + int64_t(c0) * lda;
+        double bl[kPSB], xs[kPSB];
+#pragma unroll
+        for (int jj = 0; jj < kPSB; ++jj) bl[jj] = base[int64_t(min(j0 + jj, L - 1)) * lda];
+#pragma unroll
+        for (int jj = 0; jj < kPSB; ++jj) xs[jj] = cb[min(j0 + jj, L - 1)];
+        double acc0 = 0.0, acc1 = 0.0;
+#pragma unroll
+        for (int jj = 0; jj < kPSB; jj += 2) {
+          const int j = j0 + jj;
+          acc0 += bl[jj] * (j == 0 ? x0m : (j < L ? xs[jj] : 0.0));
+          acc1 += bl[jj + 1] * (j + 1 < L ? xs[jj + 1] : 0.0);
+        }
+        if (r < L) pwv[r] += acc0 + acc1;
+      }
+      // w_i.x (t < s) and v_i.x (t >= s) of the panel's earlier steps.
+      for (int t = wv; t < 2 * s; t += kTW) {
+        const int cidx = t < s ? kPNB + t : t - s;  // w_t, else v_{t-s}
+        double acc = 0.0;
+        for (int r = lane; r < L; r += 64) acc += VW[int64_t(c0 + r) * kPSW + cidx] * (r == 0 ? x0m : cb[r]);
+        acc = wave_sum(acc);
+        if (lane == 0) coef[t < s ? t : kPNB + t - s] = acc;
+      }
+      __syncthreads();
+      // q = y - sum_i (v_i (w_i.x) + w_i (v_i.x)); q.x.
+      double qxs = 0.0;
+      for (int i = tid; i < L; i += kTT) {
+        double v = 0.0;
+        for (int ww = 0; ww < kTW; ++ww) {
+          v += pw[int64_t(ww) * m + i];
+          pw[int64_t(ww) * m + i] = 0.0;
+        }
+        const double* vr = VW + int64_t(c0 + i) * kPSW;
+#pragma unroll
+        for (int i2 = 0; i2 < kPNB; ++i2) {
+          const double t = vr[i2] * coef[i2] + vr[kPNB + i2] * coef[kPNB + i2];
+          v -= i2 < s ? t : 0.0;
+        }
+        q[i] = v;
+        qxs += v * (i == 0 ? x0m : cb[i]);
+      }
+      const double qx = block_sum1(qxs, red + slot * kTW);
+      slot ^= 1;
+      // v_s = x and w_s = tau q - (tau^2/2 q.x) x into the panel rows.
+      const double K = 0.5 * tk * tk * qx;
+      for (int i = tid; i < L; i += kTT) {
+        const double xi = i == 0 ? x0m : cb[i];
+        double* row = VW + int64_t(c0 + i) * kPSW;
+        row[s] = xi;
+        row[kPNB + s] = tk * q[i] - K * xi;
+      }
+      __syncthreads();
+    }
+    const int p1 = p0 + se;
+    // The panel's reflectors to A[k+1:, k].
+    for (int s = 0; s < se; ++s) {
+      const int k = p0 + s;
+      for (int r = k + 1 + tid; r < m; r += kTT) a(r, k) = VW[int64_t(r) * kPSW + s];
+    }
+    if (p1 >= last) {
+      // Trailing 2 x 2 of A_cur.
+      if (tid == 0) {
+        double v00 = a(m - 2, m - 2), v10 = a(m - 1, m - 2), v11 = a(m - 1, m - 1);
+        const double* r2 = VW + int64_t(m - 2) * kPSW;
+        const double* r1 = VW + int64_t(m - 1) * kPSW;
+        for (int i = 0; i < se; ++i) {
+          v00 -= 2.0 * r2[i] * r2[kPNB + i];
+          v10 -= r1[i] * r2[kPNB + i] + r1[kPNB + i] * r2[i];
+          v11 -= 2.0 * r1[i] * r1[kPNB + i];
+        }
+        dl[m - 2] = v00;
+        el[m - 2] = v10;
+        dl[m - 1] = v11;
+        tl[m - 2] = 0.0;
+        tl[m - 1] = 0.0;
+      }
+      break;
+    }
+    // Trailing block A[p1:, p1:] -= sum_i (v_i w_i^T + w_i v_i^T), both
+    // triangles: tiles of 64 rows (a lane per row) x kPUC columns dealt to
+    // the waves in turn; a tile's loads are issued before its stores.
+    {
+      const int Lp = m - p1;
+      const int nrc = (Lp + 63) / 64, ntile = nrc * ((Lp + kPUC - 1) / kPUC);
+      for (int t = wv; t < ntile; t += kTW) {
+        const int rc = t % nrc, J0 = p1 + (t / nrc) * kPUC;
+        const int R = p1 + 64 * rc + lane, Rc = min(R, m - 1);
+        double rv[2 * kPNB], val[kPUC];
+        const double* vr = VW + int64_t(Rc) * kPSW;
+#pragma unroll
+        for (int i = 0; i < 2 * kPNB; ++i) rv[i] = vr[i];
+#pragma unroll
+        for (int jj = 0; jj < kPUC; ++jj) val[jj] = A[Rc + int64_t(min(J0 + jj, m - 1)) * lda];
+#pragma unroll
+        for (int jj = 0; jj < kPUC; ++jj) {
+          const int J = J0 + jj;
+          const double* vc = VW + int64_t(min(J, m - 1)) * kPSW;
+          // Each product rounded on its own, so that (R, J) and (J, R) get
+          // the same value: the trailing block stays exactly symmetric.
+          double dlt = 0.0;
+#pragma unroll
+          for (int i = 0; i < kPNB; ++i)
+            dlt = __dadd_rn(dlt, __dadd_rn(__dmul_rn(rv[i], vc[kPNB + i]), __dmul_rn(rv[kPNB + i], vc[i])));
+          if (R < m && J < m) A[R + int64_t(J) * lda] = val[jj] - dlt;
+        }
+      }
+    }
+    __syncthreads();
+    p0 = p1;
+  }
+  __syncthreads();
+  for (int i = tid; i < m; i += kTT) {
+    d[i] = dl[i];
+    tau[i] = tl[i];
+    if (i < m - 1) e[i] = el[i];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // 2. Bisection: one wave per wanted eigenvalue index j (ascending).
 // ---------------------------------------------------------------------------
@@ -818,15 +1050,8 @@ int64_t sym_eigen_work(int m, int want) {
   return 3 * int64_t(m) + 8 + 5 * mw + (mw + 7) / 8 + 8 + int64_t(want) * want + mw + gemm_tn_work(want, want, m);
 }
 
-void sym_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, double* values, double* Y,
-                      int64_t ldy, double* work, int* info) {
-  if (m <= 0 || want <= 0 || want > m || lda < m || ldy < m || lda * m >= (int64_t(1) << 31))
-    fail(BD_ERR_INVALID_PARAMETER, "sym_eigen_lowest: shape");
-  if (size_t(5 * m + 2 * kTW) * 8 > 160 * 1024 || size_t(3 * m) * 8 > 64 * 1024 || m > 1025 ||
-      size_t((4 + kBtChunk) * m + kBtChunk) * 8 > 64 * 1024)
-    fail(BD_ERR_INVALID_PARAMETER, "sym_eigen_lowest: subspace too large for the LDS kernels");
-  const EigenLayout L = eigen_layout(m, want, work);
-  hipLaunchKernelGGL(k_zero_i, dim3(1), dim3(64), 0, s, info, 1);
+// One step at a time (k_tridiag): for subspaces whose panel LDS does not fit.
+void tridiag_unblocked(hipStream_t s, int m, double* A, int64_t lda, const EigenLayout& L) {
   const size_t tri_lds_lower = size_t(5 * m + 2 * kTW + kTW * m) * 8;
   const bool lower = tri_lds_lower <= 160 * 1024;
   const size_t tri_lds = lower ? tri_lds_lower : size_t(5 * m + 2 * kTW) * 8;
@@ -839,6 +1064,28 @@ void sym_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, do
     hipLaunchKernelGGL(k_tridiag<true>, dim3(1), dim3(kTT), tri_lds, s, m, A, lda, L.d, L.e, L.tau);
   else
     hipLaunchKernelGGL(k_tridiag<false>, dim3(1), dim3(kTT), tri_lds, s, m, A, lda, L.d, L.e, L.tau);
+}
+
+void sym_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, double* values, double* Y,
+                      int64_t ldy, double* work, int* info) {
+  if (m <= 0 || want <= 0 || want > m || lda < m || ldy < m || lda * m >= (int64_t(1) << 31))
+    fail(BD_ERR_INVALID_PARAMETER, "sym_eigen_lowest: shape");
+  if (size_t(5 * m + 2 * kTW) * 8 > 160 * 1024 || size_t(3 * m) * 8 > 64 * 1024 || m > 1025 ||
+      size_t((4 + kBtChunk) * m + kBtChunk) * 8 > 64 * 1024)
+    fail(BD_ERR_INVALID_PARAMETER, "sym_eigen_lowest: subspace too large for the LDS kernels");
+  const EigenLayout L = eigen_layout(m, want, work);
+  hipLaunchKernelGGL(k_zero_i, dim3(1), dim3(64), 0, s, info, 1);
+  const size_t panel_lds = tridiag_panel_lds(m);
+  if (m >= 3 && panel_lds <= 160 * 1024) {
+    if (panel_lds > 64 * 1024 &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tridiag_panel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, int(panel_lds)) != hipSuccess)
+      fail(BD_ERR_DEVICE, "k_tridiag_panel LDS attribute");
+    hipLaunchKernelGGL(k_lower_to_upper, dim3((m + 255) / 256, m), dim3(256), 0, s, m, A, lda);
+    hipLaunchKernelGGL(k_tridiag_panel, dim3(1), dim3(kTT), panel_lds, s, m, A, lda, L.d, L.e, L.tau);
+  } else {
+    tridiag_unblocked(s, m, A, lda, L);
+  }
   hipLaunchKernelGGL(k_bisect, dim3((want + 3) / 4), dim3(256), size_t(3 * m) * 8, s, m, want, L.d, L.e, values,
                      L.tnorm);
   const size_t invit_lds = size_t(2 * m + 5 * int64_t(m) * kInvLds) * 8 + size_t(m) * kInvLds;
