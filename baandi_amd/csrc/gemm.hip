@@ -1,6 +1,7 @@
 // FP64 tiled GEMM kernels on the matrix cores: 64 x BN output tiles (BN 64;
 // 128 is built but slower here), 256 threads (4 waves, a 32 x BN/2 quadrant each), K
-// staged through LDS 16 at a time, products by v_mfma_f64_16x16x4f64 (lane l
+// staged through LDS 16 at a time (the next slice loaded into registers while
+// the current one is multiplied), products by v_mfma_f64_16x16x4f64 (lane l
 // holds A[l%16][l/16], B[l/16][l%16] and D[l/16 + 4v][l%16], v = 0..3;
 // pinned by the probe below).
 #include <hip/hip_runtime.h>
@@ -39,6 +40,45 @@ __device__ __forceinline__ void mfma_tile(const double (*As)[TM + 1], const doub
   }
 }
 
+// Slice loaders: every thread owns four consecutive elements of the A slice
+// and 4 * BN/64 of the B slice (consecutive along the contiguous index), so
+// the addresses are fixed per thread and advance by a constant per slice.
+// Interior slices load unchecked; edge slices check every element (zero
+// padding). The next slice is loaded into registers while the current one is
+// multiplied out of LDS.
+template <int BN>
+struct SliceRegs {
+  double a[4];
+  double b[4 * (BN / 64)];
+};
+
+// B slice (TK k x BN columns, k contiguous): thread t holds k = 4 (t & 3) + u
+// of column g * 64 + (t >> 2).
+template <int BN>
+__device__ __forceinline__ void load_b(SliceRegs<BN>& r, const double* __restrict__ B, int64_t ldb, int64_t k0,
+                                       int64_t kend, int64_t col0, int64_t N, bool interior) {
+  const int tid = threadIdx.x;
+  const int kb = (tid & 3) * 4;
+#pragma unroll
+  for (int g = 0; g < BN / 64; ++g) {
+    const int64_t gj = col0 + g * 64 + (tid >> 2);
+    const double* p = B + (k0 + kb) + gj * ldb;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      r.b[g * 4 + u] = interior ? p[u] : ((k0 + kb + u < kend && gj < N) ? p[u] : 0.0);
+  }
+}
+
+template <int BN>
+__device__ __forceinline__ void stash_b(const SliceRegs<BN>& r, double (*Bs)[BN + 1]) {
+  const int tid = threadIdx.x;
+  const int kb = (tid & 3) * 4;
+#pragma unroll
+  for (int g = 0; g < BN / 64; ++g)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) Bs[kb + u][g * 64 + (tid >> 2)] = r.b[g * 4 + u];
+}
+
 // NN: C = alpha A B + beta C; 64 x BN output tile per workgroup.
 template <int BN>
 __global__ __launch_bounds__(256) void k_gemm_nn(int64_t M, int64_t N, int64_t K, double alpha,
@@ -56,22 +96,38 @@ __global__ __launch_bounds__(256) void k_gemm_nn(int64_t M, int64_t N, int64_t K
   for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
     for (int bj = 0; bj < NB; ++bj) acc[bi][bj] = f64x4{0.0, 0.0, 0.0, 0.0};
-  for (int64_t k0 = 0; k0 < K; k0 += TK) {
-    // A tile: 64 rows x 16 k, coalesced along rows.
-    for (int e = tid; e < TM * TK; e += 256) {
-      int i = e % TM, k = e / TM;
-      int64_t gi = row0 + i, gk = k0 + k;
-      As[k][i] = (gi < M && gk < K) ? A[gi + gk * lda] : 0.0;
-    }
-    // B tile: 16 k x BN cols, coalesced along k.
-    for (int e = tid; e < TK * BN; e += 256) {
-      int k = e % TK, j = e / TK;
-      int64_t gk = k0 + k, gj = col0 + j;
-      Bs[k][j] = (gk < K && gj < N) ? B[gk + gj * ldb] : 0.0;
-    }
+  // A slice (64 rows x TK k, rows contiguous): thread t holds rows
+  // 4 (t & 15) + u of k = t >> 4.
+  const int ka = tid >> 4, ia = (tid & 15) * 4;
+  const bool rows_in = row0 + TM <= M, cols_in = col0 + BN <= N;
+  SliceRegs<BN> r;
+  auto fetch = [&](int64_t k0) {
+    const bool interior = rows_in && cols_in && k0 + TK <= K;
+    const double* p = A + (row0 + ia) + (k0 + ka) * lda;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      r.a[u] = interior ? p[u] : ((row0 + ia + u < M && k0 + ka < K) ? p[u] : 0.0);
+    load_b<BN>(r, B, ldb, k0, K, col0, N, interior);
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) As[ka][ia + u] = r.a[u];
+    stash_b<BN>(r, Bs);
+  };
+  if (K > 0) {
+    fetch(0);
+    stash();
     __syncthreads();
+  }
+  for (int64_t k0 = 0; k0 < K; k0 += TK) {
+    const bool more = k0 + TK < K;
+    if (more) fetch(k0 + TK);
     mfma_tile<BN>(As, Bs, lane, wr, wc, acc);
     __syncthreads();
+    if (more) {
+      stash();
+      __syncthreads();
+    }
   }
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi)
@@ -107,20 +163,38 @@ __global__ __launch_bounds__(256) void k_gemm_tn_part(int64_t M, int64_t N, int6
   for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
     for (int bj = 0; bj < NB; ++bj) acc[bi][bj] = f64x4{0.0, 0.0, 0.0, 0.0};
-  for (int64_t k0 = kbeg; k0 < kend; k0 += TK) {
-    for (int e = tid; e < TM * TK; e += 256) {
-      int k = e % TK, i = e / TK;
-      int64_t gi = row0 + i, gk = k0 + k;
-      As[k][i] = (gi < M && gk < kend) ? A[gk + gi * lda] : 0.0;
-    }
-    for (int e = tid; e < TK * BN; e += 256) {
-      int k = e % TK, j = e / TK;
-      int64_t gk = k0 + k, gj = col0 + j;
-      Bs[k][j] = (gk < kend && gj < N) ? B[gk + gj * ldb] : 0.0;
-    }
+  // A slice (TK k x 64 columns i of A, k contiguous): thread t holds
+  // k = 4 (t & 3) + u of column i = t >> 2.
+  const int ia = tid >> 2, ka = (tid & 3) * 4;
+  const bool rows_in = row0 + TM <= M, cols_in = col0 + BN <= N;
+  SliceRegs<BN> r;
+  auto fetch = [&](int64_t k0) {
+    const bool interior = rows_in && cols_in && k0 + TK <= kend;
+    const double* p = A + (k0 + ka) + (row0 + ia) * lda;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      r.a[u] = interior ? p[u] : ((k0 + ka + u < kend && row0 + ia < M) ? p[u] : 0.0);
+    load_b<BN>(r, B, ldb, k0, kend, col0, N, interior);
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) As[ka + u][ia] = r.a[u];
+    stash_b<BN>(r, Bs);
+  };
+  if (kbeg < kend) {
+    fetch(kbeg);
+    stash();
     __syncthreads();
+  }
+  for (int64_t k0 = kbeg; k0 < kend; k0 += TK) {
+    const bool more = k0 + TK < kend;
+    if (more) fetch(k0 + TK);
     mfma_tile<BN>(As, Bs, lane, wr, wc, acc);
     __syncthreads();
+    if (more) {
+      stash();
+      __syncthreads();
+    }
   }
   double* out = part + int64_t(blockIdx.z) * M * N;
 #pragma unroll
