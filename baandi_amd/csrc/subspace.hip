@@ -22,7 +22,15 @@ namespace bd {
 
 namespace {
 constexpr int kCT = 1024;
+constexpr int kInvRows = 3;  // k_chol_inv_lds: rows per lane of the substitution (k <= 192)
 constexpr double kPivotRatioMin = DBL_EPSILON / 1e-12;  // lean_davidson.rs:62-64
+
+// Lane l's value of v (l uniform across the wave).
+__device__ __forceinline__ double read_lane(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -86,7 +94,8 @@ __global__ __launch_bounds__(kCT) void k_chol_inv(int k, double* __restrict__ G,
 // The same with the block in LDS (packed lower triangle, column-major):
 // an LDL^T elimination with one barrier per column (column j is only read
 // while the columns right of it are updated), then L = column j over
-// sqrt(d_j), then L^{-1} by forward substitution as above.
+// sqrt(d_j), then L^{-1} by forward substitution, a wave per column with the
+// right-hand side in registers.
 __global__ __launch_bounds__(kCT) void k_chol_inv_lds(int k, const double* __restrict__ G,
                                                       const double* __restrict__ sdiag, int64_t sstride,
                                                       double* __restrict__ T22, int64_t ldt, int* __restrict__ flag,
@@ -125,21 +134,41 @@ __global__ __launch_bounds__(kCT) void k_chol_inv_lds(int k, const double* __res
     for (int i = l + 1 + lane; i < k; i += 64) P[pk(i, l)] /= sd;
   }
   __syncthreads();
-  for (int l = tid; l < k; l += kCT) P[pk(l, l)] = sqrt(P[pk(l, l)]);
+  for (int l = tid; l < k; l += kCT) {
+    const double sd = sqrt(P[pk(l, l)]);
+    P[pk(l, l)] = sd;
+    rhsall[l] = 1.0 / sd;  // reciprocal diagonal for the substitution
+  }
   __syncthreads();
   if (tid == 0) min_ratio[0] = mr;
-  double* rhs = rhsall + int64_t(wave) * k;
-  for (int c = wave; c < k; c += nw) {
-    for (int i = lane; i < k; i += 64) rhs[i] = i == c ? 1.0 : 0.0;
+  // Column c of L^{-1}: forward substitution L x = e_c, one wave per column,
+  // the right-hand side in registers (lane owns rows lane + 64 t, t <
+  // kInvRows), x_l broadcast from its owner lane: the serial chain per row is
+  // a lane read, a multiply and the update, with no LDS round trip.
+  const double* rd = rhsall;
+  for (int c = __builtin_amdgcn_readfirstlane(wave); c < k; c += nw) {
+    double r[kInvRows];
+#pragma unroll
+    for (int t = 0; t < kInvRows; ++t) r[t] = lane + 64 * t == c ? 1.0 : 0.0;
     for (int l = c; l < k; ++l) {
-      const int base_l = pk(0, l);
-      double xl = 0.0;
-      if ((l & 63) == lane) xl = rhs[l] / P[base_l + l];
-      xl = __shfl(xl, l & 63, 64);
-      if ((l & 63) == lane) rhs[l] = xl;
-      for (int i = l + 1 + ((lane - (l + 1)) & 63); i < k; i += 64) rhs[i] -= P[base_l + i] * xl;
+      const int tl = l >> 6, ll = l & 63;
+      double own = r[0];
+#pragma unroll
+      for (int t = 1; t < kInvRows; ++t) own = tl == t ? r[t] : own;
+      const double x = read_lane(own, ll) * rd[l];
+      const int base_l = pk(0, l);  // P[base_l + i] is in bounds for every i < k
+#pragma unroll
+      for (int t = 0; t < kInvRows; ++t) {
+        const int i = lane + 64 * t;
+        const double upd = r[t] - P[base_l + min(i, k - 1)] * x;
+        r[t] = i == l ? x : ((i > l && i < k) ? upd : r[t]);
+      }
     }
-    for (int i = lane; i < k; i += 64) T22[i + int64_t(c) * ldt] = i >= c ? rhs[i] : 0.0;
+#pragma unroll
+    for (int t = 0; t < kInvRows; ++t) {
+      const int i = lane + 64 * t;
+      if (i < k) T22[i + int64_t(c) * ldt] = i >= c ? r[t] : 0.0;
+    }
   }
 }
 
@@ -217,7 +246,7 @@ void extend_inverse_cholesky_dev(hipStream_t s, const double* S, double* T, int 
   if (from > 0) gemm_tn(s, k, k, from, -1.0, fw.W, from, fw.W, from, 1.0, fw.G, k, fw.gwork);
   double* T22 = T + from + int64_t(from) * cap;
   const size_t lds_packed = (size_t(k) * (k + 1) / 2 + size_t(kCT / 64) * k) * 8;
-  if (lds_packed <= 160 * 1024) {
+  if (lds_packed <= 160 * 1024 && k <= 64 * kInvRows) {
     if (lds_packed > 64 * 1024 &&
         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_chol_inv_lds),
                             hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_packed)) != hipSuccess)
