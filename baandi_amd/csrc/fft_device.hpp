@@ -1,11 +1,11 @@
-// LDS-resident mixed-radix (2,3,4,5) Stockham FFT for gfx950 workgroups.
+// LDS-resident mixed-radix (2,3,4,5,8,9) Stockham FFT for gfx950 workgroups.
 //
 // A workgroup transforms `nlines` lines of length n held in LDS; element
 // (line, i) lives at lds[line * lstride + i * estride]. Each pass reads its
 // radix-R butterflies into registers, synchronises, and writes the
 // self-sorted outputs back in place, so the whole workgroup must hold every
-// butterfly of a pass at once: nthreads * MAXB(R) >= nlines * n / R (checked
-// by the host before launch, see fft_fits()).
+// butterfly of a pass at once: nthreads * pass_mb(R, nthreads) >= nlines * n / R
+// (checked by the host before launch, see fft_fits()).
 //
 // Convention (crates/dft-numerics/src/lib.rs:46-56): forward uses e^{-i},
 // inverse e^{+i}; both unscaled.
@@ -20,18 +20,18 @@ struct c2 {
 };
 
 template <typename T>
-__device__ __forceinline__ c2<T> cadd(c2<T> a, c2<T> b) { return {a.x + b.x, a.y + b.y}; }
+__host__ __device__ __forceinline__ c2<T> cadd(c2<T> a, c2<T> b) { return {a.x + b.x, a.y + b.y}; }
 template <typename T>
-__device__ __forceinline__ c2<T> csub(c2<T> a, c2<T> b) { return {a.x - b.x, a.y - b.y}; }
+__host__ __device__ __forceinline__ c2<T> csub(c2<T> a, c2<T> b) { return {a.x - b.x, a.y - b.y}; }
 template <typename T>
-__device__ __forceinline__ c2<T> cmul(c2<T> a, c2<T> b) {
+__host__ __device__ __forceinline__ c2<T> cmul(c2<T> a, c2<T> b) {
   return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
 template <typename T>
-__device__ __forceinline__ c2<T> cscale(c2<T> a, T s) { return {a.x * s, a.y * s}; }
+__host__ __device__ __forceinline__ c2<T> cscale(c2<T> a, T s) { return {a.x * s, a.y * s}; }
 // Multiply by -i (forward) or +i (inverse).
 template <bool INV, typename T>
-__device__ __forceinline__ c2<T> rot_i(c2<T> a) {
+__host__ __device__ __forceinline__ c2<T> rot_i(c2<T> a) {
   return INV ? c2<T>{-a.y, a.x} : c2<T>{a.y, -a.x};
 }
 
@@ -42,17 +42,22 @@ struct RadixPlan {
   int radix[12];
 };
 
-// Most butterflies one thread holds per pass, by radix (<= 24 complex values).
-template <int R>
-struct MaxB;
-template <> struct MaxB<2> { static constexpr int v = 12; };
-template <> struct MaxB<3> { static constexpr int v = 8; };
-template <> struct MaxB<4> { static constexpr int v = 6; };
-template <> struct MaxB<5> { static constexpr int v = 4; };
+// Most butterflies one thread holds per pass, by radix and workgroup size
+// (at most 24 complex values per thread at 256 threads). Radices 2-5 hold
+// the same lines at 256 and 512 threads (half the butterflies per thread at
+// 512); radices 8 and 9 keep 16-18 values per thread at 512 threads.
+__host__ __device__ constexpr int pass_mb(int R, int NT) {
+  return R == 2 ? 12 * 256 / NT
+         : R == 3 ? 8 * 256 / NT
+         : R == 4 ? 6 * 256 / NT
+         : R == 5 ? 4 * 256 / NT
+         : R == 8 ? (NT == 256 ? 3 : 2)
+                  : 2;  // R == 9
+}
 
 // In-register DFT of length R with root e^{-2 pi i/R} (e^{+} for INV).
 template <int R, bool INV, typename T>
-__device__ __forceinline__ void dft(c2<T>* v) {
+__host__ __device__ __forceinline__ void dft(c2<T>* v) {
   if constexpr (R == 2) {
     c2<T> a = v[0], b = v[1];
     v[0] = cadd(a, b);
@@ -88,19 +93,68 @@ __device__ __forceinline__ void dft(c2<T>* v) {
     v[4] = csub(a1, b1);
     v[2] = cadd(a2, b2);
     v[3] = csub(a2, b2);
+  } else if constexpr (R == 8) {
+    // X[2m] = DFT4(v_j + v_{j+4})[m], X[2m+1] = DFT4(w^j (v_j - v_{j+4}))[m],
+    // w = e^{-2 pi i/8} (e^{+} for INV).
+    const T h = T(0.70710678118654752440084436210484904);  // 1/sqrt(2)
+    c2<T> a[4], b[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      a[j] = cadd(v[j], v[j + 4]);
+      b[j] = csub(v[j], v[j + 4]);
+    }
+    // w^1 = h(1 -+ i), w^2 = -+i, w^3 = h(-1 -+ i).
+    b[1] = INV ? c2<T>{h * (b[1].x - b[1].y), h * (b[1].x + b[1].y)}
+               : c2<T>{h * (b[1].x + b[1].y), h * (b[1].y - b[1].x)};
+    b[2] = rot_i<INV>(b[2]);
+    b[3] = INV ? c2<T>{-h * (b[3].x + b[3].y), h * (b[3].x - b[3].y)}
+               : c2<T>{h * (b[3].y - b[3].x), -h * (b[3].x + b[3].y)};
+    dft<4, INV>(a);
+    dft<4, INV>(b);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      v[2 * m] = a[m];
+      v[2 * m + 1] = b[m];
+    }
+  } else if constexpr (R == 9) {
+    // 3 x 3: column DFTs over v[n1 + 3 n2], twiddles w^(n1 k1), row DFTs;
+    // X[k1 + 3 k2]. w = e^{-2 pi i/9} (e^{+} for INV).
+    const T c1 = T(0.76604444311897803520239265055541667);   // cos(2 pi/9)
+    const T s1 = T(0.64278760968653932632264340990726344);   // sin(2 pi/9)
+    const T c2_ = T(0.17364817766693034885171662676931480);  // cos(4 pi/9)
+    const T s2 = T(0.98480775301220805936674302458952301);   // sin(4 pi/9)
+    const T c4 = T(-0.93969262078590838405410927732473147);  // cos(8 pi/9)
+    const T s4 = T(0.34202014332566873304409961468225958);   // sin(8 pi/9)
+    c2<T> col[3][3];
+#pragma unroll
+    for (int n1 = 0; n1 < 3; ++n1) {
+#pragma unroll
+      for (int n2 = 0; n2 < 3; ++n2) col[n1][n2] = v[n1 + 3 * n2];
+      dft<3, INV>(col[n1]);  // col[n1][k1]
+    }
+    const T sg = INV ? T(1) : T(-1);  // w^p = cos + sg i sin
+    col[1][1] = cmul(col[1][1], c2<T>{c1, sg * s1});
+    col[1][2] = cmul(col[1][2], c2<T>{c2_, sg * s2});
+    col[2][1] = cmul(col[2][1], c2<T>{c2_, sg * s2});
+    col[2][2] = cmul(col[2][2], c2<T>{c4, sg * s4});
+#pragma unroll
+    for (int k1 = 0; k1 < 3; ++k1) {
+      c2<T> row[3] = {col[0][k1], col[1][k1], col[2][k1]};
+      dft<3, INV>(row);
+#pragma unroll
+      for (int k2 = 0; k2 < 3; ++k2) v[k1 + 3 * k2] = row[k2];
+    }
   }
 }
 
 // One in-place Stockham pass (radix R, Ns = product of earlier radices).
 // `roots[k]` = e^{-2 pi i k/n} (forward); the inverse conjugates. A workgroup
-// of NT threads holds MaxB<R>::v * 256 / NT butterflies per thread, so the
-// lines a pass can hold (and fft_fits) are the same for 256 and 512 threads
-// while the wider workgroup needs half the registers.
+// of NT threads holds pass_mb(R, NT) butterflies per thread.
 template <int R, bool INV, typename T, int NT>
 __device__ __forceinline__ void stockham_pass(c2<T>* lds, int nlines, int lstride, int estride,
                                               int n, int Ns, const c2<T>* roots) {
-  static_assert(MaxB<R>::v * 256 % NT == 0, "butterflies per thread must be whole");
-  constexpr int MB = MaxB<R>::v * 256 / NT;
+  static_assert(NT == 256 || NT == 512, "workgroups of 256 or 512 threads");
+  constexpr int MB = pass_mb(R, NT);
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int m = n / R;
   const int total = nlines * m;
@@ -150,7 +204,9 @@ __device__ void lds_fft(c2<T>* lds, int nlines, int lstride, int estride, const 
   int Ns = 1;
   for (int p = 0; p < rp.count; ++p) {
     const int R = rp.radix[p];
-    if (R == 4) stockham_pass<4, INV, T, NT>(lds, nlines, lstride, estride, rp.n, Ns, roots);
+    if (R == 8) stockham_pass<8, INV, T, NT>(lds, nlines, lstride, estride, rp.n, Ns, roots);
+    else if (R == 9) stockham_pass<9, INV, T, NT>(lds, nlines, lstride, estride, rp.n, Ns, roots);
+    else if (R == 4) stockham_pass<4, INV, T, NT>(lds, nlines, lstride, estride, rp.n, Ns, roots);
     else if (R == 2) stockham_pass<2, INV, T, NT>(lds, nlines, lstride, estride, rp.n, Ns, roots);
     else if (R == 3) stockham_pass<3, INV, T, NT>(lds, nlines, lstride, estride, rp.n, Ns, roots);
     else stockham_pass<5, INV, T, NT>(lds, nlines, lstride, estride, rp.n, Ns, roots);

@@ -34,44 +34,21 @@ constexpr int kDensityGroups = 8;  // band-density pair groups per x plane
 constexpr double kSqrt2 = 1.41421356237309504880168872420969808;
 constexpr double kInvSqrt2 = 0.70710678118654752440084436210484904;
 
-template <int R>
-bool pass_fits(int nlines, int n) {
-  return int64_t(nlines) * (n / R) <= int64_t(MaxB<R>::v) * kThreads;
-}
-// A workgroup holds MaxB<R>::v * 256 butterflies of a pass whatever its
-// size (stockham_pass scales the butterflies per thread with the size), so
-// `threads` only has to be one of the sizes the kernels are built for.
-template <int R>
-bool pass_fits_t(int nlines, int n, int threads) {
-  if (threads != 256 && threads != 512) return false;
-  return int64_t(nlines) * (n / R) <= int64_t(MaxB<R>::v) * 256;
+bool pass_fits(int R, int nlines, int n, int threads) {
+  return int64_t(nlines) * (n / R) <= int64_t(pass_mb(R, threads)) * threads;
 }
 }  // namespace
 
+// The kernels are built for workgroups of 256 and 512 threads.
 bool fft_fits_threads(int nlines, const RadixPlan& rp, int threads) {
-  for (int p = 0; p < rp.count; ++p) {
-    int R = rp.radix[p];
-    bool ok = R == 2 ? pass_fits_t<2>(nlines, rp.n, threads)
-              : R == 3 ? pass_fits_t<3>(nlines, rp.n, threads)
-              : R == 4 ? pass_fits_t<4>(nlines, rp.n, threads)
-                       : pass_fits_t<5>(nlines, rp.n, threads);
-    if (!ok) return false;
-  }
+  if (threads != 256 && threads != 512) return false;
+  for (int p = 0; p < rp.count; ++p)
+    if (!pass_fits(rp.radix[p], nlines, rp.n, threads)) return false;
   return true;
 }
 
 namespace {
-bool fft_fits(int nlines, const RadixPlan& rp) {
-  for (int p = 0; p < rp.count; ++p) {
-    int R = rp.radix[p];
-    bool ok = R == 2 ? pass_fits<2>(nlines, rp.n)
-              : R == 3 ? pass_fits<3>(nlines, rp.n)
-              : R == 4 ? pass_fits<4>(nlines, rp.n)
-                       : pass_fits<5>(nlines, rp.n);
-    if (!ok) return false;
-  }
-  return true;
-}
+bool fft_fits(int nlines, const RadixPlan& rp) { return fft_fits_threads(nlines, rp, kThreads); }
 
 // ---------------------------------------------------------------------------
 // Kernels
@@ -347,7 +324,9 @@ __global__ void k_coupling(const double* __restrict__ q, double* __restrict__ w,
 // ---------------------------------------------------------------------------
 // Plans
 // ---------------------------------------------------------------------------
-RadixPlan radix_plan(int n) {
+// Passes of radix 9 and 8 first when `big` (fewer LDS round trips), then 4,
+// 2, 3, 5.
+RadixPlan radix_plan(int n, bool big) {
   RadixPlan rp{};
   rp.n = n;
   int r = n;
@@ -358,6 +337,10 @@ RadixPlan radix_plan(int n) {
       r /= f;
     }
   };
+  if (big) {
+    take(9);
+    take(8);
+  }
   take(4);
   take(2);
   take(3);
@@ -371,9 +354,9 @@ StickPlan build_stick_plan(const System& s) {
   p.n0 = s.dims[0];
   p.n1 = s.dims[1];
   p.n2 = s.dims[2];
-  p.rx = radix_plan(p.n0);
-  p.ry = radix_plan(p.n1);
-  p.rz = radix_plan(p.n2);
+  p.rx = radix_plan(p.n0, true);
+  p.ry = radix_plan(p.n1, true);
+  p.rz = radix_plan(p.n2, true);
   auto split = [&](int64_t f, int& i0, int& i1, int& i2) {
     i2 = int(f % p.n2);
     i1 = int((f / p.n2) % p.n1);
@@ -398,6 +381,10 @@ StickPlan build_stick_plan(const System& s) {
   std::map<int, int> zc_of;
   for (int i = 0; i < p.Z; ++i) zc_of[p.z_of_zc[i]] = i;
   // Tile size: the largest power of two <= 32 whose x transform fits a workgroup.
+  // Radix-9/8 passes only where the batches fit as with radices 2-5.
+  if (!fft_fits(32, p.rx)) p.rx = radix_plan(p.n0, false);
+  if (!fft_fits(p.Z, p.ry)) p.ry = radix_plan(p.n1, false);
+  if (!fft_fits(std::min(16, p.n1), p.rz)) p.rz = radix_plan(p.n2, false);
   p.T = 32;
   while (p.T > 1 && !fft_fits(p.T, p.rx)) p.T /= 2;
   if (!fft_fits(p.T, p.rx)) fail(BD_ERR_INVALID_PARAMETER, "x FFT does not fit a workgroup");
@@ -460,9 +447,9 @@ StickPlan build_k_stick_plan(const System& s, const KBasis& kb) {
   p.n0 = s.dims[0];
   p.n1 = s.dims[1];
   p.n2 = s.dims[2];
-  p.rx = radix_plan(p.n0);
-  p.ry = radix_plan(p.n1);
-  p.rz = radix_plan(p.n2);
+  p.rx = radix_plan(p.n0, true);
+  p.ry = radix_plan(p.n1, true);
+  p.rz = radix_plan(p.n2, true);
   auto split = [&](int64_t f, int& i0, int& i1, int& i2) {
     i2 = int(f % p.n2);
     i1 = int((f / p.n2) % p.n1);
@@ -484,6 +471,10 @@ StickPlan build_k_stick_plan(const System& s, const KBasis& kb) {
   p.Z = int(p.z_of_zc.size());
   std::map<int, int> zc_of;
   for (int i = 0; i < p.Z; ++i) zc_of[p.z_of_zc[i]] = i;
+  // Radix-9/8 passes only where the batches fit as with radices 2-5.
+  if (!fft_fits(32, p.rx)) p.rx = radix_plan(p.n0, false);
+  if (!fft_fits(p.Z, p.ry)) p.ry = radix_plan(p.n1, false);
+  if (!fft_fits(std::min(16, p.n1), p.rz)) p.rz = radix_plan(p.n2, false);
   p.T = 32;
   while (p.T > 1 && !fft_fits(p.T, p.rx)) p.T /= 2;
   if (!fft_fits(p.T, p.rx)) fail(BD_ERR_INVALID_PARAMETER, "x FFT does not fit a workgroup");

@@ -155,7 +155,7 @@ bool fft_fits_threads(int nlines, const RadixPlan& rp, int threads);
 
 StickPlan build_stick_plan(const System& s);
 StickPlan build_k_stick_plan(const System& s, const KBasis& kb);
-RadixPlan radix_plan(int n);
+RadixPlan radix_plan(int n, bool big);
 
 }  // namespace bd
 
