@@ -31,8 +31,8 @@ os.environ["GPU_MAX_HW_QUEUES"] = "16"
 HBM_PEAK_GBS = 8000.0
 # Measured HBM bytes per pair of k_plane at Si64 FP64 (autotuned 512-thread shape):
 # rocprofv3 FETCH_SIZE (x2.000, the 256 MiB copy calibration) + WRITE_SIZE (x1.000)
-# of an 80-pair launch, 203917.2 KiB / 80 pairs (profiles/k_plane_si64_fp64.txt).
-PLANE_TRAFFIC_BYTES_PER_PAIR = 203917.2 * 1024 / 80
+# of an 80-pair launch, 204066.0 KiB / 80 pairs (profiles/k_plane_si64_fp64.txt).
+PLANE_TRAFFIC_BYTES_PER_PAIR = 204066.0 * 1024 / 80
 METRIC = "Si64 SCF wall-time to 1e-8 Ha/atom: mixed+learned vs plain FP64, 1 MI355X"
 
 
