@@ -241,7 +241,9 @@ int col_tile(int64_t) { return 64; }
 int tn_splits(int64_t M, int64_t N, int64_t K) {
   const int bn = col_tile(N);
   const int64_t tiles = ((M + TM - 1) / TM) * ((N + bn - 1) / bn);
-  int64_t want = std::max<int64_t>(1, 1024 / std::max<int64_t>(tiles, 1));
+  // ~512 workgroups (two per CU): more splits cost more in the partial sums
+  // than they gain in occupancy on the Davidson shapes.
+  int64_t want = std::max<int64_t>(1, 512 / std::max<int64_t>(tiles, 1));
   int64_t maxs = std::max<int64_t>(1, K / 256);
   return int(std::min<int64_t>({want, maxs, 256}));
 }
