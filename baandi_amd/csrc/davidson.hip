@@ -587,15 +587,6 @@ __global__ __launch_bounds__(kT) void k_fix_weak(double* __restrict__ E, int64_t
   }
 }
 
-// dst[:, j] = src[:, j] / |src[:, j]| with |.|^2 = norm2[j] (0 for a zero column).
-__global__ void k_unit_cols(const double* __restrict__ src, int64_t n, const double* __restrict__ norm2,
-                            double* __restrict__ dst) {
-  const int j = blockIdx.y;
-  const double v = norm2[j];
-  const double sc = v > 0.0 ? 1.0 / sqrt(v) : 0.0;
-  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
-    dst[int64_t(j) * n + i] = src[int64_t(j) * n + i] * sc;
-}
 }  // namespace
 
 // Canonical orthonormalisation of a small Gram matrix: given G = U diag(l) U^T
