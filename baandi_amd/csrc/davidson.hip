@@ -293,18 +293,25 @@ struct Solver {
   }
 
   // A = T H_r T^T, lowest nb eigenpairs, C = T^T Y (workspace.rs:435-499).
+  // If the inverse iteration cannot separate the vectors of a tight cluster
+  // (the exact pairs of a realified k-point matrix can land so), A is formed
+  // again and reduced by the one-step tridiagonalisation, whose rounding
+  // differs; only a second failure is an error.
   void eigen(int m) {
     auto t0 = std::chrono::steady_clock::now();
-    transpose(s, m, m, w.T.p, cap, w.Tt.p, m);
-    gemm_nn(s, m, m, m, 1.0, w.H.p, cap, w.Tt.p, m, 0.0, w.small3.p, m);  // H_r T^T
-    gemm_nn(s, m, m, m, 1.0, w.T.p, cap, w.small3.p, m, 0.0, w.A.p, m);   // A
-    symmetrize(s, m, w.A.p, m);
-    sym_eigen_lowest(s, m, nb, w.A.p, m, w.vals.p, w.Y.p, m, w.eigw.p, w.info);
-    gemm_nn(s, m, nb, m, 1.0, w.Tt.p, m, w.Y.p, m, 0.0, w.C.p, m);  // C
     int bad = 0;
-    BD_HIP(hipMemcpyAsync(values.data(), w.vals.p, size_t(nb) * 8, hipMemcpyDeviceToHost, s));
-    BD_HIP(hipMemcpyAsync(&bad, w.info, sizeof(int), hipMemcpyDeviceToHost, s));
-    sync();
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      transpose(s, m, m, w.T.p, cap, w.Tt.p, m);
+      gemm_nn(s, m, m, m, 1.0, w.H.p, cap, w.Tt.p, m, 0.0, w.small3.p, m);  // H_r T^T
+      gemm_nn(s, m, m, m, 1.0, w.T.p, cap, w.small3.p, m, 0.0, w.A.p, m);   // A
+      symmetrize(s, m, w.A.p, m);
+      sym_eigen_lowest(s, m, nb, w.A.p, m, w.vals.p, w.Y.p, m, w.eigw.p, w.info, attempt == 0);
+      gemm_nn(s, m, nb, m, 1.0, w.Tt.p, m, w.Y.p, m, 0.0, w.C.p, m);  // C
+      BD_HIP(hipMemcpyAsync(values.data(), w.vals.p, size_t(nb) * 8, hipMemcpyDeviceToHost, s));
+      BD_HIP(hipMemcpyAsync(&bad, w.info, sizeof(int), hipMemcpyDeviceToHost, s));
+      sync();
+      if (!bad) break;
+    }
     t_dense += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (bad) fail(BD_ERR_APPLICATION, "reference reduced eigensolve failed");
   }

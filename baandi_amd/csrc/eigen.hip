@@ -1067,7 +1067,7 @@ void tridiag_unblocked(hipStream_t s, int m, double* A, int64_t lda, const Eigen
 }
 
 void sym_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, double* values, double* Y,
-                      int64_t ldy, double* work, int* info) {
+                      int64_t ldy, double* work, int* info, bool blocked) {
   if (m <= 0 || want <= 0 || want > m || lda < m || ldy < m || lda * m >= (int64_t(1) << 31))
     fail(BD_ERR_INVALID_PARAMETER, "sym_eigen_lowest: shape");
   if (size_t(5 * m + 2 * kTW) * 8 > 160 * 1024 || size_t(3 * m) * 8 > 64 * 1024 || m > 1025 ||
@@ -1076,7 +1076,7 @@ void sym_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, do
   const EigenLayout L = eigen_layout(m, want, work);
   hipLaunchKernelGGL(k_zero_i, dim3(1), dim3(64), 0, s, info, 1);
   const size_t panel_lds = tridiag_panel_lds(m);
-  if (m >= 3 && panel_lds <= 160 * 1024) {
+  if (blocked && m >= 3 && panel_lds <= 160 * 1024) {
     if (panel_lds > 64 * 1024 &&
         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tridiag_panel),
                             hipFuncAttributeMaxDynamicSharedMemorySize, int(panel_lds)) != hipSuccess)

@@ -15,9 +15,10 @@ int64_t sym_eigen_work(int m, int want);
 // leading dimension lda; overwritten by the Householder vectors). values
 // (device, `want`) ascending; Y (m x want, ldy) orthonormal eigenvectors.
 // *info (device) is set to 0, or to 1 + j when vector j could not be made
-// orthogonal to its cluster.
+// orthogonal to its cluster. `blocked` selects the panel tridiagonalisation
+// where it fits (else the one-step kernel).
 void sym_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, double* values, double* Y,
-                      int64_t ldy, double* work, int* info);
+                      int64_t ldy, double* work, int* info, bool blocked = true);
 
 // Device state of the bordered inverse Cholesky of the subspace overlap.
 struct FactorWork {
