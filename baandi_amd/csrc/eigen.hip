@@ -915,16 +915,33 @@ __global__ __launch_bounds__(256) void k_cluster_gs(int m, int want, const doubl
         break;
       }
       // Most of the vector lay in the span of its cluster: restart from the
-      // remainder plus a fresh direction and apply (T - lambda_j I)^{-1}.
+      // remainder plus a fresh direction and apply (T - lambda_j I)^{-1},
+      // projecting the cluster's accepted vectors out after every solve. A
+      // nearly singular factor sends every right-hand side towards the same
+      // direction; without the projection in between, a second solve would
+      // swamp the wanted direction in that one.
       const double rs = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
       for (int i = lane; i < m; i += 64) y[i] = y[i] * rs + 0.1 * hash_unit(i, j, 1 + attempt);
-      __threadfence_block();
-      if (lane == 0) {
-        for (int it = 0; it < 2; ++it) {
+      for (int it = 0; it < 2; ++it) {
+        __threadfence_block();
+        if (lane == 0) {
           const double big = lu_solve(m, want, j, f, y, 1);
           const double sc = big > 0.0 ? 1.0 / big : 1.0;
           for (int i = 0; i < m; ++i) y[i] *= sc;
         }
+        __threadfence_block();
+        for (int l = h; l < j; ++l) {
+          const double* yl = Y + int64_t(l) * ldy;
+          double acc = 0.0;
+          for (int i = lane; i < m; i += 64) acc += yl[i] * y[i];
+          const double c = wave_sum(acc);
+          for (int i = lane; i < m; i += 64) y[i] -= c * yl[i];
+        }
+        double n2 = 0.0;
+        for (int i = lane; i < m; i += 64) n2 += y[i] * y[i];
+        n2 = wave_sum(n2);
+        const double sc = n2 > 0.0 ? 1.0 / sqrt(n2) : 0.0;
+        for (int i = lane; i < m; i += 64) y[i] *= sc;
       }
       __threadfence_block();
     }
