@@ -1,6 +1,6 @@
 """Train the learned Davidson preconditioner on the GPU and save its weights.
 
-    python -m baandi_amd.train_preconditioner [--cells 8] [--steps 3000] [--out PATH]
+    python -m baandi_amd.train_preconditioner [--workload si8|si64] [--cells 8] [--steps 3000] [--out PATH]
 
 Training pairs come from the FP64 GPU Davidson on displaced diamond-silicon
 cells (Gamma, 30 Ry, synthetic Z=4 pseudopotential): each cell gets atomic
@@ -60,6 +60,8 @@ def cell_pairs(workload, seed, maps=12, log=print):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=("si8", "si64"), default="si8",
+                    help="cell the displaced training cells are drawn from")
     ap.add_argument("--cells", type=int, default=8)
     ap.add_argument("--sigma", type=float, default=0.05, help="displacement s.d., Bohr")
     ap.add_argument("--steps", type=int, default=3000)
@@ -70,7 +72,7 @@ def main(argv=None):
     args = ap.parse_args(argv)
     from . import workloads
     from .learned import Network, train, tpa_loss, loss_and_grad
-    base = workloads.si8()
+    base = workloads.si8() if args.workload == "si8" else workloads.si64()
     pairs = None
     for c in range(args.cells):
         p = cell_pairs(displaced(base, args.sigma, 1000 + c), seed=c)
@@ -84,7 +86,7 @@ def main(argv=None):
           f"{final:.4f} (TPA {tpa_loss(pairs):.4f}) over {pairs.bands} pairs", flush=True)
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     net.save(args.out, loss=final, tpa_loss=tpa_loss(pairs), pairs=pairs.bands, cells=args.cells,
-             sigma=args.sigma, steps=args.steps)
+             sigma=args.sigma, steps=args.steps, workload=args.workload)
     print(f"[train] saved {args.out}")
     return 0
 

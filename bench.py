@@ -219,6 +219,8 @@ def main():
                     help="Al 6x6x6 k-point SCFs timed (0 skips the k-point leg)")
     ap.add_argument("--kpoint-only", action="store_true",
                     help="profile mode: only the Al k-point leg, its JSON object printed")
+    ap.add_argument("--mlp", default=None,
+                    help="learned-preconditioner weights (.npz) for the learned legs (default: the packaged file)")
     ap.add_argument("--hx-only", type=int, default=0,
                     help="profile mode: N applies of H to 160 Si64 columns, then a 256 MiB copy")
     args = ap.parse_args()
@@ -269,7 +271,7 @@ def main():
         from baandi_amd import lib as L
         from baandi_amd.learned import Network
         from baandi_amd.train_preconditioner import DEFAULT_OUT
-        net = Network.load(DEFAULT_OUT)
+        net = Network.load(args.mlp or DEFAULT_OUT)
         natoms = len(system.get("atoms_cart"))
         occ = int(round(system.electrons)) // 2
         legs = {"mixed": ScfOptions(bands=wl["bands"], threshold_ry=1e-8, precision=L.BD_MIXED,
