@@ -9,13 +9,17 @@ from conftest import fcc_rows
 pytestmark = pytest.mark.gpu
 
 CASES = {
-    # name: (rows, atoms, z, ecut, columns)
+    # name: (rows, atoms, z, ecut, columns[, grid override])
     "kat1_si2_5ha": (fcc_rows(10.261212856717933), [[0, 0, 0], [0.25] * 3], 4.0, 5.0, 8),
     "si2_20ry": (fcc_rows(10.26), [[0, 0, 0], [0.25] * 3], 4.0, 10.0, 9),
     "kat2_skew": ([[5.0, 0, 0], [0.2, 5.0, 0], [0, 0.1, 5.0]], [[0.13, 0.21, 0.32]], 2.0, 1.0, 4),
     "si8_cubic_15ha": ([[10.26, 0, 0], [0, 10.26, 0], [0, 0, 10.26]],
                        [[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0],
                         [.25, .25, .25], [.25, .75, .75], [.75, .25, .75], [.75, .75, .25]], 4.0, 15.0, 33),
+    # FFT lengths through the radix-8 and radix-9 passes: 16 = 8*2, 27 = 9*3,
+    # 36 = 9*4, and 72 = 9*8 on every axis.
+    "kat1_grid_16_27_36": (fcc_rows(10.261212856717933), [[0, 0, 0], [0.25] * 3], 4.0, 5.0, 5, (16, 27, 36)),
+    "kat1_grid_72": (fcc_rows(10.261212856717933), [[0, 0, 0], [0.25] * 3], 4.0, 5.0, 4, (72, 72, 72)),
 }
 
 
@@ -23,10 +27,13 @@ def build(case):
     import torch
     from baandi_amd.hamiltonian import GammaSystem, DistributedRealHamiltonian
     from oracle import physics as P
-    rows, atoms, z, ecut, m = CASES[case]
+    rows, atoms, z, ecut, m = CASES[case][:5]
+    grid = CASES[case][5] if len(CASES[case]) > 5 else None
     points = 161 if z == 2.0 else 801
-    sysg = GammaSystem(rows, atoms, z, ecut, mesh_points=points)
-    orc = P.build_gamma_system(rows, atoms, P.synthetic_pseudo(z, 1.0, points), ecut)
+    sysg = GammaSystem(rows, atoms, z, ecut, mesh_points=points, grid_override=grid)
+    orc = P.build_gamma_system(rows, atoms, P.synthetic_pseudo(z, 1.0, points), ecut, grid_override=grid)
+    if grid is not None:
+        assert tuple(sysg.dims) == tuple(grid) == tuple(orc.dims)
     assert np.array_equal(sysg.get("miller"), orc.basis.miller)
     op = DistributedRealHamiltonian(sysg, capacity=m)
     rng = np.random.default_rng(7)
