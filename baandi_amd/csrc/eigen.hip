@@ -18,6 +18,10 @@
 //  5. Back-transformation by the stored reflectors, one wave per vector.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <set>
+#include <string>
+
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -1062,6 +1066,20 @@ EigenLayout eigen_layout(int m, int want, double* work) {
 }
 }  // namespace
 
+// The attribute is per kernel and process-wide: set per call to the call's
+// own size, it would race between host threads that launch the same kernel
+// with different sizes (a launch above the current maximum fails). Setting
+// the one largest value once avoids that.
+void allow_full_lds(const void* fn, const char* name) {
+  static std::mutex mu;
+  static std::set<const void*> done;
+  std::lock_guard<std::mutex> lock(mu);
+  if (done.count(fn)) return;
+  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess)
+    fail(BD_ERR_DEVICE, std::string(name) + " LDS attribute");
+  done.insert(fn);
+}
+
 int64_t sym_eigen_work(int m, int want) {
   const int64_t mw = int64_t(m) * want;
   return 3 * int64_t(m) + 8 + 5 * mw + (mw + 7) / 8 + 8 + int64_t(want) * want + mw + gemm_tn_work(want, want, m);
@@ -1074,9 +1092,7 @@ void tridiag_unblocked(hipStream_t s, int m, double* A, int64_t lda, const Eigen
   const size_t tri_lds = lower ? tri_lds_lower : size_t(5 * m + 2 * kTW) * 8;
   const void* tri_fn = lower ? reinterpret_cast<const void*>(&k_tridiag<true>)
                              : reinterpret_cast<const void*>(&k_tridiag<false>);
-  if (tri_lds > 64 * 1024 &&
-      hipFuncSetAttribute(tri_fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(tri_lds)) != hipSuccess)
-    fail(BD_ERR_DEVICE, "k_tridiag LDS attribute");
+  if (tri_lds > 64 * 1024) allow_full_lds(tri_fn, "k_tridiag");
   if (lower)
     hipLaunchKernelGGL(k_tridiag<true>, dim3(1), dim3(kTT), tri_lds, s, m, A, lda, L.d, L.e, L.tau);
   else
@@ -1094,10 +1110,7 @@ void sym_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, do
   hipLaunchKernelGGL(k_zero_i, dim3(1), dim3(64), 0, s, info, 1);
   const size_t panel_lds = tridiag_panel_lds(m);
   if (blocked && m >= 3 && panel_lds <= 160 * 1024) {
-    if (panel_lds > 64 * 1024 &&
-        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tridiag_panel),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, int(panel_lds)) != hipSuccess)
-      fail(BD_ERR_DEVICE, "k_tridiag_panel LDS attribute");
+    if (panel_lds > 64 * 1024) allow_full_lds(reinterpret_cast<const void*>(&k_tridiag_panel), "k_tridiag_panel");
     hipLaunchKernelGGL(k_lower_to_upper, dim3((m + 255) / 256, m), dim3(256), 0, s, m, A, lda);
     hipLaunchKernelGGL(k_tridiag_panel, dim3(1), dim3(kTT), panel_lds, s, m, A, lda, L.d, L.e, L.tau);
   } else {
@@ -1107,10 +1120,7 @@ void sym_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, do
                      L.tnorm);
   const size_t invit_lds = size_t(2 * m + 5 * int64_t(m) * kInvLds) * 8 + size_t(m) * kInvLds;
   if (invit_lds <= 160 * 1024) {
-    if (invit_lds > 64 * 1024 &&
-        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_invit_lds), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            int(invit_lds)) != hipSuccess)
-      fail(BD_ERR_DEVICE, "k_invit_lds LDS attribute");
+    if (invit_lds > 64 * 1024) allow_full_lds(reinterpret_cast<const void*>(&k_invit_lds), "k_invit_lds");
     hipLaunchKernelGGL(k_invit_lds, dim3((want + kInvLds - 1) / kInvLds), dim3(kInvLds), invit_lds, s, m, want, L.d,
                        L.e, values, L.tnorm, L.f, L.Z);
   } else {

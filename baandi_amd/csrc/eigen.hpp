@@ -7,6 +7,10 @@
 
 namespace bd {
 
+// Lets launches of kernel `fn` ask for up to the whole 160 KB of LDS (set once
+// per kernel; `name` labels the error).
+void allow_full_lds(const void* fn, const char* name);
+
 // Doubles of workspace needed by sym_eigen_lowest for an m x m matrix and
 // `want` eigenpairs.
 int64_t sym_eigen_work(int m, int want);

@@ -529,16 +529,14 @@ static void allow_lds(const StickPlan& plan) {
                         reinterpret_cast<const void*>(&k_plane<0, F, 512>),
                         reinterpret_cast<const void*>(&k_plane<1, F, 512>)})
     BD_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-  if (stick_lds(plan, e) > 65536) {
-    BD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_stick_inverse<F>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, int(stick_lds(plan, e))));
-    BD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_stick_forward<F>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, int(stick_lds(plan, e))));
-    BD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_kstick_inverse<F>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, int(stick_lds(plan, e))));
-    BD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_kstick_forward<F>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, int(stick_lds(plan, e))));
-  }
+  // The same largest value for every op: the attribute is per kernel and
+  // process-wide, so a per-plan size set by one op could undercut another's.
+  if (stick_lds(plan, e) > 65536)
+    for (const void* f : {reinterpret_cast<const void*>(&k_stick_inverse<F>),
+                          reinterpret_cast<const void*>(&k_stick_forward<F>),
+                          reinterpret_cast<const void*>(&k_kstick_inverse<F>),
+                          reinterpret_cast<const void*>(&k_kstick_forward<F>)})
+      BD_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
 }
 
 GammaOp::GammaOp(const System& s, const KBasis* k, int cap, int prec) : sys(&s), kb(k) {

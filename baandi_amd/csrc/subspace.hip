@@ -247,19 +247,13 @@ void extend_inverse_cholesky_dev(hipStream_t s, const double* S, double* T, int 
   double* T22 = T + from + int64_t(from) * cap;
   const size_t lds_packed = (size_t(k) * (k + 1) / 2 + size_t(kCT / 64) * k) * 8;
   if (lds_packed <= 160 * 1024 && k <= 64 * kInvRows) {
-    if (lds_packed > 64 * 1024 &&
-        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_chol_inv_lds),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_packed)) != hipSuccess)
-      fail(BD_ERR_DEVICE, "k_chol_inv_lds LDS attribute");
+    if (lds_packed > 64 * 1024) allow_full_lds(reinterpret_cast<const void*>(&k_chol_inv_lds), "k_chol_inv_lds");
     hipLaunchKernelGGL(k_chol_inv_lds, dim3(1), dim3(kCT), lds_packed, s, k, fw.G, S + from + int64_t(from) * cap,
                        int64_t(cap) + 1, T22, int64_t(cap), fw.flag, fw.min_ratio);
   } else {
     const size_t lds = size_t(kCT / 64) * k * 8;
     if (lds > 160 * 1024) fail(BD_ERR_INVALID_PARAMETER, "inverse Cholesky block too large");
-    if (lds > 64 * 1024 &&
-        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_chol_inv), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            int(lds)) != hipSuccess)
-      fail(BD_ERR_DEVICE, "k_chol_inv LDS attribute");
+    if (lds > 64 * 1024) allow_full_lds(reinterpret_cast<const void*>(&k_chol_inv), "k_chol_inv");
     hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(kCT), lds, s, k, fw.G, S + from + int64_t(from) * cap,
                        int64_t(cap) + 1, T22, int64_t(cap), fw.flag, fw.min_ratio);
   }
