@@ -1103,8 +1103,9 @@ void sym_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, do
                       int64_t ldy, double* work, int* info, bool blocked) {
   if (m <= 0 || want <= 0 || want > m || lda < m || ldy < m || lda * m >= (int64_t(1) << 31))
     fail(BD_ERR_INVALID_PARAMETER, "sym_eigen_lowest: shape");
+  const size_t bt_lds = size_t((4 + kBtChunk) * m + kBtChunk) * 8;
   if (size_t(5 * m + 2 * kTW) * 8 > 160 * 1024 || size_t(3 * m) * 8 > 64 * 1024 || m > 1025 ||
-      size_t((4 + kBtChunk) * m + kBtChunk) * 8 > 64 * 1024)
+      bt_lds > 160 * 1024)
     fail(BD_ERR_INVALID_PARAMETER, "sym_eigen_lowest: subspace too large for the LDS kernels");
   const EigenLayout L = eigen_layout(m, want, work);
   hipLaunchKernelGGL(k_zero_i, dim3(1), dim3(64), 0, s, info, 1);
@@ -1133,8 +1134,9 @@ void sym_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, do
   gemm_tn(s, want, want, m, 1.0, Y, ldy, Y, ldy, 0.0, L.G, want, L.gwork);
   hipLaunchKernelGGL(k_loewdin_factor, dim3((want + 255) / 256, want), dim3(256), 0, s, want, L.G);
   gemm_nn(s, m, want, want, 1.0, Y, ldy, L.G, want, 0.0, L.Ytmp, m);
-  hipLaunchKernelGGL(k_back_transform, dim3((want + 3) / 4), dim3(256), size_t((4 + kBtChunk) * m + kBtChunk) * 8, s, m,
-                     want, A, lda, L.tau, L.Ytmp, int64_t(m));
+  if (bt_lds > 64 * 1024) allow_full_lds(reinterpret_cast<const void*>(&k_back_transform), "k_back_transform");
+  hipLaunchKernelGGL(k_back_transform, dim3((want + 3) / 4), dim3(256), bt_lds, s, m, want, A, lda, L.tau, L.Ytmp,
+                     int64_t(m));
   copy_columns(s, m, want, L.Ytmp, m, Y, ldy);
 }
 

@@ -35,7 +35,11 @@ def _check_pairs(A, want, label):
     assert orth < 1e-12 * max(A.shape[0], 10)
 
 
-@pytest.mark.parametrize("m,want", [(1, 1), (2, 2), (3, 3), (5, 2), (64, 64), (160, 160), (320, 160), (321, 100)])
+# 700 exceeds the blocked tridiagonalisation's and the LDS inverse iteration's
+# LDS and takes the one-step and global-memory kernels, with a back-transform
+# above 64 KB of LDS.
+@pytest.mark.parametrize("m,want", [(1, 1), (2, 2), (3, 3), (5, 2), (64, 64), (160, 160), (320, 160), (321, 100),
+                                    (700, 60)])
 def test_eigen_random_symmetric(m, want):
     rng = np.random.default_rng(m * 7 + want)
     B = rng.standard_normal((m, m))
