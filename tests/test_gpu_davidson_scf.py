@@ -1,7 +1,7 @@
 """GPU parity of the reference Davidson and the Gamma SCF against the FP64
 oracle and the reference goldens, through the C ABI.
 
-Bars (north star): eigenvalues <= 1e-10 Ha against a dense FP64 eigensolve;
+Bars: eigenvalues <= 1e-10 Ha against a dense FP64 eigensolve;
 SCF total energy <= 1e-8 Ha/atom against the stored goldens (KAT-1 certified
 -4.684561184277879 Ha; KAT-2 -0.8646074276373095 Ha)."""
 import numpy as np
