@@ -549,7 +549,7 @@ GammaOp::GammaOp(const System& s, const KBasis* k, int cap, int prec) : sys(&s),
   hipDeviceProp_t prop;
   BD_HIP(hipGetDeviceProperties(&prop, dev));
   if (std::string(prop.gcnArchName).rfind("gfx950", 0) != 0)
-    fail(BD_ERR_DEVICE, std::string("this build targets gfx950, device is ") + prop.gcnArchName);
+    fail(BD_ERR_DEVICE, std::string("the library is compiled for gfx950, device is ") + prop.gcnArchName);
   n = kb ? 2 * kb->nk : s.n;
   nrep = kb ? kb->nk : s.nrep;
   for (int i = 0; i < 3; ++i) dims[i] = s.dims[i];
