@@ -260,7 +260,15 @@ void gemm_nn(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const
     hipLaunchKernelGGL(k_gemm_nn<64>, grid, dim3(256), 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
-int64_t gemm_tn_work(int64_t M, int64_t N, int64_t K) { return int64_t(tn_splits(M, N, K)) * M * N; }
+// Covers the partial sums of every product M' x N' with M' <= M, N' <= N at
+// this K, so one buffer serves every call up to its sizing shape:
+// tn_splits is not monotone (fewer tiles take more splits). A split count
+// above one is at most 512 / tiles with tiles >= M'N' / (TM * 64), and at
+// most min(K / 256, 256).
+int64_t gemm_tn_work(int64_t M, int64_t N, int64_t K) {
+  const int64_t smax = std::min<int64_t>(std::max<int64_t>(1, K / 256), 256);
+  return std::max(M * N, std::min(int64_t(512) * TM * col_tile(N), smax * M * N));
+}
 
 void gemm_tn(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
              int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,

@@ -14,7 +14,8 @@ void gemm_nn(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const
 // C(MxN, ldc) = alpha * A^T B + beta * C with A (KxM, lda), B (KxN, ldb): a
 // tall-skinny reduction over K, split into fixed chunks whose partial tiles
 // are summed in ascending chunk order (deterministic). `work` must hold
-// gemm_tn_work(M, N, K) doubles.
+// gemm_tn_work(M, N, K) doubles, a size that also covers every smaller M, N
+// at the same K.
 void gemm_tn(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
              int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
              double* work);

@@ -154,7 +154,13 @@ void gemm_nn_bf16x3(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha
                      ldc, nullptr);
 }
 
-int64_t gemm_tn_bf16x3_work(int64_t M, int64_t N, int64_t K) { return int64_t(bf_splits(M, N, K)) * M * N; }
+// Covers every product M' x N' with M' <= M, N' <= N at this K (bf_splits
+// is not monotone in the shape): a split count above one is at most
+// 1024 / tiles with tiles >= M'N' / (BM * BN), and at most min(K / 256, 256).
+int64_t gemm_tn_bf16x3_work(int64_t M, int64_t N, int64_t K) {
+  const int64_t smax = std::min<int64_t>(std::max<int64_t>(1, K / 256), 256);
+  return std::max(M * N, std::min(int64_t(1024) * BM * BN, smax * M * N));
+}
 
 void gemm_tn_bf16x3(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
                     const double* B, int64_t ldb, double beta, double* C, int64_t ldc, double* work) {

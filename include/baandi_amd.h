@@ -280,7 +280,8 @@ bd_status bd_block_vectors_dev(bd_block* block, double* x_dev, double* hx_dev, i
 /* ------------------------------------------------------------------------
  * Dense products of the path as stand-alone entry points
  * C = alpha op(A) B + beta C, column-major, device pointers; op(A) = A^T when
- * trans_a (then a split-K reduction needs `work`, bd_gemm_workspace doubles).
+ * trans_a (then a split-K reduction needs `work`, bd_gemm_workspace doubles;
+ * the size returned for M, N also covers every smaller M, N at the same K).
  * mode 0: FP64; mode 1: BF16x3 split products on MFMA, FP32 accumulation.
  * These are the products behind PᵀX / P·W (distributed_real_gamma.rs:957-1251)
  * and the Davidson border Grams / rotations (ref_davidson/workspace.rs:300-381,

@@ -81,3 +81,23 @@ def test_operator_needs_a_gfx950_device():
     with pytest.raises(L.BaandiError) as e:
         DistributedRealHamiltonian(s, capacity=4)
     assert e.value.status == L.BD_ERR_DEVICE
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_gemm_workspace_covers_smaller_shapes(mode):
+    """One split-K buffer serves every A^T B up to its sizing shape: the size
+    is monotone in M and N at a fixed K and never below M N. (Fewer output
+    tiles take more K splits, so the split count alone is not monotone: the
+    Davidson and operator buffers are sized once at the subspace capacity.)"""
+    from baandi_amd import lib as L
+    lib = L.load()
+    for K in (100, 500, 23847, 36525, 200000):
+        for mmax, nmax in ((640, 576), (576, 576), (320, 160), (64, 64)):
+            top = lib.bd_gemm_workspace(mode, 1, mmax, nmax, K)
+            for m in list(range(1, mmax + 1, 7)) + [mmax]:
+                prev = 0
+                for n in list(range(1, nmax + 1, 5)) + [nmax]:
+                    w = lib.bd_gemm_workspace(mode, 1, m, n, K)
+                    assert m * n <= w <= top, (mode, K, mmax, nmax, m, n, w, top)
+                    assert w >= prev
+                    prev = w

@@ -156,3 +156,36 @@ def test_mixed_davidson_eigenvalues_vs_fp64(precond):
     # BF16x3 Grams carry ~2^-16 relative noise: the mixed tier's eigenvalue
     # floor is ~2e-5 Ha, which is why the SCF finishes in FP64.
     assert err < 5e-5
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_gemm_workspace_sized_at_capacity_serves_narrow_products(mode):
+    """A^T B with a workspace sized for (640, 576) and products of 64..576
+    columns, as the operator's projector product P^T X runs them: the partial
+    sums stay inside the sized region (a guard band after it keeps its fill)
+    and each product matches the FP64 host product."""
+    import torch
+    from baandi_amd import lib as L
+    lib = L.load()
+    K, M, cap = 36525, 640, 576
+    rng = np.random.default_rng(7)
+    A = rng.standard_normal((K, M))
+    B = rng.standard_normal((K, cap))
+    Ad = torch.from_numpy(np.asfortranarray(A).ravel(order="F")).cuda()
+    Bd = torch.from_numpy(np.asfortranarray(B).ravel(order="F")).cuda()
+    size = lib.bd_gemm_workspace(mode, 1, M, cap, K)
+    guard = 1 << 20
+    W = torch.full((size + guard,), 7.25, dtype=torch.float64, device="cuda")
+    for n in (64, 96, 128, 288, 576):
+        print(f"[gemm workspace] mode={mode} {M}x{n}x{K} sized {size}", flush=True)
+        Cd = torch.zeros(M * n, dtype=torch.float64, device="cuda")
+        L.check(lib.bd_gemm_dev(mode, 1, M, n, K, 1.0, L.ptr(Ad), K, L.ptr(Bd), K, 0.0, L.ptr(Cd), M, L.ptr(W)))
+        torch.cuda.synchronize()
+        assert bool((W[size:] == 7.25).all()), n
+        ref = A.T @ B[:, :n]
+        out = Cd.cpu().numpy().reshape(n, M).T
+        err = np.abs(out - ref).max()
+        if mode == 0:
+            assert err < 1e-12 * np.abs(ref).max()
+        else:  # the bf16x3 bound of test_bf16x3_gemm_error_bound
+            assert err < 2.0 ** -15 * (np.abs(A.T) @ np.abs(B[:, :n])).max()
