@@ -98,6 +98,11 @@ class ScfOptions:
     # Optional fixed-point verification map (driver.rs:727-757): one more
     # Davidson at the converged potential with every band at this threshold (Ha).
     final_threshold: float = 0.0
+    # Fermi-Dirac smearing width (Ha) for cells whose gap is too small for
+    # fixed occupations (occupations.rs:71-232); 0 keeps the lowest
+    # electrons/2 bands doubly occupied. With smearing every band is held to
+    # the Davidson threshold and the energy is the free energy E - TS.
+    theta: float = 0.0
 
 
 class DensityField:
@@ -217,6 +222,13 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
     occupied = int(round(electrons)) // 2
     occ = np.zeros(bands)
     occ[:occupied] = 2.0
+    smeared = options.theta > 0.0
+    converge = bands if smeared else occupied
+    mu, ts = None, 0.0
+    if smeared:
+        from .kpoint import fermi_dirac
+        if 2 * bands <= electrons:
+            raise L.BaandiError(L.BD_ERR_INVALID_PARAMETER, "smearing needs more bands than electrons/2")
     field = DensityField(system)
     vion_g = torch.from_numpy(system.get("ionic_density_re") + 1j * system.get("ionic_density_im")).to(dev)
     vion = field.to_grid(vion_g).real.contiguous()
@@ -283,7 +295,7 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
             diag = diag_static + vmean
             totals["fields_s"] += time.perf_counter() - tf
             td = time.perf_counter()
-            opts = ref_davidson.Options(bands=bands, threshold=ethr / 2.0, occupied=occupied,
+            opts = ref_davidson.Options(bands=bands, threshold=ethr / 2.0, occupied=converge,
                                         first_step=(iteration == 1 and not redo),
                                         max_iterations=options.davidson_max_iterations, return_hx=True,
                                         preconditioner=options.preconditioner, mlp=options.mlp,
@@ -297,6 +309,9 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
             totals["dense_s"] += sol.record["seconds_dense"]
             totals["apply_s"] += sol.record["seconds_apply"]
             tf = time.perf_counter()
+            if smeared:
+                occ, mu, ts = fermi_dirac(sol.eigenvalues[None, :], np.ones(1), electrons, options.theta)
+                occ = occ[0]
             cur.band_density_sum(xout, occ, rho_dev)
             rho_out = rho_dev.view(field.dims)
             y = field.from_grid(rho_out)
@@ -330,12 +345,15 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
     if options.final_threshold > 0.0:
         vmean = float(potential.mean())
         diag = diag_static + vmean
-        opts = ref_davidson.Options(bands=bands, threshold=options.final_threshold, occupied=occupied,
+        opts = ref_davidson.Options(bands=bands, threshold=options.final_threshold, occupied=converge,
                                     first_step=True, max_iterations=200, return_hx=True,
                                     preconditioner=options.preconditioner, mlp=options.mlp,
                                     mlp_vbar=vmean)
         sol = ref_davidson.solve_with_initial_hx(op, orbitals, diag, opts, out=(xout, hxout))
         orbitals, xout = xout, orbitals
+        if smeared:
+            occ, mu, ts = fermi_dirac(sol.eigenvalues[None, :], np.ones(1), electrons, options.theta)
+            occ = occ[0]
         op.band_density_sum(orbitals, occ, rho_dev)
         rho_out = rho_dev.view(field.dims)
     # Energy audit of the converged map (refscf/map.rs:466-540).
@@ -349,10 +367,11 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
     eps_o, _ = lda_pz_torch(rho_out.abs())
     xc = dv * float((rho_out.abs() * eps_o).sum())
     local = dv * float((rho_out * vion).sum())
-    total = kin + local + nonlocal_ + hartree + xc + ewald
+    total = kin + local + nonlocal_ + hartree + xc + ewald - ts
     energy = {"kinetic": kin, "local": local, "nonlocal": nonlocal_, "hartree": hartree, "xc": xc,
-              "ewald": ewald, "total": total, "band_sum": float(np.dot(sol.eigenvalues, occ))}
-    out = dict(energy=energy, eigenvalues=sol.eigenvalues.copy(), history=history, wall_s=wall,
+              "ewald": ewald, "entropy_term": ts, "total": total, "band_sum": float(np.dot(sol.eigenvalues, occ))}
+    out = dict(energy=energy, eigenvalues=sol.eigenvalues.copy(), occupations=occ.copy(), mu=mu,
+               history=history, wall_s=wall,
                setup_s=t_setup, iterations=len(history), totals=totals, orbitals=orbitals, op=op,
                plan=op.plan_info(), potential=potential.cpu().numpy())
     if profile_kernels:

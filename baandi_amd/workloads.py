@@ -32,6 +32,18 @@ def si64(a=20.52, ecut_ha=15.0):
                 bands=160)
 
 
+def si128(a=10.26, ecut_ha=12.5, sigma=0.05, seed=128):
+    """Si128 cell (4x2x2 conventional cells), Gamma, 25 Ry, 288 bands, atoms
+    displaced by N(0, sigma) Bohr (seeded): the learned-preconditioner cell."""
+    import numpy as np
+    rows = [[4 * a, 0.0, 0.0], [0.0, 2 * a, 0.0], [0.0, 0.0, 2 * a]]
+    atoms = np.array(supercell((4, 2, 2)), dtype=np.float64)
+    rng = np.random.default_rng(seed)
+    atoms = atoms + rng.normal(0.0, sigma, atoms.shape) / np.array([4 * a, 2 * a, 2 * a])
+    return dict(name="si128_gamma_25ry_displaced", rows=rows, atoms=atoms.tolist(), z=4.0, ecut_ha=ecut_ha,
+                bands=288)
+
+
 def si8(a=10.26, ecut_ha=15.0):
     rows = [[a, 0.0, 0.0], [0.0, a, 0.0], [0.0, 0.0, a]]
     return dict(name="si8_gamma_30ry", rows=rows, atoms=CONVENTIONAL, z=4.0, ecut_ha=ecut_ha, bands=20)

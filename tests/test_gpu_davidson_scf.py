@@ -127,6 +127,35 @@ def test_scf_kat2_golden():
     assert abs(r["energy"]["total"] - (-0.8646074276373095)) < 1e-9
 
 
+def test_scf_smeared_matches_oracle_k_at_gamma(kat1):
+    """KAT-1 cell, 8 bands, Fermi-Dirac 0.01 Ha: the real Gamma SCF with
+    smearing against the oracle's k SCF on the 1x1x1 mesh (dense solves of
+    the complex H at k = 0). The cell is metallic at Gamma, so five bands
+    carry fractional occupations."""
+    from baandi_amd.hamiltonian import GammaSystem
+    from baandi_amd.scf import run_scf, ScfOptions
+    from oracle import physics as P
+    from oracle import kpoint as OK
+    rows, atoms = fcc_rows(kat1["cell_a_bohr"]), [[0, 0, 0], [0.25] * 3]
+    s = GammaSystem(rows, atoms, 4.0, kat1["ecut_ha"])
+    r = run_scf(s, ScfOptions(bands=8, theta=0.01, threshold_ry=1e-11))
+    ref = OK.scf_k(rows, atoms, P.synthetic_pseudo(4.0, 1.0, 801), kat1["ecut_ha"], (1, 1, 1), 8, 0.01,
+                   threshold_ry=1e-11)
+    de = abs(r["energy"]["total"] - ref["energy"]["total"])
+    deig = np.abs(r["eigenvalues"] - np.ravel(ref["eigenvalues"])).max()
+    docc = np.abs(r["occupations"] - np.ravel(ref["occupations"])).max()
+    print(f"[smeared gamma] E={r['energy']['total']:.12f} oracle={ref['energy']['total']:.12f} dE={de:.2e} "
+          f"TS={r['energy']['entropy_term']:.3e} mu={r['mu']:.8f}/{ref['mu']:.8f} eig err {deig:.2e} "
+          f"occ err {docc:.2e} maps={r['iterations']}", flush=True)
+    assert abs(r["occupations"].sum() - 8.0) < 1e-10
+    assert 0.0 < (r["occupations"] * (2.0 - r["occupations"])).sum()  # fractional bands present
+    for key in ("kinetic", "nonlocal", "local", "hartree", "xc", "entropy_term"):
+        assert abs(r["energy"][key] - ref["energy"][key]) < 1e-5, key
+    assert de < 1e-8 * len(atoms)
+    assert deig < 1e-6 and docc < 1e-4
+    assert abs(r["mu"] - ref["mu"]) < 1e-6
+
+
 def test_scf_si8_matches_oracle_scf():
     """Cubic Si8 at 6 Ha: GPU SCF vs oracle SCF from their own seeded starts."""
     from baandi_amd.hamiltonian import GammaSystem
