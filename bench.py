@@ -206,6 +206,42 @@ def kpoint_leg(steps, threads=16):
     }
 
 
+def si128_leg(mlp_path=None, theta=0.01):
+    """Si128 (4x2x2 conventional cells, displaced N(0, 0.05 Bohr), Gamma,
+    25 Ry, 288 bands): SCF to dr2 < 1e-8 Ry in FP64, mixed, learned and
+    mixed + learned. The cell is nearly gapless at Gamma under the synthetic
+    pseudopotential, so every leg uses Fermi-Dirac smearing (theta Ha)."""
+    import numpy as np
+    from baandi_amd import workloads, lib as L
+    from baandi_amd.scf import run_scf, ScfOptions
+    from baandi_amd.learned import Network
+    from baandi_amd.train_preconditioner import DEFAULT_OUT
+    wl = workloads.si128()
+    system = workloads.build(wl)
+    net = Network.load(mlp_path or DEFAULT_OUT)
+    base = dict(bands=wl["bands"], threshold_ry=1e-8, preconditioning="thomas_fermi", autotune=True, theta=theta)
+    legs = {"fp64": ScfOptions(**base),
+            "mixed": ScfOptions(precision=L.BD_MIXED, **base),
+            "learned": ScfOptions(preconditioner=L.BD_PRECOND_MLP, mlp=net, **base),
+            "mixed_learned": ScfOptions(precision=L.BD_MIXED, preconditioner=L.BD_PRECOND_MLP, mlp=net, **base)}
+    out, ref = {"workload": f"{wl['name']}: n={system.n}, grid {list(system.dims)}, {system.nproj} projectors, "
+                            f"{wl['bands']} bands, Fermi-Dirac {theta} Ha, Thomas-Fermi mixing, dr2<1e-8 Ry"}, None
+    for name, opts in legs.items():
+        r = run_scf(system, opts)
+        ref = ref or r
+        print(f"[bench] si128 {name}: {r['wall_s']:.3f}s maps={r['iterations']} E={r['energy']['total']:.12f}",
+              file=sys.stderr, flush=True)
+        t = r["totals"]
+        out[name] = {"wall_s": round(r["wall_s"], 3), "scf_iterations": r["iterations"],
+                     "h_columns": t["h_columns"], "davidson_iterations": t["davidson_iterations"],
+                     "dense_s": round(t["dense_s"], 3), "apply_s": round(t["apply_s"], 3),
+                     "mixed_maps": t["mixed_maps"], "mlp_fallbacks": t["mlp_fallbacks"],
+                     "energy_ha": r["energy"]["total"], "entropy_term_ha": r["energy"]["entropy_term"],
+                     "energy_diff_per_atom_ha": abs(r["energy"]["total"] - ref["energy"]["total"]) / 128,
+                     "eigenvalue_max_diff_ha": float(np.abs(r["eigenvalues"] - ref["eigenvalues"]).max())}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,6 +255,8 @@ def main():
                     help="Al 6x6x6 k-point SCFs timed (0 skips the k-point leg)")
     ap.add_argument("--kpoint-only", action="store_true",
                     help="profile mode: only the Al k-point leg, its JSON object printed")
+    ap.add_argument("--si128-only", action="store_true",
+                    help="profile mode: only the Si128 smeared SCF legs, their JSON object printed")
     ap.add_argument("--mlp", default=None,
                     help="learned-preconditioner weights (.npz) for the learned legs (default: the packaged file)")
     ap.add_argument("--hx-only", type=int, default=0,
@@ -234,6 +272,9 @@ def main():
     from baandi_amd.scf import run_scf, ScfOptions
     if args.kpoint_only:
         print(json.dumps(kpoint_leg(max(args.kpoint_steps, 1))))
+        return
+    if args.si128_only:
+        print(json.dumps(si128_leg(args.mlp)))
         return
 
     wl = workloads.si64()
