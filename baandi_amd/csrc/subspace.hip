@@ -10,6 +10,7 @@
 // eps * (sum_j |T_ij| sqrt(S_jj))^2 <= sqrt(eps).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -44,63 +45,20 @@ __global__ void k_copy_block(int k, const double* __restrict__ S, int64_t lds, d
   if (i < k) G[i + int64_t(j) * k] = S[i + int64_t(j) * lds];
 }
 
-// In-place right-looking Cholesky of G with the pivot-ratio test, then
-// T22 = L^{-1} into T (ld ldt), zero above the diagonal.
-__global__ __launch_bounds__(kCT) void k_chol_inv(int k, double* __restrict__ G, const double* __restrict__ sdiag,
-                                                  int64_t sstride, double* __restrict__ T22, int64_t ldt,
-                                                  int* __restrict__ flag, double* __restrict__ min_ratio) {
-  extern __shared__ double b[];  // kCT/64 waves x k doubles
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  constexpr int nw = kCT / 64;
-  double mr = INFINITY;
-  for (int j = 0; j < k; ++j) {
-    const double dj = G[j + int64_t(j) * k];
-    const double ratio = dj / sdiag[int64_t(j) * sstride];
-    if (isnan(ratio) || ratio < kPivotRatioMin) {
-      if (tid == 0) {
-        flag[0] = 1;
-        min_ratio[0] = isnan(ratio) ? 0.0 : fmin(mr, ratio);
-      }
-      return;  // every thread read the same pivot: uniform exit
-    }
-    mr = fmin(mr, ratio);
-    const double inv = 1.0 / sqrt(dj);
-    __syncthreads();  // all threads hold dj before column j changes
-    for (int i = j + tid; i < k; i += kCT) G[i + int64_t(j) * k] = i == j ? sqrt(dj) : G[i + int64_t(j) * k] * inv;
-    __syncthreads();
-    for (int l = j + 1 + wave; l < k; l += nw) {
-      const double llj = G[l + int64_t(j) * k];
-      for (int i = l + lane; i < k; i += 64) G[i + int64_t(l) * k] -= G[i + int64_t(j) * k] * llj;
-    }
-    __syncthreads();
-  }
-  if (tid == 0) min_ratio[0] = mr;
-  // Column c of L^{-1}: forward substitution L x = e_c, one wave per column;
-  // lane owns rows i = lane (mod 64) of its right-hand side in LDS.
-  double* rhs = b + int64_t(wave) * k;
-  for (int c = wave; c < k; c += nw) {
-    for (int i = lane; i < k; i += 64) rhs[i] = i == c ? 1.0 : 0.0;
-    for (int l = c; l < k; ++l) {
-      double xl = 0.0;
-      if ((l & 63) == lane) xl = rhs[l] / G[l + int64_t(l) * k];
-      xl = __shfl(xl, l & 63, 64);
-      if ((l & 63) == lane) rhs[l] = xl;
-      for (int i = l + 1 + ((lane - (l + 1)) & 63); i < k; i += 64) rhs[i] -= G[i + int64_t(l) * k] * xl;
-    }
-    for (int i = lane; i < k; i += 64) T22[i + int64_t(c) * ldt] = i >= c ? rhs[i] : 0.0;
-  }
-}
-
-// The same with the block in LDS (packed lower triangle, column-major):
-// an LDL^T elimination with one barrier per column (column j is only read
-// while the columns right of it are updated), then L = column j over
-// sqrt(d_j), then L^{-1} by forward substitution, a wave per column with the
-// right-hand side in registers.
+// Cholesky of the k x k block G (k <= 64 * kInvRows) with the pivot-ratio
+// test, then T22 = L^{-1} into T (ld ldt), zero above the diagonal. The block
+// sits in LDS as a packed lower triangle (column-major): an LDL^T elimination
+// with one barrier per column (column j is only read while the columns right
+// of it are updated), then L = column j over sqrt(d_j), then L^{-1} by
+// forward substitution, a wave per column with the right-hand side in
+// registers. A flag already set by an earlier block of the same border
+// leaves the block untouched; min_ratio keeps the minimum over the blocks.
 __global__ __launch_bounds__(kCT) void k_chol_inv_lds(int k, const double* __restrict__ G,
                                                       const double* __restrict__ sdiag, int64_t sstride,
                                                       double* __restrict__ T22, int64_t ldt, int* __restrict__ flag,
                                                       double* __restrict__ min_ratio) {
   extern __shared__ double sh[];
+  if (flag[0]) return;  // an earlier block of this border failed: uniform exit
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   constexpr int nw = kCT / 64;
   double* P = sh;
@@ -116,7 +74,7 @@ __global__ __launch_bounds__(kCT) void k_chol_inv_lds(int k, const double* __res
     if (isnan(ratio) || ratio < kPivotRatioMin) {
       if (tid == 0) {
         flag[0] = 1;
-        min_ratio[0] = isnan(ratio) ? 0.0 : fmin(mr, ratio);
+        min_ratio[0] = isnan(ratio) ? 0.0 : fmin(min_ratio[0], fmin(mr, ratio));
       }
       return;  // every thread read the same pivot: uniform exit
     }
@@ -140,7 +98,7 @@ __global__ __launch_bounds__(kCT) void k_chol_inv_lds(int k, const double* __res
     rhsall[l] = 1.0 / sd;  // reciprocal diagonal for the substitution
   }
   __syncthreads();
-  if (tid == 0) min_ratio[0] = mr;
+  if (tid == 0) min_ratio[0] = fmin(min_ratio[0], mr);
   // Column c of L^{-1}: forward substitution L x = e_c, one wave per column,
   // the right-hand side in registers (lane owns rows lane + 64 t, t <
   // kInvRows), x_l broadcast from its owner lane: the serial chain per row is
@@ -235,33 +193,45 @@ __global__ void k_zero_flag(int* flag, double* min_ratio) {
 // with M, N, K <= cap.
 int64_t factor_gwork(int cap) { return int64_t(cap < 256 ? 1 : cap / 256) * cap * cap; }
 
-void extend_inverse_cholesky_dev(hipStream_t s, const double* S, double* T, int cap, int from, int to,
-                                 const FactorWork& fw) {
+namespace {
+// Widest border block k_chol_inv_lds holds: its rows live in the kInvRows
+// registers of a lane, and k(k+1)/2 + k doubles fit the 160 KB of LDS.
+constexpr int kFactorBlock = 64 * kInvRows;
+
+// T[from:to, 0:to] for one block of at most kFactorBlock new columns:
+// W = T11 S12, G = S22 - W^T W, T22 = chol(G)^{-1}, T21 = -T22 W^T T11.
+void extend_block(hipStream_t s, const double* S, double* T, int cap, int from, int to, const FactorWork& fw) {
   const int k = to - from;
-  hipLaunchKernelGGL(k_zero_flag, dim3(1), dim3(1), 0, s, fw.flag, fw.min_ratio);
-  if (k <= 0) return;
   if (from > 0) gemm_nn(s, from, k, from, 1.0, T, cap, S + int64_t(from) * cap, cap, 0.0, fw.W, from);
   hipLaunchKernelGGL(k_copy_block, dim3((k + 255) / 256, k), dim3(256), 0, s, k,
                      S + from + int64_t(from) * cap, int64_t(cap), fw.G);
   if (from > 0) gemm_tn(s, k, k, from, -1.0, fw.W, from, fw.W, from, 1.0, fw.G, k, fw.gwork);
   double* T22 = T + from + int64_t(from) * cap;
-  const size_t lds_packed = (size_t(k) * (k + 1) / 2 + size_t(kCT / 64) * k) * 8;
-  if (lds_packed <= 160 * 1024 && k <= 64 * kInvRows) {
-    if (lds_packed > 64 * 1024) allow_full_lds(reinterpret_cast<const void*>(&k_chol_inv_lds), "k_chol_inv_lds");
-    hipLaunchKernelGGL(k_chol_inv_lds, dim3(1), dim3(kCT), lds_packed, s, k, fw.G, S + from + int64_t(from) * cap,
-                       int64_t(cap) + 1, T22, int64_t(cap), fw.flag, fw.min_ratio);
-  } else {
-    const size_t lds = size_t(kCT / 64) * k * 8;
-    if (lds > 160 * 1024) fail(BD_ERR_INVALID_PARAMETER, "inverse Cholesky block too large");
-    if (lds > 64 * 1024) allow_full_lds(reinterpret_cast<const void*>(&k_chol_inv), "k_chol_inv");
-    hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(kCT), lds, s, k, fw.G, S + from + int64_t(from) * cap,
-                       int64_t(cap) + 1, T22, int64_t(cap), fw.flag, fw.min_ratio);
-  }
+  const size_t lds = (size_t(k) * (k + 1) / 2 + size_t(k)) * 8;
+  if (k > kFactorBlock || lds > 160 * 1024) fail(BD_ERR_INVALID_PARAMETER, "inverse Cholesky block too large");
+  if (lds > 64 * 1024) allow_full_lds(reinterpret_cast<const void*>(&k_chol_inv_lds), "k_chol_inv_lds");
+  hipLaunchKernelGGL(k_chol_inv_lds, dim3(1), dim3(kCT), lds, s, k, fw.G, S + from + int64_t(from) * cap,
+                     int64_t(cap) + 1, T22, int64_t(cap), fw.flag, fw.min_ratio);
   if (from > 0) {
     gemm_tn(s, k, from, from, 1.0, fw.W, from, T, cap, 0.0, fw.U, k, fw.gwork);         // U = W^T T11
     gemm_nn(s, k, from, k, -1.0, T22, cap, fw.U, k, 0.0, T + from, cap);                  // T21 = -T22 U
   }
   hipLaunchKernelGGL(k_growth_guard, dim3((k + 3) / 4), dim3(256), 0, s, S, T, cap, from, to, fw.flag);
+}
+}  // namespace
+
+// A border wider than one block is factored as successive extensions in
+// near-equal blocks: the inverse Cholesky factor of S[0:b, 0:b] extended by
+// the next columns of S is that of S[0:b', 0:b'] (the pivot of each column is
+// the same Schur complement, so the pivot-ratio test is unchanged).
+void extend_inverse_cholesky_dev(hipStream_t s, const double* S, double* T, int cap, int from, int to,
+                                 const FactorWork& fw) {
+  const int k = to - from;
+  hipLaunchKernelGGL(k_zero_flag, dim3(1), dim3(1), 0, s, fw.flag, fw.min_ratio);
+  if (k <= 0) return;
+  const int blocks = (k + kFactorBlock - 1) / kFactorBlock;
+  const int width = (k + blocks - 1) / blocks;
+  for (int a = from; a < to; a += width) extend_block(s, S, T, cap, a, std::min(to, a + width), fw);
 }
 
 void mirror_border(hipStream_t s, double* M, int cap, int from, int to) {

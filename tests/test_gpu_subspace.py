@@ -130,3 +130,44 @@ def test_inverse_cholesky_rejects_dependent_border():
     ok_ref, _ = Davidson._factor(None, S, 32, 64, None)
     print(f"[factor] dependent border: accepted={ok}, min ratio {ratio:.3e}, oracle accepted={ok_ref}")
     assert not ok and not ok_ref
+
+
+def test_inverse_cholesky_wide_borders_against_oracle():
+    """Borders wider than one LDS block (288 and 300 new columns are each
+    factored as two extensions of 144 and 150) against the oracle's factor."""
+    from oracle.davidson import Davidson
+    rng = np.random.default_rng(4)
+    cap, n = 600, 2000
+    V = rng.standard_normal((n, cap)) * rng.uniform(0.5, 2.0, cap)[None, :]
+    S = V.T @ V
+    T = np.zeros((cap, cap))
+    for frm, to in [(0, 288), (288, 588), (588, 600)]:
+        print(f"[factor wide] {frm}->{to}", flush=True)
+        ok, ratio, T = _factor_dev(S, T, cap, frm, to)
+        ok_ref, Tref = Davidson._factor(None, S, frm, to, None)
+        assert ok and ok_ref
+        err = np.abs(T[:to, :to] - Tref).max() / np.abs(Tref).max()
+        ident = np.abs(T[:to, :to] @ S[:to, :to] @ T[:to, :to].T - np.eye(to)).max()
+        print(f"[factor wide] {frm}->{to}: T vs oracle {err:.2e}, |T S T^T - I| {ident:.2e}, min ratio {ratio:.3e}")
+        assert err < 1e-11 and ident < 1e-11
+        assert np.all(T[:to, :to][np.triu_indices(to, 1)] == 0.0)
+        assert 0.0 < ratio <= 1.0
+
+
+@pytest.mark.parametrize("dependent", [60, 250])
+def test_inverse_cholesky_wide_border_rejects_dependence(dependent):
+    """A 300-column border (two blocks of 150) with one column numerically
+    dependent on an earlier one, in the first or the second block: rejected,
+    as the oracle rejects it, with the failing pivot ratio reported."""
+    from oracle.davidson import Davidson
+    rng = np.random.default_rng(5)
+    cap, n = 320, 1200
+    V = rng.standard_normal((n, cap))
+    V[:, dependent] = V[:, 10] + 1e-9 * V[:, dependent + 1]
+    S = V.T @ V
+    T = np.zeros((cap, cap))
+    ok, ratio, _ = _factor_dev(S, T, cap, 0, 300)
+    ok_ref, _ = Davidson._factor(None, S, 0, 300, None)
+    print(f"[factor wide] dependent column {dependent}: accepted={ok}, min ratio {ratio:.3e}, oracle {ok_ref}")
+    assert not ok and not ok_ref
+    assert ratio < np.finfo(float).eps / 1e-12
