@@ -35,11 +35,12 @@ def _check_pairs(A, want, label):
     assert orth < 1e-12 * max(A.shape[0], 10)
 
 
-# 700 exceeds the blocked tridiagonalisation's and the LDS inverse iteration's
-# LDS and takes the one-step and global-memory kernels, with a back-transform
-# above 64 KB of LDS.
+# The blocked tridiagonalisation holds m <= 523 in LDS (523 and 524 straddle
+# that limit) and the LDS inverse iteration m <= 476; above those the one-step
+# and global-memory kernels run, with a back-transform above 64 KB of LDS
+# from m = 683.
 @pytest.mark.parametrize("m,want", [(1, 1), (2, 2), (3, 3), (5, 2), (64, 64), (160, 160), (320, 160), (321, 100),
-                                    (700, 60)])
+                                    (523, 60), (524, 100), (576, 288), (659, 40), (660, 30), (700, 60)])
 def test_eigen_random_symmetric(m, want):
     rng = np.random.default_rng(m * 7 + want)
     B = rng.standard_normal((m, m))
