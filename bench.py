@@ -206,11 +206,12 @@ def kpoint_leg(steps, threads=16):
     }
 
 
-def si128_leg(mlp_path=None, theta=0.01):
+def si128_leg(mlp_path=None, theta=0.01, variants=True):
     """Si128 (4x2x2 conventional cells, displaced N(0, 0.05 Bohr), Gamma,
-    25 Ry, 288 bands): SCF to dr2 < 1e-8 Ry in FP64, mixed, learned and
-    mixed + learned. The cell is nearly gapless at Gamma under the synthetic
-    pseudopotential, so every leg uses Fermi-Dirac smearing (theta Ha)."""
+    25 Ry, 288 bands): SCF to dr2 < 1e-8 Ry in FP64, then (variants) mixed,
+    learned and mixed + learned. The cell is nearly gapless at Gamma under
+    the synthetic pseudopotential, so every leg uses Fermi-Dirac smearing
+    (theta Ha)."""
     import numpy as np
     from baandi_amd import workloads, lib as L
     from baandi_amd.scf import run_scf, ScfOptions
@@ -224,6 +225,8 @@ def si128_leg(mlp_path=None, theta=0.01):
             "mixed": ScfOptions(precision=L.BD_MIXED, **base),
             "learned": ScfOptions(preconditioner=L.BD_PRECOND_MLP, mlp=net, **base),
             "mixed_learned": ScfOptions(precision=L.BD_MIXED, preconditioner=L.BD_PRECOND_MLP, mlp=net, **base)}
+    if not variants:
+        legs = {"fp64": legs["fp64"]}
     out, ref = {"workload": f"{wl['name']}: n={system.n}, grid {list(system.dims)}, {system.nproj} projectors, "
                             f"{wl['bands']} bands, Fermi-Dirac {theta} Ha, Thomas-Fermi mixing, dr2<1e-8 Ry"}, None
     for name, opts in legs.items():
@@ -256,7 +259,8 @@ def main():
     ap.add_argument("--kpoint-only", action="store_true",
                     help="profile mode: only the Al k-point leg, its JSON object printed")
     ap.add_argument("--si128-only", action="store_true",
-                    help="profile mode: only the Si128 smeared SCF legs, their JSON object printed")
+                    help="profile mode: only the Si128 smeared SCF legs, their JSON object printed "
+                         "(--mixed-steps 0: the FP64 leg alone)")
     ap.add_argument("--mlp", default=None,
                     help="learned-preconditioner weights (.npz) for the learned legs (default: the packaged file)")
     ap.add_argument("--hx-only", type=int, default=0,
@@ -274,7 +278,7 @@ def main():
         print(json.dumps(kpoint_leg(max(args.kpoint_steps, 1))))
         return
     if args.si128_only:
-        print(json.dumps(si128_leg(args.mlp)))
+        print(json.dumps(si128_leg(args.mlp, variants=args.mixed_steps > 0)))
         return
 
     wl = workloads.si64()
