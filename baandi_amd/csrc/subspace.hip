@@ -197,6 +197,9 @@ namespace {
 // Widest border block k_chol_inv_lds holds: its rows live in the kInvRows
 // registers of a lane, and k(k+1)/2 + k doubles fit the 160 KB of LDS.
 constexpr int kFactorBlock = 64 * kInvRows;
+// Widest block a border is split into: the elimination costs ~k^3 / 3 in
+// barriered column steps, so narrower blocks plus the extension GEMMs cost less.
+constexpr int kFactorSplit = 96;
 
 // T[from:to, 0:to] for one block of at most kFactorBlock new columns:
 // W = T11 S12, G = S22 - W^T W, T22 = chol(G)^{-1}, T21 = -T22 W^T T11.
@@ -229,7 +232,7 @@ void extend_inverse_cholesky_dev(hipStream_t s, const double* S, double* T, int 
   const int k = to - from;
   hipLaunchKernelGGL(k_zero_flag, dim3(1), dim3(1), 0, s, fw.flag, fw.min_ratio);
   if (k <= 0) return;
-  const int blocks = (k + kFactorBlock - 1) / kFactorBlock;
+  const int blocks = (k + kFactorSplit - 1) / kFactorSplit;
   const int width = (k + blocks - 1) / blocks;
   for (int a = from; a < to; a += width) extend_block(s, S, T, cap, a, std::min(to, a + width), fw);
 }

@@ -134,8 +134,8 @@ def test_inverse_cholesky_rejects_dependent_border():
 
 
 def test_inverse_cholesky_wide_borders_against_oracle():
-    """Borders wider than one LDS block (288 and 300 new columns are each
-    factored as two extensions of 144 and 150) against the oracle's factor."""
+    """Borders wider than one factor block (288 new columns as three blocks
+    of 96, 300 as four of 75) against the oracle's factor."""
     from oracle.davidson import Davidson
     rng = np.random.default_rng(4)
     cap, n = 600, 2000
@@ -157,8 +157,8 @@ def test_inverse_cholesky_wide_borders_against_oracle():
 
 @pytest.mark.parametrize("dependent", [60, 250])
 def test_inverse_cholesky_wide_border_rejects_dependence(dependent):
-    """A 300-column border (two blocks of 150) with one column numerically
-    dependent on an earlier one, in the first or the second block: rejected,
+    """A 300-column border (four blocks of 75) with one column numerically
+    dependent on an earlier one, in the first or the last block: rejected,
     as the oracle rejects it, with the failing pivot ratio reported."""
     from oracle.davidson import Davidson
     rng = np.random.default_rng(5)
