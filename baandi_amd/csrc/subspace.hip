@@ -23,7 +23,7 @@ namespace bd {
 
 namespace {
 constexpr int kCT = 1024;
-constexpr int kInvRows = 3;  // k_chol_inv_lds: rows per lane of the substitution (k <= 192)
+constexpr int kInvRows = 2;  // k_chol_inv_lds: rows per lane of the substitution (k <= 128)
 constexpr double kPivotRatioMin = DBL_EPSILON / 1e-12;  // lean_davidson.rs:62-64
 
 // Lane l's value of v (l uniform across the wave).
