@@ -1,6 +1,8 @@
 """Host logic of the k-point driver (baandi_amd/kpoint.py) against the oracle:
 Monkhorst-Pack points and weights, Fermi-Dirac occupations, chemical
-potential and entropy, including the capacity-1 form over realified levels."""
+potential and entropy, including the capacity-1 form over realified levels;
+and the Gamma SCF's check that smeared occupations have room for every
+electron."""
 import numpy as np
 
 from baandi_amd import kpoint as K
@@ -37,3 +39,16 @@ def test_capacity_one_over_doubled_levels_equals_capacity_two():
     occ1, mu1, ts1 = K.fermi_dirac(np.repeat(e, 2, axis=1), w, 5.0, 0.02, capacity=1.0)
     assert abs(mu1 - mu2) < 1e-12 and abs(ts1 - ts2) < 1e-13
     assert np.abs(occ1[:, 0::2] + occ1[:, 1::2] - occ2).max() < 1e-12
+
+
+def test_gamma_smearing_needs_more_bands_than_electron_pairs():
+    """Fermi-Dirac occupations cannot hold every electron in electrons/2
+    bands: the Gamma SCF rejects that before any device work."""
+    import pytest
+    from baandi_amd import lib as L
+    from baandi_amd.hamiltonian import GammaSystem
+    from baandi_amd.scf import run_scf, ScfOptions
+    from conftest import fcc_rows
+    s = GammaSystem(fcc_rows(10.26), [[0, 0, 0], [0.25] * 3], 4.0, 5.0)
+    with pytest.raises(L.BaandiError, match="more bands"):
+        run_scf(s, ScfOptions(bands=4, theta=0.01))
