@@ -836,7 +836,8 @@ __global__ __launch_bounds__(64) void k_invit(int m, int want, const double* __r
 // The same with kInvLds vectors per workgroup whose factors and vector stay in
 // LDS (a dependent chain of LDS accesses instead of L2 round trips), copied
 // to the global layout at the end (the cluster re-iteration reads them there).
-constexpr int kInvLds = 8;
+// 8 vectors per workgroup up to m = 476, 4 from there to m = 910.
+template <int kInvLds>
 __global__ __launch_bounds__(kInvLds) void k_invit_lds(int m, int want, const double* __restrict__ d,
                                                        const double* __restrict__ e,
                                                        const double* __restrict__ values,
@@ -1119,11 +1120,15 @@ void sym_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, do
   }
   hipLaunchKernelGGL(k_bisect, dim3((want + 3) / 4), dim3(256), size_t(3 * m) * 8, s, m, want, L.d, L.e, values,
                      L.tnorm);
-  const size_t invit_lds = size_t(2 * m + 5 * int64_t(m) * kInvLds) * 8 + size_t(m) * kInvLds;
-  if (invit_lds <= 160 * 1024) {
-    if (invit_lds > 64 * 1024) allow_full_lds(reinterpret_cast<const void*>(&k_invit_lds), "k_invit_lds");
-    hipLaunchKernelGGL(k_invit_lds, dim3((want + kInvLds - 1) / kInvLds), dim3(kInvLds), invit_lds, s, m, want, L.d,
-                       L.e, values, L.tnorm, L.f, L.Z);
+  auto invit_lds = [m](int v) { return size_t(2 * m + 5 * int64_t(m) * v) * 8 + size_t(m) * v; };
+  if (invit_lds(8) <= 160 * 1024) {
+    if (invit_lds(8) > 64 * 1024) allow_full_lds(reinterpret_cast<const void*>(&k_invit_lds<8>), "k_invit_lds<8>");
+    hipLaunchKernelGGL((k_invit_lds<8>), dim3((want + 7) / 8), dim3(8), invit_lds(8), s, m, want, L.d, L.e, values,
+                       L.tnorm, L.f, L.Z);
+  } else if (invit_lds(4) <= 160 * 1024) {
+    allow_full_lds(reinterpret_cast<const void*>(&k_invit_lds<4>), "k_invit_lds<4>");
+    hipLaunchKernelGGL((k_invit_lds<4>), dim3((want + 3) / 4), dim3(4), invit_lds(4), s, m, want, L.d, L.e, values,
+                       L.tnorm, L.f, L.Z);
   } else {
     hipLaunchKernelGGL(k_invit, dim3((want + 63) / 64), dim3(64), size_t(2 * m) * 8, s, m, want, L.d, L.e, values,
                        L.tnorm, L.f, L.Z);

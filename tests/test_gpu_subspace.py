@@ -36,9 +36,9 @@ def _check_pairs(A, want, label):
 
 
 # The blocked tridiagonalisation holds m <= 523 in LDS (523 and 524 straddle
-# that limit) and the LDS inverse iteration m <= 476; above those the one-step
-# and global-memory kernels run, with a back-transform above 64 KB of LDS
-# from m = 683.
+# that limit); the LDS inverse iteration holds 8 vectors per workgroup up to
+# m = 476 and 4 up to 910. Above those the one-step and global-memory kernels
+# run, with a back-transform above 64 KB of LDS from m = 683.
 @pytest.mark.parametrize("m,want", [(1, 1), (2, 2), (3, 3), (5, 2), (64, 64), (160, 160), (320, 160), (321, 100),
                                     (523, 60), (524, 100), (576, 288), (659, 40), (660, 30), (700, 60)])
 def test_eigen_random_symmetric(m, want):
