@@ -311,6 +311,7 @@ struct Solver {
       BD_HIP(hipMemcpyAsync(&bad, w.info, sizeof(int), hipMemcpyDeviceToHost, s));
       sync();
       if (!bad) break;
+      rec.eigen_retries += 1;
     }
     t_dense += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (bad) fail(BD_ERR_APPLICATION, "reference reduced eigensolve failed");
@@ -684,7 +685,8 @@ struct BlockGraph {
     hipStream_t s = op.stream;
     op.gemm.tn(s, m, m, n, 1.0, V.p, n, HV.p, n, 0.0, A.p, m, gwork.p);
     symmetrize(s, m, A.p, m);
-    hipLaunchKernelGGL(k_lift_dropped, dim3(1), dim3(256), 0, s, A.p, int64_t(m), m - nb, dropped, nb, 1e6);
+    hipLaunchKernelGGL(k_lift_dropped, dim3(unsigned((nb + 255) / 256)), dim3(256), 0, s, A.p, int64_t(m), m - nb,
+                       dropped, nb, 1e6);
     sym_eigen_lowest(s, m, nb, A.p, m, theta.p, Y.p, m, eigw.p, info);
     op.gemm.nn(s, n, nb, m, 1.0, V.p, n, Y.p, m, 0.0, X.p, n);
     op.gemm.nn(s, n, nb, m, 1.0, HV.p, n, Y.p, m, 0.0, HX.p, n);

@@ -460,6 +460,29 @@ extern "C" bd_status bd_gamma_op_autotune(bd_gamma_op* h, int32_t m, int32_t rep
   });
 }
 
+// Read (set = 0) or set (set = 1) the plane kernel's launch shape on the
+// stick/plane path: inout2 = (threads, y rows per z-FFT chunk). Set accepts
+// exactly the shapes the autotuner may pick.
+extern "C" bd_status bd_gamma_op_plane_shape(bd_gamma_op* h, int32_t* inout2, int32_t set) {
+  BD_GUARD({
+    if (!h || !inout2) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    bd::GammaOp& o = h->op;
+    if (o.kb && o.fused) bd::fail(BD_ERR_INVALID_PARAMETER, "op runs the fused k-point kernel");
+    const bd::StickPlan& p = o.plan;
+    if (set) {
+      const int t = inout2[0], c = inout2[1];
+      if (c < 1 || c > p.n1 || !bd::fft_fits_threads(p.Z, p.ry, t) || !bd::fft_fits_threads(c, p.rz, t) ||
+          o.plane_lds_bytes(c) > 160 * 1024)
+        bd::fail(BD_ERR_INVALID_PARAMETER, "invalid plane shape");
+      o.plane_threads = t;
+      o.plan.chunk = c;
+    } else {
+      inout2[0] = o.plane_threads;
+      inout2[1] = p.chunk;
+    }
+  });
+}
+
 // Read or set the fused shape (threads, xlines, zrows, xplanes); set validates it.
 // Setting threads = 0 switches the op to the three-stage path; reading gives
 // threads = 0 on that path.

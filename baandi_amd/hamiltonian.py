@@ -193,6 +193,17 @@ class DistributedRealHamiltonian:
         L.check(L.load().bd_gamma_op_autotune(self.handle, int(columns), int(reps), L.ptr(out)))
         return tuple(out)
 
+    def plane_shape(self):
+        """(threads, y rows per z-FFT chunk) of the plane kernel."""
+        out = np.zeros(2, dtype=np.int32)
+        L.check(L.load().bd_gamma_op_plane_shape(self.handle, L.ptr(out), 0))
+        return int(out[0]), int(out[1])
+
+    def set_plane_shape(self, threads, chunk):
+        """Force the plane kernel's shape (one the autotuner may pick)."""
+        v = np.array([threads, chunk], dtype=np.int32)
+        L.check(L.load().bd_gamma_op_plane_shape(self.handle, L.ptr(v), 1))
+
     def set_profiling(self, on=True):
         L.check(L.load().bd_gamma_op_set_profiling(self.handle, 1 if on else 0))
 

@@ -381,7 +381,8 @@ def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
     ethr = 1e-2
     last_dr2 = math.inf
     history = []
-    totals = dict(h_columns=0, davidson_iterations=0, dense_s=0.0, apply_s=0.0, davidson_s=0.0, fields_s=0.0)
+    totals = dict(h_columns=0, davidson_iterations=0, dense_s=0.0, apply_s=0.0, davidson_s=0.0, fields_s=0.0,
+                  eigen_retries=0)
     result = None
     for iteration in range(1, options.max_iterations + 1):
         if iteration > 1:
@@ -408,6 +409,7 @@ def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
                                            threads=options.threads)
             for sol in sols:
                 totals["h_columns"] += sol.record["h_columns"]
+                totals["eigen_retries"] += sol.record["eigen_retries"]
                 totals["davidson_iterations"] += sol.record["iterations"]
                 totals["dense_s"] += sol.record["seconds_dense"]
                 totals["apply_s"] += sol.record["seconds_apply"]

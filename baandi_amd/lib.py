@@ -26,6 +26,7 @@ EXPORTS = [
     "bd_block_start_dev", "bd_block_step", "bd_block_step_async", "bd_block_finish", "bd_block_vectors_dev",
     "bd_k_op_create", "bd_k_op_info", "bd_k_op_get", "bd_k_batch_create", "bd_k_batch_destroy",
     "bd_k_batch_stream", "bd_k_batch_apply_dev", "bd_gamma_op_autotune", "bd_gamma_op_fused_shape",
+    "bd_gamma_op_plane_shape",
     "bd_gemm_workspace", "bd_gemm_dev", "bd_mfma_probe_dev", "bd_mfma_f64_probe_dev",
     "bd_sym_eigen_workspace", "bd_sym_eigen_dev", "bd_inverse_cholesky_dev",
     "bd_mlp_param_count", "bd_mlp_blob_floats", "bd_mlp_quantize_dev", "bd_mlp_forward_dev",
@@ -71,7 +72,7 @@ class DavidsonRecord(C.Structure):
         ("iterations", C.c_int32), ("h_calls", C.c_int32), ("h_columns", C.c_int32),
         ("repairs", C.c_int32), ("restarts", C.c_int32), ("all_converged", C.c_int32),
         ("seconds_total", C.c_double), ("seconds_apply", C.c_double), ("seconds_dense", C.c_double),
-        ("mlp_fallback", C.c_int32),
+        ("mlp_fallback", C.c_int32), ("eigen_retries", C.c_int32),
     ]
 
 
@@ -111,6 +112,7 @@ def load():
         "bd_k_batch_apply_dev": [P, P, P, P, P, P],
         "bd_gamma_op_autotune": [P, I32, I32, P],
         "bd_gamma_op_fused_shape": [P, P, I32],
+        "bd_gamma_op_plane_shape": [P, P, I32],
         "bd_gamma_op_set_potential": [P, P, U64],
         "bd_gamma_op_set_potential_dev": [P, P, U64],
         "bd_gamma_op_apply": [P, P, I64, I32, P, I64],

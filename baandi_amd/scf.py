@@ -274,7 +274,7 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
     last_dr2 = math.inf
     history = []
     totals = dict(h_columns=0, davidson_iterations=0, dense_s=0.0, apply_s=0.0, davidson_s=0.0, fields_s=0.0,
-                  mixed_maps=0, mlp_fallbacks=0)
+                  mixed_maps=0, mlp_fallbacks=0, eigen_retries=0)
     result = None
     for iteration in range(1, options.max_iterations + 1):
         if iteration > 1:
@@ -305,6 +305,7 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
             totals["mixed_maps"] += int(use_mixed)
             totals["mlp_fallbacks"] += int(sol.record.get("mlp_fallback", -1) >= 0)
             totals["h_columns"] += sol.record["h_columns"]
+            totals["eigen_retries"] += sol.record["eigen_retries"]
             totals["davidson_iterations"] += sol.record["iterations"]
             totals["dense_s"] += sol.record["seconds_dense"]
             totals["apply_s"] += sol.record["seconds_apply"]

@@ -191,6 +191,10 @@ bd_status bd_gamma_op_autotune(bd_gamma_op* op, int32_t m, int32_t reps, double*
 /* Read (set = 0) or set the fused shape (threads, xlines, zrows, xplanes);
  * threads = 0 selects the three-stage path. */
 bd_status bd_gamma_op_fused_shape(bd_gamma_op* op, int32_t* inout4, int32_t set);
+/* Read (set = 0) or set the plane kernel's shape on the stick/plane path
+ * (threads 256 or 512, y rows per z-FFT chunk); set accepts the shapes the
+ * autotuner may pick. All shapes give the same bits. */
+bd_status bd_gamma_op_plane_shape(bd_gamma_op* op, int32_t* inout2, int32_t set);
 
 /* ------------------------------------------------------------------------
  * Reference block Davidson on the op
@@ -227,6 +231,8 @@ typedef struct {
   double seconds_apply;
   double seconds_dense;  /* subspace step: factor + reduced eigensolve (GPU, synchronised) */
   int32_t mlp_fallback;  /* BD_PRECOND_MLP: iteration that fell back to TPA, -1 if none */
+  int32_t eigen_retries; /* reduced eigensolves that lost a vector of a tight cluster and were
+                            redone with the one-step tridiagonalisation */
 } bd_davidson_record;
 
 /* X0 (n x bands, device), optional HX0 (device or NULL), diagonal (n, device).

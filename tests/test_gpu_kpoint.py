@@ -172,7 +172,8 @@ def test_k_scf_matches_oracle(kat1):
     deig = np.abs(r["eigenvalues"] - ref["eigenvalues"]).max()
     print(f"[k scf] E={r['energy']['total']:.12f} oracle={ref['energy']['total']:.12f} dE={de:.2e} "
           f"TS={r['energy']['entropy_term']:.3e} mu={r['mu']:.8f}/{ref['mu']:.8f} eig err {deig:.2e} "
-          f"maps={r['iterations']} wall={r['wall_s']:.2f}s", flush=True)
+          f"maps={r['iterations']} wall={r['wall_s']:.2f}s eigen retries={r['totals']['eigen_retries']}",
+          flush=True)
     # The two SCFs stop at different densities within dr2 < 1e-11 Ry: the
     # terms and eigenvalues differ to first order in that residual, the
     # stationary free energy only to second order.

@@ -40,7 +40,8 @@ def _check_pairs(A, want, label):
 # m = 476 and 4 up to 910. Above those the one-step and global-memory kernels
 # run, with a back-transform above 64 KB of LDS from m = 683.
 @pytest.mark.parametrize("m,want", [(1, 1), (2, 2), (3, 3), (5, 2), (64, 64), (160, 160), (320, 160), (321, 100),
-                                    (523, 60), (524, 100), (576, 288), (659, 40), (660, 30), (700, 60)])
+                                    (523, 60), (524, 100), (576, 288), (659, 40), (660, 30), (700, 60),
+                                    (910, 40), (911, 40), (974, 30), (1025, 30)])
 def test_eigen_random_symmetric(m, want):
     rng = np.random.default_rng(m * 7 + want)
     B = rng.standard_normal((m, m))
@@ -57,6 +58,16 @@ def test_eigen_realified_hermitian(half):
     R = np.block([[A.real, -A.imag], [A.imag, A.real]])
     _check_pairs(R, half, "realified")
     _check_pairs(R, 2 * half, "realified")
+
+
+def test_eigen_realified_above_lds_inverse_iteration():
+    """m = 912 (above the LDS inverse iteration's 910): the realified
+    Hermitian matrix, every eigenvalue exactly twice, through the
+    global-memory inverse iteration."""
+    rng = np.random.default_rng(912)
+    B = rng.standard_normal((456, 456)) + 1j * rng.standard_normal((456, 456))
+    A = 0.5 * (B + B.conj().T)
+    _check_pairs(np.block([[A.real, -A.imag], [A.imag, A.real]]), 40, "realified")
 
 
 @pytest.mark.parametrize("m", [12, 96, 240])

@@ -64,6 +64,56 @@ def test_denominator_identity():
     assert np.abs(a - b).max() <= 8 * np.finfo(float).eps * np.abs(b).max()
 
 
+def test_denominator_reference_ry_form():
+    """The reference's independent Ry expression (crates/dft-solvers/tests/
+    ref_davidson.rs:238-256): with x = 2(h - e), d_Ry = (1 + x + sqrt(1 +
+    (x-1)^2))/2 equals 2 d_Ha; d depends on h - e only; at e = 0 the direct
+    form 0.25 (1 + 2d + sqrt(1 + (2d - 1)^2)) holds from -1e8 to 100. A slip
+    of the Ha -> Ry factor in x fails the first check."""
+    for h, e in [(-0.7, -0.2), (0.35, -0.15), (2.1, 0.6)]:
+        x = 2.0 * (h - e)
+        d_ry = (1.0 + x + math.sqrt(1.0 + (x - 1.0) ** 2)) / 2.0
+        for f in (D.sqrt_denominator, D.preconditioner_denominator):
+            a = float(f(np.array(h), np.array(e)))
+            assert abs(2.0 * a - d_ry) < 2e-15
+            assert abs(a - float(f(np.array(h + 0.25), np.array(e + 0.25)))) < 2e-15
+    for d in [-1e8, -10.0, -1.0, 0.0, 0.5, 1.0, 100.0]:
+        direct = 0.25 * (1.0 + 2.0 * d + math.sqrt(1.0 + (2.0 * d - 1.0) ** 2))
+        for f in (D.sqrt_denominator, D.preconditioner_denominator):
+            p = float(f(np.array(d), np.array(0.0)))
+            assert np.isfinite(p) and p > 0.0
+            assert abs(p - direct) < 1e-8
+
+
+def test_tpa_weight_closed_form():
+    """Teter-Payne-Allan (Phys. Rev. B 40, 12255 (1989)): K(x) = p/(p + 16 x^4),
+    p = 27 + 18x + 12x^2 + 8x^3, x = T_G/<T>, evaluated here in exact rational
+    arithmetic. The oracle's two branches (x <= 1 direct, x > 1 in y = 1/x;
+    crates/dft-solvers/src/lib.rs:197-207) match it to 2 ulp, with K(0) = 1,
+    K(1) = 65/81, K ~ 1/(2x) for large x, and the kinetic scale's 0.1 Ha floor
+    (lib.rs:178-193)."""
+    from fractions import Fraction as Fr
+
+    def exact(x):
+        x = Fr(x)
+        p = 27 + 18 * x + 12 * x ** 2 + 8 * x ** 3
+        return p / (p + 16 * x ** 4)
+
+    xs = [0.0, 1e-3, 0.25, 0.5, 0.999, 1.0, 1.001, 1.5, 2.0, 7.25, 40.0, 1e3]
+    scale = 0.375
+    got = D.tpa_weight(np.array(xs) * scale, scale)
+    for x, k in zip(xs, got):
+        ref = float(exact(Fr(x * scale) / Fr(scale)))
+        assert abs(k - ref) <= 2 * np.finfo(float).eps * ref, (x, k, ref)
+    assert D.tpa_weight(np.array([0.0]), 1.0)[0] == 1.0
+    assert abs(D.tpa_weight(np.array([1.0]), 1.0)[0] - 65.0 / 81.0) < 1e-16
+    big = D.tpa_weight(np.array([1e6]), 1.0)[0]
+    assert abs(big * 2e6 - 1.0) < 1e-5
+    w = np.array([1.0, 3.0])
+    assert D.kinetic_scale(w, np.array([0.2, 1.0])) == pytest.approx((0.2 + 3.0) / 4.0, rel=1e-15)
+    assert D.kinetic_scale(w, np.array([0.01, 0.02])) == 0.1
+
+
 @pytest.mark.parametrize("pc", ["qe", "tpa"])
 def test_davidson_matches_dense_eigensolve(si2_20ry, pc):
     """Reference Davidson vs dense EVD, 1e-11 Ha (crates/dft-solvers/tests/ref_davidson.rs:223)."""
