@@ -14,6 +14,8 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
+#include <mutex>
+#include <string>
 
 #include "common.hpp"
 #include "eigen.hpp"
@@ -259,6 +261,69 @@ void set_identity(hipStream_t s, int m, int n, double* A, int64_t lda) {
 
 }  // namespace bd
 
+namespace bd {
+namespace {
+// Factor scratch of one call at capacity cap: W, G, U (cap x cap each), the
+// split-K partials, then min_ratio and the flag.
+constexpr int64_t factor_scratch_doubles(int cap) {
+  return 3 * int64_t(cap) * cap + int64_t(cap < 256 ? 1 : cap / 256) * cap * cap + 2;
+}
+
+FactorWork factor_layout(double* base, int cap) {
+  const int64_t c2 = int64_t(cap) * cap;
+  FactorWork fw;
+  fw.W = base;
+  fw.G = fw.W + c2;
+  fw.U = fw.G + c2;
+  fw.gwork = fw.U + c2;
+  fw.min_ratio = fw.gwork + factor_gwork(cap);
+  fw.flag = reinterpret_cast<int*>(fw.min_ratio + 1);
+  return fw;
+}
+
+// One border extension on the null stream with its scratch at `base`; the
+// verdict and the smallest pivot ratio are read back (synchronous).
+void factor_on_null_stream(const double* S, double* T, int cap, int from, int to, double* base,
+                           int32_t* accepted, double* min_ratio) {
+  const FactorWork fw = factor_layout(base, cap);
+  extend_inverse_cholesky_dev(nullptr, S, T, cap, from, to, fw);
+  int flag = 0;
+  if (hipMemcpy(&flag, fw.flag, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(min_ratio, fw.min_ratio, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    fail(BD_ERR_DEVICE, "inverse Cholesky launch failed");
+  *accepted = flag == 0 ? 1 : 0;
+}
+
+// Allocation-free variant: the scratch of every call on a GPU is this
+// file-scope device array (each GPU holds its own copy until the process
+// ends). Its host-side name is not a device address: kernels get the address
+// from hipGetSymbolAddress, taken once per device with that device current.
+// Calls that overlap on one GPU share the array and race.
+constexpr int kStaticFactorCap = 640;
+constexpr int64_t kStaticFactorDoubles = factor_scratch_doubles(kStaticFactorCap);
+static __device__ __align__(256) double scratch[kStaticFactorDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+double* scratch_dev[kMaxDevices] = {};
+
+double* static_factor_scratch() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) fail(BD_ERR_DEVICE, "no current HIP device");
+  if (dev < 0 || dev >= kMaxDevices) fail(BD_ERR_DEVICE, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_dev[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      fail(BD_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    scratch_dev[dev] = static_cast<double*>(ptr);
+  }
+  return scratch_dev[dev];
+}
+}  // namespace
+}  // namespace bd
+
 extern "C" bd_status bd_inverse_cholesky_dev(const double* S, double* T, int32_t cap, int32_t from, int32_t to,
                                              int32_t* accepted, double* min_ratio) {
   struct Scratch {
@@ -270,21 +335,22 @@ extern "C" bd_status bd_inverse_cholesky_dev(const double* S, double* T, int32_t
   BD_GUARD({
     if (!S || !T || !accepted || !min_ratio || cap <= 0 || from < 0 || to < from || to > cap)
       bd::fail(BD_ERR_INVALID_PARAMETER, "inverse Cholesky arguments");
-    const int64_t c2 = int64_t(cap) * cap;
-    const int64_t total = 3 * c2 + bd::factor_gwork(cap) + 2;
+    const int64_t total = bd::factor_scratch_doubles(cap);
     if (hipMalloc(&buf.p, size_t(total) * sizeof(double)) != hipSuccess) bd::fail(BD_ERR_DEVICE, "scratch");
-    bd::FactorWork fw;
-    fw.W = buf.p;
-    fw.G = fw.W + c2;
-    fw.U = fw.G + c2;
-    fw.gwork = fw.U + c2;
-    fw.min_ratio = fw.gwork + bd::factor_gwork(cap);
-    fw.flag = reinterpret_cast<int*>(fw.min_ratio + 1);
-    bd::extend_inverse_cholesky_dev(nullptr, S, T, cap, from, to, fw);
-    int flag = 0;
-    if (hipMemcpy(&flag, fw.flag, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(min_ratio, fw.min_ratio, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
-      bd::fail(BD_ERR_DEVICE, "inverse Cholesky launch failed");
-    *accepted = flag == 0 ? 1 : 0;
+    bd::factor_on_null_stream(S, T, cap, from, to, buf.p, accepted, min_ratio);
+  });
+}
+
+extern "C" bd_status bd_inverse_cholesky_static_dev(const double* S, double* T, int32_t cap, int32_t from,
+                                                    int32_t to, int32_t* accepted, double* min_ratio) {
+  BD_GUARD({
+    if (!S || !T || !accepted || !min_ratio || cap <= 0 || from < 0 || to < from || to > cap)
+      bd::fail(BD_ERR_INVALID_PARAMETER, "inverse Cholesky arguments");
+    if (cap > bd::kStaticFactorCap)
+      bd::fail(BD_ERR_INVALID_PARAMETER, "bd_inverse_cholesky_static_dev: cap " + std::to_string(cap) +
+                                             " exceeds the static scratch limit of " +
+                                             std::to_string(bd::kStaticFactorCap));
+    double* base = bd::static_factor_scratch();
+    bd::factor_on_null_stream(S, T, cap, from, to, base, accepted, min_ratio);
   });
 }

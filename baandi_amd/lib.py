@@ -28,7 +28,7 @@ EXPORTS = [
     "bd_k_batch_stream", "bd_k_batch_apply_dev", "bd_gamma_op_autotune", "bd_gamma_op_fused_shape",
     "bd_gamma_op_plane_shape",
     "bd_gemm_workspace", "bd_gemm_dev", "bd_mfma_probe_dev", "bd_mfma_f64_probe_dev",
-    "bd_sym_eigen_workspace", "bd_sym_eigen_dev", "bd_inverse_cholesky_dev",
+    "bd_sym_eigen_workspace", "bd_sym_eigen_dev", "bd_inverse_cholesky_dev", "bd_inverse_cholesky_static_dev",
     "bd_mlp_param_count", "bd_mlp_blob_floats", "bd_mlp_quantize_dev", "bd_mlp_forward_dev",
     "bd_mlp_train_workspace", "bd_mlp_loss_grad_dev", "bd_mlp_adam_dev", "bd_mlp_train_dev",
 ]
@@ -138,6 +138,7 @@ def load():
         "bd_mfma_f64_probe_dev": [P, P, P],
         "bd_sym_eigen_dev": [I32, I32, P, I64, P, P, I64, P],
         "bd_inverse_cholesky_dev": [P, P, I32, I32, I32, P, P],
+        "bd_inverse_cholesky_static_dev": [P, P, I32, I32, I32, P, P],
         "bd_mlp_quantize_dev": [P, P],
         "bd_mlp_forward_dev": [I64, P, F, F, F, P, P],
         "bd_mlp_loss_grad_dev": [I64, I32, P, P, P, P, P, I32, P, P, P, P],

@@ -324,6 +324,15 @@ bd_status bd_sym_eigen_dev(int32_t m, int32_t want, double* A, int64_t lda, doub
                            int64_t ldy, double* work);
 bd_status bd_inverse_cholesky_dev(const double* S, double* T, int32_t cap, int32_t from, int32_t to,
                                   int32_t* accepted, double* min_ratio);
+/* bd_inverse_cholesky_static_dev: the same extension with the same results,
+ * allocating nothing. No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Its scratch is one file-scope
+ * device array per GPU (16.4 MB, held until the process ends), so cap must
+ * be <= 640 (BD_ERR_INVALID_PARAMETER otherwise). Calls that overlap on one
+ * GPU share that array and race; serialise them. */
+bd_status bd_inverse_cholesky_static_dev(const double* S, double* T, int32_t cap, int32_t from, int32_t to,
+                                         int32_t* accepted, double* min_ratio);
 
 /* ------------------------------------------------------------------------
  * Learned Davidson preconditioner (no reference counterpart: it takes the
