@@ -123,6 +123,49 @@ __global__ void k_finite(const double* __restrict__ p, int64_t count, int* __res
 
 __global__ void k_zero_int(int* p) { *p = 0; }
 
+// |V[:, j]|^2 per column, one workgroup per column (fixed order).
+__global__ __launch_bounds__(kT) void k_norm2_cols(const double* __restrict__ V, int64_t n, double* __restrict__ out) {
+  __shared__ double red[16];
+  const double* v = V + int64_t(blockIdx.x) * n;
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) acc += v[i] * v[i];
+  const double t = block_sum(acc, red);
+  if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
+
+// M[0:k, 0:k] = diag(values) (ld ldm).
+__global__ void k_set_diag(int k, const double* __restrict__ values, double* __restrict__ M, int64_t ldm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i < k) M[i + int64_t(j) * ldm] = i == j ? values[i] : 0.0;
+}
+
+// max |S[i, j] - delta_ij| over the leading k x k block (one workgroup; a
+// non-finite entry gives +inf).
+__global__ __launch_bounds__(kT) void k_identity_defect(int k, const double* __restrict__ S, int64_t lds,
+                                                        double* __restrict__ out) {
+  __shared__ double red[16];
+  double mx = 0.0;
+  for (int64_t t = threadIdx.x; t < int64_t(k) * k; t += blockDim.x) {
+    const int i = int(t % k), j = int(t / k);
+    const double v = S[i + int64_t(j) * lds] - (i == j ? 1.0 : 0.0);
+    mx = isfinite(v) ? fmax(mx, fabs(v)) : INFINITY;
+  }
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0.0;
+    for (int w = 0; w < int(blockDim.x >> 6); ++w) r = fmax(r, red[w]);
+    out[0] = r;
+  }
+}
+
+// Restart guard of ref_davidson/ideal_reset.rs: column norms within this of 1
+// allow the ideal reset; a measured |S - I| above it disables it.
+constexpr double kOrthogonalityLimit = 1e-12;
+// Every REFRESH_INTERVAL-th contraction is a measured refresh.
+constexpr int kRefreshInterval = 4;
+
 struct Dev {
   double* p = nullptr;
   void alloc(size_t count) { BD_HIP(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(double))); }
@@ -215,6 +258,10 @@ struct Solver {
   std::vector<double> prev_r2;
   bd_davidson_record rec{};
   double t_apply = 0.0, t_dense = 0.0;
+  // Restart state (workspace.rs:772-935): contractions of this solve, and
+  // whether a guard trip has switched the ideal reset off for its rest.
+  int contractions = 0;
+  bool reset_disabled = false;
   hipEvent_t ev0, ev1;
 
   Solver(GammaOp& op_, DavidsonWork& w_, const bd_davidson_options& o_, const double* diag_)
@@ -323,14 +370,61 @@ struct Solver {
     if (!factor(0, m)) fail(BD_ERR_APPLICATION, "reference contracted basis is dependent");
   }
 
-  // V <- V C (nb columns), HV <- HV C, measured refresh, C = I (workspace.rs:772-935).
+  // Column norms^2 of the nb restarted columns all within 1e-12 of 1
+  // (reset_norm_guard, workspace.rs:895-918).
+  bool norm_guard() {
+    hipLaunchKernelGGL(k_norm2_cols, dim3(nb), dim3(kT), 0, s, V, n, w.norm2.p);
+    std::vector<double> nr(nb);
+    down(nr.data(), w.norm2.p, nb);
+    for (double v : nr)
+      if (!std::isfinite(v) || std::fabs(v - 1.0) > kOrthogonalityLimit) return false;
+    return true;
+  }
+
+  // V <- V C (nb columns), HV <- HV C, then C = I and either the ideal reset
+  // (S = I, H_r = diag(theta), T = I: the restarted columns are the Ritz
+  // vectors) or a measured refresh (workspace.rs:772-935, ideal_reset.rs):
+  // measured every kRefreshInterval-th contraction, and for the rest of the
+  // solve once a guard trips (column norms off 1, or a measured |S - I| above
+  // 1e-12).
   void contract(int m) {
     op.gemm.nn(s, n, nb, m, 1.0, V, n, w.C.p, m, 0.0, X, n);
     op.gemm.nn(s, n, nb, m, 1.0, HV, n, w.C.p, m, 0.0, HX, n);
     std::swap(V, X);
     std::swap(HV, HX);
-    refresh(nb);
+    contractions += 1;
+    bool ideal = false;
+    if (reset_disabled) {
+    } else if (contractions % kRefreshInterval == 0) {
+      rec.periodic_refreshes += 1;
+    } else if (norm_guard()) {
+      ideal = true;
+      rec.ideal_resets += 1;
+    } else {
+      reset_disabled = true;
+      rec.guard_fallbacks += 1;
+    }
     set_identity(s, nb, nb, w.C.p, nb);
+    if (ideal) {
+      auto t0 = std::chrono::steady_clock::now();
+      set_identity(s, nb, nb, w.S.p, cap);
+      set_identity(s, nb, nb, w.T.p, cap);
+      hipLaunchKernelGGL(k_set_diag, dim3((nb + 255) / 256, nb), dim3(256), 0, s, nb, w.vals.p, w.H.p,
+                         int64_t(cap));
+      BD_HIP(hipGetLastError());
+      t_dense += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    } else {
+      refresh(nb);
+      if (!reset_disabled) {
+        hipLaunchKernelGGL(k_identity_defect, dim3(1), dim3(kT), 0, s, nb, w.S.p, int64_t(cap), w.norm2.p);
+        double defect = 0.0;
+        down(&defect, w.norm2.p, 1);
+        if (!(defect <= kOrthogonalityLimit)) {
+          reset_disabled = true;
+          rec.guard_fallbacks += 1;
+        }
+      }
+    }
     rec.restarts += 1;
   }
 
@@ -514,6 +608,17 @@ struct Solver {
       for (int i = 0; i < nb; ++i) {
         converged[i] = std::fabs(values[i] - old[i]) < band_threshold(i);
         if (!converged[i]) active.push_back(i);
+      }
+      if (o.profile) {
+        // Solution::ritz_changes_ha / ritz_values_ha (ref_davidson.rs:86-89).
+        for (int i = 0; i < nb; ++i) {
+          const int64_t at = int64_t(step) * nb + i;
+          if (o.ritz_changes_ha) o.ritz_changes_ha[at] = std::fabs(values[i] - old[i]);
+          if (o.ritz_values_ha) {
+            o.ritz_values_ha[2 * at] = old[i];
+            o.ritz_values_ha[2 * at + 1] = values[i];
+          }
+        }
       }
       iterations = step + 1;
       if (active.empty() || iterations == o.max_iterations) break;

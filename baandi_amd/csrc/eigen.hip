@@ -18,6 +18,7 @@
 //  5. Back-transformation by the stored reflectors, one wave per vector.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <mutex>
 #include <set>
 #include <string>
@@ -589,6 +590,188 @@ __global__ __launch_bounds__(kTT) void k_tridiag_panel(int m, double* __restrict
 }
 
 // ---------------------------------------------------------------------------
+// 1c. Tridiagonalisation across the chip, for subspaces whose trailing block
+// one CU cannot stream fast enough (the Si512 subspace of 2252). The panel
+// scheme of k_tridiag_panel with kMB reflectors per panel, one launch per
+// reflector for its column (one workgroup: the corrections of the earlier
+// panel reflectors, the reflector and, for the step before, w) and one for
+// the product of the panel-start trailing block with it (a wave per column,
+// every CU), then a tiled rank-2kMB update of the trailing block that writes
+// both triangles from one value (the block stays exactly symmetric). V and W
+// (m x kMB, ld m) hold the panel's v_i, w_i, zero above their first row; y
+// the last product; sc[0] the last tau. Same outputs as k_tridiag.
+// ---------------------------------------------------------------------------
+constexpr int kMB = 16;  // reflectors per panel
+
+// Finish w of step s - 1 (when s > 0): q = y - sum_{t < s-1} (v_t (w_t.v) +
+// w_t (v_t.v)), w = tau q - (tau^2/2 q.v) v. Then by `mode`: 0, column k =
+// p0 + s of A_cur = A - sum_{t < s} (v_t w_t^T + w_t v_t^T): d[k], and the
+// reflector of its rows k+1.. into V[:, s], A[k+1:, k], e[k], tau[k]; 1,
+// nothing more (the panel's last w); 2 (k = m - 2), the trailing 2 x 2 of
+// A_cur: d[m-2], e[m-2], d[m-1].
+__global__ __launch_bounds__(kTT) void k_td_column(int m, double* __restrict__ A, int64_t lda, int p0, int s,
+                                                   int mode, double* __restrict__ V, double* __restrict__ W,
+                                                   const double* __restrict__ y, double* __restrict__ sc,
+                                                   double* __restrict__ d, double* __restrict__ e,
+                                                   double* __restrict__ tau) {
+  extern __shared__ double sh[];
+  double* wl = sh;                // w of step s-1 (rows >= k)
+  double* xl = sh + m;            // the new reflector
+  double* coef = sh + 2 * m;      // 2 kMB: w_t.v at t, v_t.v at kMB + t
+  double* rowk = coef + 2 * kMB;  // 2 kMB: V[k, t] at t, W[k, t] at kMB + t
+  double* red = rowk + 2 * kMB;   // kTW
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int k = p0 + s;
+  if (s > 0) {
+    const int c0 = k;  // first trailing row of the previous step
+    const double* v = V + int64_t(s - 1) * m;
+    const double t1 = sc[0];
+    for (int t = wave; t < 2 * (s - 1); t += kTW) {
+      const double* a = t < s - 1 ? W + int64_t(t) * m : V + int64_t(t - (s - 1)) * m;
+      double acc = 0.0;
+      for (int r = c0 + lane; r < m; r += 64) acc += a[r] * v[r];
+      acc = wave_sum(acc);
+      if (lane == 0) coef[t < s - 1 ? t : kMB + t - (s - 1)] = acc;
+    }
+    __syncthreads();
+    double qx = 0.0;
+    for (int r = c0 + tid; r < m; r += kTT) {
+      double q = y[r];
+      for (int t = 0; t < s - 1; ++t) q -= V[r + int64_t(t) * m] * coef[t] + W[r + int64_t(t) * m] * coef[kMB + t];
+      wl[r] = q;
+      qx += q * v[r];
+    }
+    qx = block_sum(qx, red);
+    const double K = 0.5 * t1 * t1 * qx;
+    double* wg = W + int64_t(s - 1) * m;
+    for (int r = tid; r < m; r += kTT) {
+      const double w = r >= c0 ? t1 * wl[r] - K * v[r] : 0.0;
+      wl[r] = w;
+      wg[r] = w;
+    }
+    __syncthreads();
+  }
+  if (mode == 1) return;  // panel end: only w of the last step
+  // Row k of the panel's V and W (t < s; t = s-1 from LDS).
+  for (int t = tid; t < s; t += kTT) {
+    rowk[t] = V[k + int64_t(t) * m];
+    rowk[kMB + t] = t == s - 1 ? wl[k] : W[k + int64_t(t) * m];
+  }
+  __syncthreads();
+  auto acur = [&](int r, int j) {  // A_cur[r, j] for r, j >= k (j == k or the final 2 x 2)
+    double a = A[r + int64_t(j) * lda];
+    for (int t = 0; t < s; ++t) {
+      const double vr = V[r + int64_t(t) * m], vj = V[j + int64_t(t) * m];
+      const double wr = t == s - 1 ? wl[r] : W[r + int64_t(t) * m];
+      const double wj = t == s - 1 ? wl[j] : W[j + int64_t(t) * m];
+      a -= vr * wj + wr * vj;
+    }
+    return a;
+  };
+  if (mode == 2) {
+    if (tid == 0) {
+      d[m - 2] = acur(m - 2, m - 2);
+      e[m - 2] = acur(m - 1, m - 2);
+      d[m - 1] = acur(m - 1, m - 1);
+      tau[m - 2] = 0.0;
+      tau[m - 1] = 0.0;
+    }
+    return;
+  }
+  double ss = 0.0;
+  for (int r = k + tid; r < m; r += kTT) {
+    double a = A[r + int64_t(k) * lda];
+    for (int t = 0; t < s; ++t) {
+      const double wr = t == s - 1 ? wl[r] : W[r + int64_t(t) * m];
+      a -= V[r + int64_t(t) * m] * rowk[kMB + t] + wr * rowk[t];
+    }
+    if (r == k) {
+      d[k] = a;
+    } else {
+      xl[r] = a;
+      ss += a * a;
+    }
+  }
+  ss = block_sum(ss, red);
+  const double x0 = xl[k + 1];
+  const double sigma = sqrt(ss);
+  double alpha = 0.0, tk = 0.0;
+  if (sigma > 0.0) {
+    alpha = x0 >= 0.0 ? -sigma : sigma;
+    tk = 1.0 / (sigma * (sigma + fabs(x0)));  // 2 / |x - alpha e1|^2
+  }
+  double* vs = V + int64_t(s) * m;
+  for (int r = tid; r < m; r += kTT) {
+    double x = 0.0;
+    if (r > k) x = (r == k + 1 && sigma > 0.0) ? x0 - alpha : xl[r];
+    vs[r] = x;
+    if (r > k) A[r + int64_t(k) * lda] = x;
+  }
+  if (tid == 0) {
+    e[k] = alpha;
+    tau[k] = tk;
+    sc[0] = tk;
+  }
+}
+
+// y[r] = sum_{j > k} A[j, r] v[j] for rows r > k (column r of the symmetric
+// panel-start block against v, a wave per column).
+__global__ __launch_bounds__(256) void k_td_symv(int m, const double* __restrict__ A, int64_t lda, int k,
+                                                 const double* __restrict__ v, double* __restrict__ y) {
+  const int lane = threadIdx.x & 63;
+  const int r = k + 1 + blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= m) return;
+  const double* col = A + int64_t(r) * lda;
+  double acc = 0.0;
+  for (int j = k + 1 + lane; j < m; j += 64) acc += col[j] * v[j];
+  acc = wave_sum(acc);
+  if (lane == 0) y[r] = acc;
+}
+
+// A[p1:, p1:] -= sum_{t < se} (v_t w_t^T + w_t v_t^T): 64 x 64 tiles of the
+// lower triangle, each written to (i, j) and, transposed, to (j, i).
+__global__ __launch_bounds__(256) void k_td_update(int m, double* __restrict__ A, int64_t lda, int p1, int se,
+                                                   const double* __restrict__ V, const double* __restrict__ W) {
+  __shared__ double rv[64][kMB], rw[64][kMB], cv[64][kMB], cw[64][kMB];
+  __shared__ double tile[64][65];
+  // Lower-triangle tile index b -> (bi, bj), bi >= bj.
+  int bi = int((sqrt(8.0 * double(blockIdx.x) + 1.0) - 1.0) * 0.5);
+  while ((bi + 1) * (bi + 2) / 2 <= int(blockIdx.x)) ++bi;
+  while (bi * (bi + 1) / 2 > int(blockIdx.x)) --bi;
+  const int bj = int(blockIdx.x) - bi * (bi + 1) / 2;
+  const int i0 = p1 + 64 * bi, j0 = p1 + 64 * bj;
+  for (int e = threadIdx.x; e < 64 * kMB; e += 256) {
+    const int r = e / kMB, t = e - r * kMB;
+    const int ri = min(i0 + r, m - 1), rj = min(j0 + r, m - 1);
+    const bool ok = t < se;
+    rv[r][t] = ok ? V[ri + int64_t(t) * m] : 0.0;
+    rw[r][t] = ok ? W[ri + int64_t(t) * m] : 0.0;
+    cv[r][t] = ok ? V[rj + int64_t(t) * m] : 0.0;
+    cw[r][t] = ok ? W[rj + int64_t(t) * m] : 0.0;
+  }
+  __syncthreads();
+  const int li = threadIdx.x & 63;  // row within the tile (coalesced)
+  for (int lj = threadIdx.x >> 6; lj < 64; lj += 4) {
+    const int i = i0 + li, j = j0 + lj;
+    if (i < m && j < m && i >= j) {
+      double dlt = 0.0;
+#pragma unroll
+      for (int t = 0; t < kMB; ++t) dlt += rv[li][t] * cw[lj][t] + rw[li][t] * cv[lj][t];
+      const double nv = A[i + int64_t(j) * lda] - dlt;
+      A[i + int64_t(j) * lda] = nv;
+      tile[li][lj] = nv;
+    }
+  }
+  __syncthreads();
+  // Mirror: (j, i) = (i, j), rows j of the tile's columns written coalesced.
+  const int lj = threadIdx.x & 63;
+  for (int li2 = threadIdx.x >> 6; li2 < 64; li2 += 4) {
+    const int i = i0 + li2, j = j0 + lj;
+    if (i < m && j < m && i > j) A[j + int64_t(i) * lda] = tile[li2][lj];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // 2. Bisection: one wave per wanted eigenvalue index j (ascending).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int sturm_count(int m, const double* ds, const double* e2, double x, double pivmin) {
@@ -965,11 +1148,12 @@ __global__ void k_loewdin_factor(int n, double* __restrict__ G) {
 // 5. Y <- H_0 H_1 ... H_{m-3} Y, one wave per column (its column in LDS,
 // lane-owned rows); the block stages kBtChunk reflectors at a time in LDS.
 // ---------------------------------------------------------------------------
-constexpr int kBtChunk = 8;
-
+// CH reflectors staged at a time: 8, or 4 where (4 + 8) m doubles exceed the LDS.
+template <int CH>
 __global__ __launch_bounds__(256) void k_back_transform(int m, int want, const double* __restrict__ A, int64_t lda,
                                                         const double* __restrict__ tau, double* __restrict__ Y,
                                                         int64_t ldy) {
+  constexpr int kBtChunk = CH;
   extern __shared__ double sh[];
   const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
   const int col = blockIdx.x * (blockDim.x >> 6) + wl;
@@ -1045,6 +1229,7 @@ struct EigenLayout {
   double* G;      // want x want (the Loewdin step)
   double* Ytmp;   // m x want
   double* gwork;  // split-K workspace of Y^T Y
+  double* td;     // tridiag_multi: V, W (m x kMB), y (m), a scalar
 };
 
 EigenLayout eigen_layout(int m, int want, double* work) {
@@ -1063,6 +1248,7 @@ EigenLayout eigen_layout(int m, int want, double* work) {
   L.G = L.Z + mw + (mw + 7) / 8 + 8;
   L.Ytmp = L.G + int64_t(want) * want;
   L.gwork = L.Ytmp + mw;
+  L.td = L.gwork + gemm_tn_work(want, want, m);
   return L;
 }
 }  // namespace
@@ -1083,8 +1269,12 @@ void allow_full_lds(const void* fn, const char* name) {
 
 int64_t sym_eigen_work(int m, int want) {
   const int64_t mw = int64_t(m) * want;
-  return 3 * int64_t(m) + 8 + 5 * mw + (mw + 7) / 8 + 8 + int64_t(want) * want + mw + gemm_tn_work(want, want, m);
+  return 3 * int64_t(m) + 8 + 5 * mw + (mw + 7) / 8 + 8 + int64_t(want) * want + mw + gemm_tn_work(want, want, m) +
+         2 * int64_t(m) * kMB + m + 8;
 }
+
+// Subspaces of at least this size are tridiagonalised across the chip.
+std::atomic<int> multi_from{1026};
 
 // One step at a time (k_tridiag): for subspaces whose panel LDS does not fit.
 void tridiag_unblocked(hipStream_t s, int m, double* A, int64_t lda, const EigenLayout& L) {
@@ -1100,18 +1290,49 @@ void tridiag_unblocked(hipStream_t s, int m, double* A, int64_t lda, const Eigen
     hipLaunchKernelGGL(k_tridiag<false>, dim3(1), dim3(kTT), tri_lds, s, m, A, lda, L.d, L.e, L.tau);
 }
 
+// Across the chip (k_td_column / k_td_symv / k_td_update), for m >= 3; the
+// scratch T holds V, W (m x kMB each), y (m) and one scalar.
+void tridiag_multi(hipStream_t s, int m, double* A, int64_t lda, const EigenLayout& L, double* scratch) {
+  double* V = scratch;
+  double* W = V + int64_t(m) * kMB;
+  double* y = W + int64_t(m) * kMB;
+  double* sc = y + m;
+  const size_t col_lds = size_t(2 * m + 4 * kMB + kTW) * 8;
+  if (col_lds > 64 * 1024) allow_full_lds(reinterpret_cast<const void*>(&k_td_column), "k_td_column");
+  hipLaunchKernelGGL(k_lower_to_upper, dim3((m + 255) / 256, m), dim3(256), 0, s, m, A, lda);
+  const int last = m - 2;  // reflector steps k = 0 .. m-3
+  for (int p0 = 0; p0 < last; p0 += kMB) {
+    const int se = std::min(kMB, last - p0);
+    for (int st = 0; st < se; ++st) {
+      const int k = p0 + st;
+      hipLaunchKernelGGL(k_td_column, dim3(1), dim3(kTT), col_lds, s, m, A, lda, p0, st, 0, V, W, y, sc, L.d, L.e,
+                         L.tau);
+      hipLaunchKernelGGL(k_td_symv, dim3((m - k - 1 + 3) / 4), dim3(256), 0, s, m, A, lda, k, V + int64_t(st) * m,
+                         y);
+    }
+    hipLaunchKernelGGL(k_td_column, dim3(1), dim3(kTT), col_lds, s, m, A, lda, p0, se, 1, V, W, y, sc, L.d, L.e,
+                       L.tau);
+    const int p1 = p0 + se, tiles = (m - p1 + 63) / 64;
+    hipLaunchKernelGGL(k_td_update, dim3(tiles * (tiles + 1) / 2), dim3(256), 0, s, m, A, lda, p1, se, V, W);
+  }
+  hipLaunchKernelGGL(k_td_column, dim3(1), dim3(kTT), col_lds, s, m, A, lda, m - 2, 0, 2, V, W, y, sc, L.d, L.e,
+                     L.tau);
+}
+
 void sym_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, double* values, double* Y,
                       int64_t ldy, double* work, int* info, bool blocked) {
   if (m <= 0 || want <= 0 || want > m || lda < m || ldy < m || lda * m >= (int64_t(1) << 31))
     fail(BD_ERR_INVALID_PARAMETER, "sym_eigen_lowest: shape");
-  const size_t bt_lds = size_t((4 + kBtChunk) * m + kBtChunk) * 8;
-  if (size_t(5 * m + 2 * kTW) * 8 > 160 * 1024 || size_t(3 * m) * 8 > 64 * 1024 || m > 1025 ||
-      bt_lds > 160 * 1024)
-    fail(BD_ERR_INVALID_PARAMETER, "sym_eigen_lowest: subspace too large for the LDS kernels");
+  auto bt_lds = [m](int ch) { return size_t((4 + ch) * m + ch) * 8; };
+  const int bt_ch = bt_lds(8) <= 160 * 1024 ? 8 : 4;
+  if (size_t(3 * m) * 8 > 64 * 1024 || bt_lds(bt_ch) > 160 * 1024)
+    fail(BD_ERR_INVALID_PARAMETER, "sym_eigen_lowest: subspace too large for the LDS kernels (m <= 2730)");
   const EigenLayout L = eigen_layout(m, want, work);
   hipLaunchKernelGGL(k_zero_i, dim3(1), dim3(64), 0, s, info, 1);
   const size_t panel_lds = tridiag_panel_lds(m);
-  if (blocked && m >= 3 && panel_lds <= 160 * 1024) {
+  if (m >= 3 && m >= multi_from.load()) {
+    tridiag_multi(s, m, A, lda, L, L.td);
+  } else if (blocked && m >= 3 && panel_lds <= 160 * 1024) {
     if (panel_lds > 64 * 1024) allow_full_lds(reinterpret_cast<const void*>(&k_tridiag_panel), "k_tridiag_panel");
     hipLaunchKernelGGL(k_lower_to_upper, dim3((m + 255) / 256, m), dim3(256), 0, s, m, A, lda);
     hipLaunchKernelGGL(k_tridiag_panel, dim3(1), dim3(kTT), panel_lds, s, m, A, lda, L.d, L.e, L.tau);
@@ -1139,9 +1360,15 @@ void sym_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, do
   gemm_tn(s, want, want, m, 1.0, Y, ldy, Y, ldy, 0.0, L.G, want, L.gwork);
   hipLaunchKernelGGL(k_loewdin_factor, dim3((want + 255) / 256, want), dim3(256), 0, s, want, L.G);
   gemm_nn(s, m, want, want, 1.0, Y, ldy, L.G, want, 0.0, L.Ytmp, m);
-  if (bt_lds > 64 * 1024) allow_full_lds(reinterpret_cast<const void*>(&k_back_transform), "k_back_transform");
-  hipLaunchKernelGGL(k_back_transform, dim3((want + 3) / 4), dim3(256), bt_lds, s, m, want, A, lda, L.tau, L.Ytmp,
-                     int64_t(m));
+  if (bt_ch == 8) {
+    if (bt_lds(8) > 64 * 1024) allow_full_lds(reinterpret_cast<const void*>(&k_back_transform<8>), "k_back_transform");
+    hipLaunchKernelGGL(k_back_transform<8>, dim3((want + 3) / 4), dim3(256), bt_lds(8), s, m, want, A, lda, L.tau,
+                       L.Ytmp, int64_t(m));
+  } else {
+    allow_full_lds(reinterpret_cast<const void*>(&k_back_transform<4>), "k_back_transform");
+    hipLaunchKernelGGL(k_back_transform<4>, dim3((want + 3) / 4), dim3(256), bt_lds(4), s, m, want, A, lda, L.tau,
+                       L.Ytmp, int64_t(m));
+  }
   copy_columns(s, m, want, L.Ytmp, m, Y, ldy);
 }
 
@@ -1153,6 +1380,18 @@ void zero_doubles(hipStream_t s, double* p, int64_t count) {
 }
 
 }  // namespace bd
+
+extern "C" bd_status bd_sym_eigen_multi_from(int32_t* inout, int32_t set) {
+  BD_GUARD({
+    if (!inout) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    if (set) {
+      if (*inout < 3) bd::fail(BD_ERR_INVALID_PARAMETER, "threshold must be >= 3");
+      bd::multi_from.store(*inout);
+    } else {
+      *inout = bd::multi_from.load();
+    }
+  });
+}
 
 extern "C" int64_t bd_sym_eigen_workspace(int32_t m, int32_t want) {
   if (m <= 0 || want <= 0 || want > m) return 0;

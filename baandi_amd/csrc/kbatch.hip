@@ -360,10 +360,7 @@ extern "C" bd_status bd_gamma_op_autotune(bd_gamma_op* h, int32_t m, int32_t rep
       for (int t : {256, 512})
         for (int c = 1; c <= p.n1; ++c) {
           const bool useful = c == p.n1 || (c % 4 == 0) || c == 1;
-          if (!useful) continue;
-          if (!bd::fft_fits_threads(p.Z, p.ry, t) || !bd::fft_fits_threads(c, p.rz, t)) continue;
-          if (o.plane_lds_bytes(c) > 160 * 1024) continue;
-          cands.push_back({t, c});
+          if (useful && o.plane_shape_valid(t, c)) cands.push_back({t, c});
         }
       if (cands.empty()) bd::fail(BD_ERR_APPLICATION, "no valid plane shape");
       BD_HIP(hipMemsetAsync(o.d_x.p, 0, size_t(o.n) * m * 8, o.stream));
@@ -471,14 +468,42 @@ extern "C" bd_status bd_gamma_op_plane_shape(bd_gamma_op* h, int32_t* inout2, in
     const bd::StickPlan& p = o.plan;
     if (set) {
       const int t = inout2[0], c = inout2[1];
-      if (c < 1 || c > p.n1 || !bd::fft_fits_threads(p.Z, p.ry, t) || !bd::fft_fits_threads(c, p.rz, t) ||
-          o.plane_lds_bytes(c) > 160 * 1024)
-        bd::fail(BD_ERR_INVALID_PARAMETER, "invalid plane shape");
+      if (!o.plane_shape_valid(t, c)) bd::fail(BD_ERR_INVALID_PARAMETER, "invalid plane shape");
       o.plane_threads = t;
       o.plan.chunk = c;
     } else {
       inout2[0] = o.plane_threads;
       inout2[1] = p.chunk;
+    }
+  });
+}
+
+// Read (set = 0) or set the plane pass: 0 = the fused one-workgroup plane
+// kernel (only where the plane fits), zb > 0 = the split pass with zb
+// z-lines per y-transform batch. Setting resets the shape to 256 threads.
+extern "C" bd_status bd_gamma_op_plane_split(bd_gamma_op* h, int32_t* inout, int32_t set) {
+  BD_GUARD({
+    if (!h || !inout) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    bd::GammaOp& o = h->op;
+    if (o.kb && o.fused) bd::fail(BD_ERR_INVALID_PARAMETER, "op runs the fused k-point kernel");
+    if (!set) {
+      inout[0] = o.plan.zb;
+      return BD_OK;
+    }
+    if (inout[0] < 0) bd::fail(BD_ERR_INVALID_PARAMETER, "z-line batch must be >= 0");
+    if (inout[0] == 0) {
+      const int zb = o.plan.zb;
+      o.plan.zb = 0;
+      int c = std::min(16, o.plan.n1);
+      while (c > 1 && !o.plane_shape_valid(256, c)) --c;
+      if (!o.plane_shape_valid(256, c)) {
+        o.plan.zb = zb;
+        bd::fail(BD_ERR_INVALID_PARAMETER, "the plane does not fit one workgroup: the split pass is required");
+      }
+      o.plane_threads = 256;
+      o.plan.chunk = c;
+    } else {
+      o.enable_split(inout[0]);
     }
   });
 }

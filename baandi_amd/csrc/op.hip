@@ -104,7 +104,7 @@ __global__ __launch_bounds__(kThreads) void k_stick_forward(
     const double* __restrict__ x, int64_t ldx, double* __restrict__ y, int64_t ldy, int m,
     int n0, int T, int Spad, const int* __restrict__ tile_off, const int4* __restrict__ entries,
     const c2<double>* __restrict__ roots_x, RadixPlan rx, const c2<F>* __restrict__ sticks,
-    const double* __restrict__ kinetic, double inv_n) {
+    const double* __restrict__ kinetic, double inv_n, int accumulate) {
   extern __shared__ char smem[];
   c2<F>* L = reinterpret_cast<c2<F>*>(smem);
   c2<F>* rt = L + T * n0;
@@ -129,10 +129,17 @@ __global__ __launch_bounds__(kThreads) void k_stick_forward(
     const int i = en.x;
     const c2<F> lp = L[en.y];
     const c2<double> hp = {double(lp.x) * inv_n, double(lp.y) * inv_n};
+    // accumulate: y += the local part alone (the warm start's dV X);
+    // otherwise y = local part + T_G x.
     if (i == 0) {
-      const double k0 = kinetic[0];
-      ya[0] = hp.x + k0 * xa[0];
-      if (hb) yb[0] = hp.y + k0 * xb[0];
+      if (accumulate) {
+        ya[0] += hp.x;
+        if (hb) yb[0] += hp.y;
+      } else {
+        const double k0 = kinetic[0];
+        ya[0] = hp.x + k0 * xa[0];
+        if (hb) yb[0] = hp.y + k0 * xb[0];
+      }
     } else {
       const c2<F> lm = L[en.z];
       const c2<double> hm = {double(lm.x) * inv_n, double(lm.y) * inv_n};
@@ -140,12 +147,21 @@ __global__ __launch_bounds__(kThreads) void k_stick_forward(
       const double er = 0.5 * (hp.x + hm.x), ei = 0.5 * (hp.y - hm.y);
       const double orr = 0.5 * (hp.x - hm.x), oi = 0.5 * (hp.y + hm.y);
       const int r1 = 2 * i - 1, r2 = 2 * i;
-      const double k = kinetic[r1];
-      ya[r1] = er * kSqrt2 + k * xa[r1];
-      ya[r2] = ei * kSqrt2 + k * xa[r2];
-      if (hb) {
-        yb[r1] = oi * kSqrt2 + k * xb[r1];
-        yb[r2] = -orr * kSqrt2 + k * xb[r2];
+      if (accumulate) {
+        ya[r1] += er * kSqrt2;
+        ya[r2] += ei * kSqrt2;
+        if (hb) {
+          yb[r1] += oi * kSqrt2;
+          yb[r2] += -orr * kSqrt2;
+        }
+      } else {
+        const double k = kinetic[r1];
+        ya[r1] = er * kSqrt2 + k * xa[r1];
+        ya[r2] = ei * kSqrt2 + k * xa[r2];
+        if (hb) {
+          yb[r1] = oi * kSqrt2 + k * xb[r1];
+          yb[r2] = -orr * kSqrt2 + k * xb[r2];
+        }
       }
     }
   }
@@ -222,7 +238,7 @@ __global__ __launch_bounds__(NT) void k_plane(
     const int2* __restrict__ stick_yz, const int* __restrict__ z_of_zc,
     const c2<double>* __restrict__ roots, RadixPlan ry, RadixPlan rz,
     const F* __restrict__ potential, int pairs, const double* __restrict__ weights,
-    double inv_volume, double* __restrict__ rho) {
+    double inv_volume, double* __restrict__ rho, c2<F>* __restrict__ keep) {
   extern __shared__ char smem[];
   c2<F>* P = reinterpret_cast<c2<F>*>(smem);  // [Z][n1]
   c2<F>* Rb = P + Z * n1;                     // [chunk][n2]
@@ -274,10 +290,13 @@ __global__ __launch_bounds__(NT) void k_plane(
         __syncthreads();
       } else {
         double* out = rho + (int64_t(xi) * n1 + y0) * n2;
+        // With `keep`: the pair's real-space field too, [pair][x][y][z].
+        c2<F>* kp = keep ? keep + int64_t(t) * n0 * n1 * n2 + (int64_t(xi) * n1 + y0) * n2 : nullptr;
         for (int i = threadIdx.x; i < c * n2; i += blockDim.x) {
           const double re = double(Rb[i].x), im = double(Rb[i].y);
           const double d = wa * re * re + wb * im * im;
           out[i] = t == tbeg ? d : out[i] + d;
+          if (kp) kp[i] = Rb[i];
         }
         __syncthreads();
       }
@@ -288,6 +307,142 @@ __global__ __launch_bounds__(NT) void k_plane(
         const int2 yz = stick_yz[s];
         if (yz.x >= 0) col[s] = P[yz.y * n1 + yz.x];
       }
+    }
+  }
+}
+
+// Kept-field pass of the warm start (apply_local_change,
+// distributed_real_gamma.rs:1370-1438): from a pair's kept real-space field
+// psi(r) (written by the band density of the same columns), dV psi, forward z
+// per row chunk, forward y, back to the sticks. Only the r -> G half runs.
+template <typename F, int NT>
+__global__ __launch_bounds__(NT) void k_plane_kept(
+    c2<F>* __restrict__ sticks, int n0, int n1, int n2, int Spad, int Z, int chunk,
+    const int2* __restrict__ stick_yz, const int* __restrict__ z_of_zc, const c2<double>* __restrict__ roots,
+    RadixPlan ry, RadixPlan rz, const F* __restrict__ dv, const c2<F>* __restrict__ kept) {
+  extern __shared__ char smem[];
+  c2<F>* P = reinterpret_cast<c2<F>*>(smem);  // [Z][n1]
+  c2<F>* Rb = P + Z * n1;                     // [chunk][n2]
+  c2<F>* rty = Rb + chunk * n2;
+  c2<F>* rtz = rty + n1;
+  const int t = blockIdx.x, xi = blockIdx.y;
+  stage_roots(rty, roots + n0, n1);
+  stage_roots(rtz, roots + n0 + n1, n2);
+  const c2<F>* kp = kept + int64_t(t) * n0 * n1 * n2 + int64_t(xi) * n1 * n2;
+  for (int y0 = 0; y0 < n1; y0 += chunk) {
+    const int c = min(chunk, n1 - y0);
+    const F* v = dv + (int64_t(xi) * n1 + y0) * n2;
+    __syncthreads();
+    for (int i = threadIdx.x; i < c * n2; i += blockDim.x) Rb[i] = cscale(kp[int64_t(y0) * n2 + i], v[i]);
+    __syncthreads();
+    lds_fft<false, F, NT>(Rb, c, n2, 1, rz, rtz);
+    for (int i = threadIdx.x; i < c * Z; i += blockDim.x) {
+      const int r = i / Z, zc = i - r * Z;
+      P[zc * n1 + y0 + r] = Rb[r * n2 + z_of_zc[zc]];
+    }
+  }
+  __syncthreads();
+  lds_fft<false, F, NT>(P, Z, n1, 1, ry, rty);
+  c2<F>* col = sticks + int64_t(t) * n0 * Spad + int64_t(xi) * Spad;
+  for (int s = threadIdx.x; s < Spad; s += blockDim.x) {
+    const int2 yz = stick_yz[s];
+    if (yz.x >= 0) col[s] = P[yz.y * n1 + yz.x];
+  }
+}
+
+// Split plane pass, for planes whose Z x n1 values do not fit one workgroup's
+// LDS (the Si512 grid's 144-point planes in FP64): the same transforms as
+// k_plane through a plane buffer [pair][x][Z][n1] in HBM. k_plane_y does the
+// y transforms of zb z-lines at a time (inverse: sticks -> buffer; forward:
+// buffer -> sticks); k_plane_z the z transforms of a chunk of y rows (MODE 0:
+// inverse, * V(r), forward, back into the buffer; MODE 1: inverse and the
+// density sum). `pair0` is the first pair of the batch the buffer holds.
+template <bool INV, typename F, int NT>
+__global__ __launch_bounds__(NT) void k_plane_y(c2<F>* __restrict__ sticks, int n0, int n1, int Spad, int Z, int zb,
+                                                const int2* __restrict__ stick_yz, const c2<double>* __restrict__ roots,
+                                                RadixPlan ry, int pair0, c2<F>* __restrict__ planes) {
+  extern __shared__ char smem[];
+  c2<F>* P = reinterpret_cast<c2<F>*>(smem);  // [zb][n1]
+  c2<F>* rty = P + zb * n1;
+  const int tl = blockIdx.x, xi = blockIdx.y, z0 = blockIdx.z * zb;
+  const int nz = min(zb, Z - z0);
+  stage_roots(rty, roots + n0, n1);
+  c2<F>* col = sticks + (int64_t(pair0 + tl) * n0 + xi) * Spad;
+  c2<F>* pl = planes + (int64_t(tl) * n0 + xi) * Z * n1 + int64_t(z0) * n1;
+  if (INV) {
+    for (int i = threadIdx.x; i < nz * n1; i += blockDim.x) P[i] = {F(0), F(0)};
+    __syncthreads();
+    for (int s = threadIdx.x; s < Spad; s += blockDim.x) {
+      const int2 yz = stick_yz[s];
+      if (yz.x >= 0 && yz.y >= z0 && yz.y < z0 + nz) P[(yz.y - z0) * n1 + yz.x] = col[s];
+    }
+    __syncthreads();
+    lds_fft<true, F, NT>(P, nz, n1, 1, ry, rty);
+    for (int i = threadIdx.x; i < nz * n1; i += blockDim.x) pl[i] = P[i];
+  } else {
+    for (int i = threadIdx.x; i < nz * n1; i += blockDim.x) P[i] = pl[i];
+    __syncthreads();
+    lds_fft<false, F, NT>(P, nz, n1, 1, ry, rty);
+    for (int s = threadIdx.x; s < Spad; s += blockDim.x) {
+      const int2 yz = stick_yz[s];
+      if (yz.x >= 0 && yz.y >= z0 && yz.y < z0 + nz) col[s] = P[(yz.y - z0) * n1 + yz.x];
+    }
+  }
+}
+
+// MODE 0: grid (pairs of the batch, n0, y chunks). MODE 1: grid (n0, y chunks,
+// groups); group g sums pairs g, g + groups, ... of the batch into its partial
+// density rho + g N, after the earlier batches (`first`: the batch that
+// writes instead of adding).
+template <int MODE, typename F, int NT>
+__global__ __launch_bounds__(NT) void k_plane_z(c2<F>* __restrict__ planes, int n0, int n1, int n2, int Z, int chunk,
+                                                const int* __restrict__ z_of_zc, const c2<double>* __restrict__ roots,
+                                                RadixPlan rz, const F* __restrict__ potential, int batch_pairs,
+                                                int pair0, int total_pairs, const double* __restrict__ weights,
+                                                double inv_volume, int first, double* __restrict__ rho) {
+  extern __shared__ char smem[];
+  c2<F>* Rb = reinterpret_cast<c2<F>*>(smem);  // [chunk][n2]
+  c2<F>* rtz = Rb + chunk * n2;
+  const int xi = MODE == 0 ? blockIdx.y : blockIdx.x;
+  const int y0 = (MODE == 0 ? blockIdx.z : blockIdx.y) * chunk;
+  const int c = min(chunk, n1 - y0);
+  stage_roots(rtz, roots + n0 + n1, n2);
+  const int groups = MODE == 0 ? 1 : gridDim.z;
+  const int tbeg = MODE == 0 ? blockIdx.x : blockIdx.z;
+  const int tend = MODE == 0 ? tbeg + 1 : batch_pairs;
+  if (MODE == 1) rho += int64_t(blockIdx.z) * n0 * n1 * n2;
+  for (int tl = tbeg; tl < tend; tl += groups) {
+    c2<F>* pl = planes + (int64_t(tl) * n0 + xi) * Z * n1;
+    for (int i = threadIdx.x; i < c * n2; i += blockDim.x) Rb[i] = {F(0), F(0)};
+    __syncthreads();
+    for (int i = threadIdx.x; i < c * Z; i += blockDim.x) {
+      const int zc = i / c, r = i - zc * c;
+      Rb[r * n2 + z_of_zc[zc]] = pl[zc * n1 + y0 + r];
+    }
+    __syncthreads();
+    lds_fft<true, F, NT>(Rb, c, n2, 1, rz, rtz);
+    if (MODE == 0) {
+      const F* v = potential + (int64_t(xi) * n1 + y0) * n2;
+      for (int i = threadIdx.x; i < c * n2; i += blockDim.x) Rb[i] = cscale(Rb[i], v[i]);
+      __syncthreads();
+      lds_fft<false, F, NT>(Rb, c, n2, 1, rz, rtz);
+      for (int i = threadIdx.x; i < c * Z; i += blockDim.x) {
+        const int zc = i / c, r = i - zc * c;
+        pl[zc * n1 + y0 + r] = Rb[r * n2 + z_of_zc[zc]];
+      }
+      __syncthreads();
+    } else {
+      const int t = pair0 + tl;
+      const double wa = weights[2 * t] * inv_volume;
+      const double wb = (2 * t + 1 < 2 * total_pairs) ? weights[2 * t + 1] * inv_volume : 0.0;
+      double* out = rho + (int64_t(xi) * n1 + y0) * n2;
+      const bool write = first && tl == tbeg;
+      for (int i = threadIdx.x; i < c * n2; i += blockDim.x) {
+        const double re = double(Rb[i].x), im = double(Rb[i].y);
+        const double d = wa * re * re + wb * im * im;
+        out[i] = write ? d : out[i] + d;
+      }
+      __syncthreads();
     }
   }
 }
@@ -388,7 +543,6 @@ StickPlan build_stick_plan(const System& s) {
   p.T = 32;
   while (p.T > 1 && !fft_fits(p.T, p.rx)) p.T /= 2;
   if (!fft_fits(p.T, p.rx)) fail(BD_ERR_INVALID_PARAMETER, "x FFT does not fit a workgroup");
-  if (!fft_fits(p.Z, p.ry)) fail(BD_ERR_INVALID_PARAMETER, "y FFT plane does not fit a workgroup");
   p.chunk = 16;
   while (p.chunk > 1 && !fft_fits(p.chunk, p.rz)) p.chunk /= 2;
   p.chunk = std::min(p.chunk, p.n1);
@@ -478,7 +632,6 @@ StickPlan build_k_stick_plan(const System& s, const KBasis& kb) {
   p.T = 32;
   while (p.T > 1 && !fft_fits(p.T, p.rx)) p.T /= 2;
   if (!fft_fits(p.T, p.rx)) fail(BD_ERR_INVALID_PARAMETER, "x FFT does not fit a workgroup");
-  if (!fft_fits(p.Z, p.ry)) fail(BD_ERR_INVALID_PARAMETER, "y FFT plane does not fit a workgroup");
   p.chunk = 16;
   while (p.chunk > 1 && !fft_fits(p.chunk, p.rz)) p.chunk /= 2;
   p.chunk = std::min(p.chunk, p.n1);
@@ -510,6 +663,54 @@ size_t GammaOp::plane_lds_bytes(int chunk) const {
   return size_t(plan.Z * plan.n1 + chunk * plan.n2 + plan.n1 + plan.n2) * e;
 }
 
+size_t GammaOp::elem_bytes() const { return precision == BD_MIXED ? sizeof(c2<float>) : sizeof(c2<double>); }
+
+// A (threads, chunk) shape of the plane pass: the fused k_plane holds the
+// whole Z x n1 plane; the split pass holds zb z-lines (y transforms) or a
+// chunk of y rows (z transforms) at a time.
+bool GammaOp::plane_shape_valid(int threads, int chunk) const {
+  const StickPlan& p = plan;
+  const size_t e = elem_bytes(), lim = 160 * 1024;
+  if ((threads != 256 && threads != 512) || chunk < 1 || chunk > p.n1) return false;
+  if (!fft_fits_threads(chunk, p.rz, threads)) return false;
+  if (p.zb > 0)
+    return fft_fits_threads(p.zb, p.ry, threads) && size_t(p.zb * p.n1 + p.n1) * e <= lim &&
+           size_t(chunk * p.n2 + p.n2) * e <= lim;
+  return fft_fits_threads(p.Z, p.ry, threads) && plane_lds_bytes(chunk) <= lim;
+}
+
+// Switch to the split plane pass with zb z-lines per y batch (0: the largest
+// that fits), and size its plane buffer: up to ~2 GiB of pairs per batch, at
+// least kDensityGroups pairs (or all).
+void GammaOp::enable_split(int zb) {
+  const StickPlan& p = plan;
+  const size_t e = elem_bytes(), lim = 160 * 1024;
+  auto fits = [&](int z) {
+    return z >= 1 && z <= p.Z && fft_fits_threads(z, p.ry, 256) && size_t(z * p.n1 + p.n1) * e <= lim;
+  };
+  if (zb == 0) {
+    zb = p.Z;
+    while (zb > 1 && !fits(zb)) --zb;
+  }
+  if (!fits(zb)) fail(BD_ERR_INVALID_PARAMETER, "no y-line batch of the split plane pass fits a workgroup");
+  const size_t plane = size_t(p.n0) * p.Z * p.n1 * e;
+  const int pairs = slots(capacity);
+  int batch = int(std::max<size_t>(1, (size_t(2) << 30) / plane));
+  batch = std::min(pairs, std::max(batch, kDensityGroups));
+  if (d_planes.n < (size_t(batch) * plane + 15) / 16) {
+    DevBuf<c2<double>> fresh;
+    fresh.alloc((size_t(batch) * plane + 15) / 16);
+    std::swap(d_planes.p, fresh.p);
+    std::swap(d_planes.n, fresh.n);
+  }
+  plan.zb = zb;
+  plane_batch = batch;
+  plane_threads = 256;
+  int c = std::min(plan.chunk, p.n1);
+  while (c > 1 && !plane_shape_valid(256, c)) --c;
+  plan.chunk = c;
+}
+
 // The k kernels move a coefficient's (Re, Im) as one 16-byte access.
 static bool complex_aligned(const double* p, int64_t ld) {
   return (reinterpret_cast<uintptr_t>(p) & 15) == 0 && (ld & 1) == 0;
@@ -527,7 +728,17 @@ static void allow_lds(const StickPlan& plan) {
   for (const void* f : {reinterpret_cast<const void*>(&k_plane<0, F, 256>),
                         reinterpret_cast<const void*>(&k_plane<1, F, 256>),
                         reinterpret_cast<const void*>(&k_plane<0, F, 512>),
-                        reinterpret_cast<const void*>(&k_plane<1, F, 512>)})
+                        reinterpret_cast<const void*>(&k_plane<1, F, 512>),
+                        reinterpret_cast<const void*>(&k_plane_y<true, F, 256>),
+                        reinterpret_cast<const void*>(&k_plane_y<false, F, 256>),
+                        reinterpret_cast<const void*>(&k_plane_y<true, F, 512>),
+                        reinterpret_cast<const void*>(&k_plane_y<false, F, 512>),
+                        reinterpret_cast<const void*>(&k_plane_z<0, F, 256>),
+                        reinterpret_cast<const void*>(&k_plane_z<1, F, 256>),
+                        reinterpret_cast<const void*>(&k_plane_z<0, F, 512>),
+                        reinterpret_cast<const void*>(&k_plane_z<1, F, 512>),
+                        reinterpret_cast<const void*>(&k_plane_kept<F, 256>),
+                        reinterpret_cast<const void*>(&k_plane_kept<F, 512>)})
     BD_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
   // The same largest value for every op: the attribute is per kernel and
   // process-wide, so a per-plan size set by one op could undercut another's.
@@ -561,8 +772,7 @@ GammaOp::GammaOp(const System& s, const KBasis* k, int cap, int prec) : sys(&s),
   plan = kb ? build_k_stick_plan(s, *kb) : build_stick_plan(s);
   size_t lds_max = size_t(prop.sharedMemPerBlock);
   const size_t elem = precision == BD_MIXED ? sizeof(c2<float>) : sizeof(c2<double>);
-  if (stick_lds(plan, elem) > lds_max || plane_lds(plan, elem) > lds_max)
-    fail(BD_ERR_INVALID_PARAMETER, "FFT workgroup LDS exceeds the device limit");
+  if (stick_lds(plan, elem) > lds_max) fail(BD_ERR_INVALID_PARAMETER, "x FFT tile exceeds the device LDS");
   if (precision == BD_MIXED) allow_lds<float>(plan);
   else allow_lds<double>(plan);
   BD_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -600,6 +810,8 @@ GammaOp::GammaOp(const System& s, const KBasis* k, int cap, int prec) : sys(&s),
   }
   const int pairs = slots(cap);
   d_sticks.alloc(size_t(pairs) * plan.n0 * plan.Spad);
+  // Planes that do not fit one workgroup go through the split pass.
+  if (!fft_fits(plan.Z, plan.ry) || plane_lds(plan, elem) > lds_max) enable_split(0);
   // Host-pointer entry points stage through these (no allocation after create).
   d_x.alloc(size_t(n) * cap);
   d_y.alloc(size_t(n) * cap);
@@ -650,6 +862,35 @@ void GammaOp::refresh_potential32() {
   BD_HIP(hipGetLastError());
 }
 
+// The split plane pass over all `pairs` FFT slots, plane_batch at a time:
+// MODE 0 transforms the sticks in place with V(r); MODE 1 sums the band
+// density into kDensityGroups-style group partials at rho_part.
+template <int MODE, typename F, int NT>
+void GammaOp::split_pass(int pairs, const F* pot, const double* w, double* rho_part) {
+  const StickPlan& p = plan;
+  const size_t e = sizeof(c2<F>);
+  c2<F>* sticks = reinterpret_cast<c2<F>*>(d_sticks.p);
+  c2<F>* planes = reinterpret_cast<c2<F>*>(d_planes.p);
+  const size_t ly = size_t(p.zb * p.n1 + p.n1) * e, lz = size_t(p.chunk * p.n2 + p.n2) * e;
+  const unsigned zbatches = unsigned((p.Z + p.zb - 1) / p.zb), ychunks = unsigned((p.n1 + p.chunk - 1) / p.chunk);
+  const int groups = std::min(pairs, kDensityGroups);
+  for (int t0 = 0; t0 < pairs; t0 += plane_batch) {
+    const int bp = std::min(plane_batch, pairs - t0);
+    hipLaunchKernelGGL((k_plane_y<true, F, NT>), dim3(bp, p.n0, zbatches), dim3(NT), ly, stream, sticks, p.n0, p.n1,
+                       p.Spad, p.Z, p.zb, d_stick_yz.p, d_roots.p, p.ry, t0, planes);
+    if (MODE == 0) {
+      hipLaunchKernelGGL((k_plane_z<0, F, NT>), dim3(bp, p.n0, ychunks), dim3(NT), lz, stream, planes, p.n0, p.n1,
+                         p.n2, p.Z, p.chunk, d_zc.p, d_roots.p, p.rz, pot, bp, t0, pairs, nullptr, 0.0, 0, nullptr);
+      hipLaunchKernelGGL((k_plane_y<false, F, NT>), dim3(bp, p.n0, zbatches), dim3(NT), ly, stream, sticks, p.n0,
+                         p.n1, p.Spad, p.Z, p.zb, d_stick_yz.p, d_roots.p, p.ry, t0, planes);
+    } else {
+      hipLaunchKernelGGL((k_plane_z<1, F, NT>), dim3(p.n0, ychunks, groups), dim3(NT), lz, stream, planes, p.n0,
+                         p.n1, p.n2, p.Z, p.chunk, d_zc.p, d_roots.p, p.rz, nullptr, bp, t0, pairs, w,
+                         1.0 / sys->volume, t0 == 0 ? 1 : 0, rho_part);
+    }
+  }
+}
+
 template <typename F>
 void GammaOp::local_apply(const double* x, int64_t ldx, int m, double* y, int64_t ldy) {
   const StickPlan& p = plan;
@@ -667,14 +908,18 @@ void GammaOp::local_apply(const double* x, int64_t ldx, int m, double* y, int64_
     hipLaunchKernelGGL(k_stick_inverse<F>, dim3(p.ntiles, pairs), dim3(kThreads), stick_lds(p, e), stream, x,
                        ldx, m, p.n0, p.T, p.Spad, d_tile_off.p, d_entries.p, d_roots.p, p.rx, sticks);
   if (profiling) BD_HIP(hipEventRecord(ev[1], stream));
-  if (plane_threads == 512)
+  if (p.zb > 0) {
+    if (plane_threads == 512) split_pass<0, F, 512>(pairs, pot, nullptr, nullptr);
+    else split_pass<0, F, 256>(pairs, pot, nullptr, nullptr);
+  } else if (plane_threads == 512) {
     hipLaunchKernelGGL((k_plane<0, F, 512>), dim3(pairs, p.n0), dim3(512), plane_lds(p, e), stream, sticks, p.n0,
                        p.n1, p.n2, p.Spad, p.Z, p.chunk, d_stick_yz.p, d_zc.p, d_roots.p, p.ry, p.rz, pot, pairs,
-                       nullptr, 0.0, nullptr);
-  else
+                       nullptr, 0.0, nullptr, nullptr);
+  } else {
     hipLaunchKernelGGL((k_plane<0, F, 256>), dim3(pairs, p.n0), dim3(256), plane_lds(p, e), stream, sticks, p.n0,
                        p.n1, p.n2, p.Spad, p.Z, p.chunk, d_stick_yz.p, d_zc.p, d_roots.p, p.ry, p.rz, pot, pairs,
-                       nullptr, 0.0, nullptr);
+                       nullptr, 0.0, nullptr, nullptr);
+  }
   if (profiling) BD_HIP(hipEventRecord(ev[2], stream));
   if (kb)
     hipLaunchKernelGGL(k_kstick_forward<F>, dim3(p.ntiles, pairs), dim3(kThreads), stick_lds(p, e), stream, x,
@@ -683,7 +928,7 @@ void GammaOp::local_apply(const double* x, int64_t ldx, int m, double* y, int64_
   else
     hipLaunchKernelGGL(k_stick_forward<F>, dim3(p.ntiles, pairs), dim3(kThreads), stick_lds(p, e), stream, x,
                        ldx, y, ldy, m, p.n0, p.T, p.Spad, d_tile_off.p, d_entries.p, d_roots.p, p.rx,
-                       sticks, d_kinetic.p, 1.0 / double(N));
+                       sticks, d_kinetic.p, 1.0 / double(N), 0);
   if (profiling) BD_HIP(hipEventRecord(ev[3], stream));
 }
 
@@ -700,16 +945,75 @@ void GammaOp::density(const double* x, int64_t ldx, int m, const double* w, doub
     hipLaunchKernelGGL(k_stick_inverse<F>, dim3(p.ntiles, pairs), dim3(kThreads), stick_lds(p, e), stream, x,
                        ldx, m, p.n0, p.T, p.Spad, d_tile_off.p, d_entries.p, d_roots.p, p.rx, sticks);
   const int groups = std::min(pairs, kDensityGroups);
-  if (plane_threads == 512)
+  // Warm start: keep the pairs' real-space fields (Gamma, fused plane pass).
+  c2<F>* keep = kept_on ? reinterpret_cast<c2<F>*>(d_kept.p) : nullptr;
+  kept_valid = keep != nullptr;
+  kept_x = x;
+  kept_ldx = ldx;
+  kept_m = m;
+  if (p.zb > 0) {
+    if (plane_threads == 512) split_pass<1, F, 512>(pairs, nullptr, w, d_rho_part.p);
+    else split_pass<1, F, 256>(pairs, nullptr, w, d_rho_part.p);
+  } else if (plane_threads == 512)
     hipLaunchKernelGGL((k_plane<1, F, 512>), dim3(p.n0, groups), dim3(512), plane_lds(p, e), stream, sticks, p.n0,
                        p.n1, p.n2, p.Spad, p.Z, p.chunk, d_stick_yz.p, d_zc.p, d_roots.p, p.ry, p.rz, nullptr,
-                       pairs, w, 1.0 / sys->volume, d_rho_part.p);
+                       pairs, w, 1.0 / sys->volume, d_rho_part.p, keep);
   else
     hipLaunchKernelGGL((k_plane<1, F, 256>), dim3(p.n0, groups), dim3(256), plane_lds(p, e), stream, sticks, p.n0,
                        p.n1, p.n2, p.Spad, p.Z, p.chunk, d_stick_yz.p, d_zc.p, d_roots.p, p.ry, p.rz, nullptr,
-                       pairs, w, 1.0 / sys->volume, d_rho_part.p);
+                       pairs, w, 1.0 / sys->volume, d_rho_part.p, keep);
   hipLaunchKernelGGL(k_sum_groups, dim3(unsigned((N + 255) / 256)), dim3(256), 0, stream, d_rho_part.p, N,
                      groups, rho);
+}
+
+// Keep (on = true) the real-space pair fields of every later band density,
+// for apply_kept: pairs_cap x N values of the op's precision.
+void GammaOp::enable_kept(bool on) {
+  if (on && (kb || plan.zb > 0))
+    fail(BD_ERR_INVALID_PARAMETER, "kept fields: Gamma ops on the one-workgroup plane pass only");
+  if (on && !d_kept.p) d_kept.alloc((size_t(slots(capacity)) * N * elem_bytes() + 15) / 16);
+  kept_on = on;
+  kept_valid = false;
+}
+
+// y += dV X, from the fields kept by the last band density, which must have
+// been of these same columns (x, ldx, m): the r -> G half of the local apply
+// (apply_local_change, distributed_real_gamma.rs:1370-1438).
+template <typename F>
+void GammaOp::kept_apply(const double* x, int64_t ldx, int m, const double* dv, double* y, int64_t ldy) {
+  const StickPlan& p = plan;
+  const int pairs = slots(m);
+  const size_t e = sizeof(c2<F>);
+  c2<F>* sticks = reinterpret_cast<c2<F>*>(d_sticks.p);
+  const F* dvf;
+  if constexpr (std::is_same_v<F, float>) {
+    hipLaunchKernelGGL(k_to_float, dim3(unsigned((N + 255) / 256)), dim3(256), 0, stream, dv, N, d_potential32.p);
+    dvf = d_potential32.p;
+  } else {
+    dvf = dv;
+  }
+  const c2<F>* kept = reinterpret_cast<const c2<F>*>(d_kept.p);
+  if (plane_threads == 512)
+    hipLaunchKernelGGL((k_plane_kept<F, 512>), dim3(pairs, p.n0), dim3(512), plane_lds(p, e), stream, sticks, p.n0,
+                       p.n1, p.n2, p.Spad, p.Z, p.chunk, d_stick_yz.p, d_zc.p, d_roots.p, p.ry, p.rz, dvf, kept);
+  else
+    hipLaunchKernelGGL((k_plane_kept<F, 256>), dim3(pairs, p.n0), dim3(256), plane_lds(p, e), stream, sticks, p.n0,
+                       p.n1, p.n2, p.Spad, p.Z, p.chunk, d_stick_yz.p, d_zc.p, d_roots.p, p.ry, p.rz, dvf, kept);
+  hipLaunchKernelGGL(k_stick_forward<F>, dim3(p.ntiles, pairs), dim3(kThreads), stick_lds(p, e), stream, x, ldx, y,
+                     ldy, m, p.n0, p.T, p.Spad, d_tile_off.p, d_entries.p, d_roots.p, p.rx, sticks, d_kinetic.p,
+                     1.0 / double(N), 1);
+  if constexpr (std::is_same_v<F, float>) refresh_potential32();  // the float copy is V(r) again
+}
+
+void GammaOp::apply_kept(const double* x, int64_t ldx, int m, const double* dv, double* y, int64_t ldy) {
+  if (!kept_on) fail(BD_ERR_INVALID_PARAMETER, "kept fields are off (bd_gamma_op_keep_fields)");
+  if (!kept_valid || x != kept_x || ldx != kept_ldx || m != kept_m)
+    fail(BD_ERR_INVALID_PARAMETER, "no kept fields of these columns (band density of the same X first)");
+  if (ldy < n) fail(BD_ERR_INVALID_PARAMETER, "leading dimension smaller than rows");
+  if (m == 0) return;
+  if (precision == BD_MIXED) kept_apply<float>(x, ldx, m, dv, y, ldy);
+  else kept_apply<double>(x, ldx, m, dv, y, ldy);
+  BD_HIP(hipGetLastError());
 }
 
 void GammaOp::projections(const double* x, int64_t ldx, int m) {
@@ -907,6 +1211,21 @@ extern "C" bd_status bd_gamma_op_band_density_dev(bd_gamma_op* op, const double*
     o.d_weights.upload(hw.data(), hw.size(), o.stream);
     o.band_density(x, ldx, m, o.d_weights.p, rho);
     BD_HIP(hipStreamSynchronize(o.stream));
+  });
+}
+
+extern "C" bd_status bd_gamma_op_keep_fields(bd_gamma_op* op, int32_t on) {
+  BD_GUARD({
+    if (!op) bd::fail(BD_ERR_INVALID_PARAMETER, "null op");
+    op->op.enable_kept(on != 0);
+  });
+}
+
+extern "C" bd_status bd_gamma_op_apply_kept_dev(bd_gamma_op* op, const double* x, int64_t ldx, int32_t m,
+                                                const double* delta_v_dev, double* y, int64_t ldy) {
+  BD_GUARD({
+    if (!op || (m > 0 && (!x || !y || !delta_v_dev))) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    op->op.apply_kept(x, ldx, m, delta_v_dev, y, ldy);
   });
 }
 

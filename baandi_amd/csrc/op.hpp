@@ -45,6 +45,7 @@ struct StickPlan {
   int Spad;     // ntiles * T
   int Z;        // occupied z planes
   int chunk;    // y rows per z-pass chunk in the plane kernel
+  int zb = 0;   // split plane pass: z lines per y-transform batch (0: fused k_plane)
   RadixPlan rx, ry, rz;
   std::vector<int> tile_off;       // ntiles+1 offsets into entries
   std::vector<int4> entries;       // Gamma: (rep, plus slot*n0+x, minus slot*n0+x, 0)
@@ -116,6 +117,26 @@ struct GammaOp {
   bool fused = false;
   FusedShape fshape;
   int plane_threads = 256;  // k_plane workgroup size (256 or 512; tuned with plan.chunk)
+  // Split plane pass (plan.zb > 0): plane buffer [pair][x][Z][n1] for
+  // plane_batch FFT slots at a time.
+  DevBuf<c2<double>> d_planes;
+  int plane_batch = 0;
+  size_t elem_bytes() const;
+  bool plane_shape_valid(int threads, int chunk) const;
+  void enable_split(int zb);
+  template <int MODE, typename F, int NT>
+  void split_pass(int pairs, const F* pot, const double* w, double* rho_part);
+  // Warm start (KeptHx): real-space pair fields of the last band density,
+  // [pair][N] of the op's precision, and the columns they belong to.
+  DevBuf<c2<double>> d_kept;
+  bool kept_on = false, kept_valid = false;
+  const double* kept_x = nullptr;
+  int64_t kept_ldx = 0;
+  int kept_m = 0;
+  void enable_kept(bool on);
+  void apply_kept(const double* x, int64_t ldx, int m, const double* dv, double* y, int64_t ldy);
+  template <typename F>
+  void kept_apply(const double* x, int64_t ldx, int m, const double* dv, double* y, int64_t ldy);
   DevBuf<int> d_fpos;
   KWork kwork(const double* x, int64_t ldx, int m, double* y, int64_t ldy, int first) const;
   size_t fused_lds(const FusedShape& f) const;

@@ -174,7 +174,7 @@ int64_t flat_of(const std::array<int32_t, 3>& m, const int dims[3]) {
 }
 }  // namespace
 
-System::System(const bd_system_desc& d) {
+System::System(const bd_system_desc& d, bool on_device) : device(on_device) {
   if (!(d.ecut_ha > 0.0) || d.natoms <= 0 || d.mesh_points < 4 || d.z_valence <= 0.0 || d.rc <= 0.0)
     fail(BD_ERR_INVALID_PARAMETER, "system descriptor: cutoff, atoms, mesh, charge or rc");
   // Cell: user rows are lattice vectors -> columns (crates/dft-geometry/src/lib.rs:46-49).
@@ -383,10 +383,11 @@ std::vector<std::array<int, 3>> channel_list(const System& s) {
 }
 
 // Complex projector columns at the momenta q (hamiltonian.rs:930-1000):
-// (-i)^l Y_lm(q^) beta_i(|q|)/sqrt(Omega) e^{-i q.tau}, atom-major, each
-// column n long. Radial transforms are cached per exact |q|^2 bit pattern.
-std::vector<Cplx> projector_columns(const System& s, const std::vector<std::array<double, 3>>& q,
-                                    const std::vector<double>& q2) {
+// (-i)^l Y_lm(q^) beta_i(|q|)/sqrt(Omega) e^{-i q.tau}. The atom-independent
+// part, channel-major (nc x n): radial transforms cached per exact |q|^2 bit
+// pattern.
+std::vector<Cplx> projector_table(const System& s, const std::vector<std::array<double, 3>>& q,
+                                  const std::vector<double>& q2) {
   const auto channels = channel_list(s);
   const int npj = int(s.projs.size());
   const int nc = int(channels.size());
@@ -425,6 +426,16 @@ std::vector<Cplx> projector_columns(const System& s, const std::vector<std::arra
       table[size_t(c) * n + g] = il * (real_ylm(l, m, dir) * radial[size_t(gslot[g]) * npj + i] / sv);
     }
   }
+  return table;
+}
+
+// The table times each atom's structure-factor phase e^{-i q.tau}: the
+// columns atom-major (hamiltonian.rs:930-1000).
+std::vector<Cplx> projector_columns(const System& s, const std::vector<std::array<double, 3>>& q,
+                                    const std::vector<double>& q2) {
+  const int nc = int(channel_list(s).size());
+  const int64_t n = int64_t(q2.size());
+  const std::vector<Cplx> table = projector_table(s, q, q2);
   std::vector<Cplx> full(size_t(n) * nc * s.atoms.size());
   for (size_t a = 0; a < s.atoms.size(); ++a) {
     for (int64_t g = 0; g < n; ++g) {
@@ -456,6 +467,26 @@ void System::build_projectors() {
   const int nc = int(channels.size());
   const int npj = int(projs.size());
   nproj = int64_t(nc) * int64_t(atoms.size());
+  // Block-diagonal D over equal (l, m) channels (hamiltonian.rs:988-995).
+  std::vector<double> block(size_t(nc) * nc, 0.0);
+  for (int a = 0; a < nc; ++a)
+    for (int b = 0; b < nc; ++b)
+      if (channels[a][1] == channels[b][1] && channels[a][2] == channels[b][2])
+        block[a * nc + b] = dij_ha[channels[a][0] * npj + channels[b][0]];
+  channel_count = nc;
+  block_d = block;
+  coupling.assign(size_t(nproj) * nproj, 0.0);
+  for (size_t at = 0; at < atoms.size(); ++at) {
+    blocks.push_back({int64_t(at * nc), nc});
+    for (int a = 0; a < nc; ++a)
+      for (int b = 0; b < nc; ++b)
+        coupling[(at * nc + b) * nproj + at * nc + a] = block[a * nc + b];  // column-major
+  }
+  if (device) {
+    // The atom loops (phases, ORC packing, the Davidson diagonal) on the GPU.
+    device_projectors(*this, projector_table(*this, cart, g2));
+    return;
+  }
   std::vector<Cplx> full = projector_columns(*this, cart, g2);
   // Per-atom columns packed to ORC rows (distributed_real_gamma.rs:363-402).
   projectors_orc.assign(size_t(n) * nproj, 0.0);
@@ -469,21 +500,6 @@ void System::build_projectors() {
       o[2 * i] = s2 * f[plus[i]].imag();
     }
   }
-  // Block-diagonal D over equal (l, m) channels (hamiltonian.rs:988-995).
-  coupling.assign(size_t(nproj) * nproj, 0.0);
-  std::vector<double> block(size_t(nc) * nc, 0.0);
-  for (int a = 0; a < nc; ++a)
-    for (int b = 0; b < nc; ++b)
-      if (channels[a][1] == channels[b][1] && channels[a][2] == channels[b][2])
-        block[a * nc + b] = dij_ha[channels[a][0] * npj + channels[b][0]];
-  for (size_t at = 0; at < atoms.size(); ++at) {
-    blocks.push_back({int64_t(at * nc), nc});
-    for (int a = 0; a < nc; ++a)
-      for (int b = 0; b < nc; ++b)
-        coupling[(at * nc + b) * nproj + at * nc + a] = block[a * nc + b];  // column-major
-  }
-  channel_count = nc;
-  block_d = block;
   // Davidson diagonal T + sum_ij D_ij Re(b_i b_j*) per representative
   // (refstart.rs:605-650), shared by its two ORC rows.
   diagonal_static.resize(n);
@@ -505,6 +521,7 @@ void System::build_ionic() {
   const size_t nd = dens_g2.size();
   vion_re.assign(nd, 0.0);
   vion_im.assign(nd, 0.0);
+  std::vector<double> vg(nd);
   for (size_t g = 0; g < nd; ++g) {
     double v;
     if (dens_miller[g] == std::array<int32_t, 3>{0, 0, 0}) {
@@ -522,6 +539,14 @@ void System::build_ionic() {
         cache.emplace(k, v);
       }
     }
+    vg[g] = v;
+  }
+  if (device) {
+    device_ionic(*this, vg);  // structure factors on the GPU
+    return;
+  }
+  for (size_t g = 0; g < nd; ++g) {
+    const double v = vg[g];
     double sr = 0.0, si = 0.0;
     for (auto& t : atoms) {
       double ang = dens_cart[g][0] * t[0] + dens_cart[g][1] * t[1] + dens_cart[g][2] * t[2];
@@ -615,7 +640,7 @@ KBasis::KBasis(const System& s, const double k_fractional[3]) {
 // ---------------------------------------------------------------------------
 struct bd_system {
   bd::System s;
-  explicit bd_system(const bd_system_desc& d) : s(d) {}
+  bd_system(const bd_system_desc& d, bool device) : s(d, device) {}
 };
 
 namespace bd {
@@ -625,7 +650,14 @@ const System& system_of(const bd_system* s) { return s->s; }
 extern "C" bd_status bd_system_create(const bd_system_desc* desc, bd_system** out) {
   BD_GUARD({
     if (!desc || !out) bd::fail(BD_ERR_INVALID_PARAMETER, "null descriptor");
-    *out = new bd_system(*desc);
+    *out = new bd_system(*desc, false);
+  });
+}
+
+extern "C" bd_status bd_system_create_dev(const bd_system_desc* desc, bd_system** out) {
+  BD_GUARD({
+    if (!desc || !out) bd::fail(BD_ERR_INVALID_PARAMETER, "null descriptor");
+    *out = new bd_system(*desc, true);
   });
 }
 

@@ -2,6 +2,7 @@
 #pragma once
 #include <array>
 #include <cstdint>
+#include <complex>
 #include <vector>
 
 #include "../../include/baandi_amd.h"
@@ -53,7 +54,10 @@ struct System {
   std::vector<double> vion_re, vion_im;
   double ewald = 0.0;  // Ha
 
-  explicit System(const bd_system_desc& d);
+  // on_device: the atom loops of the set-up (projector phases and ORC rows,
+  // the Davidson diagonal, the ionic structure factors) run on the GPU.
+  bool device = false;
+  System(const bd_system_desc& d, bool on_device);
 
  private:
   void build_pseudo(const bd_system_desc& d);
@@ -62,6 +66,13 @@ struct System {
   void build_ionic();
   void build_ewald();
 };
+
+// Device halves of the set-up (setup_dev.hip): from the atom-independent
+// projector table (nc x n complex, channel-major), the ORC projector rows and
+// the Davidson diagonal; from V_loc(|G|) per density-sphere G, the ionic
+// potential V_ion(G) = V_loc sum_a e^{-i G.tau_a}.
+void device_projectors(System& s, const std::vector<std::complex<double>>& table);
+void device_ionic(System& s, const std::vector<double>& vloc);
 
 // The Bloch basis of one k point of a System (k != 0 in general): the same
 // cell, grid, pseudopotential and D blocks; complex coefficients stored as

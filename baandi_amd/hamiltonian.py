@@ -28,7 +28,8 @@ class GammaSystem:
     """One Gamma cell with one synthetic species, built natively."""
 
     def __init__(self, lattice_rows, frac_positions, z_valence, ecut_ha, rc=1.0, mesh_points=801,
-                 betas=SYNTHETIC_BETAS, dij_ry=SYNTHETIC_DIJ_RY, grid_override=None):
+                 betas=SYNTHETIC_BETAS, dij_ry=SYNTHETIC_DIJ_RY, grid_override=None, device=False):
+        """``device``: run the set-up's atom loops on the GPU (bd_system_create_dev)."""
         lib = L.load()
         d = L.SystemDesc()
         for i in range(3):
@@ -55,7 +56,8 @@ class GammaSystem:
         for i in range(3):
             d.grid_override[i] = 0 if grid_override is None else int(grid_override[i])
         h = C.c_void_p()
-        L.check(lib.bd_system_create(C.byref(d), C.byref(h)))
+        create = lib.bd_system_create_dev if device else lib.bd_system_create
+        L.check(create(C.byref(d), C.byref(h)))
         self.handle = h
         info = L.SystemInfo()
         L.check(lib.bd_system_get_info(h, C.byref(info)))
@@ -173,6 +175,32 @@ class DistributedRealHamiltonian:
         L.check(L.load().bd_gamma_op_band_density_dev(self.handle, L.ptr(x), x.shape[1], x.shape[0],
                                                        L.ptr(w), L.ptr(out)))
         self.leave()
+
+    def keep_fields(self, on=True):
+        """Keep the real-space pair fields of every later band density (the
+        warm start's slabs; bd_gamma_op_keep_fields)."""
+        L.check(L.load().bd_gamma_op_keep_fields(self.handle, 1 if on else 0))
+
+    def apply_kept(self, x, delta_v, y):
+        """y += dV x from the fields kept by the last band density of this
+        same tensor x (apply_local_change, distributed_real_gamma.rs:1370-1438)."""
+        m, n = x.shape
+        assert n == self.global_len() and y.shape == x.shape and x.is_contiguous() and y.is_contiguous()
+        dv = delta_v.reshape(-1).contiguous()
+        self.enter()
+        L.check(L.load().bd_gamma_op_apply_kept_dev(self.handle, L.ptr(x), n, m, L.ptr(dv), L.ptr(y), n))
+        self.leave()
+
+    def plane_split(self):
+        """0 on the one-workgroup plane pass, else the split pass's z lines per batch."""
+        v = np.zeros(1, dtype=np.int32)
+        L.check(L.load().bd_gamma_op_plane_split(self.handle, L.ptr(v), 0))
+        return int(v[0])
+
+    def set_plane_split(self, zb):
+        """0: one workgroup per plane (where it fits); zb > 0: the split pass."""
+        v = np.array([zb], dtype=np.int32)
+        L.check(L.load().bd_gamma_op_plane_split(self.handle, L.ptr(v), 1))
 
     def nonlocal_band_energies(self, x):
         out = np.empty(x.shape[0])

@@ -17,7 +17,7 @@ BD_PRECOND_QE, BD_PRECOND_TPA, BD_PRECOND_MLP = 0, 1, 2
 
 # Every function include/baandi_amd.h declares.
 EXPORTS = [
-    "bd_last_error", "bd_system_create", "bd_system_destroy", "bd_system_get_info", "bd_system_get",
+    "bd_last_error", "bd_system_create", "bd_system_create_dev", "bd_system_destroy", "bd_system_get_info", "bd_system_get",
     "bd_gamma_op_create", "bd_gamma_op_destroy", "bd_gamma_op_set_potential",
     "bd_gamma_op_set_potential_dev", "bd_gamma_op_apply", "bd_gamma_op_apply_dev",
     "bd_gamma_op_band_density_dev", "bd_gamma_op_nonlocal_energies_dev", "bd_gamma_op_stream",
@@ -26,7 +26,10 @@ EXPORTS = [
     "bd_block_start_dev", "bd_block_step", "bd_block_step_async", "bd_block_finish", "bd_block_vectors_dev",
     "bd_k_op_create", "bd_k_op_info", "bd_k_op_get", "bd_k_batch_create", "bd_k_batch_destroy",
     "bd_k_batch_stream", "bd_k_batch_apply_dev", "bd_gamma_op_autotune", "bd_gamma_op_fused_shape",
-    "bd_gamma_op_plane_shape",
+    "bd_gamma_op_plane_shape", "bd_gamma_op_plane_split", "bd_gamma_op_keep_fields", "bd_gamma_op_apply_kept_dev",
+    "bd_sym_eigen_multi_from", "bd_field_create", "bd_field_destroy", "bd_field_stream", "bd_field_to_grid_dev",
+    "bd_field_from_grid_dev", "bd_field_potential_dev", "bd_field_residual_dev", "bd_field_gram_dev",
+    "bd_field_mix_dev", "bd_field_energies_dev",
     "bd_gemm_workspace", "bd_gemm_dev", "bd_mfma_probe_dev", "bd_mfma_f64_probe_dev",
     "bd_sym_eigen_workspace", "bd_sym_eigen_dev", "bd_inverse_cholesky_dev", "bd_inverse_cholesky_static_dev",
     "bd_mlp_param_count", "bd_mlp_blob_floats", "bd_mlp_quantize_dev", "bd_mlp_forward_dev",
@@ -64,6 +67,7 @@ class DavidsonOptions(C.Structure):
         ("first_step", C.c_int32), ("capacity", C.c_int32), ("max_iterations", C.c_int32),
         ("return_hx", C.c_int32), ("preconditioner", C.c_int32),
         ("mlp_blob", C.c_void_p), ("mlp_vbar", C.c_double),
+        ("profile", C.c_int32), ("ritz_changes_ha", C.c_void_p), ("ritz_values_ha", C.c_void_p),
     ]
 
 
@@ -73,6 +77,7 @@ class DavidsonRecord(C.Structure):
         ("repairs", C.c_int32), ("restarts", C.c_int32), ("all_converged", C.c_int32),
         ("seconds_total", C.c_double), ("seconds_apply", C.c_double), ("seconds_dense", C.c_double),
         ("mlp_fallback", C.c_int32), ("eigen_retries", C.c_int32),
+        ("ideal_resets", C.c_int32), ("periodic_refreshes", C.c_int32), ("guard_fallbacks", C.c_int32),
     ]
 
 
@@ -102,6 +107,7 @@ def load():
     lib.bd_last_error.restype = C.c_char_p
     sig = {
         "bd_system_create": [C.POINTER(SystemDesc), C.POINTER(P)],
+        "bd_system_create_dev": [C.POINTER(SystemDesc), C.POINTER(P)],
         "bd_system_get_info": [P, C.POINTER(SystemInfo)],
         "bd_system_get": [P, C.c_char_p, P, I64],
         "bd_gamma_op_create": [P, I32, I32, C.POINTER(P)],
@@ -113,6 +119,18 @@ def load():
         "bd_gamma_op_autotune": [P, I32, I32, P],
         "bd_gamma_op_fused_shape": [P, P, I32],
         "bd_gamma_op_plane_shape": [P, P, I32],
+        "bd_gamma_op_plane_split": [P, P, I32],
+        "bd_gamma_op_keep_fields": [P, I32],
+        "bd_gamma_op_apply_kept_dev": [P, P, I64, I32, P, P, I64],
+        "bd_sym_eigen_multi_from": [P, I32],
+        "bd_field_create": [P, C.POINTER(P)],
+        "bd_field_to_grid_dev": [P, P, P],
+        "bd_field_from_grid_dev": [P, P, P],
+        "bd_field_potential_dev": [P, P, P, P, P],
+        "bd_field_residual_dev": [P, P, P, P, P],
+        "bd_field_gram_dev": [P, I32, P, P, P, P],
+        "bd_field_mix_dev": [P, I32, P, P, P, P, P, D, I32, P],
+        "bd_field_energies_dev": [P, P, P, P],
         "bd_gamma_op_set_potential": [P, P, U64],
         "bd_gamma_op_set_potential_dev": [P, P, U64],
         "bd_gamma_op_apply": [P, P, I64, I32, P, I64],
@@ -157,6 +175,10 @@ def load():
     lib.bd_gamma_op_stream.restype = P
     lib.bd_block_destroy.argtypes = [P]
     lib.bd_block_destroy.restype = None
+    lib.bd_field_destroy.argtypes = [P]
+    lib.bd_field_destroy.restype = None
+    lib.bd_field_stream.argtypes = [P]
+    lib.bd_field_stream.restype = P
     lib.bd_k_batch_destroy.argtypes = [P]
     lib.bd_k_batch_destroy.restype = None
     lib.bd_k_batch_stream.argtypes = [P]

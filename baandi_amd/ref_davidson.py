@@ -27,6 +27,7 @@ class Options:
     preconditioner: int = L.BD_PRECOND_QE  # BD_PRECOND_QE, BD_PRECOND_TPA or BD_PRECOND_MLP
     mlp: object = None        # BD_PRECOND_MLP: a baandi_amd.learned.Network
     mlp_vbar: float = 0.0     # BD_PRECOND_MLP: mean local potential (Ha)
+    profile: bool = False     # record the per-iteration Ritz history (ref_davidson.rs:52-54)
 
     def band_threshold(self, band):
         """ref_davidson.rs:70-77."""
@@ -43,6 +44,10 @@ class Solution:
     converged: np.ndarray
     band_thresholds_ha: list
     record: dict = field(default_factory=dict)
+    # Per correction (iteration) and band: |Ritz change| (Ha) and the
+    # (previous, current) Ritz values; empty unless Options.profile.
+    ritz_changes_ha: list = field(default_factory=list)
+    ritz_values_ha: list = field(default_factory=list)
 
 
 class NotConverged(L.BaandiError):
@@ -67,12 +72,18 @@ def solve_with_initial_hx(operator, start, hamiltonian_diagonal, options: Option
     hx = torch.empty_like(start) if options.return_hx else None
     if options.return_hx and out is not None:
         hx = out[1]
+    hist_c = hist_v = None
+    if options.profile:
+        hist_c = np.zeros((options.max_iterations, nb))
+        hist_v = np.zeros((options.max_iterations, nb, 2))
     o = L.DavidsonOptions(bands=nb, threshold=float(options.threshold), occupied=int(options.occupied),
                           first_step=int(bool(options.first_step)), capacity=int(options.capacity),
                           max_iterations=int(options.max_iterations), return_hx=int(bool(options.return_hx)),
                           preconditioner=int(options.preconditioner),
                           mlp_blob=(L.ptr(options.mlp.blob) if options.mlp is not None else None),
-                          mlp_vbar=float(options.mlp_vbar))
+                          mlp_vbar=float(options.mlp_vbar), profile=int(bool(options.profile)),
+                          ritz_changes_ha=(hist_c.ctypes.data if hist_c is not None else None),
+                          ritz_values_ha=(hist_v.ctypes.data if hist_v is not None else None))
     if options.preconditioner == L.BD_PRECOND_MLP and options.mlp is None:
         raise ValueError("BD_PRECOND_MLP needs a trained network (Options.mlp)")
     eig = np.zeros(nb)
@@ -88,6 +99,10 @@ def solve_with_initial_hx(operator, start, hamiltonian_diagonal, options: Option
     record = {f: getattr(rec, f) for f, _ in L.DavidsonRecord._fields_}
     sol = Solution(eigenvalues=eig, eigenvectors=x, hx=hx, converged=conv.astype(bool),
                    band_thresholds_ha=[options.band_threshold(i) for i in range(nb)], record=record)
+    if options.profile:
+        its = record["iterations"]
+        sol.ritz_changes_ha = [hist_c[i].copy() for i in range(its)]
+        sol.ritz_values_ha = [hist_v[i].copy() for i in range(its)]
     if st == L.BD_NOT_CONVERGED and raise_not_converged:
         raise NotConverged(sol, "reference Davidson reached the iteration limit")
     return sol

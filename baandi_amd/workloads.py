@@ -44,6 +44,19 @@ def si128(a=10.26, ecut_ha=12.5, sigma=0.05, seed=128):
                 bands=288)
 
 
+def si512(a=10.26, ecut_ha=15.0, sigma=0.03, seed=512):
+    """Si512 cell (4x4x4 conventional cells, 41.04 Bohr cube), Gamma, 30 Ry,
+    1126 bands (1024 occupied), atoms displaced by N(0, sigma) Bohr (seeded):
+    the combined-at-scale cell (n = 191,711, 144^3 grid, 2560 projectors)."""
+    import numpy as np
+    rows = [[4 * a, 0.0, 0.0], [0.0, 4 * a, 0.0], [0.0, 0.0, 4 * a]]
+    atoms = np.array(supercell((4, 4, 4)), dtype=np.float64)
+    rng = np.random.default_rng(seed)
+    atoms = atoms + rng.normal(0.0, sigma, atoms.shape) / (4 * a)
+    return dict(name="si512_gamma_30ry_displaced", rows=rows, atoms=atoms.tolist(), z=4.0, ecut_ha=ecut_ha,
+                bands=1126)
+
+
 def si8(a=10.26, ecut_ha=15.0):
     rows = [[a, 0.0, 0.0], [0.0, a, 0.0], [0.0, 0.0, a]]
     return dict(name="si8_gamma_30ry", rows=rows, atoms=CONVENTIONAL, z=4.0, ecut_ha=ecut_ha, bands=20)
@@ -65,7 +78,9 @@ def al4(a=7.65, ecut_ha=12.5, mesh=(6, 6, 6)):
                 bands=16, mesh=tuple(mesh), theta=0.01, betas=AL_BETAS, dij_ry=AL_DIJ_RY)
 
 
-def build(workload):
+def build(workload, device=False):
+    """The native set-up of a workload; ``device`` runs its atom loops on the GPU."""
     from .hamiltonian import GammaSystem, SYNTHETIC_BETAS, SYNTHETIC_DIJ_RY
     return GammaSystem(workload["rows"], workload["atoms"], workload["z"], workload["ecut_ha"],
-                       betas=workload.get("betas", SYNTHETIC_BETAS), dij_ry=workload.get("dij_ry", SYNTHETIC_DIJ_RY))
+                       betas=workload.get("betas", SYNTHETIC_BETAS), dij_ry=workload.get("dij_ry", SYNTHETIC_DIJ_RY),
+                       device=device)

@@ -50,39 +50,31 @@ def fused_bytes(plan, pairs, c=16, r=8):
     return 2 * c * pairs * n0 * plan["sticks"] + r * n0 * n1 * n2
 
 
-def cpu_baseline(system, seconds=12.0):
-    """Oracle (NumPy FP64) H·X columns/s on the Si64 set-up, one BLAS thread."""
+def cpu_baseline(system, columns=160, reps=3):
+    """The oracle's C++ FP64 H·X (OpenMP over band pairs, pruned stick/plane
+    FFTs; NumPy BLAS for P D P^T X) on the Si64 set-up: the median of `reps`
+    applies to a `columns`-column block on all the host threads this process
+    may use, and on one. Returns (ms all threads, ms one thread, threads)."""
     import numpy as np
     from threadpoolctl import threadpool_limits
-    from oracle import physics as P
-    from oracle.hamiltonian import apply_orc
-    from oracle.fields import reciprocal_g2  # noqa: F401  (oracle import check)
-
-    class View:  # the oracle's apply reads only these set-up arrays
-        pass
-
-    v = View()
-    v.dims = system.dims
-    miller = system.get("miller")
-    basis = P.Basis(miller=miller, cart=None, g2=system.get("g2"))
-    v.half = P.half_sphere(basis)
-    v.fft_index = P.flat_index(P.map_miller(miller, v.dims), v.dims)
-    v.kinetic_orc = system.get("kinetic_orc")
-    v.orc_p = system.get("projectors_orc")
-    cpl = system.get("coupling")
-    nc = 5
-    v.coupling = [(s, nc, cpl[s:s + nc, s:s + nc].copy()) for s in range(0, system.nproj, nc)]
+    from oracle.cpu_hx import CpuHamiltonian
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    h = CpuHamiltonian(system, threads=threads)
     rng = np.random.default_rng(0)
-    pot = rng.standard_normal(v.dims) * 0.1
-    cols = 8
-    x = rng.standard_normal((system.n, cols))
-    done, t0 = 0, time.perf_counter()
-    with threadpool_limits(1):
-        while time.perf_counter() - t0 < seconds:
-            apply_orc(v, x, pot)
-            done += cols
-    dt = time.perf_counter() - t0
-    return done / dt, done, dt
+    h.set_potential(rng.standard_normal(system.dims) * 0.1)
+    x = np.asfortranarray(rng.standard_normal((system.n, columns)))
+    out = {}
+    for t in (threads, 1):
+        h.set_threads(t)
+        times = []
+        with threadpool_limits(t):
+            h.apply(x)  # warm-up (first-touch of the per-thread grids)
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                h.apply(x)
+                times.append(time.perf_counter() - t0)
+        out[t] = 1e3 * float(np.median(times))
+    return out[threads], out[1], threads
 
 
 def hx_profile(system, wl, count):
@@ -251,7 +243,6 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--mixed-steps", type=int, default=1,
                     help="SCFs timed with the mixed tier (loose maps in FP32/BF16x3, final maps in FP64)")
     ap.add_argument("--kpoint-steps", type=int, default=1,
@@ -392,14 +383,17 @@ def main():
     if args.kpoint_steps > 0:
         out["kpoint"] = kpoint_leg(args.kpoint_steps)
     if not args.no_cpu:
-        cps, cols, dt = cpu_baseline(system, args.cpu_seconds)
-        gpu_cps = st["columns"] / (sum(st[k] for k in ("stick_inverse", "plane", "stick_forward", "projections",
-                                                        "back_product")) * 1e-3)
+        ms_all, ms_one, threads = cpu_baseline(system)
+        gpu_ms = sum(st[k] for k in ("stick_inverse", "plane", "stick_forward", "projections",
+                                     "back_product")) / max(st["columns"], 1) * 160
         out["cpu_baseline"] = {
-            "value": round(cps, 2), "unit": "H·X columns/s (Si64 FP64)", "cores": 1, "kind": "port",
-            "sample": f"oracle apply_orc on {cols} Si64 ORC columns in {dt:.1f}s (NumPy FFT, 1 BLAS thread)",
-            "gpu_value": round(gpu_cps, 1),
-            "scf_hx_only_estimate_s": round(last["totals"]["h_columns"] / cps, 1),
+            "value": round(ms_all, 1), "unit": "ms per Si64 FP64 H·X of 160 columns", "cores": threads,
+            "kind": "port",
+            "sample": "oracle/cpu_hx (C++ FP64, OpenMP over band pairs, pruned stick/plane FFTs; NumPy BLAS "
+                      "for P D P^T X) on one 160-column Si64 block, median of 3 applies",
+            "one_core_ms": round(ms_one, 1), "nproc": os.cpu_count(),
+            "gpu_ms_same_block": round(gpu_ms, 3),
+            "scf_hx_only_estimate_s": round(last["totals"]["h_columns"] * ms_all / 160 / 1e3, 2),
         }
     print(json.dumps(out))
 

@@ -77,6 +77,12 @@ typedef struct {
 } bd_system_info;
 
 bd_status bd_system_create(const bd_system_desc* desc, bd_system** out);
+/* The same set-up with its atom loops on the GPU (the projector columns'
+ * structure-factor phases and ORC packing, the Davidson diagonal, the ionic
+ * potential's structure factors; hamiltonian.rs:885-1035, 211-340): the host
+ * keeps the radial transforms and Y_lm. Same arrays to rounding; for large
+ * cells (the Si512 cell's 192k rows and 1.5M density G against 512 atoms). */
+bd_status bd_system_create_dev(const bd_system_desc* desc, bd_system** out);
 void bd_system_destroy(bd_system* sys);
 bd_status bd_system_get_info(const bd_system* sys, bd_system_info* out);
 /* Copy one named set-up array into `out` (count elements; error if count
@@ -124,6 +130,18 @@ bd_status bd_gamma_op_apply_dev(bd_gamma_op* op, const double* x_dev, int64_t ld
  * grid (band_density_sum, distributed_real_gamma.rs:741-769). */
 bd_status bd_gamma_op_band_density_dev(bd_gamma_op* op, const double* x_dev, int64_t ldx,
                                        int32_t m, const double* weights, double* rho_dev);
+/* Warm start of an SCF map (KeptHx: crates/dft-engine/src/refscf/kept_hx.rs,
+ * apply_local_change of distributed_real_gamma.rs:1370-1438). With
+ * keep_fields on, every band density also keeps its columns' real-space pair
+ * fields on the device (capacity/2 x n0 n1 n2 complex values; Gamma ops on
+ * the one-workgroup plane pass). apply_kept then adds dV X to Y for those
+ * same columns (x_dev, ldx, m as passed to the density; an error otherwise):
+ * only the r -> G half of the local transform runs, with no kinetic or
+ * nonlocal term, so H_new X = H_old X + dV X with dV = V_new - V_old (full
+ * grid, Ha). */
+bd_status bd_gamma_op_keep_fields(bd_gamma_op* op, int32_t on);
+bd_status bd_gamma_op_apply_kept_dev(bd_gamma_op* op, const double* x_dev, int64_t ldx, int32_t m,
+                                     const double* delta_v_dev, double* y_dev, int64_t ldy);
 /* x_b^T P D P^T x_b per column (nonlocal_band_energies, :878-928). */
 bd_status bd_gamma_op_nonlocal_energies_dev(bd_gamma_op* op, const double* x_dev, int64_t ldx,
                                             int32_t m, double* out_host);
@@ -195,6 +213,14 @@ bd_status bd_gamma_op_fused_shape(bd_gamma_op* op, int32_t* inout4, int32_t set)
  * (threads 256 or 512, y rows per z-FFT chunk); set accepts the shapes the
  * autotuner may pick. All shapes give the same bits. */
 bd_status bd_gamma_op_plane_shape(bd_gamma_op* op, int32_t* inout2, int32_t set);
+/* Read (set = 0) or set the plane pass: 0 = one workgroup per y-z plane
+ * (only where the Z x n1 plane fits the LDS), zb > 0 = the split pass, which
+ * moves each plane through HBM and transforms zb z-lines (then a chunk of y
+ * rows) at a time. The op picks the split pass at create when the plane does
+ * not fit (the 144^3 grid of the Si512 cell in FP64); setting it on a smaller
+ * cell allocates its plane buffer. Both give the same bits. Replaces the
+ * plane stage of distributed_fft.rs:956-1080, which handles any plane size. */
+bd_status bd_gamma_op_plane_split(bd_gamma_op* op, int32_t* inout, int32_t set);
 
 /* ------------------------------------------------------------------------
  * Reference block Davidson on the op
@@ -218,6 +244,13 @@ typedef struct {
   int32_t preconditioner;/* bd_preconditioner */
   const float* mlp_blob; /* BD_PRECOND_MLP: device blob from bd_mlp_quantize_dev */
   double mlp_vbar;       /* BD_PRECOND_MLP: mean local potential (Ha), a feature */
+  int32_t profile;       /* nonzero: fill the Ritz history below (Options::profile) */
+  /* Host arrays of max_iterations x bands (row per iteration), or NULL:
+   * |theta_new - theta_old| per band (Solution::ritz_changes_ha), and the
+   * pairs (theta_old, theta_new) (Solution::ritz_values_ha, 2 per entry).
+   * Rows past record.iterations are left untouched. */
+  double* ritz_changes_ha;
+  double* ritz_values_ha;
 } bd_davidson_options;
 
 typedef struct {
@@ -233,6 +266,12 @@ typedef struct {
   int32_t mlp_fallback;  /* BD_PRECOND_MLP: iteration that fell back to TPA, -1 if none */
   int32_t eigen_retries; /* reduced eigensolves that lost a vector of a tight cluster and were
                             redone with the one-step tridiagonalisation */
+  /* Restarts by kind (ResetStats, ref_davidson/ideal_reset.rs): ideal resets
+   * (S = I, H_r = diag(theta)), periodic measured refreshes (every 4th), and
+   * guard trips (after which the solve refreshes by measurement). */
+  int32_t ideal_resets;
+  int32_t periodic_refreshes;
+  int32_t guard_fallbacks;
 } bd_davidson_record;
 
 /* X0 (n x bands, device), optional HX0 (device or NULL), diagonal (n, device).
@@ -322,6 +361,11 @@ bd_status bd_mfma_probe_dev(const uint16_t* a_bits, const uint16_t* b_bits, floa
 int64_t bd_sym_eigen_workspace(int32_t m, int32_t want);
 bd_status bd_sym_eigen_dev(int32_t m, int32_t want, double* A, int64_t lda, double* values, double* Y,
                            int64_t ldy, double* work);
+/* Read (set = 0) or set the subspace size from which the tridiagonalisation
+ * runs across the chip (one launch per reflector for its column, one for the
+ * product with the trailing block on every CU, a tiled panel update) instead
+ * of in one workgroup; default 1026. The solver takes m <= 2730. */
+bd_status bd_sym_eigen_multi_from(int32_t* inout, int32_t set);
 bd_status bd_inverse_cholesky_dev(const double* S, double* T, int32_t cap, int32_t from, int32_t to,
                                   int32_t* accepted, double* min_ratio);
 /* bd_inverse_cholesky_static_dev: the same extension with the same results,
@@ -333,6 +377,38 @@ bd_status bd_inverse_cholesky_dev(const double* S, double* T, int32_t cap, int32
  * GPU share that array and race; serialise them. */
 bd_status bd_inverse_cholesky_static_dev(const double* S, double* T, int32_t cap, int32_t from, int32_t to,
                                          int32_t* accepted, double* min_ratio);
+
+/* ------------------------------------------------------------------------
+ * SCF field step (the caller side of the path: the potential of an input
+ * density, the residual and its metric, the mixing update and the energy
+ * terms; crates/dft-engine/src/refscf/map.rs:41-198, metric.rs,
+ * broyden.rs:106-113, 317-445; xc.rs:48-73 LDA-PZ). Coefficients live on the
+ * density sphere (4 Ecut, bd_system_info.ndensity complex values, interleaved
+ * Re/Im); fields on the full n0 n1 n2 grid (real). Device pointers; each call
+ * returns when its work is done.
+ * ---------------------------------------------------------------------- */
+typedef struct bd_field bd_field;
+bd_status bd_field_create(const bd_system* sys, bd_field** out);
+void bd_field_destroy(bd_field* field);
+void* bd_field_stream(bd_field* field);
+/* grid (complex, N) = unnormalised inverse transform of sphere coefficients. */
+bd_status bd_field_to_grid_dev(bd_field* field, const double* coeff, double* grid);
+/* coeff = forward transform / N of a real grid field, on the sphere. */
+bd_status bd_field_from_grid_dev(bd_field* field, const double* real_grid, double* coeff);
+/* v = V_ion + V_H[x] + v_xc(|rho_in|) of input coefficients x; *vmean = mean(v). */
+bd_status bd_field_potential_dev(bd_field* field, const double* x, const double* vion, double* v, double* vmean);
+/* f = coefficients of rho_out (G = 0 set to electrons/volume) - x;
+ * *dr2 = half-sphere Hartree-metric norm of f, Ry (metric.rs). */
+bd_status bd_field_residual_dev(bd_field* field, const double* rho_out, const double* x, double* f, double* dr2);
+/* a (p x p, row-major) = metric products of p <= 9 rows (p x nd complex),
+ * c (p) = their products with f. */
+bd_status bd_field_gram_dev(bd_field* field, int32_t p, const double* rows, const double* f, double* a, double* c);
+/* out = (x - sum gamma_i dX_i) + beta (f - sum gamma_i dF_i) [* G^2/(G^2 + a_TF)
+ * when screening]; gamma is a host array of p values. */
+bd_status bd_field_mix_dev(bd_field* field, int32_t p, const double* dX, const double* dF, const double* gamma,
+                           const double* x, const double* f, double beta, int32_t screening, double* out);
+/* out3 = (Hartree, XC, local) energies (Ha) of a real grid density. */
+bd_status bd_field_energies_dev(bd_field* field, const double* rho, const double* vion, double* out3);
 
 /* ------------------------------------------------------------------------
  * Learned Davidson preconditioner (no reference counterpart: it takes the

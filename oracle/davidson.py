@@ -5,8 +5,10 @@ TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
 Follows crates/dft-solvers/src/ref_davidson.rs:117-366 with the workspace of
 ref_davidson/workspace.rs, the streaming correction of ref_davidson/stream.rs
 and the bordered inverse Cholesky of lean_davidson.rs:164-444. Contraction
-always takes the measured refresh (workspace.rs:772-935 "refresh" arm); the
-ideal-reset shortcut changes only the trajectory, never the fixed point.
+(workspace.rs:772-935) takes the ideal reset (S = I, H_r = diag(theta)) when
+the restarted columns have unit norms to 1e-12, a measured refresh every
+fourth contraction, and measured refreshes for the rest of the solve once a
+guard trips (ref_davidson/ideal_reset.rs:7-8).
 
 Parity: converged eigenvalues against a dense LAPACK eigensolve of the dense
 ORC H (<=1e-11 Ha, the tolerance of crates/dft-solvers/tests/ref_davidson.rs:223)
@@ -22,6 +24,8 @@ EPS = np.finfo(np.float64).eps
 ORTHOGONALITY_GATE = 1e-12  # lean_davidson.rs:62
 PIVOT_RATIO_MIN = EPS / ORTHOGONALITY_GATE  # lean_davidson.rs:64
 DEPENDENCE_TOLERANCE = 1e-12  # crates/dft-numerics/src/linalg.rs:12
+ORTHOGONALITY_LIMIT = 1e-12  # ref_davidson/ideal_reset.rs:7
+REFRESH_INTERVAL = 4  # ref_davidson/ideal_reset.rs:8
 
 
 def sqrt_denominator(h, e):
@@ -66,7 +70,7 @@ class NotConverged(Exception):
 
 class Davidson:
     def __init__(self, apply, diag, bands, threshold, occupied, first_step=False,
-                 capacity=0, max_iterations=20, preconditioner="qe", kinetic=None):
+                 capacity=0, max_iterations=20, preconditioner="qe", kinetic=None, ideal_reset=True):
         self.apply = apply
         self.diag = np.asarray(diag, dtype=np.float64)
         self.n = bands
@@ -82,6 +86,10 @@ class Davidson:
         self.h_columns = 0
         self.repairs = 0
         self.restarts = 0
+        self.ideal_reset = ideal_reset
+        self.ideal_resets = 0
+        self.periodic_refreshes = 0
+        self.guard_fallbacks = 0
 
     def band_threshold(self, i):
         """Options::band_threshold (ref_davidson.rs:70-77)."""
@@ -202,11 +210,35 @@ class Davidson:
         return k
 
     def _contract(self, m):
+        """Restart to the Ritz block (workspace.rs:772-935)."""
+        n = self.n
         self.restarts += 1
-        self.V[:, : self.n] = self.V[:, :m] @ self.C
-        self.HV[:, : self.n] = self.HV[:, :m] @ self.C
-        self._refresh(self.n)
-        self.C = np.eye(self.n)
+        self.V[:, :n] = self.V[:, :m] @ self.C
+        self.HV[:, :n] = self.HV[:, :m] @ self.C
+        self.contractions += 1
+        ideal = False
+        if self.reset_disabled:
+            pass
+        elif self.contractions % REFRESH_INTERVAL == 0:
+            self.periodic_refreshes += 1
+        elif np.all(np.abs(np.sum(self.V[:, :n] ** 2, axis=0) - 1.0) <= ORTHOGONALITY_LIMIT):
+            ideal = True  # reset_norm_guard, workspace.rs:895-918 (NaN fails the test)
+            self.ideal_resets += 1
+        else:
+            self.reset_disabled = True
+            self.guard_fallbacks += 1
+        self.C = np.eye(n)
+        if ideal:
+            self.S[:n, :n] = np.eye(n)
+            self.Hr[:n, :n] = np.diag(self.values[:n])
+            self.T[:n, :n] = np.eye(n)
+        else:
+            self._refresh(n)
+            if not self.reset_disabled:
+                defect = np.abs(self.S[:n, :n] - np.eye(n)).max()
+                if not defect <= ORTHOGONALITY_LIMIT:
+                    self.reset_disabled = True
+                    self.guard_fallbacks += 1
 
     # -- driver -------------------------------------------------------------
     def solve(self, x0, hx0=None):
@@ -219,6 +251,8 @@ class Davidson:
         self.S = np.zeros((cap, cap))
         self.Hr = np.zeros((cap, cap))
         self.T = np.zeros((cap, cap))
+        self.contractions = 0
+        self.reset_disabled = not self.ideal_reset
         self.V[:, :n] = x0
         self.HV[:, :n] = self._h(x0) if hx0 is None else hx0
         self._set_border(self.S, self._border(self.V, self.V, 0, n), 0, n)
@@ -279,7 +313,8 @@ class Davidson:
         HX = self.HV[:, :m] @ self.C
         return dict(eigenvalues=self.values[:n].copy(), x=X, hx=HX, converged=list(converged),
                     iterations=self.iterations, h_calls=self.h_calls, h_columns=self.h_columns,
-                    repairs=self.repairs, restarts=self.restarts)
+                    repairs=self.repairs, restarts=self.restarts, ideal_resets=self.ideal_resets,
+                    periodic_refreshes=self.periodic_refreshes, guard_fallbacks=self.guard_fallbacks)
 
 
 def random_orc_start(rows, bands, seed):

@@ -88,23 +88,117 @@ def test_davidson_first_iterations_match_oracle(cell, bands, occupied, precond):
     g /= np.linalg.norm(g, axis=1, keepdims=True)
     diag = s.get("diagonal_static") + pot.mean()
     code = L.BD_PRECOND_QE if precond == "qe" else L.BD_PRECOND_TPA
-    for its in (1, 2, 3):
+    for its in (1, 2, 3, 6):
         opts = ref_davidson.Options(bands=bands, threshold=1e-15, occupied=occupied, max_iterations=its,
                                     preconditioner=code)
         sol = ref_davidson.solve(op, torch.from_numpy(g).cuda(), torch.from_numpy(diag).cuda(), opts)
         ref = OD.Davidson(lambda x: apply_orc(o, x, pot), diag, bands, 1e-15, occupied, max_iterations=its,
                           preconditioner=precond, kinetic=s.get("kinetic_orc")).solve(g.T.copy())
         err = np.abs(sol.eigenvalues - ref["eigenvalues"]).max()
+        rec = sol.record
         print(f"[davidson trajectory] {cell} {precond} iterations={its}: max |dE| {err:.2e} Ha, H columns "
-              f"{sol.record['h_columns']}/{ref['h_columns']}", flush=True)
-        assert sol.record["iterations"] == ref["iterations"] == its
-        assert sol.record["h_columns"] == ref["h_columns"]
+              f"{rec['h_columns']}/{ref['h_columns']}, restarts {rec['restarts']}/{ref['restarts']} "
+              f"(ideal {rec['ideal_resets']}/{ref['ideal_resets']}, periodic "
+              f"{rec['periodic_refreshes']}/{ref['periodic_refreshes']})", flush=True)
+        assert rec["iterations"] == ref["iterations"] == its
+        assert rec["h_columns"] == ref["h_columns"]
+        for key in ("restarts", "repairs", "ideal_resets", "periodic_refreshes", "guard_fallbacks"):
+            assert rec[key] == ref[key], key
         assert err < 1e-10
     # A different preconditioner moves the iteration-3 values well above that bar.
     other = OD.Davidson(lambda x: apply_orc(o, x, pot), diag, bands, 1e-15, occupied, max_iterations=3,
                         preconditioner="tpa" if precond == "qe" else "qe",
                         kinetic=s.get("kinetic_orc")).solve(g.T.copy())
     assert np.abs(other["eigenvalues"] - ref["eigenvalues"]).max() > 1e-6
+
+
+def test_davidson_rejects_dependent_start():
+    """Equal start columns: InvalidParameter before any iteration
+    (crates/dft-solvers/tests/ref_davidson.rs:345-356)."""
+    from baandi_amd.hamiltonian import DistributedRealHamiltonian
+    from baandi_amd import ref_davidson, lib as L
+    torch, s, o = si2()
+    op = DistributedRealHamiltonian(s, capacity=4)
+    op.set_owned_potential(o.ionic_grid.reshape(-1))
+    x0 = torch.ones((4, s.n), dtype=torch.float64).cuda() / np.sqrt(s.n)
+    diag = torch.from_numpy(s.get("diagonal_static")).cuda()
+    with pytest.raises(L.BaandiError) as e:
+        ref_davidson.solve(op, x0, diag, ref_davidson.Options(bands=4, threshold=1e-8, occupied=2))
+    assert e.value.status == L.BD_ERR_INVALID_PARAMETER
+    assert "dependent" in str(e.value)
+
+
+@pytest.mark.parametrize("precond", ["qe", "tpa"])
+def test_davidson_rank_repair_when_capacity_is_short(precond):
+    """A basis of 59 plane waves (Si2 at 2.5 Ha) and 40 bands: the capacity
+    is the dimension, so the 40 corrections of the first iteration do not
+    fit beside the 40 columns and the rank repair runs (workspace.rs:586-771).
+    The first iteration equals the oracle's (same repair, H columns and Ritz
+    values; the repaired basis spans the whole space, so later corrections are
+    rounding noise whose 32 eps drops differ run to run), and the solve
+    converges to the dense spectrum."""
+    from baandi_amd.hamiltonian import DistributedRealHamiltonian
+    from baandi_amd import ref_davidson, lib as L
+    from oracle.hamiltonian import apply_orc, dense_orc
+    from oracle import davidson as OD
+    torch, s, o = si2(ecut=2.5)
+    assert s.n == 59
+    bands, occupied = 40, 4
+    pot = o.ionic_grid + 0.05 * np.cos(np.arange(o.ionic_grid.size)).reshape(o.dims)
+    op = DistributedRealHamiltonian(s, capacity=bands)
+    op.set_owned_potential(pot.reshape(-1))
+    g = np.random.default_rng(8).standard_normal((bands, s.n))
+    g /= np.linalg.norm(g, axis=1, keepdims=True)
+    diag = s.get("diagonal_static") + pot.mean()
+    code = L.BD_PRECOND_QE if precond == "qe" else L.BD_PRECOND_TPA
+    for its in (1,):
+        sol = ref_davidson.solve(op, torch.from_numpy(g).cuda(), torch.from_numpy(diag).cuda(),
+                                 ref_davidson.Options(bands=bands, threshold=1e-15, occupied=occupied,
+                                                      max_iterations=its, preconditioner=code))
+        ref = OD.Davidson(lambda x: apply_orc(o, x, pot), diag, bands, 1e-15, occupied, max_iterations=its,
+                          preconditioner=precond, kinetic=s.get("kinetic_orc")).solve(g.T.copy())
+        err = np.abs(sol.eigenvalues - ref["eigenvalues"]).max()
+        print(f"[repair] {precond} iterations={its}: repairs {sol.record['repairs']}/{ref['repairs']}, "
+              f"H columns {sol.record['h_columns']}/{ref['h_columns']}, max |dE| {err:.2e}", flush=True)
+        assert sol.record["repairs"] == ref["repairs"] >= 1
+        assert sol.record["h_columns"] == ref["h_columns"]
+        assert err < 1e-10
+    w = np.linalg.eigvalsh(dense_orc(o, pot))
+    sol = ref_davidson.solve(op, torch.from_numpy(g).cuda(), torch.from_numpy(diag).cuda(),
+                             ref_davidson.Options(bands=bands, threshold=1e-12, occupied=bands, max_iterations=100,
+                                                  preconditioner=code, return_hx=True))
+    print(f"[repair] {precond} converged: record={sol.record}", flush=True)
+    assert sol.converged.all() and sol.record["repairs"] >= 1
+    assert np.abs(sol.eigenvalues - w[:bands]).max() < 1e-10
+
+
+def test_davidson_profile_history():
+    """Options.profile: one row of Ritz changes and (previous, current)
+    values per iteration (Solution::ritz_changes_ha / ritz_values_ha); the
+    last row's current values are the returned eigenvalues and its changes
+    fall under each converged band's threshold."""
+    from baandi_amd.hamiltonian import DistributedRealHamiltonian
+    from baandi_amd import ref_davidson
+    torch, s, o = si8()
+    bands, occupied = 24, 16
+    op = DistributedRealHamiltonian(s, capacity=bands)
+    op.set_owned_potential(o.ionic_grid.reshape(-1))
+    g = np.random.default_rng(4).standard_normal((bands, s.n))
+    x0 = torch.from_numpy(g / np.linalg.norm(g, axis=1, keepdims=True)).cuda()
+    diag = torch.from_numpy(s.get("diagonal_static") + o.ionic_grid.mean()).cuda()
+    sol = ref_davidson.solve(op, x0, diag, ref_davidson.Options(bands=bands, threshold=1e-9, occupied=occupied,
+                                                                max_iterations=60, profile=True))
+    its = sol.record["iterations"]
+    print(f"[profile] iterations={its}, record={sol.record}", flush=True)
+    assert len(sol.ritz_changes_ha) == len(sol.ritz_values_ha) == its > 1
+    last = sol.ritz_values_ha[-1]
+    assert np.array_equal(last[:, 1], sol.eigenvalues)
+    assert np.array_equal(sol.ritz_changes_ha[-1], np.abs(last[:, 1] - last[:, 0]))
+    assert np.all(sol.ritz_changes_ha[-1][sol.converged] < np.array(sol.band_thresholds_ha)[sol.converged])
+    for a, b in zip(sol.ritz_values_ha[:-1], sol.ritz_values_ha[1:]):
+        assert np.array_equal(a[:, 1], b[:, 0])
+    assert sol.record["ideal_resets"] > 0
+    assert sol.record["restarts"] >= sol.record["ideal_resets"] + sol.record["periodic_refreshes"]
 
 
 def test_davidson_rejects_nonfinite_start():

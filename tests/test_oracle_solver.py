@@ -129,6 +129,46 @@ def test_davidson_matches_dense_eigensolve(si2_20ry, pc):
     assert np.sqrt((res ** 2).sum(0)).max() < 1e-5
 
 
+def test_ideal_reset_keeps_the_fixed_point(si2_20ry):
+    """The ideal reset (S = I, H_r = diag(theta) after a restart whose columns
+    have unit norms; ref_davidson/ideal_reset.rs) against measured refreshes
+    at every restart: the same converged eigenvalues (1e-11 Ha vs dense), with
+    ideal resets taken, every fourth contraction measured and no guard trip."""
+    s = si2_20ry
+    hd = H.dense_orc(s, s.ionic_grid)
+    w = np.linalg.eigvalsh(hd)
+    out = {}
+    for ideal in (True, False):
+        dav = D.Davidson(lambda x: H.apply_orc(s, x, s.ionic_grid), np.diag(hd), 8, 1e-13, 4, first_step=True,
+                         max_iterations=200, preconditioner="qe", kinetic=s.kinetic_orc, ideal_reset=ideal)
+        out[ideal] = dav.solve(D.random_orc_start(len(s.basis.g2), 8, 1))
+        assert np.abs(out[ideal]["eigenvalues"] - w[:8]).max() < 1e-11
+    r = out[True]
+    assert r["ideal_resets"] > 0 and r["guard_fallbacks"] == 0
+    assert r["periodic_refreshes"] == r["restarts"] // 4
+    assert r["ideal_resets"] + r["periodic_refreshes"] == r["restarts"]
+    assert out[False]["ideal_resets"] == 0 and out[False]["periodic_refreshes"] == 0
+
+
+def test_ideal_reset_guard_trips_on_unnormalised_columns(si2_20ry):
+    """Columns off unit norm by more than 1e-12 at a restart: the norm guard
+    trips and every later restart is a measured refresh (workspace.rs:814-835)."""
+    s = si2_20ry
+    hd = H.dense_orc(s, s.ionic_grid)
+    dav = D.Davidson(lambda x: H.apply_orc(s, x, s.ionic_grid), np.diag(hd), 8, 1e-13, 4, first_step=True,
+                     max_iterations=200, preconditioner="qe", kinetic=s.kinetic_orc)
+    original = dav._contract
+
+    def skewed(m):  # scale the Ritz coefficients of the first restart by 1 + 1e-9
+        if dav.contractions == 0:
+            dav.C = dav.C * (1.0 + 1e-9)
+        original(m)
+    dav._contract = skewed
+    r = dav.solve(D.random_orc_start(len(s.basis.g2), 8, 1))
+    assert r["guard_fallbacks"] == 1 and r["ideal_resets"] == 0 and r["restarts"] > 1
+    assert np.abs(r["eigenvalues"] - np.linalg.eigvalsh(hd)[:8]).max() < 1e-11
+
+
 def test_kat1_scf_fixed_point(kat1):
     """KAT-1: the stored certified energy and eigenvalues of reference-1."""
     pp = P.synthetic_pseudo(4.0, 1.0, 801)
