@@ -695,6 +695,8 @@ extern "C" bd_status bd_system_get(const bd_system* sys, const char* name, void*
     else if (k == "coupling") put(s.coupling.data(), s.coupling.size(), 8);
     else if (k == "plus_index") put(s.plus_flat.data(), s.plus_flat.size(), 8);
     else if (k == "minus_index") put(s.minus_flat.data(), s.minus_flat.size(), 8);
+    else if (k == "plus_basis") put(s.plus.data(), s.plus.size(), 8);
+    else if (k == "minus_basis") put(s.minus.data(), s.minus.size(), 8);
     else if (k == "density_miller") put(s.dens_miller.data(), s.dens_miller.size() * 3, 4);
     else if (k == "density_g2") put(s.dens_g2.data(), s.dens_g2.size(), 8);
     else if (k == "density_index") put(s.dens_flat.data(), s.dens_flat.size(), 8);

@@ -9,8 +9,29 @@ sphere (4 Ecut), fields real float64 CUDA tensors on the FFT grid.
 from __future__ import annotations
 
 import ctypes as C
+import math
+
+import numpy as np
 
 from . import lib as L
+
+# bd_field_gram_dev and bd_field_mix_dev take at most this many history rows.
+MAX_HISTORY = 9
+
+
+def gaussian_start(system):
+    """Start density on the density sphere: superposed Gaussian atomic charges
+    Z (alpha/pi)^1.5 e^{-alpha r^2}, alpha = 1/rc^2, i.e. coefficients
+    Z/Omega e^{-G^2 rc^2/4} sum_a e^{-i G.tau_a}, with G = 0 holding
+    electrons/Omega (complex128 numpy array, density-sphere order)."""
+    atoms = system.get("atoms_cart")
+    recip = 2.0 * math.pi * np.linalg.inv(np.array(system.lattice_rows)).T
+    miller = system.get("density_miller")
+    g2 = system.get("density_g2")
+    sfac = np.exp(-1j * ((miller.astype(np.float64) @ recip) @ atoms.T)).sum(axis=1)
+    x = system.z_valence / system.volume * np.exp(-g2 * system.rc ** 2 / 4.0) * sfac
+    x[np.all(miller == 0, axis=1)] = system.electrons / system.volume
+    return x
 
 
 class DensityField:
@@ -127,3 +148,52 @@ class DensityField:
             except (TypeError, AttributeError):
                 pass
             self.handle = None
+
+
+class Mixer:
+    """Difference-history Anderson update in the half-sphere Hartree metric
+    (crates/dft-engine/src/refscf/broyden.rs:317-445) on the field kernels.
+
+    Keeps the last ``history`` pairs (x_prev - x, f_prev - f); the mixing
+    coefficients solve the p x p metric Gram system of the residual
+    differences by an eigendecomposition on the host (a rank-deficient history
+    drops its oldest pair and tries again). ``preconditioning``
+    "thomas_fermi" scales the history-reduced residual by |G|^2/(|G|^2 + a),
+    a = (12/pi)^(2/3)/r_s (broyden.rs:106-113, applied to the residual term
+    only, :439); "plain" (profile.rs:17-26, the default) leaves it unscaled."""
+
+    def __init__(self, field: DensityField, beta, history, preconditioning="plain"):
+        if preconditioning not in ("plain", "thomas_fermi"):
+            raise ValueError(f"unknown preconditioning {preconditioning!r}")
+        if not 0 <= int(history) <= MAX_HISTORY:
+            raise ValueError(f"mixing history must be 0..{MAX_HISTORY}")
+        self.field, self.beta, self.history = field, float(beta), int(history)
+        self.screening = preconditioning == "thomas_fermi"
+        self.pairs = []  # (dx, dF), newest last
+        self.prev = None
+
+    def update(self, x, f):
+        """Next input density coefficients from input x and residual f."""
+        import torch
+        if self.prev is not None:
+            px, pf = self.prev
+            self.pairs.append((px - x, pf - f))
+            if len(self.pairs) > self.history:
+                self.pairs.pop(0)
+        self.prev = (x.clone(), f.clone())
+        gamma = dX = dF = None
+        while self.pairs:
+            p = len(self.pairs)
+            dF = torch.stack([df for _, df in self.pairs])
+            a, c = self.field.gram(dF, f)
+            a = 0.5 * (a + a.T)
+            lam, q = np.linalg.eigh(a)
+            if lam[-1] <= 0.0 or lam[0] <= math.sqrt(p * np.finfo(float).eps) * lam[-1]:
+                self.pairs.pop(0)  # rank-deficient history: drop the oldest pair
+                dF = None
+                continue
+            gamma = q @ ((q.T @ c) / lam)
+            dX = torch.stack([dx for dx, _ in self.pairs])
+            break
+        out = torch.empty_like(x)
+        return self.field.mix(dX, dF, gamma, x, f, self.beta, self.screening, out)

@@ -80,6 +80,7 @@ class GammaSystem:
             "projectors_orc": (self.n * self.nproj, np.float64),
             "coupling": (self.nproj * self.nproj, np.float64),
             "plus_index": (self.nrep, np.int64), "minus_index": (self.nrep, np.int64),
+            "plus_basis": (self.nrep, np.int64), "minus_basis": (self.nrep, np.int64),
             "density_miller": (3 * self.ndensity, np.int32), "density_g2": (self.ndensity, np.float64),
             "density_index": (self.ndensity, np.int64),
             "ionic_density_re": (self.ndensity, np.float64), "ionic_density_im": (self.ndensity, np.float64),

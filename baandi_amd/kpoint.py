@@ -338,7 +338,8 @@ def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
     electron (its pair partner the other), so the occupations, the density
     weights and the entropy are taken over the real levels with capacity 1."""
     import torch
-    from .scf import DensityField, Mixer, lda_pz_torch, next_ethr_ry
+    from .field import DensityField, Mixer, gaussian_start
+    from .scf import next_ethr_ry
     dev = torch.device("cuda")
     t_setup = time.perf_counter()
     b = int(options.bands)
@@ -361,19 +362,14 @@ def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
     diag_static = [torch.from_numpy(op.get("diagonal_static")).to(dev) for op in ops]
     kinetic = [torch.from_numpy(op.get("kinetic")).to(dev) for op in ops]
     ewald = float(system.get("ewald")[0])
-    atoms = system.get("atoms_cart")
-    rows = np.array(system.lattice_rows)
-    recip = 2.0 * math.pi * np.linalg.inv(rows).T
-    dcart = system.get("density_miller").astype(np.float64) @ recip
-    sfac = np.exp(-1j * (dcart @ atoms.T)).sum(axis=1)
-    x = system.z_valence / system.volume * np.exp(-field.g2 * system.rc ** 2 / 4.0) * sfac
-    x[field.zero_np] = electrons / system.volume
-    x = torch.from_numpy(x).to(dev)
-    mixer = Mixer(field, options.beta, options.history, options.preconditioning, system.volume, electrons)
+    x = torch.from_numpy(gaussian_start(system)).to(dev)
+    mixer = Mixer(field, options.beta, options.history, options.preconditioning)
     orbitals = [realified_start(_complex_start(op.nk, b, options.seed + i).to(dev)) for i, op in enumerate(ops)]
     outs = [torch.empty_like(o) for o in orbitals]
     rho_dev = torch.empty(field.size, dtype=torch.float64, device=dev)
     rho_k = torch.empty_like(rho_dev)
+    potential = torch.empty_like(rho_dev)
+    fres = torch.empty_like(x)
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t_setup
 
@@ -390,12 +386,7 @@ def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
         redo = False
         while True:
             tf = time.perf_counter()
-            vh_g = field.coulomb * x
-            both = field.to_grid(x + 1j * vh_g)
-            rho_in, vh = both.real, both.imag
-            _, vxc = lda_pz_torch(rho_in.abs())
-            potential = (vion + vh + vxc).reshape(-1).contiguous()
-            vmean = float(potential.mean())  # also orders torch's work before the ops' streams
+            vmean = field.potential(x, vion, potential)  # V_ion + V_H + v_xc of the input density
             for op in ops:
                 op.set_owned_potential(potential)
             totals["fields_s"] += time.perf_counter() - tf
@@ -422,10 +413,7 @@ def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
                 op.band_density_sum(outs[i], weights[i] * occ[i], rho_k)
                 rho_dev += rho_k
             rho_out = rho_dev.view(field.dims)
-            y = field.from_grid(rho_out)
-            y[field.zero] = electrons / system.volume
-            f = y - x
-            dr2 = field.dot(f, f)
+            dr2 = field.residual(rho_out, x, fres)
             totals["fields_s"] += time.perf_counter() - tf
             if iteration == 1 and not redo and dr2 < ethr * max(electrons, 1.0):
                 ethr = next_ethr_ry(ethr, dr2, electrons)
@@ -445,7 +433,7 @@ def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
         if dr2 < options.threshold_ry:
             result = (levels, occ, mu, ts, rho_out)
             break
-        x = mixer.update(x, f)
+        x = mixer.update(x, fres)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     if result is None:
@@ -457,13 +445,7 @@ def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
         xo = orbitals[i]
         kin += float(((xo * xo * kinetic[i][None, :]).sum(dim=1).cpu().numpy() * w).sum())
         nl += float(np.dot(op.nonlocal_band_energies(xo), w))
-    dv = system.volume / field.size
-    yo = field.from_grid(rho_out)
-    vh_out = field.to_grid(field.coulomb * yo).real
-    hartree = 0.5 * dv * float((rho_out * vh_out).sum())
-    eps_o, _ = lda_pz_torch(rho_out.abs())
-    xc = dv * float((rho_out.abs() * eps_o).sum())
-    local = dv * float((rho_out * vion).sum())
+    hartree, xc, local = field.energies(rho_out, vion)
     total = kin + nl + local + hartree + xc + ewald - ts
     energy = {"kinetic": kin, "nonlocal": nl, "local": local, "hartree": hartree, "xc": xc, "ewald": ewald,
               "entropy_term": ts, "total": total}

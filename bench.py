@@ -10,8 +10,10 @@ already resident on the GPU. Prints one JSON line.
 
 The roofline object covers the dominant H·X kernel (k_plane: y/z transforms
 with the V(r) multiply), timed with HIP events on the operator's stream over
-the timed region. The CPU baseline times the FP64 oracle's H·X (NumPy, one
-thread) on a bounded sample of the same Si64 columns.
+the timed region. The CPU baseline times the oracle's C++ FP64 H·X (all host
+threads, and one) on a bounded sample of the same Si64 columns. Side legs
+(untimed by the step clock): the subspace GEMMs, the Al 6x6x6 k-point SCF,
+and the Si512 cell's set-up, 1126-column H·X and first Davidson iterations.
 """
 from __future__ import annotations
 
@@ -237,6 +239,87 @@ def si128_leg(mlp_path=None, theta=0.01, variants=True):
     return out
 
 
+def si512_leg(iterations=2, reps=2):
+    """Si512 (4x4x4 conventional cells, displaced N(0, 0.03 Bohr), Gamma,
+    30 Ry, 1126 bands, 1024 occupied) at the potential of the Gaussian start
+    density: the set-up with its atom loops on the GPU, the H·X of the full
+    1126-column block (split plane pass, 144^3 grid) in FP64 and in the mixed
+    tier, and `iterations` FP64 Davidson iterations from a seeded start
+    (subspace up to 2252), each timed."""
+    import numpy as np
+    import torch
+    from baandi_amd import workloads, lib as L, ref_davidson
+    from baandi_amd.field import DensityField, gaussian_start
+    from baandi_amd.hamiltonian import DistributedRealHamiltonian
+    wl = workloads.si512()
+    t = time.perf_counter()
+    system = workloads.build(wl, device=True)
+    setup_s = time.perf_counter() - t
+    bands = wl["bands"]
+    print(f"[bench] si512: n={system.n} grid={system.dims} proj={system.nproj} device set-up {setup_s:.2f}s",
+          file=sys.stderr, flush=True)
+    field = DensityField(system)
+    vion_g = torch.from_numpy(system.get("ionic_density_re") + 1j * system.get("ionic_density_im")).cuda()
+    vion = field.to_grid(vion_g).real.contiguous()
+    pot = torch.empty(field.size, dtype=torch.float64, device="cuda")
+    vmean = field.potential(torch.from_numpy(gaussian_start(system)).cuda(), vion, pot)
+    g = torch.Generator(device="cpu").manual_seed(1)
+    x = torch.randn(bands, system.n, generator=g, dtype=torch.float64)
+    x = (x / x.norm(dim=1, keepdim=True)).cuda()
+    y = torch.empty_like(x)
+    out = {"workload": f"{wl['name']}: n={system.n}, grid {list(system.dims)}, {system.nproj} projectors, "
+                       f"{bands} bands, potential of the Gaussian start density",
+           "setup_s": round(setup_s, 3)}
+    ref_y = None
+    for name, prec in (("fp64", L.BD_FP64), ("mixed", L.BD_MIXED)):
+        op = DistributedRealHamiltonian(system, capacity=bands, precision=prec)
+        op.set_owned_potential(pot)
+        op.apply_block(x, y)  # first call: plans, scratch
+        torch.cuda.synchronize()
+        op.set_profiling(True)
+        t = time.perf_counter()
+        for _ in range(reps):
+            op.apply_block(x, y)
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t) / reps
+        st = op.stage_totals()
+        plan = op.plan_info()
+        plane_ms = st["plane"] / reps
+        c = 16 if prec == L.BD_FP64 else 8
+        model = plane_bytes_model(plan, st["pairs"] / reps, c=c, r=c // 2)
+        entry = {"hx_ms": round(ms, 2), "columns": bands, "plane_split_z_lines": op.plane_split(),
+                 "stage_ms": {k: round(v / reps, 3) for k, v in st.items()
+                              if k not in ("applies", "pairs", "columns")},
+                 "plane_model_GBs": round(model / (plane_ms * 1e-3) / 1e9, 1) if plane_ms > 0 else None,
+                 "plane_frac_hbm": round(model / (plane_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if plane_ms > 0 else None}
+        if ref_y is None:
+            ref_y = y.clone()
+            entry["plan"] = plan
+        else:
+            entry["max_rel_diff_vs_fp64"] = float(((y - ref_y).abs().max() / ref_y.abs().max()).item())
+        out[name] = entry
+        print(f"[bench] si512 {name} H·X of {bands} columns: {ms:.1f} ms {entry['stage_ms']}", file=sys.stderr,
+              flush=True)
+        del op
+    op = DistributedRealHamiltonian(system, capacity=bands)
+    op.set_owned_potential(pot)
+    diag = torch.from_numpy(system.get("diagonal_static")).cuda() + vmean
+    opts = ref_davidson.Options(bands=bands, threshold=1e-6, occupied=1024, first_step=True,
+                                max_iterations=iterations)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    sol = ref_davidson.solve(op, x, diag, opts)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    rec = sol.record
+    out["davidson"] = {"iterations": rec["iterations"], "wall_s": round(wall, 3),
+                       "dense_s": round(rec["seconds_dense"], 3), "apply_s": round(rec["seconds_apply"], 3),
+                       "h_columns": rec["h_columns"], "lowest_ha": float(sol.eigenvalues[0]),
+                       "eigen_retries": rec["eigen_retries"]}
+    print(f"[bench] si512 davidson: {out['davidson']}", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -254,6 +337,10 @@ def main():
                          "(--mixed-steps 0: the FP64 leg alone)")
     ap.add_argument("--mlp", default=None,
                     help="learned-preconditioner weights (.npz) for the learned legs (default: the packaged file)")
+    ap.add_argument("--si512-only", action="store_true",
+                    help="profile mode: only the Si512 leg (device set-up, 1126-column H·X, Davidson iterations)")
+    ap.add_argument("--si512-iterations", type=int, default=2,
+                    help="FP64 Davidson iterations of the Si512 leg (0 skips the leg)")
     ap.add_argument("--hx-only", type=int, default=0,
                     help="profile mode: N applies of H to 160 Si64 columns, then a 256 MiB copy")
     args = ap.parse_args()
@@ -270,6 +357,9 @@ def main():
         return
     if args.si128_only:
         print(json.dumps(si128_leg(args.mlp, variants=args.mixed_steps > 0)))
+        return
+    if args.si512_only:
+        print(json.dumps(si512_leg(max(args.si512_iterations, 1))))
         return
 
     wl = workloads.si64()
@@ -382,6 +472,8 @@ def main():
     out["subspace_gemms"] = gemm_leg()
     if args.kpoint_steps > 0:
         out["kpoint"] = kpoint_leg(args.kpoint_steps)
+    if args.si512_iterations > 0:
+        out["si512"] = si512_leg(args.si512_iterations)
     if not args.no_cpu:
         ms_all, ms_one, threads = cpu_baseline(system)
         gpu_ms = sum(st[k] for k in ("stick_inverse", "plane", "stick_forward", "projections",

@@ -94,6 +94,8 @@ bd_status bd_system_get_info(const bd_system* sys, bd_system_info* out);
  *   "projectors_orc"   double, n*nproj  ORC rows, column-major
  *   "coupling"         double, nproj*nproj  block-diagonal D (Ha), column-major
  *   "plus_index", "minus_index"  int64, nrep  flat FFT index of +G / -G of each rep
+ *   "plus_basis", "minus_basis"  int64, nrep  basis index of +G / -G of each rep
+ *                      (HalfSphere plus()/minus(), crates/dft-basis/src/half.rs:41-79)
  *   "density_miller"   int32, 3*ndensity; "density_g2" double, ndensity
  *   "density_index"    int64, ndensity  flat FFT index
  *   "ionic_density_re", "ionic_density_im" double, ndensity  V_ion(G), Ha

@@ -187,11 +187,14 @@ def orc_pack(half: HalfSphere, full):
 
 
 def orc_unpack(half: HalfSphere, real):
-    """HalfSphere::unpack (half.rs:140-153): Hermitian completion."""
+    """HalfSphere::unpack (half.rs:140-153): Hermitian completion; each
+    component divided by sqrt 2 on its own, as Complex64 / f64 does."""
     real = np.asarray(real, dtype=np.float64)
     full = np.empty(real.shape, dtype=np.complex128)
     full[0] = real[0]
-    z = (real[1::2] + 1j * real[2::2]) / math.sqrt(2.0)
+    z = np.empty(real[1::2].shape, dtype=np.complex128)
+    z.real = real[1::2] / math.sqrt(2.0)
+    z.imag = real[2::2] / math.sqrt(2.0)
     full[half.plus[1:]] = z
     full[half.minus[1:]] = np.conj(z)
     return full

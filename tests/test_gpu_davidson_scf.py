@@ -252,6 +252,26 @@ def test_scf_kat1_golden(kat1):
         assert abs(e[k] - kat1["energy_terms_ha"][k]) < 5e-7, k
 
 
+def test_scf_warm_start_matches_fresh(kat1):
+    """KeptHx: the SCF whose maps start from H_old X + dV X (the kept pair
+    fields of the previous band density) and the SCF with a fresh H·X of
+    every start block reach the same fixed point; every map after the first
+    starts warm."""
+    from baandi_amd.hamiltonian import GammaSystem
+    from baandi_amd.scf import run_scf, ScfOptions
+    s = GammaSystem(fcc_rows(kat1["cell_a_bohr"]), [[0, 0, 0], [0.25] * 3], 4.0, kat1["ecut_ha"])
+    warm = run_scf(s, ScfOptions(bands=8, threshold_ry=1e-13))
+    fresh = run_scf(s, ScfOptions(bands=8, threshold_ry=1e-13, kept_hx=False))
+    print(f"[kept] KAT-1 warm {warm['energy']['total']!r} ({warm['totals']['warm_starts']} warm maps, "
+          f"{warm['totals']['h_columns']} H columns) fresh {fresh['energy']['total']!r} "
+          f"({fresh['totals']['h_columns']} H columns)", flush=True)
+    assert fresh["totals"]["warm_starts"] == 0
+    assert warm["totals"]["warm_starts"] >= warm["iterations"] - 1
+    assert all(h["warm_start"] for h in warm["history"][1:])
+    assert abs(warm["energy"]["total"] - fresh["energy"]["total"]) < 1e-10
+    assert np.abs(warm["eigenvalues"][:4] - fresh["eigenvalues"][:4]).max() < 1e-7
+
+
 def test_scf_kat2_golden():
     from baandi_amd.hamiltonian import GammaSystem
     from baandi_amd.scf import run_scf, ScfOptions

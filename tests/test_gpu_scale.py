@@ -12,8 +12,8 @@ MI355X through the C ABI:
   equals a fresh H_new X to 1e-12;
 * the SCF field kernels (grid transforms, V_ion + V_H + V_xc, residual and
   metric, mixing products and update, energy sums) against the torch
-  reference of the same formulas (baandi_amd.scf's DensityField/Mixer, pinned
-  by the KAT-1 SCF golden).
+  reference of the same formulas (tests/torch_fields.py), and the field
+  mixer's update against the torch mixer's over a sequence of residuals.
 """
 import numpy as np
 import pytest
@@ -255,7 +255,7 @@ def test_kept_fields_warm_start():
 def _field_pair(kat1):
     from baandi_amd.hamiltonian import GammaSystem
     from baandi_amd.field import DensityField
-    from baandi_amd.scf import DensityField as TorchField
+    from torch_fields import DensityField as TorchField
     s = GammaSystem(fcc_rows(kat1["cell_a_bohr"]), [[0, 0, 0], [0.25] * 3], 4.0, kat1["ecut_ha"])
     return s, DensityField(s), TorchField(s)
 
@@ -263,7 +263,7 @@ def _field_pair(kat1):
 def test_field_kernels_match_torch_reference(kat1):
     import math
     import torch
-    from baandi_amd.scf import lda_pz_torch, Mixer
+    from torch_fields import lda_pz_torch, Mixer
     s, f, t = _field_pair(kat1)
     rng = np.random.default_rng(9)
     nd = s.ndensity
@@ -330,3 +330,39 @@ def test_field_kernels_match_torch_reference(kat1):
     for got, want in ((h2, hartree), (x2, xc), (l2, local)):
         assert abs(got - want) < 1e-12 * max(abs(want), 1e-300) + 1e-15
     assert math.isfinite(dr2)
+
+
+def test_field_mixer_matches_torch_mixer(kat1):
+    """The field mixer (Gram products and update in the field kernels, the
+    small eigensolve on the host) against the torch mixer over a sequence of
+    residuals: history 3 so the oldest pairs fall out, both preconditionings,
+    and a repeated (x, f) whose zero difference pair makes the history rank
+    deficient (both mixers then drop pairs down to plain mixing)."""
+    import torch
+    from baandi_amd.field import Mixer as FieldMixer
+    from torch_fields import Mixer
+    s, f, t = _field_pair(kat1)
+    rng = np.random.default_rng(11)
+    nd = s.ndensity
+    zero = torch.from_numpy(t.zero_np).cuda()
+    for pre in ("plain", "thomas_fermi"):
+        a = FieldMixer(f, 0.7, 3, pre)
+        b = Mixer(t, 0.7, 3, pre, s.volume, s.electrons)
+        x = torch.from_numpy(rng.standard_normal(nd) + 1j * rng.standard_normal(nd)).cuda() * 1e-3
+        x[zero] = s.electrons / s.volume
+        worst = 0.0
+        for step in range(7):
+            fr = torch.from_numpy(rng.standard_normal(nd) + 1j * rng.standard_normal(nd)).cuda() * 1e-3 / (step + 1)
+            fr[zero] = 0.0
+            if step == 5:
+                xa = a.update(x, fr)
+                xb = b.update(x, fr)
+            xa = a.update(x, fr)
+            xb = b.update(x, fr)
+            worst = max(worst, ((xa - xb).abs().max() / xb.abs().max()).item())
+            x = xb
+        print(f"[mixer] {pre}: field mixer vs torch mixer, max relative difference {worst:.1e}", flush=True)
+        assert worst < 1e-11
+        assert len(a.pairs) == len(b.pairs)
+    with pytest.raises(ValueError):
+        FieldMixer(f, 0.7, 10)
