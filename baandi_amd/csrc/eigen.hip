@@ -772,6 +772,211 @@ __global__ __launch_bounds__(256) void k_td_update(int m, double* __restrict__ A
 }
 
 // ---------------------------------------------------------------------------
+// 1d. Tridiagonalisation by cooperating workgroups ("workers"), for subspaces
+// whose matrix fits the workers' LDS together (m up to ~720). Column c lives
+// in the LDS of worker c mod nw, so the products and updates of every step
+// read LDS instead of streaming the trailing block through one CU. Step k:
+//  B. every worker reads reflector k (v and tau) and multiplies its columns
+//     c > k by v: its part of y = A22 v, which it publishes;
+//  C. every worker reads all of y, forms w = tau y - (tau^2/2 y.v) v (the
+//     same code on the same data: the same bits everywhere) and updates its
+//     columns by -(v w^T + w v^T), each product rounded on its own so the
+//     matrix stays exactly symmetric. The owner of column k + 1 updates that
+//     column first, forms reflector k + 1 from it (d, e, tau) and publishes
+//     it before its other columns, so the next step starts at once.
+// Values travel as two 8-byte words, each holding 32 bits of the double and
+// the 32-bit tag k + 1 of their step (written and read as single atomic
+// words at agent scope): a reader polls the words until both carry the tag,
+// with no separate flag and no cache flush. Reflectors and parts alternate
+// between two slots by the parity of k (a worker reaches step k + 2 only
+// after every worker has read step k), and the slots are cleared before the
+// launch, so a tag is never stale. The workers are blocks 0 .. nw-1, spread
+// over the XCDs by the round-robin dispatch. Every wait is bounded: a worker
+// that polls a word more than `spins` times (at once when spins < 0), or
+// sees another worker's abort, sets *info = -1 and every worker leaves (the
+// caller redoes the solve with the one-workgroup kernels). The reflectors end
+// in A[k+1:, k] as k_tridiag leaves them.
+// ---------------------------------------------------------------------------
+constexpr int kCT = 256;
+
+__device__ __forceinline__ void ll_put(uint64_t* p, double v, uint32_t tag) {
+  const uint64_t b = uint64_t(__double_as_longlong(v));
+  const uint64_t t = uint64_t(tag) << 32;
+  __hip_atomic_store(p, t | (b & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p + 1, t | (b >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The double at p once both words carry `tag`; false on a timeout or abort.
+__device__ __forceinline__ bool ll_get(const uint64_t* p, uint32_t tag, double* out, long long limit,
+                                       const int* info) {
+  if (limit < 0) return false;
+  long long spins = 0;
+  for (;;) {
+    const uint64_t lo = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t hi = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (uint32_t(lo >> 32) == tag && uint32_t(hi >> 32) == tag) {
+      *out = __longlong_as_double((long long)((hi << 32) | (lo & 0xffffffffull)));
+      return true;
+    }
+    if (++spins > limit) return false;
+    if ((spins & 63) == 0 && __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+// Sum over the kCT-thread block in a fixed order (every worker gets the same
+// bits for the same data). `red` holds kCT/64 doubles; ends with a barrier.
+__device__ double coop_sum(double v, double* red) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int i = 0; i < kCT / 64; ++i) t += red[i];
+  __syncthreads();
+  return t;
+}
+
+size_t coop_lds(int m, int nw) { return size_t((int64_t((m + nw - 1) / nw) * m + 2 * m + kCT / 64 + 2) * 8); }
+// (the last double of the reduction slots holds the workgroup's go flag)
+
+// Words of the exchange slots: two slots of (m + 1) reflector values and two
+// of m product values, two words each.
+int64_t coop_words(int m) { return 4 * int64_t(m + 1) + 4 * int64_t(m); }
+
+// Reflector k from column k (updated through step k - 1) into d, e, tau and
+// the exchange slot: the owner's whole workgroup.
+__device__ void coop_reflector(int m, int k, double* col, double* red, double* d, double* e, double* tau,
+                               uint64_t* vb) {
+  const int tid = threadIdx.x;
+  double ss = 0.0;
+  for (int r = k + 1 + tid; r < m; r += kCT) ss += col[r] * col[r];
+  ss = coop_sum(ss, red);
+  const double x0 = col[k + 1];
+  const double sigma = sqrt(ss);
+  double alpha = 0.0, tk = 0.0;
+  if (sigma > 0.0) {
+    alpha = x0 >= 0.0 ? -sigma : sigma;
+    tk = 1.0 / (sigma * (sigma + fabs(x0)));
+  }
+  const uint32_t tag = uint32_t(k + 1);
+  for (int r = k + 1 + tid; r < m; r += kCT) ll_put(vb + 2 * r, r == k + 1 && sigma > 0.0 ? x0 - alpha : col[r], tag);
+  if (tid == 0) {
+    d[k] = col[k];
+    e[k] = alpha;
+    tau[k] = tk;
+    ll_put(vb + 2 * m, tk, tag);
+  }
+  __syncthreads();
+  if (tid == 0) col[k + 1] = sigma > 0.0 ? x0 - alpha : x0;
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kCT) void k_td_coop(int m, int nw, double* __restrict__ A, int64_t lda,
+                                                 double* __restrict__ d, double* __restrict__ e,
+                                                 double* __restrict__ tau, uint64_t* __restrict__ words,
+                                                 int* __restrict__ info, long long spins) {
+  const int me = blockIdx.x;
+  extern __shared__ __align__(16) double sh[];
+  const int ncl = (m - me + nw - 1) / nw;  // columns me, me + nw, ...
+  double* Al = sh;                         // ncl columns of m rows
+  double* vv = Al + int64_t((m + nw - 1) / nw) * m;
+  double* yw = vv + m;
+  double* red = yw + m;
+  constexpr int NWV = kCT / 64;
+  int& go = *reinterpret_cast<int*>(red + NWV + 1);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  uint64_t* vslot[2] = {words, words + 2 * int64_t(m + 1)};
+  uint64_t* yslot[2] = {words + 4 * int64_t(m + 1), words + 4 * int64_t(m + 1) + 2 * int64_t(m)};
+  if (tid == 0) go = 1;
+  for (int lc = 0; lc < ncl; ++lc) {
+    const int c = me + lc * nw;
+    for (int r = tid; r < m; r += kCT)
+      Al[int64_t(lc) * m + r] = r >= c ? A[r + int64_t(c) * lda] : A[c + int64_t(r) * lda];
+  }
+  __syncthreads();
+  if (me == 0 && m > 2) coop_reflector(m, 0, Al, red, d, e, tau, vslot[0]);
+  for (int k = 0; k + 2 < m; ++k) {
+    const int L = m - k - 1;
+    const uint32_t tag = uint32_t(k + 1);
+    uint64_t* vb = vslot[k & 1];
+    uint64_t* yb = yslot[k & 1];
+    // B. reflector k, and this worker's part of y = A22 v.
+    bool ok = true;
+    for (int i = tid; i < L && ok; i += kCT) ok = ll_get(vb + 2 * (k + 1 + i), tag, &vv[i], spins, info);
+    if (tid == 0 && ok) ok = ll_get(vb + 2 * m, tag, &red[NWV], spins, info);
+    if (!ok) go = 0;
+    __syncthreads();
+    if (!go) break;
+    const double tk = red[NWV];
+    const int lc0 = k + 1 <= me ? 0 : (k + 1 - me + nw - 1) / nw;
+    for (int lc = lc0 + wave; lc < ncl; lc += NWV) {
+      const double* col = Al + int64_t(lc) * m + k + 1;
+      double acc = 0.0;
+      for (int i = lane; i < L; i += 64) acc += col[i] * vv[i];
+      acc = wave_sum(acc);
+      if (lane == 0) ll_put(yb + 2 * (me + lc * nw), acc, tag);
+    }
+    // C. all of y, then w, then the update (column k + 1 and its reflector
+    // first on the owner of k + 1).
+    double s = 0.0;
+    for (int i = tid; i < L && ok; i += kCT) {
+      ok = ll_get(yb + 2 * (k + 1 + i), tag, &yw[i], spins, info);
+      s += yw[i] * vv[i];
+    }
+    if (!ok) go = 0;
+    s = coop_sum(s, red);
+    if (!go) break;
+    const double K = 0.5 * tk * tk * s;
+    for (int i = tid; i < L; i += kCT) yw[i] = tk * yw[i] - K * vv[i];
+    __syncthreads();
+    int first = lc0;
+    if (k + 1 < m - 2 && (k + 1) % nw == me) {
+      double* col = Al + int64_t(lc0) * m;  // column k + 1
+      for (int i = tid; i < L; i += kCT)
+        col[k + 1 + i] = col[k + 1 + i] - __dadd_rn(__dmul_rn(vv[i], yw[0]), __dmul_rn(yw[i], vv[0]));
+      __syncthreads();
+      coop_reflector(m, k + 1, col, red, d, e, tau, vslot[(k + 1) & 1]);
+      first = lc0 + 1;
+    }
+    const int ncols = ncl - first;
+    for (int t = tid; t < ncols * L; t += kCT) {
+      const int lc = first + t / L, i = t - (t / L) * L;
+      const int j = me + lc * nw - (k + 1);
+      double* a = Al + int64_t(lc) * m + k + 1 + i;
+      *a = *a - __dadd_rn(__dmul_rn(vv[i], yw[j]), __dmul_rn(yw[i], vv[j]));
+    }
+    __syncthreads();
+  }
+  if (!go && m > 2) {
+    // Abandoned: the failure is in *info; a zero tridiagonal keeps the
+    // later stages' inputs finite.
+    if (tid == 0) __hip_atomic_store(info, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int i = tid; i < m; i += kCT) {
+      d[i] = 0.0;
+      e[i] = 0.0;
+      tau[i] = 0.0;
+    }
+    return;
+  }
+  // The trailing 2 x 2 and the reflectors into A.
+  for (int lc = 0; lc < ncl; ++lc) {
+    const int c = me + lc * nw;
+    const double* col = Al + int64_t(lc) * m;
+    if (c == m - 2 && tid == 0) {
+      d[m - 2] = col[m - 2];
+      e[m - 2] = col[m - 1];
+      tau[m - 2] = 0.0;
+    } else if (c == m - 1 && tid == 0) {
+      d[m - 1] = col[m - 1];
+      tau[m - 1] = 0.0;
+    } else if (c < m - 2) {
+      for (int r = c + 1 + tid; r < m; r += kCT) A[r + int64_t(c) * lda] = col[r];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // 2. Bisection: one wave per wanted eigenvalue index j (ascending).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int sturm_count(int m, const double* ds, const double* e2, double x, double pivmin) {
@@ -1230,6 +1435,7 @@ struct EigenLayout {
   double* Ytmp;   // m x want
   double* gwork;  // split-K workspace of Y^T Y
   double* td;     // tridiag_multi: V, W (m x kMB), y (m), a scalar
+  uint64_t* words;  // k_td_coop: the exchange slots (coop_words(m))
 };
 
 EigenLayout eigen_layout(int m, int want, double* work) {
@@ -1249,6 +1455,7 @@ EigenLayout eigen_layout(int m, int want, double* work) {
   L.Ytmp = L.G + int64_t(want) * want;
   L.gwork = L.Ytmp + mw;
   L.td = L.gwork + gemm_tn_work(want, want, m);
+  L.words = reinterpret_cast<uint64_t*>(L.td + 2 * int64_t(m) * kMB + m + 8);
   return L;
 }
 }  // namespace
@@ -1270,7 +1477,25 @@ void allow_full_lds(const void* fn, const char* name) {
 int64_t sym_eigen_work(int m, int want) {
   const int64_t mw = int64_t(m) * want;
   return 3 * int64_t(m) + 8 + 5 * mw + (mw + 7) / 8 + 8 + int64_t(want) * want + mw + gemm_tn_work(want, want, m) +
-         2 * int64_t(m) * kMB + m + 8;
+         2 * int64_t(m) * kMB + m + 8 + coop_words(m);
+}
+
+// Cooperative tridiagonalisation: worker count (0: off; -1: m/10 clamped to
+// 8..32, the fastest measured from m = 160 to 576 on MI355X) and the
+// subspace sizes it takes [from, to]; the count grows where the columns do
+// not fit the workers' LDS.
+std::atomic<int> coop_workers{-1};
+std::atomic<int> coop_from{3};
+std::atomic<int> coop_to{1025};
+std::atomic<int> coop_spins{1 << 21};
+
+int coop_worker_count(int m) {
+  int w = coop_workers.load();
+  if (w < 0) w = std::min(32, std::max(8, m / 10));
+  if (w == 0 || m < 3 || m < coop_from.load() || m > coop_to.load()) return 0;
+  for (int nw = w; nw <= 32; ++nw)
+    if (coop_lds(m, nw) <= 160 * 1024) return nw;
+  return 0;
 }
 
 // Subspaces of at least this size are tridiagonalised across the chip.
@@ -1330,8 +1555,14 @@ void sym_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, do
   const EigenLayout L = eigen_layout(m, want, work);
   hipLaunchKernelGGL(k_zero_i, dim3(1), dim3(64), 0, s, info, 1);
   const size_t panel_lds = tridiag_panel_lds(m);
+  const int nw = blocked ? coop_worker_count(m) : 0;
   if (m >= 3 && m >= multi_from.load()) {
     tridiag_multi(s, m, A, lda, L, L.td);
+  } else if (nw > 0) {
+    allow_full_lds(reinterpret_cast<const void*>(&k_td_coop), "k_td_coop");
+    zero_doubles(s, reinterpret_cast<double*>(L.words), coop_words(m));
+    hipLaunchKernelGGL(k_td_coop, dim3(nw), dim3(kCT), coop_lds(m, nw), s, m, nw, A, lda, L.d, L.e, L.tau,
+                       L.words, info, (long long)coop_spins.load());
   } else if (blocked && m >= 3 && panel_lds <= 160 * 1024) {
     if (panel_lds > 64 * 1024) allow_full_lds(reinterpret_cast<const void*>(&k_tridiag_panel), "k_tridiag_panel");
     hipLaunchKernelGGL(k_lower_to_upper, dim3((m + 255) / 256, m), dim3(256), 0, s, m, A, lda);
@@ -1393,6 +1624,25 @@ extern "C" bd_status bd_sym_eigen_multi_from(int32_t* inout, int32_t set) {
   });
 }
 
+extern "C" bd_status bd_sym_eigen_coop(int32_t* inout, int32_t set) {
+  BD_GUARD({
+    if (!inout) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    if (set) {
+      if (inout[0] < -1 || inout[0] > 32 || inout[1] < 3 || inout[2] < inout[1])
+        bd::fail(BD_ERR_INVALID_PARAMETER, "workers -1..32 and 3 <= from <= to");
+      bd::coop_workers.store(inout[0]);
+      bd::coop_from.store(inout[1]);
+      bd::coop_to.store(inout[2]);
+      bd::coop_spins.store(inout[3]);
+    } else {
+      inout[0] = bd::coop_workers.load();
+      inout[1] = bd::coop_from.load();
+      inout[2] = bd::coop_to.load();
+      inout[3] = bd::coop_spins.load();
+    }
+  });
+}
+
 extern "C" int64_t bd_sym_eigen_workspace(int32_t m, int32_t want) {
   if (m <= 0 || want <= 0 || want > m) return 0;
   return bd::sym_eigen_work(m, want) + 1;  // + the status word
@@ -1407,6 +1657,7 @@ extern "C" bd_status bd_sym_eigen_dev(int32_t m, int32_t want, double* A, int64_
     int bad = 0;
     if (hipMemcpy(&bad, info, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
       bd::fail(BD_ERR_DEVICE, "eigensolver launch failed");
+    if (bad < 0) bd::fail(BD_ERR_APPLICATION, "cooperative tridiagonalisation abandoned (a worker waited too long)");
     if (bad) bd::fail(BD_ERR_APPLICATION, "eigenvector " + std::to_string(bad - 1) + " lost orthogonality");
   });
 }

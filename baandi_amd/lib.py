@@ -27,7 +27,7 @@ EXPORTS = [
     "bd_k_op_create", "bd_k_op_info", "bd_k_op_get", "bd_k_batch_create", "bd_k_batch_destroy",
     "bd_k_batch_stream", "bd_k_batch_apply_dev", "bd_gamma_op_autotune", "bd_gamma_op_fused_shape",
     "bd_gamma_op_plane_shape", "bd_gamma_op_plane_split", "bd_gamma_op_keep_fields", "bd_gamma_op_apply_kept_dev",
-    "bd_sym_eigen_multi_from", "bd_field_create", "bd_field_destroy", "bd_field_stream", "bd_field_to_grid_dev",
+    "bd_sym_eigen_multi_from", "bd_sym_eigen_coop", "bd_field_create", "bd_field_destroy", "bd_field_stream", "bd_field_to_grid_dev",
     "bd_field_from_grid_dev", "bd_field_potential_dev", "bd_field_residual_dev", "bd_field_gram_dev",
     "bd_field_mix_dev", "bd_field_energies_dev",
     "bd_gemm_workspace", "bd_gemm_dev", "bd_mfma_probe_dev", "bd_mfma_f64_probe_dev",
@@ -123,6 +123,7 @@ def load():
         "bd_gamma_op_keep_fields": [P, I32],
         "bd_gamma_op_apply_kept_dev": [P, P, I64, I32, P, P, I64],
         "bd_sym_eigen_multi_from": [P, I32],
+        "bd_sym_eigen_coop": [P, I32],
         "bd_field_create": [P, C.POINTER(P)],
         "bd_field_to_grid_dev": [P, P, P],
         "bd_field_from_grid_dev": [P, P, P],

@@ -368,6 +368,16 @@ bd_status bd_sym_eigen_dev(int32_t m, int32_t want, double* A, int64_t lda, doub
  * product with the trailing block on every CU, a tiled panel update) instead
  * of in one workgroup; default 1026. The solver takes m <= 2730. */
 bd_status bd_sym_eigen_multi_from(int32_t* inout, int32_t set);
+/* Read (set = 0) or set {workers, from, to, spins} of the cooperative
+ * tridiagonalisation: for from <= m <= to (and below the multi_from size) the
+ * matrix is split by columns over the LDS of `workers` workgroups (more
+ * where the columns do not fit, at most 32; -1: m/10 clamped to 8..32),
+ * which pass each reflector and each product through tagged words in device
+ * memory; workers = 0 keeps the one-workgroup panel kernel. A worker that
+ * polls a word more than `spins` times (at once when spins < 0) abandons the
+ * solve, which then fails like any eigensolve (the Davidson redoes it with
+ * the one-step kernel). Default {-1, 3, 1025, 2097152}. */
+bd_status bd_sym_eigen_coop(int32_t* inout, int32_t set);
 bd_status bd_inverse_cholesky_dev(const double* S, double* T, int32_t cap, int32_t from, int32_t to,
                                   int32_t* accepted, double* min_ratio);
 /* bd_inverse_cholesky_static_dev: the same extension with the same results,

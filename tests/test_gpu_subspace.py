@@ -183,3 +183,75 @@ def test_inverse_cholesky_wide_border_rejects_dependence(dependent):
     print(f"[factor wide] dependent column {dependent}: accepted={ok}, min ratio {ratio:.3e}, oracle {ok_ref}")
     assert not ok and not ok_ref
     assert ratio < np.finfo(float).eps / 1e-12
+
+
+def _coop(setting=None):
+    """Read, or set and return the previous, {workers, from, to, spins} of the
+    cooperative tridiagonalisation."""
+    from baandi_amd import lib as L
+    v = np.zeros(4, dtype=np.int32)
+    L.check(L.load().bd_sym_eigen_coop(L.ptr(v), 0))
+    old = v.copy()
+    if setting is not None:
+        v[:] = setting
+        L.check(L.load().bd_sym_eigen_coop(L.ptr(v), 1))
+    return old
+
+
+# m and the worker count: one worker (the whole matrix in one LDS), worker
+# counts that do not divide m, the Si64 subspaces, and the largest sizes the
+# workers' LDS holds (Si128's 576 and 700 over 24 and 32 workers).
+@pytest.mark.parametrize("m,want,workers", [(3, 3, 1), (5, 2, 2), (40, 40, 3), (141, 70, 1), (160, 160, 8),
+                                            (240, 160, 5), (320, 160, 8), (320, 160, 16), (321, 100, 7),
+                                            (576, 288, 24), (700, 60, 32)])
+def test_eigen_cooperative_tridiagonalisation(m, want, workers):
+    """The tridiagonalisation split by columns over cooperating workgroups
+    against LAPACK: random, degenerate (exact triples) and realified
+    (exact pairs) matrices."""
+    old = _coop([workers, 3, 1025, 1 << 21])
+    try:
+        rng = np.random.default_rng(m + workers)
+        B = rng.standard_normal((m, m))
+        _check_pairs(0.5 * (B + B.T), want, f"coop x{workers} random")
+        vals = np.sort(rng.uniform(-1.0, 3.0, m))
+        if m >= 9:
+            vals[3:6] = vals[3]
+        Q, _ = np.linalg.qr(rng.standard_normal((m, m)))
+        _check_pairs((Q * vals) @ Q.T, want, f"coop x{workers} degenerate")
+        if m % 2 == 0 and m <= 320:
+            h = m // 2
+            C = rng.standard_normal((h, h)) + 1j * rng.standard_normal((h, h))
+            H = 0.5 * (C + C.conj().T)
+            _check_pairs(np.block([[H.real, -H.imag], [H.imag, H.real]]), min(want, m), f"coop x{workers} realified")
+    finally:
+        _coop(old)
+
+
+@pytest.mark.parametrize("m,want", [(64, 64), (320, 160), (523, 60)])
+def test_eigen_panel_kernel_without_cooperation(m, want):
+    """With the cooperative path off, the one-workgroup panel kernel."""
+    old = _coop([0, 3, 1025, 1 << 21])
+    try:
+        rng = np.random.default_rng(m * 3 + 1)
+        B = rng.standard_normal((m, m))
+        _check_pairs(0.5 * (B + B.T), want, "panel")
+    finally:
+        _coop(old)
+
+
+def test_eigen_cooperative_abandon_is_reported():
+    """A worker that may not wait at all (spins < 0, forced from the host)
+    abandons the solve: bd_sym_eigen_dev reports the failure instead of
+    returning a tridiagonal form, and the next solve with the default
+    setting is correct again."""
+    from baandi_amd import lib as L
+    rng = np.random.default_rng(17)
+    B = rng.standard_normal((200, 200))
+    A = 0.5 * (B + B.T)
+    old = _coop([8, 3, 1025, -1])
+    try:
+        with pytest.raises(L.BaandiError):
+            _eigen_dev(A, 20)
+    finally:
+        _coop(old)
+    _check_pairs(A, 20, "coop after abandon")
