@@ -1485,7 +1485,7 @@ int64_t sym_eigen_work(int m, int want) {
 // subspace sizes it takes [from, to]; the count grows where the columns do
 // not fit the workers' LDS.
 std::atomic<int> coop_workers{-1};
-std::atomic<int> coop_from{3};
+std::atomic<int> coop_from{128};  // smaller subspaces (the k points') stay in one workgroup
 std::atomic<int> coop_to{1025};
 std::atomic<int> coop_spins{1 << 21};
 

@@ -53,6 +53,7 @@ class ScfOptions:
     preconditioning: str = "plain"   # density mixing: "plain" or "thomas_fermi" (refscf/profile.rs:4-7)
     capacity: int = 0                # Davidson subspace capacity (0: 2 bands, ref_davidson.rs:38-78)
     autotune: bool = False           # tune the H·X launch shapes at set-up (bit-neutral)
+    plane_shape: tuple = None        # or fix the plane kernel's (threads, z-chunk rows) instead
     # Optional fixed-point verification map (driver.rs:727-757): one more
     # Davidson at the converged potential with every band at this threshold (Ha).
     final_threshold: float = 0.0
@@ -110,7 +111,11 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
     op = DistributedRealHamiltonian(system, capacity=bands, precision=L.BD_FP64)
     op_mixed = (DistributedRealHamiltonian(system, capacity=bands, precision=L.BD_MIXED)
                 if options.precision == L.BD_MIXED else None)
-    if options.autotune:  # on the ionic potential (the shapes do not depend on V)
+    if options.plane_shape is not None:
+        for o in (op, op_mixed):
+            if o is not None:
+                o.set_plane_shape(*options.plane_shape)
+    elif options.autotune:  # on the ionic potential (the shapes do not depend on V)
         for o in (op, op_mixed):
             if o is not None:
                 o.set_owned_potential(vion.reshape(-1).contiguous())

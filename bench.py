@@ -341,6 +341,9 @@ def main():
                     help="profile mode: only the Si512 leg (device set-up, 1126-column H·X, Davidson iterations)")
     ap.add_argument("--si512-iterations", type=int, default=2,
                     help="FP64 Davidson iterations of the Si512 leg (0 skips the leg)")
+    ap.add_argument("--plane-shape", default=None,
+                    help="THREADS,ROWS: fix the plane kernel's shape instead of autotuning it (profile runs: "
+                         "the kernel's launches are then the SCF's only)")
     ap.add_argument("--hx-only", type=int, default=0,
                     help="profile mode: N applies of H to 160 Si64 columns, then a 256 MiB copy")
     args = ap.parse_args()
@@ -374,7 +377,9 @@ def main():
     # point in a third of the maps of plain mixing on this cell.
     mix = "thomas_fermi"
     # The H·X launch shapes are tuned once per op at set-up (bit-neutral).
-    opts = ScfOptions(bands=wl["bands"], threshold_ry=1e-8, preconditioning=mix, autotune=True)
+    shape = tuple(int(v) for v in args.plane_shape.split(",")) if args.plane_shape else None
+    opts = ScfOptions(bands=wl["bands"], threshold_ry=1e-8, preconditioning=mix, autotune=shape is None,
+                      plane_shape=shape)
     print(f"[bench] {wl['name']}: n={system.n} grid={system.dims} proj={system.nproj} setup={setup_s:.2f}s",
           file=sys.stderr, flush=True)
     for w in range(args.warmup):

@@ -376,7 +376,7 @@ bd_status bd_sym_eigen_multi_from(int32_t* inout, int32_t set);
  * memory; workers = 0 keeps the one-workgroup panel kernel. A worker that
  * polls a word more than `spins` times (at once when spins < 0) abandons the
  * solve, which then fails like any eigensolve (the Davidson redoes it with
- * the one-step kernel). Default {-1, 3, 1025, 2097152}. */
+ * the one-step kernel). Default {-1, 128, 1025, 2097152}. */
 bd_status bd_sym_eigen_coop(int32_t* inout, int32_t set);
 bd_status bd_inverse_cholesky_dev(const double* S, double* T, int32_t cap, int32_t from, int32_t to,
                                   int32_t* accepted, double* min_ratio);
