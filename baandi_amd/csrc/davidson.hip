@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -797,8 +798,10 @@ struct BlockGraph {
     op.gemm.nn(s, n, nb, m, 1.0, HV.p, n, Y.p, m, 0.0, HX.p, n);
   }
 
-  // The iteration's device work (captured once, then replayed).
-  void enqueue() {
+  // The iteration's device work before H·W: corrections, W = E with span(X)
+  // projected out (twice) and weak columns replaced, canonical W into the
+  // second half of V.
+  void before_apply() {
     hipStream_t s = op.stream;
     const unsigned g = unsigned(std::min<int64_t>((n * nb + 255) / 256, 4096));
     hipLaunchKernelGGL(k_corrections, dim3(nb), dim3(kT), 0, s, X.p, HX.p, n, theta.p, diag.p, op.d_kinetic.p, mode,
@@ -816,14 +819,23 @@ struct BlockGraph {
     hipLaunchKernelGGL(k_copy_d, dim3(g), dim3(256), 0, s, X.p, n * nb, V.p);
     hipLaunchKernelGGL(k_copy_d, dim3(g), dim3(256), 0, s, HX.p, n * nb, HV.p);
     canonical(E.p, V.p + n * nb);
+  }
+
+  // The iteration's device work (captured once, then replayed).
+  void enqueue() {
+    before_apply();
     op.apply(V.p + n * nb, n, nb, HV.p + n * nb, n);
     ritz(m2);
   }
 
-  void start(const double* x0, int64_t ldx0) {
+  void start_before_apply(const double* x0, int64_t ldx0) {
     hipStream_t s = op.stream;
     BD_HIP(hipMemcpy2DAsync(E.p, n * 8, x0, ldx0 * 8, n * 8, nb, hipMemcpyDeviceToDevice, s));
     canonical(E.p, V.p);
+  }
+
+  void start(const double* x0, int64_t ldx0) {
+    start_before_apply(x0, ldx0);
     op.apply(V.p, n, nb, HV.p, n);
     ritz(nb);
     check();
@@ -852,11 +864,141 @@ struct BlockGraph {
     if (v[0] || v[1]) fail(BD_ERR_APPLICATION, "block iteration: reduced eigensolve failed");
   }
 };
+// The block iterations of every k point of a map stepped together. Each
+// iteration forks from one stream into the k points' own streams for the
+// per-k work (parallel branches), joins for one fused launch of H·W over all
+// columns of all k points, and forks again for the per-k Rayleigh-Ritz; the
+// whole iteration is captured once as one HIP graph and replayed. The work
+// tables of the two batched applies (start: H·V, step: H·W) hold fixed
+// buffers, so they are uploaded once.
+struct KBlocks {
+  std::vector<GammaOp*> ops;
+  std::vector<std::unique_ptr<BlockGraph>> blocks;
+  int nb;
+  hipStream_t main = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> joins;
+  DevBuf<KWork> wstart, wstep;
+  FusedShape f;
+  size_t lds = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  KBlocks(const std::vector<GammaOp*>& o, int nb_, int mode) : ops(o), nb(nb_) {
+    const GammaOp& o0 = *ops[0];
+    f = o0.fshape;
+    for (GammaOp* op : ops) {
+      if (!op->kb || !op->fused) fail(BD_ERR_INVALID_PARAMETER, "k blocks need k-point ops on the fused path");
+      if (op->sys != o0.sys || op->precision != o0.precision)
+        fail(BD_ERR_INVALID_PARAMETER, "k blocks must share the system and precision");
+      if (!op->fused_valid(f)) fail(BD_ERR_INVALID_PARAMETER, "k blocks need a common fused shape");
+      if (nb <= 0 || 2 * int64_t(nb) > op->n || nb > op->capacity)
+        fail(BD_ERR_INVALID_PARAMETER, "block iteration: bands vs rows and capacity");
+      lds = std::max(lds, op->fused_lds(f));
+      blocks.emplace_back(new BlockGraph(*op, nb, mode));
+    }
+    const int count = int(ops.size());
+    std::vector<KWork> ws(count), wt(count);
+    for (int k = 0; k < count; ++k) {
+      BlockGraph& b = *blocks[k];
+      ws[k] = ops[k]->kwork(b.V.p, b.n, nb, b.HV.p, b.n, k * nb);
+      wt[k] = ops[k]->kwork(b.V.p + b.n * nb, b.n, nb, b.HV.p + b.n * nb, b.n, k * nb);
+    }
+    wstart.alloc(size_t(count));
+    wstep.alloc(size_t(count));
+    BD_HIP(hipMemcpy(wstart.p, ws.data(), sizeof(KWork) * count, hipMemcpyHostToDevice));
+    BD_HIP(hipMemcpy(wstep.p, wt.data(), sizeof(KWork) * count, hipMemcpyHostToDevice));
+    BD_HIP(hipStreamCreateWithFlags(&main, hipStreamNonBlocking));
+    BD_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    joins.resize(count, nullptr);
+    for (auto& e : joins) BD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  ~KBlocks() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto e : joins)
+      if (e) (void)hipEventDestroy(e);
+    if (fork) (void)hipEventDestroy(fork);
+    if (main) (void)hipStreamDestroy(main);
+  }
+
+  // fn(k) on every k point's stream after the work queued on `main`; `main`
+  // then waits for all of them.
+  template <class F>
+  void each(F fn) {
+    BD_HIP(hipEventRecord(fork, main));
+    for (size_t k = 0; k < ops.size(); ++k) {
+      BD_HIP(hipStreamWaitEvent(ops[k]->stream, fork, 0));
+      fn(int(k));
+      BD_HIP(hipEventRecord(joins[k], ops[k]->stream));
+    }
+    for (size_t k = 0; k < ops.size(); ++k) BD_HIP(hipStreamWaitEvent(main, joins[k], 0));
+  }
+
+  // `main` after everything queued on the k points' streams (potentials,
+  // diagonals set since the last call).
+  void order_after_ops() {
+    for (size_t k = 0; k < ops.size(); ++k) {
+      BD_HIP(hipEventRecord(joins[k], ops[k]->stream));
+      BD_HIP(hipStreamWaitEvent(main, joins[k], 0));
+    }
+  }
+
+  void batched_apply(const DevBuf<KWork>& w) {
+    const KWork none{};
+    fused_launch(*ops[0], f, lds, w.p, none, int(ops.size()), int(ops.size()) * nb, main);
+  }
+
+  void enqueue() {
+    each([&](int k) { blocks[k]->before_apply(); });
+    batched_apply(wstep);
+    each([&](int k) { blocks[k]->ritz(2 * nb); });
+  }
+
+  void start(const double* const* x0, const int64_t* ldx0) {
+    order_after_ops();
+    each([&](int k) { blocks[k]->start_before_apply(x0[k], ldx0[k]); });
+    batched_apply(wstart);
+    each([&](int k) { blocks[k]->ritz(nb); });
+    BD_HIP(hipStreamSynchronize(main));
+    for (auto& b : blocks) b->check();
+  }
+
+  void step(bool use_graph) {
+    if (!use_graph) {
+      enqueue();
+      return;
+    }
+    if (!exec) {
+      for (GammaOp* op : ops)
+        if (op->profiling) fail(BD_ERR_INVALID_PARAMETER, "graph capture needs profiling off");
+      BD_HIP(hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal));
+      enqueue();
+      BD_HIP(hipStreamEndCapture(main, &graph));
+      BD_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    BD_HIP(hipGraphLaunch(exec, main));
+  }
+
+  // Ritz values of every k point (count x nb, host) after the queued work;
+  // fails when any reduced eigensolve failed.
+  void values(double* out) {
+    for (size_t k = 0; k < ops.size(); ++k)
+      BD_HIP(hipMemcpyAsync(out + k * nb, blocks[k]->theta.p, size_t(nb) * 8, hipMemcpyDeviceToHost, main));
+    BD_HIP(hipStreamSynchronize(main));
+    for (auto& b : blocks) b->check();
+  }
+};
 }  // namespace bd
 
 struct bd_block {
   bd::BlockGraph g;
   bd_block(bd::GammaOp& op, int nb, int mode) : g(op, nb, mode) {}
+};
+
+struct bd_kblocks {
+  bd::KBlocks g;
+  bd_kblocks(const std::vector<bd::GammaOp*>& ops, int nb, int mode) : g(ops, nb, mode) {}
 };
 
 namespace {
@@ -1034,5 +1176,88 @@ extern "C" bd_status bd_block_vectors_dev(bd_block* b, double* x, double* hx, in
     if (hx)
       BD_HIP(hipMemcpy2DAsync(hx, ld * 8, g.HX.p, g.n * 8, g.n * 8, g.nb, hipMemcpyDeviceToDevice, g.op.stream));
     BD_HIP(hipStreamSynchronize(g.op.stream));
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Block iterations of all k points of a map: per-k work on the k points'
+// streams, one fused H·W launch for all of them, one captured graph
+// ---------------------------------------------------------------------------
+extern "C" bd_status bd_kblocks_create(bd_gamma_op* const* ops, int32_t count, int32_t bands, int32_t preconditioner,
+                                       bd_kblocks** out) {
+  BD_GUARD({
+    if (!ops || !out || count <= 0) bd::fail(BD_ERR_INVALID_PARAMETER, "empty k blocks");
+    if (preconditioner != BD_PRECOND_QE && preconditioner != BD_PRECOND_TPA)
+      bd::fail(BD_ERR_INVALID_PARAMETER, "block iteration: QE or TPA preconditioner");
+    std::vector<bd::GammaOp*> v;
+    for (int i = 0; i < count; ++i) {
+      if (!ops[i]) bd::fail(BD_ERR_INVALID_PARAMETER, "null op in k blocks");
+      for (int j = 0; j < i; ++j)
+        if (ops[j] == ops[i]) bd::fail(BD_ERR_INVALID_PARAMETER, "an op appears twice in k blocks");
+      v.push_back(&ops[i]->op);
+    }
+    *out = new bd_kblocks(v, bands, preconditioner == BD_PRECOND_QE ? 0 : 1);
+  });
+}
+
+extern "C" void bd_kblocks_destroy(bd_kblocks* b) { delete b; }
+
+// The Davidson diagonals of every k point (device, n_k rows each).
+extern "C" bd_status bd_kblocks_set_diagonals_dev(bd_kblocks* b, const double* const* diag) {
+  BD_GUARD({
+    if (!b || !diag) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    auto& g = b->g;
+    for (size_t k = 0; k < g.ops.size(); ++k) {
+      if (!diag[k]) bd::fail(BD_ERR_INVALID_PARAMETER, "null diagonal");
+      BD_HIP(hipMemcpyAsync(g.blocks[k]->diag.p, diag[k], size_t(g.blocks[k]->n) * 8, hipMemcpyDeviceToDevice,
+                            g.ops[k]->stream));
+    }
+    for (auto* op : g.ops) BD_HIP(hipStreamSynchronize(op->stream));
+  });
+}
+
+// Rayleigh-Ritz of every start block (n_k x bands, ldx0[k]); values: count x
+// bands (host).
+extern "C" bd_status bd_kblocks_start_dev(bd_kblocks* b, const double* const* x0, const int64_t* ldx0,
+                                          double* values) {
+  BD_GUARD({
+    if (!b || !x0 || !ldx0 || !values) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    auto& g = b->g;
+    for (size_t k = 0; k < g.ops.size(); ++k) {
+      if (!x0[k] || ldx0[k] < g.ops[k]->n) bd::fail(BD_ERR_INVALID_PARAMETER, "start block");
+      if (!g.ops[k]->has_potential) bd::fail(BD_ERR_INVALID_PARAMETER, "no potential installed");
+    }
+    g.start(x0, ldx0);
+    g.values(values);
+  });
+}
+
+// `steps` iterations of all k points (use_graph: captured on the first call,
+// then replayed); values (count x bands, host) of the last one.
+extern "C" bd_status bd_kblocks_step(bd_kblocks* b, int32_t steps, int32_t use_graph, double* values) {
+  BD_GUARD({
+    if (!b || !values || steps <= 0) bd::fail(BD_ERR_INVALID_PARAMETER, "block step arguments");
+    auto& g = b->g;
+    g.order_after_ops();
+    for (int i = 0; i < steps; ++i) g.step(use_graph != 0);
+    g.values(values);
+  });
+}
+
+// X (and HX when hx is not NULL) of every k point, n_k x bands with leading
+// dimension ld[k].
+extern "C" bd_status bd_kblocks_vectors_dev(bd_kblocks* b, double* const* x, double* const* hx, const int64_t* ld) {
+  BD_GUARD({
+    if (!b || !x || !ld) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    auto& g = b->g;
+    for (size_t k = 0; k < g.ops.size(); ++k) {
+      auto& bl = *g.blocks[k];
+      if (!x[k] || ld[k] < bl.n) bd::fail(BD_ERR_INVALID_PARAMETER, "vector block");
+      BD_HIP(hipMemcpy2DAsync(x[k], ld[k] * 8, bl.X.p, bl.n * 8, bl.n * 8, bl.nb, hipMemcpyDeviceToDevice, g.main));
+      if (hx && hx[k])
+        BD_HIP(hipMemcpy2DAsync(hx[k], ld[k] * 8, bl.HX.p, bl.n * 8, bl.n * 8, bl.nb, hipMemcpyDeviceToDevice,
+                                g.main));
+    }
+    BD_HIP(hipStreamSynchronize(g.main));
   });
 }

@@ -229,6 +229,64 @@ class KPointBatch:
             self.handle = None
 
 
+class KBlocks:
+    """The block iterations of all k points of a map stepped together
+    (bd_kblocks_*): per-k work on each op's stream, one fused H·W launch over
+    every column of every k point, the iteration captured once as a HIP graph.
+    Blocks are CUDA float64 tensors (bands, 2 nk_i) per k point."""
+
+    def __init__(self, ops, bands, preconditioner=L.BD_PRECOND_TPA):
+        lib = L.load()
+        self.ops = list(ops)
+        self.bands = int(bands)
+        arr = (C.c_void_p * len(self.ops))(*[op.handle.value for op in self.ops])
+        h = C.c_void_p()
+        L.check(lib.bd_kblocks_create(arr, len(self.ops), self.bands, int(preconditioner), C.byref(h)))
+        self.handle = h
+        self.values = np.zeros((len(self.ops), self.bands))
+
+    def _ptrs(self, ts):
+        return (C.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
+
+    def _enter(self):
+        for op in self.ops:
+            op.enter()
+
+    def set_diagonals(self, diags):
+        self._enter()
+        L.check(L.load().bd_kblocks_set_diagonals_dev(self.handle, self._ptrs([d.contiguous() for d in diags])))
+
+    def start(self, xs):
+        """Rayleigh-Ritz of every start block; returns the values (count, bands)."""
+        xs = [x.contiguous() for x in xs]
+        for op, x in zip(self.ops, xs):
+            assert x.shape == (self.bands, op.global_len())
+        ld = np.array([x.shape[1] for x in xs], dtype=np.int64)
+        self._enter()
+        L.check(L.load().bd_kblocks_start_dev(self.handle, self._ptrs(xs), L.ptr(ld), L.ptr(self.values)))
+        return self.values.copy()
+
+    def step(self, steps=1, use_graph=True):
+        L.check(L.load().bd_kblocks_step(self.handle, int(steps), 1 if use_graph else 0, L.ptr(self.values)))
+        return self.values.copy()
+
+    def vectors(self, outs):
+        """Copy each k point's Ritz block X into outs[k] (bands, 2 nk)."""
+        for op, o in zip(self.ops, outs):
+            assert o.is_contiguous() and o.shape == (self.bands, op.global_len())
+        ld = np.array([o.shape[1] for o in outs], dtype=np.int64)
+        L.check(L.load().bd_kblocks_vectors_dev(self.handle, self._ptrs(outs), None, L.ptr(ld)))
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            try:
+                L.load().bd_kblocks_destroy(h)
+            except (TypeError, AttributeError):
+                pass
+            self.handle = None
+
+
 def as_real(x):
     """(m, nk) complex128 -> (m, 2 nk) float64 view of the same memory."""
     import torch
@@ -317,6 +375,12 @@ class KScfOptions:
     threads: int = 8                # k points solved concurrently (host threads, one stream each)
     autotune: bool = False          # tune the fused kernel's launch shape at set-up (bit-neutral)
     preconditioning: str = "plain"  # density mixing: "plain" or "thomas_fermi"
+    # "davidson": the reference Davidson per k point, `threads` at a time;
+    # "block": the block iterations of all k points stepped together (one
+    # fused H·W launch for every column of every k point, the iteration
+    # replayed from a captured HIP graph) until the largest Ritz value change
+    # is below the map's threshold.
+    solver: str = "davidson"
 
 
 def _complex_start(nk, bands, seed):
@@ -360,6 +424,9 @@ def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
             except L.BaandiError:
                 pass
     diag_static = [torch.from_numpy(op.get("diagonal_static")).to(dev) for op in ops]
+    if options.solver not in ("davidson", "block"):
+        raise ValueError(f"unknown k-point solver {options.solver!r}")
+    blocks = KBlocks(ops, 2 * b, options.preconditioner) if options.solver == "block" else None
     kinetic = [torch.from_numpy(op.get("kinetic")).to(dev) for op in ops]
     ewald = float(system.get("ewald")[0])
     x = torch.from_numpy(gaussian_start(system)).to(dev)
@@ -384,6 +451,7 @@ def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
         if iteration > 1:
             ethr = next_ethr_ry(1e-2 if iteration == 2 else ethr, last_dr2, electrons)
         redo = False
+        hcol0, dav0 = totals["h_columns"], totals["davidson_iterations"]
         while True:
             tf = time.perf_counter()
             vmean = field.potential(x, vion, potential)  # V_ion + V_H + v_xc of the input density
@@ -396,17 +464,33 @@ def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
                                         max_iterations=options.davidson_max_iterations,
                                         preconditioner=options.preconditioner)
             diags = [d + vmean for d in diag_static]
-            sols = ref_davidson.solve_many(ops, orbitals, diags, [opts] * len(ops), [(o, None) for o in outs],
-                                           threads=options.threads)
-            for sol in sols:
-                totals["h_columns"] += sol.record["h_columns"]
-                totals["eigen_retries"] += sol.record["eigen_retries"]
-                totals["davidson_iterations"] += sol.record["iterations"]
-                totals["dense_s"] += sol.record["seconds_dense"]
-                totals["apply_s"] += sol.record["seconds_apply"]
+            if blocks is None:
+                sols = ref_davidson.solve_many(ops, orbitals, diags, [opts] * len(ops), [(o, None) for o in outs],
+                                               threads=options.threads)
+                for sol in sols:
+                    totals["h_columns"] += sol.record["h_columns"]
+                    totals["eigen_retries"] += sol.record["eigen_retries"]
+                    totals["davidson_iterations"] += sol.record["iterations"]
+                    totals["dense_s"] += sol.record["seconds_dense"]
+                    totals["apply_s"] += sol.record["seconds_apply"]
+                levels = np.stack([s.eigenvalues for s in sols])
+                unconverged = sum(int((~s.converged).sum()) for s in sols)
+            else:
+                blocks.set_diagonals(diags)
+                prev = blocks.start(orbitals)
+                steps, change = 0, math.inf
+                while steps < options.davidson_max_iterations and change >= ethr / 2.0:
+                    levels = blocks.step(1)
+                    change = float(np.abs(levels - prev).max())
+                    prev = levels
+                    steps += 1
+                levels = prev
+                blocks.vectors(outs)
+                unconverged = 0 if change < ethr / 2.0 else int(levels.size)
+                totals["h_columns"] += len(ops) * 2 * b * (steps + 1)
+                totals["davidson_iterations"] += steps
             totals["davidson_s"] += time.perf_counter() - td
             tf = time.perf_counter()
-            levels = np.stack([s.eigenvalues for s in sols])
             occ, mu, ts = fermi_dirac(levels, weights, electrons, options.theta, capacity=1.0)
             rho_dev.zero_()
             for i, op in enumerate(ops):
@@ -423,10 +507,9 @@ def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
             break
         orbitals, outs = outs, orbitals
         last_dr2 = dr2
-        unconverged = sum(int((~s.converged).sum()) for s in sols)
         history.append(dict(iteration=iteration, dr2_ry=dr2, ethr_ry=ethr, mu=mu,
-                            davidson_iterations=sum(s.record["iterations"] for s in sols),
-                            h_columns=sum(s.record["h_columns"] for s in sols), unconverged_levels=unconverged))
+                            davidson_iterations=totals["davidson_iterations"] - dav0,
+                            h_columns=totals["h_columns"] - hcol0, unconverged_levels=unconverged))
         if log:
             log(f"kscf {iteration:3d} dr2={dr2:.3e} ethr={ethr:.2e} mu={mu:.8f} "
                 f"hcols={history[-1]['h_columns']} unconverged={unconverged}")

@@ -167,6 +167,17 @@ def kpoint_leg(steps, threads=16):
         walls.append(r["wall_s"])
         print(f"[bench] al4 k-SCF step {k}: {r['wall_s']:.3f}s maps={r['iterations']} "
               f"E={r['energy']['total']:.12f}", file=sys.stderr, flush=True)
+    # The same SCF with the block iterations of all k points stepped together
+    # (one fused H·W launch per step over every column of every k point, the
+    # step replayed from a captured HIP graph).
+    from dataclasses import replace
+    rb = K.run_scf_k(system, kp, replace(opts, solver="block"))
+    print(f"[bench] al4 k-SCF block solver: {rb['wall_s']:.3f}s maps={rb['iterations']} "
+          f"E={rb['energy']['total']:.12f}", file=sys.stderr, flush=True)
+    block = {"wall_s": round(rb["wall_s"], 4), "scf_iterations": rb["iterations"],
+             "block_steps": rb["totals"]["davidson_iterations"], "h_columns": rb["totals"]["h_columns"],
+             "energy_diff_per_atom_ha": abs(rb["energy"]["total"] - r["energy"]["total"]) / 4,
+             "eigenvalue_max_diff_ha": float(np.abs(rb["eigenvalues"] - r["eigenvalues"]).max())}
     ops = r["ops"]
     xs = [o.contiguous() for o in r["orbitals"]]
     ys = [torch.empty_like(x) for x in xs]
@@ -190,7 +201,7 @@ def kpoint_leg(steps, threads=16):
         "workload": "Al fcc 4-atom cubic a=7.65 Bohr, 6x6x6 MP (108 k), 25 Ry, 16 bands, FD 0.01 Ha, "
                     "Thomas-Fermi mixing, SCF to dr2<1e-8 Ry",
         "k_points": len(kp), "nk_range": [min(o.nk for o in ops), max(o.nk for o in ops)],
-        "wall_s": round(sum(walls) / len(walls), 4), "scf_iterations": r["iterations"],
+        "wall_s": round(sum(walls) / len(walls), 4), "scf_iterations": r["iterations"], "block_solver": block,
         "setup_s": round(r["setup_s"], 3), "concurrent_solves": threads,
         "energy_ha": r["energy"]["total"], "entropy_term_ha": r["energy"]["entropy_term"], "mu_ha": r["mu"],
         "h_columns": r["totals"]["h_columns"], "fused_shape": list(ops[0].fused_shape()),

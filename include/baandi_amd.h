@@ -324,6 +324,31 @@ bd_status bd_block_finish(bd_block* block, double* values);
 /* X (and HX unless NULL), n x bands, leading dimension ld. */
 bd_status bd_block_vectors_dev(bd_block* block, double* x_dev, double* hx_dev, int64_t ld);
 
+/* The block iterations of all k points of a map stepped together (no
+ * reference counterpart: the batched form of the k map's per-k solves,
+ * crates/dft-engine/src/refscf/k_map.rs:181-208): per-k work on the k
+ * points' own streams, one fused launch of H·W over every column of every k
+ * point, the whole iteration captured once as a HIP graph. The ops must be
+ * k-point ops on the fused path of one system and precision (as for
+ * bd_k_batch_create), each at most once; bands as for bd_block_create. */
+typedef struct bd_kblocks bd_kblocks;
+bd_status bd_kblocks_create(bd_gamma_op* const* ops, int32_t count, int32_t bands, int32_t preconditioner,
+                            bd_kblocks** out);
+void bd_kblocks_destroy(bd_kblocks* blocks);
+/* The Davidson diagonal of every k point (device, n_k rows each), copied in. */
+bd_status bd_kblocks_set_diagonals_dev(bd_kblocks* blocks, const double* const* diag_dev);
+/* Rayleigh-Ritz of every start block (n_k x bands, ldx0[k]), ordered after
+ * the work queued on the ops' streams; values: count x bands (host). */
+bd_status bd_kblocks_start_dev(bd_kblocks* blocks, const double* const* x0_dev, const int64_t* ldx0,
+                               double* values);
+/* `steps` iterations of every k point; values (count x bands, host) after
+ * the last. Synchronises once. */
+bd_status bd_kblocks_step(bd_kblocks* blocks, int32_t steps, int32_t use_graph, double* values);
+/* X (and HX where hx_dev and hx_dev[k] are not NULL) of every k point,
+ * n_k x bands with leading dimension ld[k]. */
+bd_status bd_kblocks_vectors_dev(bd_kblocks* blocks, double* const* x_dev, double* const* hx_dev,
+                                 const int64_t* ld);
+
 /* ------------------------------------------------------------------------
  * Dense products of the path as stand-alone entry points
  * C = alpha op(A) B + beta C, column-major, device pointers; op(A) = A^T when

@@ -388,3 +388,79 @@ def test_plane_autotune_is_bitwise_neutral():
         print(f"[plane autotune] precision={prec} threads={int(shape[0])} chunk={int(shape[1])} "
               f"{shape[4]:.1f} us per apply", flush=True)
         assert torch.equal(y0, y1)
+
+
+def test_kblocks_equal_per_k_block_iterations():
+    """All k points of the Al 2x2x2 mesh stepped together (per-k work on the
+    ops' streams, one fused H·W launch over every column of every k point,
+    eager and replayed from the captured graph) give the bits of the per-k
+    block iterations: Ritz values of the start and three steps, and the
+    final Ritz blocks."""
+    import torch
+    from baandi_amd import ref_davidson, lib as L
+    from baandi_amd.kpoint import KBlocks, realified_start
+    system, ops = _al_ops(capacity=16)
+    nb = 16
+    rng = np.random.default_rng(3)
+    starts, diags = [], []
+    for op in ops:
+        z = rng.standard_normal((8, op.nk)) + 1j * rng.standard_normal((8, op.nk))
+        starts.append(realified_start(torch.from_numpy(z / np.linalg.norm(z, axis=1, keepdims=True)).cuda()))
+        diags.append(torch.from_numpy(op.get("diagonal_static") - 0.4).cuda())
+    ref_vals, ref_x = [], []
+    for op, x0, d in zip(ops, starts, diags):
+        it = ref_davidson.BlockIteration(op, nb, L.BD_PRECOND_TPA, graph=False)
+        it.set_diagonal(d)
+        v0 = it.start(x0)
+        vals = [v0] + [it.step() for _ in range(3)]
+        x = torch.empty_like(x0)
+        it.vectors(x)
+        torch.cuda.synchronize()
+        ref_vals.append(vals)
+        ref_x.append(x)
+    for graph in (False, True):
+        kb = KBlocks(ops, nb, L.BD_PRECOND_TPA)
+        kb.set_diagonals(diags)
+        vals = [kb.start(starts)] + [kb.step(1, use_graph=graph) for _ in range(3)]
+        xs = [torch.empty_like(x0) for x0 in starts]
+        kb.vectors(xs)
+        torch.cuda.synchronize()
+        same = all(np.array_equal(vals[s][k], ref_vals[k][s]) for k in range(len(ops)) for s in range(4)) and \
+            all(torch.equal(xs[k], ref_x[k]) for k in range(len(ops)))
+        print(f"[kblocks] {len(ops)} k points, graph={graph}: bitwise equal to the per-k iterations {same}",
+              flush=True)
+        assert same
+
+
+def test_k_scf_block_solver_matches_davidson_and_oracle(kat1):
+    """The k SCF with the block iterations of all k points stepped together
+    (one fused H·W launch per step, the step replayed from a HIP graph)
+    reaches the fixed point of the Davidson k SCF (Al 2x2x2) and of the
+    oracle's dense k SCF (KAT-1 2x2x2)."""
+    from oracle import physics as P
+    from oracle import kpoint as OK
+    from baandi_amd import workloads as W
+    from baandi_amd.hamiltonian import GammaSystem
+    from baandi_amd.kpoint import KScfOptions, run_scf_k, monkhorst_pack
+    wl = W.al4(mesh=(2, 2, 2))
+    system = W.build(wl)
+    kp = monkhorst_pack((2, 2, 2))
+    base = dict(bands=wl["bands"], theta=wl["theta"], threshold_ry=1e-10, preconditioning="thomas_fermi")
+    rd = run_scf_k(system, kp, KScfOptions(**base))
+    rb = run_scf_k(system, kp, KScfOptions(solver="block", **base))
+    de = abs(rd["energy"]["total"] - rb["energy"]["total"])
+    deig = np.abs(rd["eigenvalues"] - rb["eigenvalues"]).max()
+    print(f"[k scf block] Al 2x2x2: davidson {rd['energy']['total']:.12f} ({rd['wall_s']:.2f}s) block "
+          f"{rb['energy']['total']:.12f} ({rb['wall_s']:.2f}s) dE={de:.2e} eig {deig:.2e}", flush=True)
+    assert de < 1e-8 * 4
+    assert deig < 1e-6
+    rows = fcc_rows(kat1["cell_a_bohr"])
+    s2 = GammaSystem(rows, ATOMS, 4.0, kat1["ecut_ha"])
+    r = run_scf_k(s2, kp, KScfOptions(bands=8, theta=0.01, threshold_ry=1e-11, solver="block"))
+    ref = OK.scf_k(rows, ATOMS, P.synthetic_pseudo(4.0, 1.0, 801), kat1["ecut_ha"], (2, 2, 2), 8, 0.01,
+                   threshold_ry=1e-11)
+    de = abs(r["energy"]["total"] - ref["energy"]["total"])
+    print(f"[k scf block] KAT-1 2x2x2: {r['energy']['total']:.12f} oracle {ref['energy']['total']:.12f} "
+          f"dE={de:.2e}", flush=True)
+    assert de < 1e-8 * len(ATOMS)
+    assert np.abs(r["eigenvalues"] - ref["eigenvalues"]).max() < 1e-6
