@@ -331,6 +331,40 @@ def si512_leg(iterations=2, reps=2):
     return out
 
 
+def si512_scf_leg(theta=0.01, variants=True):
+    """Si512 SCF to dr2 < 1e-8 Ry (Fermi-Dirac theta Ha, Thomas-Fermi mixing)
+    in FP64 and, with variants, in the mixed tier: wall time, maps, and the
+    mixed run's energy and eigenvalue error against the FP64 run."""
+    import numpy as np
+    from baandi_amd import workloads, lib as L
+    from baandi_amd.scf import run_scf, ScfOptions
+    wl = workloads.si512()
+    t = time.perf_counter()
+    system = workloads.build(wl, device=True)
+    setup_s = time.perf_counter() - t
+    base = dict(bands=wl["bands"], threshold_ry=1e-8, preconditioning="thomas_fermi", theta=theta)
+    legs = {"fp64": ScfOptions(**base)}
+    if variants:
+        legs["mixed"] = ScfOptions(precision=L.BD_MIXED, **base)
+    out, ref = {"workload": f"{wl['name']}: n={system.n}, grid {list(system.dims)}, {system.nproj} projectors, "
+                            f"{wl['bands']} bands, Fermi-Dirac {theta} Ha, Thomas-Fermi mixing, dr2<1e-8 Ry",
+                "setup_s": round(setup_s, 2)}, None
+    for name, opts in legs.items():
+        r = run_scf(system, opts, log=lambda m, name=name: print(f"[bench] si512 {name} {m}", file=sys.stderr,
+                                                                    flush=True))
+        ref = ref or r
+        t_ = r["totals"]
+        out[name] = {"wall_s": round(r["wall_s"], 2), "scf_iterations": r["iterations"],
+                     "h_columns": t_["h_columns"], "davidson_iterations": t_["davidson_iterations"],
+                     "dense_s": round(t_["dense_s"], 2), "apply_s": round(t_["apply_s"], 2),
+                     "mixed_maps": t_["mixed_maps"], "energy_ha": r["energy"]["total"],
+                     "energy_diff_per_atom_ha": abs(r["energy"]["total"] - ref["energy"]["total"]) / 512,
+                     "eigenvalue_max_diff_ha": float(np.abs(r["eigenvalues"] - ref["eigenvalues"]).max())}
+        print(f"[bench] si512 {name}: {r['wall_s']:.1f}s maps={r['iterations']} E={r['energy']['total']:.10f}",
+              file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -350,6 +384,8 @@ def main():
                     help="learned-preconditioner weights (.npz) for the learned legs (default: the packaged file)")
     ap.add_argument("--si512-only", action="store_true",
                     help="profile mode: only the Si512 leg (device set-up, 1126-column H·X, Davidson iterations)")
+    ap.add_argument("--si512-scf", action="store_true",
+                    help="only the Si512 SCF legs (FP64 and mixed; --mixed-steps 0: FP64 alone), JSON printed")
     ap.add_argument("--si512-iterations", type=int, default=2,
                     help="FP64 Davidson iterations of the Si512 leg (0 skips the leg)")
     ap.add_argument("--plane-shape", default=None,
@@ -371,6 +407,9 @@ def main():
         return
     if args.si128_only:
         print(json.dumps(si128_leg(args.mlp, variants=args.mixed_steps > 0)))
+        return
+    if args.si512_scf:
+        print(json.dumps(si512_scf_leg(variants=args.mixed_steps > 0)))
         return
     if args.si512_only:
         print(json.dumps(si512_leg(max(args.si512_iterations, 1))))
