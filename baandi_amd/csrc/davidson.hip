@@ -649,6 +649,564 @@ struct Solver {
 };
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Complex reference Davidson (ref_davidson_complex.rs:641-1113) for k-point
+// ops: the columns are complex, stored as interleaved (re, im) rows (the k
+// op's own row layout, n = 2 n_k doubles per column), so H·X, the
+// corrections and the normalisation are the real kernels on complex columns.
+// The Hermitian Grams V^H W come from two real products over the
+// interleaved rows (Re = V^T W, Im = -V^T (i W)); a rotation V C with
+// complex C is one real product V [Re C, Im C] and a combination. The small
+// subspace state (overlap, projected H, T = L^{-1}, the reduced matrix, the
+// Ritz coefficients) is kept in its real 2x2-block form (z -> [[re, -im],
+// [im, re]]), where the bordered inverse Cholesky and the products are the
+// real ones; the reduced eigenproblem is solved as the complex Hermitian
+// matrix it is (herm_eigen_lowest), so no eigenvalue is doubled.
+// ---------------------------------------------------------------------------
+namespace {
+// dst[:, j] = i src[:, j] (interleaved complex columns of n doubles).
+__global__ void k_times_i(const double* __restrict__ src, int64_t n, int k, double* __restrict__ dst) {
+  const int j = blockIdx.y;
+  const double2* a = reinterpret_cast<const double2*>(src + int64_t(j) * n);
+  double2* b = reinterpret_cast<double2*>(dst + int64_t(j) * n);
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n / 2; i += int64_t(gridDim.x) * blockDim.x) {
+    const double2 v = a[i];
+    b[i] = make_double2(-v.y, v.x);
+  }
+}
+
+// Realified columns [2 from, 2 to) of M (ld ldm), rows [0, 2 to), from the
+// complex Gram G_ij = P[i, j] - i P[to k + i, j] (P: to x k real parts, then
+// to x k negated imaginary parts, ld to); a diagonal entry's imaginary part
+// is zero (G is Hermitian).
+__global__ void k_scatter_herm(const double* __restrict__ P, int to, int k, int from, double* __restrict__ M,
+                               int64_t ldm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= to) return;
+  const double gr = P[i + int64_t(j) * to];
+  const double gi = (i == from + j) ? 0.0 : -P[int64_t(to) * k + i + int64_t(j) * to];
+  double* c0 = M + int64_t(2 * (from + j)) * ldm;
+  double* c1 = c0 + ldm;
+  c0[2 * i] = gr;
+  c0[2 * i + 1] = gi;
+  c1[2 * i] = -gi;
+  c1[2 * i + 1] = gr;
+}
+
+// Complex A (m x m, ld m) from its symmetric realified form AR (ld ldr).
+__global__ void k_extract_herm(const double* __restrict__ AR, int64_t ldr, int m, double2* __restrict__ A) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= m) return;
+  const double* c0 = AR + int64_t(2 * j) * ldr;
+  const double* c1 = c0 + ldr;
+  const double re = 0.5 * (c0[2 * i] + c1[2 * i + 1]);
+  const double im = i == j ? 0.0 : 0.5 * (c0[2 * i + 1] - c1[2 * i]);
+  A[i + int64_t(j) * m] = make_double2(re, im);
+}
+
+// Realified YR (2m x 2k, ld ldr) of the complex Y (m x k, ld m).
+__global__ void k_realify(const double2* __restrict__ Y, int m, int k, double* __restrict__ YR, int64_t ldr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= m) return;
+  const double2 y = Y[i + int64_t(j) * m];
+  double* c0 = YR + int64_t(2 * j) * ldr;
+  double* c1 = c0 + ldr;
+  c0[2 * i] = y.x;
+  c0[2 * i + 1] = y.y;
+  c1[2 * i] = -y.y;
+  c1[2 * i + 1] = y.x;
+}
+
+// [Re C, Im C] (m x 2k, ld m) of the complex columns cols[j] (or j when cols
+// is null) of the realified CR (ld ldr).
+__global__ void k_coef_parts(const double* __restrict__ CR, int64_t ldr, int m, int k, const int* __restrict__ cols,
+                             double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= m) return;
+  const int c = cols ? cols[j] : j;
+  const double* c0 = CR + int64_t(2 * c) * ldr;
+  out[i + int64_t(j) * m] = c0[2 * i];
+  out[i + int64_t(k + j) * m] = c0[2 * i + 1];
+}
+
+// p[0:count] = -p[0:count].
+__global__ void k_negate(double* __restrict__ p, int64_t count) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = -p[i];
+}
+
+// Y = beta Y + alpha (T1 + i T2) for interleaved complex columns (T12: n x 2k,
+// ld n; the first k columns T1, then T2).
+__global__ void k_combine(const double* __restrict__ T12, int64_t n, int k, double alpha, double beta,
+                          double* __restrict__ Y, int64_t ldy) {
+  const int j = blockIdx.y;
+  const double2* t1 = reinterpret_cast<const double2*>(T12 + int64_t(j) * n);
+  const double2* t2 = reinterpret_cast<const double2*>(T12 + int64_t(k + j) * n);
+  double2* y = reinterpret_cast<double2*>(Y + int64_t(j) * ldy);
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n / 2; i += int64_t(gridDim.x) * blockDim.x) {
+    const double2 a = t1[i], b = t2[i];
+    const double2 v = make_double2(alpha * (a.x - b.y), alpha * (a.y + b.x));
+    y[i] = beta == 0.0 ? v : make_double2(beta * y[i].x + v.x, beta * y[i].y + v.y);
+  }
+}
+
+// out[2 i] = out[2 i + 1] = v[i]: realified diagonal values.
+__global__ void k_dup(const double* __restrict__ v, int k, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < k) {
+    out[2 * i] = v[i];
+    out[2 * i + 1] = v[i];
+  }
+}
+}  // namespace
+
+// Complex Davidson workspace: cap complex columns of n doubles for the
+// n-row blocks; the small state in realified form (capR = 2 cap).
+struct ComplexWork {
+  int64_t n = 0;
+  int nb = 0, cap = 0, capR = 0;
+  Dev V, HV, X, HX, E, E2, W2, T12, P, gwork, theta, norm2, scale, coef;
+  Dev S, H, T, Tt, A, C, YR, small1, small2, small3, vals, vals2, eigw, Ac, Yc, fW, fG, fU, fgwork, fmin;
+  int* idx = nullptr;
+  int* flag = nullptr;
+  int* info = nullptr;
+  FactorWork fw;
+  ComplexWork(int64_t n_, int nb_, int cap_) : n(n_), nb(nb_), cap(cap_), capR(2 * cap_) {
+    const size_t c2 = size_t(capR) * capR;
+    V.alloc(size_t(n) * cap);
+    HV.alloc(size_t(n) * cap);
+    X.alloc(size_t(n) * cap);
+    HX.alloc(size_t(n) * cap);
+    E.alloc(size_t(n) * nb);
+    E2.alloc(size_t(n) * nb);
+    W2.alloc(size_t(n) * cap);
+    T12.alloc(size_t(n) * 2 * cap);
+    P.alloc(size_t(2) * cap * cap);
+    gwork.alloc(size_t(std::max(Gemm::tn_work(cap, cap, n), int64_t(cap) * cap)));
+    theta.alloc(nb);
+    norm2.alloc(cap);
+    scale.alloc(nb);
+    coef.alloc(size_t(2) * cap * cap);
+    S.alloc(c2);
+    H.alloc(c2);
+    T.alloc(c2);
+    Tt.alloc(c2);
+    A.alloc(c2);
+    C.alloc(size_t(capR) * 2 * nb);
+    YR.alloc(size_t(capR) * 2 * nb);
+    small1.alloc(c2);
+    small2.alloc(c2);
+    small3.alloc(c2);
+    vals.alloc(nb);
+    vals2.alloc(2 * nb);
+    eigw.alloc(size_t(sym_eigen_work(cap, nb)) + 1);
+    Ac.alloc(size_t(2) * cap * cap);
+    Yc.alloc(size_t(2) * cap * nb);
+    fW.alloc(c2);
+    fG.alloc(c2);
+    fU.alloc(c2);
+    fgwork.alloc(size_t(factor_gwork(capR)));
+    fmin.alloc(1);
+    BD_HIP(hipMalloc(&info, 2 * sizeof(int)));
+    BD_HIP(hipMalloc(&idx, sizeof(int) * std::max(cap, 1)));
+    BD_HIP(hipMalloc(&flag, sizeof(int)));
+    fw.W = fW.p;
+    fw.G = fG.p;
+    fw.U = fU.p;
+    fw.gwork = fgwork.p;
+    fw.flag = info + 1;
+    fw.min_ratio = fmin.p;
+  }
+  ~ComplexWork() {
+    if (idx) (void)hipFree(idx);
+    if (flag) (void)hipFree(flag);
+    if (info) (void)hipFree(info);
+  }
+};
+
+ComplexWork* complex_work(GammaOp& op, int64_t n, int nb, int cap) {
+  if (!op.cdav || op.cdav->n != n || op.cdav->nb != nb || op.cdav->cap != cap) {
+    delete op.cdav;
+    op.cdav = nullptr;
+    op.cdav = new ComplexWork(n, nb, cap);
+  }
+  return op.cdav;
+}
+void destroy_complex_work(ComplexWork* w) { delete w; }
+
+namespace {
+struct ComplexSolver {
+  GammaOp& op;
+  ComplexWork& w;
+  hipStream_t s;
+  int64_t n;
+  int nb, cap, capR;
+  const bd_davidson_options& o;
+  const double* diag;
+  double* V;
+  double* HV;
+  double* X;
+  double* HX;
+  std::vector<double> values, old;
+  bd_davidson_record rec{};
+  double t_apply = 0.0, t_dense = 0.0;
+  int contractions = 0;
+  bool reset_disabled = false;
+  hipEvent_t ev0, ev1;
+
+  ComplexSolver(GammaOp& op_, ComplexWork& w_, const bd_davidson_options& o_, const double* diag_)
+      : op(op_), w(w_), s(op_.stream), n(w_.n), nb(w_.nb), cap(w_.cap), capR(w_.capR), o(o_), diag(diag_) {
+    V = w.V.p;
+    HV = w.HV.p;
+    X = w.X.p;
+    HX = w.HX.p;
+    values.assign(nb, 0.0);
+    old.assign(nb, 0.0);
+    rec.mlp_fallback = -1;
+    BD_HIP(hipEventCreate(&ev0));
+    BD_HIP(hipEventCreate(&ev1));
+  }
+  ~ComplexSolver() {
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+  }
+
+  double band_threshold(int i) const {
+    if (o.first_step || i < o.occupied) return o.threshold;
+    return std::max(5.0 * o.threshold, 5e-6);
+  }
+  void sync() { BD_HIP(hipStreamSynchronize(s)); }
+  void up_bytes(void* d, const void* h, size_t bytes) {
+    BD_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
+    sync();
+  }
+  void down(double* h, const double* d, size_t count) {
+    BD_HIP(hipMemcpyAsync(h, d, count * 8, hipMemcpyDeviceToHost, s));
+    sync();
+  }
+  unsigned row_blocks() const { return unsigned(std::min<int64_t>((n / 2 + 255) / 256, 256)); }
+
+  void apply(const double* src, int k, double* dst) {
+    BD_HIP(hipEventRecord(ev0, s));
+    op.apply(src, n, k, dst, n);
+    BD_HIP(hipEventRecord(ev1, s));
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, w.flag);
+    hipLaunchKernelGGL(k_finite, dim3(256), dim3(256), 0, s, dst, n * k, w.flag);
+    int bad = 0;
+    BD_HIP(hipMemcpyAsync(&bad, w.flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    sync();
+    float ms = 0.f;
+    BD_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+    t_apply += ms * 1e-3;
+    rec.h_calls += 1;
+    rec.h_columns += k;
+    if (bad) fail(BD_ERR_APPLICATION, "complex H application failed (non-finite output)");
+  }
+
+  // P = [Re(V[:, :rows]^H B), -Im(...)] (rows x k each, ld rows) for k
+  // complex columns B.
+  void gram(int rows, const double* B, int k, double* P) {
+    hipLaunchKernelGGL(k_times_i, dim3(row_blocks(), k), dim3(256), 0, s, B, n, k, w.W2.p);
+    op.gemm.tn(s, rows, k, n, 1.0, V, n, B, n, 0.0, P, rows, w.gwork.p);
+    op.gemm.tn(s, rows, k, n, 1.0, V, n, w.W2.p, n, 0.0, P + int64_t(rows) * k, rows, w.gwork.p);
+  }
+
+  // Y (k complex columns, ld ldy) = alpha src[:, :m] C + beta Y, with C the
+  // realified coefficients CR (ld ldr) at complex columns cols (or 0..k-1).
+  void rotate(const double* src, int m, const double* CR, int64_t ldr, int k, const int* cols, double alpha,
+              double beta, double* Y, int64_t ldy) {
+    if (k <= 0) return;
+    hipLaunchKernelGGL(k_coef_parts, dim3((m + 255) / 256, k), dim3(256), 0, s, CR, ldr, m, k, cols, w.coef.p);
+    op.gemm.nn(s, n, 2 * k, m, 1.0, src, n, w.coef.p, m, 0.0, w.T12.p, n);
+    hipLaunchKernelGGL(k_combine, dim3(row_blocks(), k), dim3(256), 0, s, w.T12.p, n, k, alpha, beta, Y, ldy);
+  }
+
+  // Border columns [from, to) of S (is_h false) or H_r, realified.
+  void border(int from, int to, bool is_h) {
+    const int k = to - from;
+    if (k <= 0) return;
+    const double* B = (is_h ? HV : V) + int64_t(from) * n;
+    double* M = is_h ? w.H.p : w.S.p;
+    gram(to, B, k, w.P.p);
+    hipLaunchKernelGGL(k_scatter_herm, dim3((to + 255) / 256, k), dim3(256), 0, s, w.P.p, to, k, from, M,
+                       int64_t(capR));
+    mirror_border(s, M, capR, 2 * from, 2 * to);
+  }
+
+  bool factor(int from, int to) {
+    auto t0 = std::chrono::steady_clock::now();
+    extend_inverse_cholesky_dev(s, w.S.p, w.T.p, capR, 2 * from, 2 * to, w.fw);
+    int verdict = 0;
+    BD_HIP(hipMemcpyAsync(&verdict, w.fw.flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    sync();
+    t_dense += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return verdict == 0;
+  }
+
+  // A = T H_r T^H (realified), the lowest nb eigenpairs of the complex A,
+  // C = T^H Y (realified).
+  void eigen(int m) {
+    auto t0 = std::chrono::steady_clock::now();
+    const int mR = 2 * m, nbR = 2 * nb;
+    transpose(s, mR, mR, w.T.p, capR, w.Tt.p, mR);
+    gemm_nn(s, mR, mR, mR, 1.0, w.H.p, capR, w.Tt.p, mR, 0.0, w.small3.p, mR);
+    gemm_nn(s, mR, mR, mR, 1.0, w.T.p, capR, w.small3.p, mR, 0.0, w.A.p, mR);
+    symmetrize(s, mR, w.A.p, mR);
+    double2* Ac = reinterpret_cast<double2*>(w.Ac.p);
+    hipLaunchKernelGGL(k_extract_herm, dim3((m + 255) / 256, m), dim3(256), 0, s, w.A.p, int64_t(mR), m, Ac);
+    herm_eigen_lowest(s, m, nb, w.Ac.p, m, w.vals.p, w.Yc.p, m, w.eigw.p, w.info);
+    hipLaunchKernelGGL(k_realify, dim3((m + 255) / 256, nb), dim3(256), 0, s,
+                       reinterpret_cast<const double2*>(w.Yc.p), m, nb, w.YR.p, int64_t(mR));
+    gemm_nn(s, mR, nbR, mR, 1.0, w.Tt.p, mR, w.YR.p, mR, 0.0, w.C.p, mR);
+    int bad = 0;
+    BD_HIP(hipMemcpyAsync(values.data(), w.vals.p, size_t(nb) * 8, hipMemcpyDeviceToHost, s));
+    BD_HIP(hipMemcpyAsync(&bad, w.info, sizeof(int), hipMemcpyDeviceToHost, s));
+    sync();
+    t_dense += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (bad) fail(BD_ERR_APPLICATION, "complex reduced eigensolve failed");
+  }
+
+  void refresh(int m) {
+    border(0, m, false);
+    border(0, m, true);
+    if (!factor(0, m)) fail(BD_ERR_APPLICATION, "complex contracted basis is dependent");
+  }
+
+  bool norm_guard() {
+    hipLaunchKernelGGL(k_norm2_cols, dim3(nb), dim3(kT), 0, s, V, n, w.norm2.p);
+    std::vector<double> nr(nb);
+    down(nr.data(), w.norm2.p, nb);
+    for (double v : nr)
+      if (!std::isfinite(v) || std::fabs(v - 1.0) > kOrthogonalityLimit) return false;
+    return true;
+  }
+
+  void contract(int m) {
+    rotate(V, m, w.C.p, 2 * m, nb, nullptr, 1.0, 0.0, X, n);
+    rotate(HV, m, w.C.p, 2 * m, nb, nullptr, 1.0, 0.0, HX, n);
+    std::swap(V, X);
+    std::swap(HV, HX);
+    contractions += 1;
+    bool ideal = false;
+    if (reset_disabled) {
+    } else if (contractions % kRefreshInterval == 0) {
+      rec.periodic_refreshes += 1;
+    } else if (norm_guard()) {
+      ideal = true;
+      rec.ideal_resets += 1;
+    } else {
+      reset_disabled = true;
+      rec.guard_fallbacks += 1;
+    }
+    const int nbR = 2 * nb;
+    set_identity(s, nbR, nbR, w.C.p, nbR);
+    if (ideal) {
+      set_identity(s, nbR, nbR, w.S.p, capR);
+      set_identity(s, nbR, nbR, w.T.p, capR);
+      hipLaunchKernelGGL(k_dup, dim3((nb + 255) / 256), dim3(256), 0, s, w.vals.p, nb, w.vals2.p);
+      hipLaunchKernelGGL(k_set_diag, dim3((nbR + 255) / 256, nbR), dim3(256), 0, s, nbR, w.vals2.p, w.H.p,
+                         int64_t(capR));
+      BD_HIP(hipGetLastError());
+    } else {
+      refresh(nb);
+      if (!reset_disabled) {
+        hipLaunchKernelGGL(k_identity_defect, dim3(1), dim3(kT), 0, s, nbR, w.S.p, int64_t(capR), w.norm2.p);
+        double defect = 0.0;
+        down(&defect, w.norm2.p, 1);
+        if (!(defect <= kOrthogonalityLimit)) {
+          reset_disabled = true;
+          rec.guard_fallbacks += 1;
+        }
+      }
+    }
+    rec.restarts += 1;
+  }
+
+  std::vector<double> corrections(int m, const std::vector<int>& active, bool contracted) {
+    const int k = int(active.size());
+    std::vector<double> th(k);
+    for (int j = 0; j < k; ++j) th[j] = values[active[j]];
+    up_bytes(w.theta.p, th.data(), size_t(k) * 8);
+    up_bytes(w.idx, active.data(), sizeof(int) * k);
+    if (contracted) {
+      hipLaunchKernelGGL(k_gather_scale, dim3(64, k), dim3(256), 0, s, V, n, w.idx, nullptr, X);
+      hipLaunchKernelGGL(k_gather_scale, dim3(64, k), dim3(256), 0, s, HV, n, w.idx, nullptr, HX);
+    } else {
+      rotate(V, m, w.C.p, 2 * m, k, w.idx, 1.0, 0.0, X, n);
+      rotate(HV, m, w.C.p, 2 * m, k, w.idx, 1.0, 0.0, HX, n);
+    }
+    const int mode = (o.preconditioner == BD_PRECOND_QE) ? 0 : 1;
+    hipLaunchKernelGGL(k_corrections, dim3(k), dim3(kT), 0, s, X, HX, n, w.theta.p, diag, op.d_kinetic.p, mode,
+                       w.E.p, w.norm2.p);
+    std::vector<double> nrm(k);
+    down(nrm.data(), w.norm2.p, k);
+    return nrm;
+  }
+
+  int normalize(const std::vector<double>& norm2, double* dst) {
+    std::vector<int> idx;
+    std::vector<double> sc;
+    for (int j = 0; j < int(norm2.size()); ++j) {
+      const double nrm = std::sqrt(norm2[j]);
+      if (!std::isfinite(nrm)) fail(BD_ERR_APPLICATION, "nonfinite complex correction");
+      if (nrm <= 32.0 * kEps) continue;
+      idx.push_back(j);
+      sc.push_back(1.0 / nrm);
+    }
+    const int kept = int(idx.size());
+    if (kept) {
+      up_bytes(w.idx, idx.data(), sizeof(int) * kept);
+      up_bytes(w.scale.p, sc.data(), size_t(kept) * 8);
+      hipLaunchKernelGGL(k_gather_scale, dim3(64, kept), dim3(256), 0, s, w.E.p, n, w.idx, w.scale.p, dst);
+    }
+    return kept;
+  }
+
+  // E[:, cols] -= V[:, :m] (V^H E) for k complex columns at E.
+  void project_out(int m, double* E, int k) {
+    gram(m, E, k, w.P.p);
+    // [Re G, Im G] (m x 2k, ld m): the Gram's second half negated.
+    hipLaunchKernelGGL(k_negate, dim3((m * k + 255) / 256), dim3(256), 0, s, w.P.p + int64_t(m) * k,
+                       int64_t(m) * k);
+    op.gemm.nn(s, n, 2 * k, m, 1.0, V, n, w.P.p, m, 0.0, w.T12.p, n);
+    hipLaunchKernelGGL(k_combine, dim3(row_blocks(), k), dim3(256), 0, s, w.T12.p, n, k, -1.0, 1.0, E, n);
+  }
+
+  // Rank repair (workspace.rs:586-771) on k normalised corrections in w.E2.
+  int repair(int m, int k) {
+    rec.repairs += 1;
+    const int mR = 2 * m, nbR = 2 * nb;
+    gemm_nn(s, mR, nbR, mR, 1.0, w.S.p, capR, w.C.p, mR, 0.0, w.small2.p, mR);
+    gemm_nn(s, mR, nbR, mR, 1.0, w.T.p, capR, w.small2.p, mR, 0.0, w.C.p, mR);
+    transpose(s, mR, mR, w.T.p, capR, w.small1.p, mR);  // T^T (realified T^H)
+    rotate(V, m, w.small1.p, mR, m, nullptr, 1.0, 0.0, X, n);
+    rotate(HV, m, w.small1.p, mR, m, nullptr, 1.0, 0.0, HX, n);
+    std::swap(V, X);
+    std::swap(HV, HX);
+    refresh(m);
+    double* E = w.E2.p;
+    for (int pass = 0; pass < 2; ++pass) project_out(m, E, k);
+    int kept = 0;
+    for (int j = 0; j < k; ++j) {
+      double* ej = E + int64_t(j) * n;
+      if (kept > 0) {
+        // Against the corrections kept so far (complex, two passes).
+        double* Vsave = V;
+        V = E;
+        for (int pass = 0; pass < 2; ++pass) project_out(kept, ej, 1);
+        V = Vsave;
+      }
+      gemm_tn(s, 1, 1, n, 1.0, ej, n, ej, n, 0.0, w.small1.p, 1, w.gwork.p);
+      double nn2 = 0.0;
+      down(&nn2, w.small1.p, 1);
+      const double nrm = std::sqrt(nn2);
+      if (nrm <= 1e-12 || m + kept == cap) continue;
+      const double inv = 1.0 / nrm;
+      up_bytes(w.idx, &j, sizeof(int));
+      up_bytes(w.scale.p, &inv, 8);
+      hipLaunchKernelGGL(k_gather_scale, dim3(64, 1), dim3(256), 0, s, E, n, w.idx, w.scale.p, E + int64_t(kept) * n);
+      sync();
+      ++kept;
+    }
+    if (kept > 0) {
+      BD_HIP(hipMemcpyAsync(V + int64_t(m) * n, E, size_t(n) * kept * 8, hipMemcpyDeviceToDevice, s));
+      border(m, m + kept, false);
+      if (!factor(m, m + kept)) fail(BD_ERR_APPLICATION, "complex overlap remains dependent after repair");
+    }
+    return kept;
+  }
+
+  bd_status run(const double* x0, int64_t ldx0, const double* hx0, double* eig, double* xout, double* hxout,
+                int64_t ldx, int32_t* conv) {
+    auto t_start = std::chrono::steady_clock::now();
+    zero_doubles(s, w.T.p, int64_t(capR) * capR);
+    BD_HIP(hipMemcpy2DAsync(V, n * 8, x0, ldx0 * 8, n * 8, nb, hipMemcpyDeviceToDevice, s));
+    if (hx0) BD_HIP(hipMemcpy2DAsync(HV, n * 8, hx0, ldx0 * 8, n * 8, nb, hipMemcpyDeviceToDevice, s));
+    else apply(V, nb, HV);
+    border(0, nb, false);
+    if (!factor(0, nb)) fail(BD_ERR_INVALID_PARAMETER, "complex Davidson start is numerically dependent");
+    border(0, nb, true);
+    eigen(nb);
+    int m = nb;
+    std::vector<int> active(nb);
+    std::iota(active.begin(), active.end(), 0);
+    std::vector<char> converged(nb, 0);
+    bool contracted = false;
+    int iterations = 0;
+    for (int step = 0; step < o.max_iterations; ++step) {
+      if (m + int(active.size()) > cap && m > nb) {
+        contract(m);
+        m = nb;
+        contracted = true;
+      }
+      std::vector<double> nrm = corrections(m, active, contracted);
+      contracted = false;
+      int k = 0;
+      bool staged = false;
+      {
+        int kk = 0;
+        for (double v : nrm) kk += (std::sqrt(v) > 32.0 * kEps) ? 1 : 0;
+        staged = m + kk <= cap && kk > 0;
+        k = normalize(nrm, staged ? V + int64_t(m) * n : w.E2.p);
+      }
+      if (!staged || k == 0) {
+        k = k > 0 ? repair(m, k) : 0;
+      } else {
+        border(m, m + k, false);
+        if (!factor(m, m + k)) {
+          BD_HIP(hipMemcpyAsync(w.E2.p, V + int64_t(m) * n, size_t(n) * k * 8, hipMemcpyDeviceToDevice, s));
+          k = repair(m, k);
+        }
+      }
+      std::copy(values.begin(), values.begin() + nb, old.begin());
+      if (k > 0) {
+        apply(V + int64_t(m) * n, k, HV + int64_t(m) * n);
+        border(m, m + k, true);
+        eigen(m + k);
+        m += k;
+      }
+      active.clear();
+      for (int i = 0; i < nb; ++i) {
+        converged[i] = std::fabs(values[i] - old[i]) < band_threshold(i);
+        if (!converged[i]) active.push_back(i);
+      }
+      if (o.profile) {
+        for (int i = 0; i < nb; ++i) {
+          const int64_t at = int64_t(step) * nb + i;
+          if (o.ritz_changes_ha) o.ritz_changes_ha[at] = std::fabs(values[i] - old[i]);
+          if (o.ritz_values_ha) {
+            o.ritz_values_ha[2 * at] = old[i];
+            o.ritz_values_ha[2 * at + 1] = values[i];
+          }
+        }
+      }
+      iterations = step + 1;
+      if (active.empty() || iterations == o.max_iterations) break;
+      if (m + int(active.size()) > cap) {
+        contract(m);
+        m = nb;
+        contracted = true;
+      }
+    }
+    rotate(V, m, w.C.p, 2 * m, nb, nullptr, 1.0, 0.0, xout, ldx);
+    if (hxout) rotate(HV, m, w.C.p, 2 * m, nb, nullptr, 1.0, 0.0, hxout, ldx);
+    sync();
+    bool all = true;
+    for (int i = 0; i < nb; ++i) {
+      eig[i] = values[i];
+      if (conv) conv[i] = converged[i];
+      all = all && converged[i];
+    }
+    rec.iterations = iterations;
+    rec.all_converged = all ? 1 : 0;
+    rec.seconds_apply = t_apply;
+    rec.seconds_dense = t_dense;
+    rec.seconds_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    return all ? BD_OK : BD_NOT_CONVERGED;
+  }
+};
+}  // namespace
+
 DavidsonWork* davidson_work(GammaOp& op, int64_t n, int nb, int cap);
 
 }  // namespace bd
@@ -1012,6 +1570,26 @@ bd_status solve_one(bd_gamma_op* h, const double* x0, int64_t ldx0, const double
     auto& op = h->op;
     const int64_t n = op.n;
     const int nb = o->bands;
+    if (o->complex_columns) {
+      // Complex columns: n / 2 complex rows per column.
+      if (!op.kb) bd::fail(BD_ERR_INVALID_PARAMETER, "complex Davidson needs a k-point op");
+      const int64_t nc = n / 2;
+      const int cap = int(std::min<int64_t>(o->capacity == 0 ? 2 * int64_t(nb) : o->capacity, nc));
+      if (nb <= 0 || nb > nc || cap < std::min<int64_t>(2 * int64_t(nb), nc) || o->occupied > nb ||
+          o->max_iterations <= 0 || !(o->threshold > 0.0) || !std::isfinite(o->threshold) || ldx0 < n ||
+          ldx < n || (o->return_hx && !hxout))
+        bd::fail(BD_ERR_INVALID_PARAMETER, "complex Davidson dimensions/threshold/start");
+      if (nb > op.capacity) bd::fail(BD_ERR_INVALID_PARAMETER, "bands exceed the operator capacity");
+      if (o->preconditioner != BD_PRECOND_QE && o->preconditioner != BD_PRECOND_TPA)
+        bd::fail(BD_ERR_INVALID_PARAMETER, "complex Davidson preconditioner (QE or TPA)");
+      if (!op.has_potential) bd::fail(BD_ERR_INVALID_PARAMETER, "no potential installed");
+      bd::ComplexWork* w = bd::complex_work(op, n, nb, cap);
+      bd::ComplexSolver solver(op, *w, *o, diag);
+      bd_status st = solver.run(x0, ldx0, hx0, eig, xout, o->return_hx ? hxout : nullptr, ldx, conv);
+      if (record) *record = solver.rec;
+      if (st == BD_NOT_CONVERGED) *msg = "complex Davidson reached the iteration limit";
+      return st;
+    }
     const int cap = int(std::min<int64_t>(o->capacity == 0 ? 2 * int64_t(nb) : o->capacity, n));
     if (nb <= 0 || nb > n || cap < std::min<int64_t>(2 * int64_t(nb), n) || o->occupied > nb ||
         o->max_iterations <= 0 || !(o->threshold > 0.0) || !std::isfinite(o->threshold) || ldx0 < n ||

@@ -977,6 +977,173 @@ __global__ __launch_bounds__(kCT) void k_td_coop(int m, int nw, double* __restri
 }
 
 // ---------------------------------------------------------------------------
+// 1e. Complex Hermitian tridiagonalisation (the k-point Davidson's reduced
+// problem): Householder reflectors H_k = I - tau_k v_k v_k^H with v_k[0] = 1
+// chosen so that H_k^H x = beta e1 with beta real (the zlarfg convention of
+// LAPACK's zhetd2), so the tridiagonal T = Q^H A Q is real symmetric and the
+// real bisection / inverse iteration serve it; eigenvectors of A are Q z.
+// One workgroup; A (complex, both triangles, interleaved re/im, ld lda
+// complex entries) stays in global memory; each step updates both triangles
+// from the same rounded products, so A stays exactly Hermitian. v_k ends in
+// A[k+1:, k] with A[k+1, k] = 1.
+// ---------------------------------------------------------------------------
+constexpr int kZT = 256;
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(__dadd_rn(__dmul_rn(a.x, b.x), -__dmul_rn(a.y, b.y)),
+                      __dadd_rn(__dmul_rn(a.x, b.y), __dmul_rn(a.y, b.x)));
+}
+// a conj(b)
+__device__ __forceinline__ double2 cmulc(double2 a, double2 b) {
+  return make_double2(__dadd_rn(__dmul_rn(a.x, b.x), __dmul_rn(a.y, b.y)),
+                      __dadd_rn(__dmul_rn(a.y, b.x), -__dmul_rn(a.x, b.y)));
+}
+
+// Sum over the kZT-thread block of (re, im) pairs, to every thread.
+__device__ double2 zblock_sum(double2 v, double* red) {
+  v.x = wave_sum(v.x);
+  v.y = wave_sum(v.y);
+  if ((threadIdx.x & 63) == 0) {
+    red[2 * (threadIdx.x >> 6)] = v.x;
+    red[2 * (threadIdx.x >> 6) + 1] = v.y;
+  }
+  __syncthreads();
+  double2 t = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int i = 0; i < kZT / 64; ++i) {
+    t.x += red[2 * i];
+    t.y += red[2 * i + 1];
+  }
+  __syncthreads();
+  return t;
+}
+
+__global__ __launch_bounds__(kZT) void k_zhe_tridiag(int m, double2* __restrict__ A, int64_t lda,
+                                                     double* __restrict__ d, double* __restrict__ e,
+                                                     double2* __restrict__ tau) {
+  extern __shared__ double2 zsh[];
+  double2* v = zsh;      // m: the reflector
+  double2* w = v + m;    // m: p, then w
+  double* red = reinterpret_cast<double*>(w + m);  // 2 * kZT / 64
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  auto a = [&](int i, int j) -> double2& { return A[i + int64_t(j) * lda]; };
+  for (int k = 0; k + 1 < m; ++k) {
+    const int L = m - k - 1;
+    double xs = 0.0;
+    for (int i = k + 2 + tid; i < m; i += kZT) {
+      const double2 x = a(i, k);
+      xs += x.x * x.x + x.y * x.y;
+    }
+    const double xnorm2 = zblock_sum(make_double2(xs, 0.0), red).x;
+    const double2 alpha = a(k + 1, k);
+    double beta = alpha.x;
+    double2 tk = make_double2(0.0, 0.0);
+    double2 scale = make_double2(0.0, 0.0);
+    if (xnorm2 > 0.0 || alpha.y != 0.0) {
+      const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xnorm2);
+      beta = alpha.x >= 0.0 ? -nrm : nrm;
+      tk = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+      // scale = 1 / (alpha - beta)
+      const double2 den = make_double2(alpha.x - beta, alpha.y);
+      const double dd = den.x * den.x + den.y * den.y;
+      scale = make_double2(den.x / dd, -den.y / dd);
+    }
+    // v = (1, x[1:] * scale) into LDS and A[k+1:, k].
+    for (int i = tid; i < L; i += kZT) {
+      double2 vi = make_double2(1.0, 0.0);
+      if (i > 0) vi = (tk.x != 0.0 || tk.y != 0.0) ? cmul(a(k + 1 + i, k), scale) : a(k + 1 + i, k);
+      v[i] = vi;
+    }
+    __syncthreads();
+    for (int i = tid; i < L; i += kZT) a(k + 1 + i, k) = v[i];
+    if (tid == 0) {
+      d[k] = a(k, k).x;
+      e[k] = beta;
+      tau[k] = tk;
+    }
+    if (tk.x == 0.0 && tk.y == 0.0) {
+      __syncthreads();
+      continue;
+    }
+    // p = tau A22 v: a wave per row i, lanes down column i (A(i, j) =
+    // conj(A(j, i)), contiguous reads).
+    for (int i = wave; i < L; i += kZT / 64) {
+      double2 acc = make_double2(0.0, 0.0);
+      for (int j = lane; j < L; j += 64) {
+        const double2 aji = a(k + 1 + j, k + 1 + i);
+        const double2 vj = v[j];
+        acc.x += aji.x * vj.x + aji.y * vj.y;
+        acc.y += aji.x * vj.y - aji.y * vj.x;
+      }
+      acc.x = wave_sum(acc.x);
+      acc.y = wave_sum(acc.y);
+      if (lane == 0) w[i] = cmul(tk, acc);
+    }
+    __syncthreads();
+    // alpha2 = -tau/2 (p^H v); w = p + alpha2 v.
+    double2 pv = make_double2(0.0, 0.0);
+    for (int i = tid; i < L; i += kZT) {
+      const double2 c = cmulc(v[i], w[i]);  // conj(p_i) v_i
+      pv.x += c.x;
+      pv.y += c.y;
+    }
+    pv = zblock_sum(pv, red);
+    const double2 al2 = cmul(make_double2(-0.5 * tk.x, -0.5 * tk.y), pv);
+    for (int i = tid; i < L; i += kZT) {
+      const double2 p = w[i], t = cmul(al2, v[i]);
+      w[i] = make_double2(p.x + t.x, p.y + t.y);
+    }
+    __syncthreads();
+    // A22 -= v w^H + w v^H, both triangles (entry (j, i) is the conjugate of
+    // entry (i, j) to the bit).
+    for (int t = tid; t < L * L; t += kZT) {
+      const int i = t % L, j = t / L;
+      const double2 p1 = cmulc(v[i], w[j]), p2 = cmulc(w[i], v[j]);
+      double2& aij = a(k + 1 + i, k + 1 + j);
+      aij = make_double2(aij.x - __dadd_rn(p1.x, p2.x), aij.y - __dadd_rn(p1.y, p2.y));
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    d[m - 1] = a(m - 1, m - 1).x;
+    tau[m - 1] = make_double2(0.0, 0.0);
+  }
+}
+
+// Y (complex, m x want) = Q Z with Z real (m x want, ld ldz): a wave per
+// vector, reflectors from A[k+1:, k] (v[0] = 1) applied last to first.
+__global__ __launch_bounds__(256) void k_zback_transform(int m, int want, const double2* __restrict__ A, int64_t lda,
+                                                         const double2* __restrict__ tau,
+                                                         const double* __restrict__ Z, int64_t ldz,
+                                                         double2* __restrict__ Y, int64_t ldy) {
+  extern __shared__ double2 ysh[];
+  const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
+  const int col = blockIdx.x * (blockDim.x >> 6) + wl;
+  if (col >= want) return;
+  double2* y = ysh + int64_t(wl) * m;
+  for (int i = lane; i < m; i += 64) y[i] = make_double2(Z[i + int64_t(col) * ldz], 0.0);
+  for (int k = m - 2; k >= 0; --k) {
+    const double2 t = tau[k];
+    if (t.x == 0.0 && t.y == 0.0) continue;
+    const double2* v = A + (k + 1) + int64_t(k) * lda;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int i = lane; i < m - k - 1; i += 64) {
+      const double2 c = cmulc(y[k + 1 + i], v[i]);  // conj(v_i) y_i
+      acc.x += c.x;
+      acc.y += c.y;
+    }
+    acc.x = wave_sum(acc.x);
+    acc.y = wave_sum(acc.y);
+    const double2 sc = cmul(t, acc);
+    for (int i = lane; i < m - k - 1; i += 64) {
+      const double2 u = cmul(sc, v[i]);
+      y[k + 1 + i] = make_double2(y[k + 1 + i].x - u.x, y[k + 1 + i].y - u.y);
+    }
+  }
+  for (int i = lane; i < m; i += 64) Y[i + int64_t(col) * ldy] = y[i];
+}
+
+// ---------------------------------------------------------------------------
 // 2. Bisection: one wave per wanted eigenvalue index j (ascending).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int sturm_count(int m, const double* ds, const double* e2, double x, double pivmin) {
@@ -1544,6 +1711,9 @@ void tridiag_multi(hipStream_t s, int m, double* A, int64_t lda, const EigenLayo
                      L.tau);
 }
 
+void tridiag_vectors(hipStream_t s, int m, int want, const EigenLayout& L, double* values, double* Ysc, int64_t ldy,
+                     int* info);
+
 void sym_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, double* values, double* Y,
                       int64_t ldy, double* work, int* info, bool blocked) {
   if (m <= 0 || want <= 0 || want > m || lda < m || ldy < m || lda * m >= (int64_t(1) << 31))
@@ -1570,6 +1740,25 @@ void sym_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, do
   } else {
     tridiag_unblocked(s, m, A, lda, L);
   }
+  tridiag_vectors(s, m, want, L, values, Y, ldy, info);
+  if (bt_ch == 8) {
+    if (bt_lds(8) > 64 * 1024) allow_full_lds(reinterpret_cast<const void*>(&k_back_transform<8>), "k_back_transform");
+    hipLaunchKernelGGL(k_back_transform<8>, dim3((want + 3) / 4), dim3(256), bt_lds(8), s, m, want, A, lda, L.tau,
+                       L.Ytmp, int64_t(m));
+  } else {
+    allow_full_lds(reinterpret_cast<const void*>(&k_back_transform<4>), "k_back_transform");
+    hipLaunchKernelGGL(k_back_transform<4>, dim3((want + 3) / 4), dim3(256), bt_lds(4), s, m, want, A, lda, L.tau,
+                       L.Ytmp, int64_t(m));
+  }
+  copy_columns(s, m, want, L.Ytmp, m, Y, ldy);
+}
+
+// Lowest `want` eigenpairs of the real tridiagonal (L.d, L.e): values, and
+// orthonormal eigenvectors in L.Ytmp (m x want, ld m); Ysc (m x want, ld
+// ldy) is scratch for the cluster Gram-Schmidt output.
+void tridiag_vectors(hipStream_t s, int m, int want, const EigenLayout& L, double* values, double* Ysc, int64_t ldy,
+                     int* info) {
+  double* Y = Ysc;
   hipLaunchKernelGGL(k_bisect, dim3((want + 3) / 4), dim3(256), size_t(3 * m) * 8, s, m, want, L.d, L.e, values,
                      L.tnorm);
   auto invit_lds = [m](int v) { return size_t(2 * m + 5 * int64_t(m) * v) * 8 + size_t(m) * v; };
@@ -1591,16 +1780,35 @@ void sym_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, do
   gemm_tn(s, want, want, m, 1.0, Y, ldy, Y, ldy, 0.0, L.G, want, L.gwork);
   hipLaunchKernelGGL(k_loewdin_factor, dim3((want + 255) / 256, want), dim3(256), 0, s, want, L.G);
   gemm_nn(s, m, want, want, 1.0, Y, ldy, L.G, want, 0.0, L.Ytmp, m);
-  if (bt_ch == 8) {
-    if (bt_lds(8) > 64 * 1024) allow_full_lds(reinterpret_cast<const void*>(&k_back_transform<8>), "k_back_transform");
-    hipLaunchKernelGGL(k_back_transform<8>, dim3((want + 3) / 4), dim3(256), bt_lds(8), s, m, want, A, lda, L.tau,
-                       L.Ytmp, int64_t(m));
-  } else {
-    allow_full_lds(reinterpret_cast<const void*>(&k_back_transform<4>), "k_back_transform");
-    hipLaunchKernelGGL(k_back_transform<4>, dim3((want + 3) / 4), dim3(256), bt_lds(4), s, m, want, A, lda, L.tau,
-                       L.Ytmp, int64_t(m));
-  }
-  copy_columns(s, m, want, L.Ytmp, m, Y, ldy);
+}
+
+// Lowest `want` eigenpairs of the complex Hermitian A (m x m, both triangles,
+// interleaved (re, im), ld lda complex entries; overwritten): the complex
+// tridiagonalisation, the real tridiagonal stage, the complex back-transform.
+// Y: m x want complex, ld ldy complex entries (ldy >= m).
+void herm_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, double* values, double* Y,
+                       int64_t ldy, double* work, int* info) {
+  if (m <= 0 || want <= 0 || want > m || lda < m || ldy < m || lda * m >= (int64_t(1) << 30))
+    fail(BD_ERR_INVALID_PARAMETER, "herm_eigen_lowest: shape");
+  if ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(Y)) & 15)
+    fail(BD_ERR_INVALID_PARAMETER, "herm_eigen_lowest: A and Y must be 16-byte aligned (complex entries)");
+  const size_t zt_lds = size_t(2 * m) * 16 + size_t(2 * (kZT / 64)) * 8;
+  const size_t zb_lds = size_t(4 * m) * 16;
+  if (size_t(3 * m) * 8 > 64 * 1024 || zt_lds > 160 * 1024 || zb_lds > 160 * 1024)
+    fail(BD_ERR_INVALID_PARAMETER, "herm_eigen_lowest: subspace too large for the LDS kernels (m <= 2560)");
+  const EigenLayout L = eigen_layout(m, want, work);
+  double2* Ac = reinterpret_cast<double2*>(A);
+  // Complex tau (m), 16-byte aligned in the across-the-chip scratch (2 m kMB + m + 8 doubles).
+  double* td = L.td + ((reinterpret_cast<uintptr_t>(L.td) & 15) ? 1 : 0);
+  double2* tc = reinterpret_cast<double2*>(td);
+  hipLaunchKernelGGL(k_zero_i, dim3(1), dim3(64), 0, s, info, 1);
+  if (zt_lds > 64 * 1024) allow_full_lds(reinterpret_cast<const void*>(&k_zhe_tridiag), "k_zhe_tridiag");
+  hipLaunchKernelGGL(k_zhe_tridiag, dim3(1), dim3(kZT), zt_lds, s, m, Ac, lda, L.d, L.e, tc);
+  // The real stage's scratch is Y itself (m x want doubles fit its 2 ldy x want).
+  tridiag_vectors(s, m, want, L, values, Y, int64_t(m), info);
+  if (zb_lds > 64 * 1024) allow_full_lds(reinterpret_cast<const void*>(&k_zback_transform), "k_zback_transform");
+  hipLaunchKernelGGL(k_zback_transform, dim3((want + 3) / 4), dim3(256), zb_lds, s, m, want, Ac, lda, tc, L.Ytmp,
+                     int64_t(m), reinterpret_cast<double2*>(Y), ldy);
 }
 
 void zero_ints(hipStream_t s, int* p, int count) {
@@ -1640,6 +1848,23 @@ extern "C" bd_status bd_sym_eigen_coop(int32_t* inout, int32_t set) {
       inout[2] = bd::coop_to.load();
       inout[3] = bd::coop_spins.load();
     }
+  });
+}
+
+// Lowest `want` eigenpairs of a complex Hermitian matrix (both triangles,
+// interleaved (re, im), ld lda complex entries; overwritten); Y complex
+// m x want (ld ldy complex entries); work: bd_sym_eigen_workspace(m, want)
+// doubles.
+extern "C" bd_status bd_herm_eigen_dev(int32_t m, int32_t want, double* A, int64_t lda, double* values, double* Y,
+                                       int64_t ldy, double* work) {
+  BD_GUARD({
+    if (!A || !values || !Y || !work) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    int* info = reinterpret_cast<int*>(work + bd::sym_eigen_work(m, want));
+    bd::herm_eigen_lowest(nullptr, m, want, A, lda, values, Y, ldy, work, info);
+    int bad = 0;
+    if (hipMemcpy(&bad, info, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+      bd::fail(BD_ERR_DEVICE, "eigensolver launch failed");
+    if (bad) bd::fail(BD_ERR_APPLICATION, "eigenvector " + std::to_string(bad - 1) + " lost orthogonality");
   });
 }
 

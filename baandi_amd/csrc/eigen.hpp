@@ -23,6 +23,11 @@ int64_t sym_eigen_work(int m, int want);
 // where it fits (else the one-step kernel).
 void sym_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, double* values, double* Y,
                       int64_t ldy, double* work, int* info, bool blocked = true);
+// The same for a complex Hermitian A (both triangles, interleaved (re, im),
+// ld lda complex entries): Y complex m x want (ld ldy complex entries); the
+// same workspace size.
+void herm_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, double* values, double* Y,
+                       int64_t ldy, double* work, int* info);
 
 // Device state of the bordered inverse Cholesky of the subspace overlap.
 struct FactorWork {

@@ -842,6 +842,7 @@ GammaOp::GammaOp(const System& s, const KBasis* k, int cap, int prec) : sys(&s),
 
 GammaOp::~GammaOp() {
   destroy_davidson_work(dav);
+  destroy_complex_work(cdav);
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);

@@ -84,6 +84,8 @@ struct FusedShape {
 
 struct DavidsonWork;
 void destroy_davidson_work(DavidsonWork* w);
+struct ComplexWork;
+void destroy_complex_work(ComplexWork* w);
 
 // The H·X operator of one Gamma cell (ORC rows), or of one k point of it
 // (kb != nullptr: rows are interleaved (Re, Im) of complex coefficients, one
@@ -92,6 +94,7 @@ struct GammaOp {
   const System* sys;
   const KBasis* kb = nullptr;
   DavidsonWork* dav = nullptr;  // Davidson workspace cached per (n, bands, cap)
+  ComplexWork* cdav = nullptr;  // complex Davidson workspace (k-point ops), likewise
   int64_t n, nrep, N;
   int dims[3];
   int capacity;

@@ -8,12 +8,13 @@ tensor of shape (m, nk) is the same memory as ``torch.view_as_real(x)``
 reshaped to (m, 2 nk), which is what the kernels read.
 
 ``solve`` is ref_davidson_complex::solve_with_initial_hx
-(crates/dft-solvers/src/ref_davidson_complex.rs:641-1113) run as the GPU
-block Davidson on the realified operator: the start block is the pairs
-(x_j, i x_j), so the real search space stays closed under multiplication by
-i and its Ritz values are the complex ones, each twice. The complex
-eigenvectors are read back from the Ritz pairs by a complex Gram-Schmidt in
-eigenvalue order.
+(crates/dft-solvers/src/ref_davidson_complex.rs:641-1113): the GPU complex
+Hermitian Davidson on complex columns (Hermitian Grams, the complex reduced
+eigenproblem, each Ritz value once). ``solve(..., realified=True)`` keeps the
+earlier form, the real Davidson on the realified operator from the pairs
+(x_j, i x_j): the real search space stays closed under multiplication by i,
+its Ritz values are the complex ones each twice, and the complex eigenvectors
+are read back from the Ritz pairs by a complex Gram-Schmidt.
 
 ``monkhorst_pack`` and ``fermi_dirac`` follow crates/dft-geometry/src/kpoints.rs
 and crates/dft-physics/src/occupations.rs:57-232; ``run_scf_k`` is the
@@ -339,17 +340,24 @@ def complex_ritz(values, vectors, bands, keep=0.5):
 class KSolution:
     eigenvalues: np.ndarray   # (bands,) complex-problem eigenvalues, Ha
     eigenvectors: object      # (bands, nk) complex128 CUDA
-    realified: object         # ref_davidson.Solution of the realified problem (2 bands)
+    realified: object         # the underlying ref_davidson.Solution (2 bands if realified)
     converged: np.ndarray
     record: dict
 
 
-def solve(operator: KPointHamiltonian, start, hamiltonian_diagonal, options: ref_davidson.Options):
+def solve(operator: KPointHamiltonian, start, hamiltonian_diagonal, options: ref_davidson.Options,
+          realified=False, initial_hx=None):
     """Lowest ``options.bands`` eigenpairs of H_k from a complex start block
     (bands, nk). ``hamiltonian_diagonal``: (2 nk,) per row. The options are the
-    complex problem's; the realified solve runs with twice the bands,
-    occupied count and capacity."""
+    complex problem's. With ``realified`` the real Davidson runs on the
+    realified operator with twice the bands, occupied count and capacity."""
     b = options.bands
+    if not realified:
+        x0 = as_real(start).contiguous()
+        hx0 = as_real(initial_hx).contiguous() if initial_hx is not None else None
+        sol = ref_davidson.solve_with_initial_hx(operator, x0, hamiltonian_diagonal.contiguous(), options,
+                                                 initial_hx=hx0, complex_columns=True)
+        return KSolution(sol.eigenvalues, as_complex(sol.eigenvectors), sol, sol.converged, sol.record)
     ro = ref_davidson.Options(bands=2 * b, threshold=options.threshold, occupied=2 * options.occupied,
                               first_step=options.first_step, capacity=2 * options.capacity,
                               max_iterations=options.max_iterations, return_hx=options.return_hx,
@@ -379,8 +387,12 @@ class KScfOptions:
     # "block": the block iterations of all k points stepped together (one
     # fused H·W launch for every column of every k point, the iteration
     # replayed from a captured HIP graph) until the largest Ritz value change
-    # is below the map's threshold.
+    # is below the map's threshold (realified columns only).
     solver: str = "davidson"
+    # "complex": the complex Hermitian Davidson on `bands` complex columns
+    # (ref_davidson_complex.rs); "realified": the real Davidson on 2 bands
+    # realified columns (the pairs (x, i x)).
+    columns: str = "complex"
 
 
 def _complex_start(nk, bands, seed):
@@ -397,10 +409,12 @@ def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
     Fermi-Dirac occupations; returns the Mermin free energy terms (Ha),
     eigenvalues (nk, bands), occupations, mu and timings.
 
-    Each k point's Davidson is the realified one (2 bands real columns, the
-    pairs (x, i x) of the complex start). A real Ritz level holds one
-    electron (its pair partner the other), so the occupations, the density
-    weights and the entropy are taken over the real levels with capacity 1."""
+    Each k point's Davidson is the complex one on `bands` complex columns
+    (levels of capacity 2), or with ``columns="realified"`` the realified one
+    (2 bands real columns, the pairs (x, i x) of the complex start): a real
+    Ritz level then holds one electron (its pair partner the other), so the
+    occupations, the density weights and the entropy are taken over the real
+    levels with capacity 1."""
     import torch
     from .field import DensityField, Mixer, gaussian_start
     from .scf import next_ethr_ry
@@ -412,26 +426,34 @@ def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
     field = DensityField(system)
     vion_g = torch.from_numpy(system.get("ionic_density_re") + 1j * system.get("ionic_density_im")).to(dev)
     vion = field.to_grid(vion_g).real.contiguous()
-    ops = [KPointHamiltonian(system, k, capacity=2 * b, precision=options.precision) for k, _ in kpoints]
+    if options.solver not in ("davidson", "block"):
+        raise ValueError(f"unknown k-point solver {options.solver!r}")
+    if options.columns not in ("complex", "realified"):
+        raise ValueError(f"unknown k-point columns {options.columns!r}")
+    cplx = options.columns == "complex" and options.solver == "davidson"
+    if options.solver == "block" and options.columns != "realified":
+        raise ValueError("the block k-point solver runs on realified columns (columns='realified')")
+    nb = b if cplx else 2 * b          # Davidson columns per k point
+    level_capacity = 2.0 if cplx else 1.0
+    ops = [KPointHamiltonian(system, k, capacity=nb, precision=options.precision) for k, _ in kpoints]
     if options.autotune and ops[0].fused_shape()[0] > 0:
         # Tuned on the first k point with the ionic potential, then shared by
         # every k point that accepts it (bases differ by a few coefficients).
         ops[0].set_owned_potential(vion.reshape(-1).contiguous())
-        shape = ops[0].autotune(2 * b, reps=3)[:4]
+        shape = ops[0].autotune(nb, reps=3)[:4]
         for op in ops[1:]:
             try:
                 op.set_fused_shape(shape)
             except L.BaandiError:
                 pass
     diag_static = [torch.from_numpy(op.get("diagonal_static")).to(dev) for op in ops]
-    if options.solver not in ("davidson", "block"):
-        raise ValueError(f"unknown k-point solver {options.solver!r}")
     blocks = KBlocks(ops, 2 * b, options.preconditioner) if options.solver == "block" else None
     kinetic = [torch.from_numpy(op.get("kinetic")).to(dev) for op in ops]
     ewald = float(system.get("ewald")[0])
     x = torch.from_numpy(gaussian_start(system)).to(dev)
     mixer = Mixer(field, options.beta, options.history, options.preconditioning)
-    orbitals = [realified_start(_complex_start(op.nk, b, options.seed + i).to(dev)) for i, op in enumerate(ops)]
+    start = as_real if cplx else realified_start
+    orbitals = [start(_complex_start(op.nk, b, options.seed + i).to(dev)).contiguous() for i, op in enumerate(ops)]
     outs = [torch.empty_like(o) for o in orbitals]
     rho_dev = torch.empty(field.size, dtype=torch.float64, device=dev)
     rho_k = torch.empty_like(rho_dev)
@@ -459,14 +481,14 @@ def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
                 op.set_owned_potential(potential)
             totals["fields_s"] += time.perf_counter() - tf
             td = time.perf_counter()
-            opts = ref_davidson.Options(bands=2 * b, threshold=ethr / 2.0, occupied=2 * b,
+            opts = ref_davidson.Options(bands=nb, threshold=ethr / 2.0, occupied=nb,
                                         first_step=(iteration == 1 and not redo),
                                         max_iterations=options.davidson_max_iterations,
                                         preconditioner=options.preconditioner)
             diags = [d + vmean for d in diag_static]
             if blocks is None:
                 sols = ref_davidson.solve_many(ops, orbitals, diags, [opts] * len(ops), [(o, None) for o in outs],
-                                               threads=options.threads)
+                                               threads=options.threads, complex_columns=cplx)
                 for sol in sols:
                     totals["h_columns"] += sol.record["h_columns"]
                     totals["eigen_retries"] += sol.record["eigen_retries"]
@@ -491,7 +513,7 @@ def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
                 totals["davidson_iterations"] += steps
             totals["davidson_s"] += time.perf_counter() - td
             tf = time.perf_counter()
-            occ, mu, ts = fermi_dirac(levels, weights, electrons, options.theta, capacity=1.0)
+            occ, mu, ts = fermi_dirac(levels, weights, electrons, options.theta, capacity=level_capacity)
             rho_dev.zero_()
             for i, op in enumerate(ops):
                 op.band_density_sum(outs[i], weights[i] * occ[i], rho_k)
@@ -532,7 +554,10 @@ def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
     total = kin + nl + local + hartree + xc + ewald - ts
     energy = {"kinetic": kin, "nonlocal": nl, "local": local, "hartree": hartree, "xc": xc, "ewald": ewald,
               "entropy_term": ts, "total": total}
-    eig = 0.5 * (levels[:, 0::2] + levels[:, 1::2])
-    return dict(energy=energy, eigenvalues=eig, levels=levels, occupations=occ[:, 0::2] + occ[:, 1::2], mu=mu,
+    if cplx:
+        eig, occupations = levels, occ
+    else:
+        eig, occupations = 0.5 * (levels[:, 0::2] + levels[:, 1::2]), occ[:, 0::2] + occ[:, 1::2]
+    return dict(energy=energy, eigenvalues=eig, levels=levels, occupations=occupations, mu=mu,
                 history=history, iterations=len(history), wall_s=wall, setup_s=t_setup, totals=totals,
                 kpoints=kpoints, ops=ops, orbitals=orbitals)

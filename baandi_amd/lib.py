@@ -27,7 +27,7 @@ EXPORTS = [
     "bd_k_op_create", "bd_k_op_info", "bd_k_op_get", "bd_k_batch_create", "bd_k_batch_destroy",
     "bd_k_batch_stream", "bd_k_batch_apply_dev", "bd_gamma_op_autotune", "bd_gamma_op_fused_shape",
     "bd_gamma_op_plane_shape", "bd_gamma_op_plane_split", "bd_gamma_op_keep_fields", "bd_gamma_op_apply_kept_dev",
-    "bd_sym_eigen_multi_from", "bd_sym_eigen_coop", "bd_field_create", "bd_field_destroy", "bd_field_stream", "bd_field_to_grid_dev",
+    "bd_sym_eigen_multi_from", "bd_herm_eigen_dev", "bd_sym_eigen_coop", "bd_field_create", "bd_field_destroy", "bd_field_stream", "bd_field_to_grid_dev",
     "bd_field_from_grid_dev", "bd_field_potential_dev", "bd_field_residual_dev", "bd_field_gram_dev",
     "bd_field_mix_dev", "bd_field_energies_dev",
     "bd_gemm_workspace", "bd_gemm_dev", "bd_mfma_probe_dev", "bd_mfma_f64_probe_dev",
@@ -68,6 +68,7 @@ class DavidsonOptions(C.Structure):
         ("return_hx", C.c_int32), ("preconditioner", C.c_int32),
         ("mlp_blob", C.c_void_p), ("mlp_vbar", C.c_double),
         ("profile", C.c_int32), ("ritz_changes_ha", C.c_void_p), ("ritz_values_ha", C.c_void_p),
+        ("complex_columns", C.c_int32),
     ]
 
 
@@ -161,6 +162,7 @@ def load():
         "bd_mfma_probe_dev": [P, P, P],
         "bd_mfma_f64_probe_dev": [P, P, P],
         "bd_sym_eigen_dev": [I32, I32, P, I64, P, P, I64, P],
+        "bd_herm_eigen_dev": [I32, I32, P, I64, P, P, I64, P],
         "bd_inverse_cholesky_dev": [P, P, I32, I32, I32, P, P],
         "bd_inverse_cholesky_static_dev": [P, P, I32, I32, I32, P, P],
         "bd_mlp_quantize_dev": [P, P],

@@ -57,12 +57,15 @@ class NotConverged(L.BaandiError):
 
 
 def solve_with_initial_hx(operator, start, hamiltonian_diagonal, options: Options, initial_hx=None,
-                          out=None, raise_not_converged=False):
+                          out=None, raise_not_converged=False, complex_columns=False):
     """Solve H X = X E for the lowest ``options.bands`` roots.
 
     ``start`` (bands, n) and ``hamiltonian_diagonal`` (n,) are CUDA float64
     tensors. Returns a Solution; an exhausted iteration limit is reported in
-    ``converged`` (and raised when ``raise_not_converged``), never hidden."""
+    ``converged`` (and raised when ``raise_not_converged``), never hidden.
+    ``complex_columns``: each row of ``start`` is a complex vector of n / 2
+    interleaved (re, im) pairs of a k-point operator, solved by the complex
+    Hermitian Davidson (ref_davidson_complex.rs:641-1113)."""
     import torch
     nb, n = start.shape
     assert nb == options.bands and n == operator.global_len()
@@ -83,7 +86,8 @@ def solve_with_initial_hx(operator, start, hamiltonian_diagonal, options: Option
                           mlp_blob=(L.ptr(options.mlp.blob) if options.mlp is not None else None),
                           mlp_vbar=float(options.mlp_vbar), profile=int(bool(options.profile)),
                           ritz_changes_ha=(hist_c.ctypes.data if hist_c is not None else None),
-                          ritz_values_ha=(hist_v.ctypes.data if hist_v is not None else None))
+                          ritz_values_ha=(hist_v.ctypes.data if hist_v is not None else None),
+                          complex_columns=int(bool(complex_columns)))
     if options.preconditioner == L.BD_PRECOND_MLP and options.mlp is None:
         raise ValueError("BD_PRECOND_MLP needs a trained network (Options.mlp)")
     eig = np.zeros(nb)
@@ -113,11 +117,12 @@ def solve(operator, start, hamiltonian_diagonal, options: Options, **kw):
     return solve_with_initial_hx(operator, start, hamiltonian_diagonal, options, None, **kw)
 
 
-def solve_many(operators, starts, diagonals, options, outs, threads=8):
+def solve_many(operators, starts, diagonals, options, outs, threads=8, complex_columns=False):
     """Independent solves on distinct operators (the k points of one map), up
     to ``threads`` at a time on the GPU, each on its operator's stream
     (bd_davidson_solve_many_dev). ``options``: one Options per solve;
-    ``outs``: (x, hx or None) per solve. Returns the Solutions."""
+    ``outs``: (x, hx or None) per solve; ``complex_columns`` as for
+    solve_with_initial_hx. Returns the Solutions."""
     import torch
     count = len(operators)
     assert len(starts) == count and len(diagonals) == count and len(options) == count and len(outs) == count
@@ -142,7 +147,7 @@ def solve_many(operators, starts, diagonals, options, outs, threads=8):
                                     max_iterations=int(o.max_iterations), return_hx=int(bool(o.return_hx)),
                                     preconditioner=int(o.preconditioner),
                                     mlp_blob=(L.ptr(o.mlp.blob).value if o.mlp is not None else None),
-                                    mlp_vbar=float(o.mlp_vbar))
+                                    mlp_vbar=float(o.mlp_vbar), complex_columns=int(bool(complex_columns)))
     ev = arr([e.ctypes.data for e in eigs])
     cv = arr([c.ctypes.data for c in convs])
     recs = (L.DavidsonRecord * count)()

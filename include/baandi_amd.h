@@ -253,6 +253,14 @@ typedef struct {
    * Rows past record.iterations are left untouched. */
   double* ritz_changes_ha;
   double* ritz_values_ha;
+  /* Nonzero: the columns are complex vectors (interleaved (re, im) rows, a
+   * k-point op's layout) and the complex Hermitian Davidson runs
+   * (ref_davidson_complex::solve_with_initial_hx,
+   * crates/dft-solvers/src/ref_davidson_complex.rs:641-1113): `bands` and
+   * `capacity` count complex columns (capacity 0: min(2 bands, n / 2)), the
+   * Grams are Hermitian, the reduced problem is the complex one (each value
+   * once). k-point ops only; QE or TPA preconditioner. */
+  int32_t complex_columns;
 } bd_davidson_options;
 
 typedef struct {
@@ -388,6 +396,16 @@ bd_status bd_mfma_probe_dev(const uint16_t* a_bits, const uint16_t* b_bits, floa
 int64_t bd_sym_eigen_workspace(int32_t m, int32_t want);
 bd_status bd_sym_eigen_dev(int32_t m, int32_t want, double* A, int64_t lda, double* values, double* Y,
                            int64_t ldy, double* work);
+/* bd_herm_eigen_dev: lowest `want` eigenpairs of a complex Hermitian m x m
+ * matrix (the k-point Davidson's reduced problem; replaces the complex
+ * RealEighWorkspace counterpart, crates/dft-numerics/src/linalg.rs): A holds
+ * both triangles as interleaved (re, im) doubles with leading dimension lda
+ * complex entries and is overwritten; Y receives m x want complex vectors
+ * (ld ldy complex entries); `work` holds bd_sym_eigen_workspace(m, want)
+ * doubles. Complex Householder reduction to a real tridiagonal, the real
+ * bisection / inverse iteration, complex back-transform. */
+bd_status bd_herm_eigen_dev(int32_t m, int32_t want, double* A, int64_t lda, double* values, double* Y,
+                            int64_t ldy, double* work);
 /* Read (set = 0) or set the subspace size from which the tridiagonalisation
  * runs across the chip (one launch per reflector for its column, one for the
  * product with the trailing block on every CU, a tiled panel update) instead

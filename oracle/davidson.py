@@ -10,9 +10,16 @@ the restarted columns have unit norms to 1e-12, a measured refresh every
 fourth contraction, and measured refreshes for the rest of the solve once a
 guard trips (ref_davidson/ideal_reset.rs:7-8).
 
+A complex start block runs the same iteration on complex columns, which is
+ref_davidson_complex::solve_with_initial_hx
+(crates/dft-solvers/src/ref_davidson_complex.rs:641-1113): Hermitian Grams
+(V^H B), the complex Hermitian reduced problem, |x|^2 weights. On real input
+every conjugation is the identity and the arithmetic is the real solver's.
+
 Parity: converged eigenvalues against a dense LAPACK eigensolve of the dense
 ORC H (<=1e-11 Ha, the tolerance of crates/dft-solvers/tests/ref_davidson.rs:223)
-and the denominator identity (stream.rs:16-22 vs ref_davidson.rs:418-422).
+and of the dense H_k for complex columns, and the denominator identity
+(stream.rs:16-22 vs ref_davidson.rs:418-422).
 """
 from __future__ import annotations
 
@@ -68,6 +75,11 @@ class NotConverged(Exception):
     pass
 
 
+def _h(a):
+    """Conjugate transpose (the transpose on real arrays)."""
+    return a.conj().T
+
+
 class Davidson:
     def __init__(self, apply, diag, bands, threshold, occupied, first_step=False,
                  capacity=0, max_iterations=20, preconditioner="qe", kinetic=None, ideal_reset=True):
@@ -114,12 +126,12 @@ class Davidson:
             L = np.linalg.cholesky(Sm)
         except np.linalg.LinAlgError:
             return False, T
-        piv = np.diag(L) ** 2 / np.diag(Sm)
+        piv = np.diag(L).real ** 2 / np.diag(Sm).real
         if np.any(~np.isfinite(piv[m_from:])) or np.any(piv[m_from:] < PIVOT_RATIO_MIN):
             return False, T
         Tn = np.linalg.inv(L)
         Tn = np.tril(Tn)
-        growth = np.abs(Tn[m_from:m_to, :]) @ np.sqrt(np.diag(Sm))
+        growth = np.abs(Tn[m_from:m_to, :]) @ np.sqrt(np.diag(Sm).real)
         if np.any(~np.isfinite(growth)) or np.any(EPS * growth * growth > math.sqrt(EPS)):
             return False, T
         return True, Tn
@@ -128,47 +140,51 @@ class Davidson:
         """A = T H_r T^T, EVD, C = T^T Y[:, :n] (workspace.rs:435-499,
         reduced_border/ritz_coefficients lean_davidson.rs:335-444)."""
         T = self.T[:m, :m]
-        A = T @ self.Hr[:m, :m] @ T.T
-        A = 0.5 * (A + A.T)
+        A = T @ self.Hr[:m, :m] @ _h(T)
+        A = 0.5 * (A + _h(A))
         w, Y = np.linalg.eigh(A)
         self.values = w
-        self.C = T.T @ Y[:, : self.n]
+        self.C = _h(T) @ Y[:, : self.n]
 
     def _border(self, V, B, m_from, m_to):
         """Border columns V^T B[:, from:to] mirrored (workspace.rs:300-381)."""
-        return V[:, :m_to].T @ B[:, m_from:m_to]
+        return _h(V[:, :m_to]) @ B[:, m_from:m_to]
 
     def _refresh(self, m):
         V, HV = self.V[:, :m], self.HV[:, :m]
-        self.S[:m, :m] = V.T @ V
-        self.Hr[:m, :m] = V.T @ HV
-        self.Hr[:m, :m] = np.triu(self.Hr[:m, :m]) + np.triu(self.Hr[:m, :m], 1).T
-        self.S[:m, :m] = np.triu(self.S[:m, :m]) + np.triu(self.S[:m, :m], 1).T
+        self.S[:m, :m] = _h(V) @ V
+        self.Hr[:m, :m] = _h(V) @ HV
+        self.Hr[:m, :m] = np.triu(self.Hr[:m, :m]) + _h(np.triu(self.Hr[:m, :m], 1))
+        self.S[:m, :m] = np.triu(self.S[:m, :m]) + _h(np.triu(self.S[:m, :m], 1))
+        if np.iscomplexobj(self.S):
+            for M in (self.S, self.Hr):
+                M[np.arange(m), np.arange(m)] = M[np.arange(m), np.arange(m)].real
         ok, self.T = self._factor(self.S, 0, m, self.T)
         if not ok:
             raise FloatingPointError("reference contracted basis is dependent")
 
     def _set_border(self, M, vals, m_from, m_to):
         for jj, j in enumerate(range(m_from, m_to)):
-            for i in range(j + 1):
+            for i in range(j):
                 M[i, j] = vals[i, jj]
-                M[j, i] = vals[i, jj]
+                M[j, i] = np.conj(vals[i, jj])
+            M[j, j] = vals[j, jj].real if np.iscomplexobj(vals) else vals[j, jj]
 
     # -- corrections --------------------------------------------------------
     def _corrections(self, X, HX, active):
-        E = np.empty((X.shape[0], len(active)))
+        E = np.empty((X.shape[0], len(active)), dtype=X.dtype)
         for j, band in enumerate(active):
             theta = self.values[band]
             r = HX[:, j] - theta * X[:, j]
             if self.preconditioner == "tpa":
-                w = X[:, j] ** 2
+                w = np.abs(X[:, j]) ** 2
                 scale = kinetic_scale(w, self.kinetic)
                 E[:, j] = -r * tpa_weight(self.kinetic, scale)
             elif self.preconditioner == "qe":
                 E[:, j] = r / sqrt_denominator(self.diag, theta)
             else:  # callable learned preconditioner (residual, x, theta) -> correction
                 E[:, j] = self.preconditioner(r, X[:, j], theta, band)
-        norms = np.sqrt(np.sum(E * E, axis=0))
+        norms = np.sqrt(np.sum(np.abs(E) ** 2, axis=0))
         if not np.all(np.isfinite(norms)):
             raise FloatingPointError("nonfinite reference correction")
         keep = norms > 32.0 * EPS
@@ -181,21 +197,21 @@ class Davidson:
         T = self.T[:m, :m]
         P = self.S[:m, :m] @ self.C
         Y = T @ P
-        self.V[:, :m] = self.V[:, :m] @ T.T
-        self.HV[:, :m] = self.HV[:, :m] @ T.T
+        self.V[:, :m] = self.V[:, :m] @ _h(T)
+        self.HV[:, :m] = self.HV[:, :m] @ _h(T)
         self.C = Y
         self._refresh(m)
         V = self.V[:, :m]
         for _ in range(2):
-            E = E - V @ (V.T @ E)
+            E = E - V @ (_h(V) @ E)
         kept = []
         for j in range(E.shape[1]):
             e = E[:, j].copy()
             for _ in range(2):
                 if kept:
                     K = np.stack(kept, 1)
-                    e -= K @ (K.T @ e)
-            nrm = math.sqrt(float(e @ e))
+                    e -= K @ (_h(K) @ e)
+            nrm = math.sqrt(float(np.sum(np.abs(e) ** 2)))
             if nrm <= DEPENDENCE_TOLERANCE or m + len(kept) == self.cap:
                 continue
             kept.append(e / nrm)
@@ -221,7 +237,7 @@ class Davidson:
             pass
         elif self.contractions % REFRESH_INTERVAL == 0:
             self.periodic_refreshes += 1
-        elif np.all(np.abs(np.sum(self.V[:, :n] ** 2, axis=0) - 1.0) <= ORTHOGONALITY_LIMIT):
+        elif np.all(np.abs(np.sum(np.abs(self.V[:, :n]) ** 2, axis=0) - 1.0) <= ORTHOGONALITY_LIMIT):
             ideal = True  # reset_norm_guard, workspace.rs:895-918 (NaN fails the test)
             self.ideal_resets += 1
         else:
@@ -246,11 +262,12 @@ class Davidson:
         cap = self.cap
         if n != self.n or n > rows or self.occupied > n or not np.all(np.isfinite(x0)):
             raise ValueError("reference Davidson dimensions/threshold/start")
-        self.V = np.zeros((rows, cap))
-        self.HV = np.zeros((rows, cap))
-        self.S = np.zeros((cap, cap))
-        self.Hr = np.zeros((cap, cap))
-        self.T = np.zeros((cap, cap))
+        dt = np.complex128 if np.iscomplexobj(x0) else np.float64
+        self.V = np.zeros((rows, cap), dtype=dt)
+        self.HV = np.zeros((rows, cap), dtype=dt)
+        self.S = np.zeros((cap, cap), dtype=dt)
+        self.Hr = np.zeros((cap, cap), dtype=dt)
+        self.T = np.zeros((cap, cap), dtype=dt)
         self.contractions = 0
         self.reset_disabled = not self.ideal_reset
         self.V[:, :n] = x0

@@ -1,6 +1,7 @@
 """k-point path on MI355X against the oracle (oracle/kpoint.py): the basis,
-projector table and diagonal at k, the complex H·X, the realified Davidson
-against the dense spectrum of H_k, k = 0 against Gamma, the band density and
+projector table and diagonal at k, the complex H·X, the complex and the
+realified Davidson against the dense spectrum of H_k, the complex Davidson's
+iterations against the oracle's, k = 0 against Gamma, the band density and
 nonlocal energies, and a Monkhorst-Pack k SCF with Fermi-Dirac smearing."""
 import numpy as np
 import pytest
@@ -71,7 +72,7 @@ def test_k_apply_matches_oracle(precision):
             assert 1e-12 < err < 2e-5  # the Gamma mixed tier's FP32/BF16x3 budget
 
 
-def _dense_solve(ecut, k, bands, precond, threshold=1e-12, seed=5):
+def _dense_solve(ecut, k, bands, precond, threshold=1e-12, seed=5, realified=False):
     import torch
     from baandi_amd import ref_davidson, lib as L
     from baandi_amd.kpoint import KPointHamiltonian, solve
@@ -87,24 +88,98 @@ def _dense_solve(ecut, k, bands, precond, threshold=1e-12, seed=5):
     x0 /= np.linalg.norm(x0, axis=1, keepdims=True)
     diag = torch.from_numpy(op.get("diagonal_static") + pot.mean()).cuda()
     sol = solve(op, torch.from_numpy(x0).cuda(), diag, ref_davidson.Options(
-        bands=bands, threshold=threshold, occupied=bands, max_iterations=300, preconditioner=precond))
+        bands=bands, threshold=threshold, occupied=bands, max_iterations=300, preconditioner=precond),
+        realified=realified)
     return op, h, w, sol
 
 
+@pytest.mark.parametrize("realified", [False, True])
 @pytest.mark.parametrize("k", [K_GENERIC, (0.5, 0.5, 0.5)])
-def test_k_davidson_matches_dense_spectrum(k):
+def test_k_davidson_matches_dense_spectrum(k, realified):
     from baandi_amd import lib as L
-    op, h, w, sol = _dense_solve(6.0, k, 8, L.BD_PRECOND_TPA)
+    op, h, w, sol = _dense_solve(6.0, k, 8, L.BD_PRECOND_TPA, realified=realified)
     err = np.abs(sol.eigenvalues - w[:8]).max()
-    levels = sol.realified.eigenvalues
-    print(f"[k davidson] k={k} nk={op.nk} eigenvalue error {err:.2e} Ha, pair split "
-          f"{np.abs(levels[0::2] - levels[1::2]).max():.1e}, record={sol.record}", flush=True)
+    split = ""
+    if realified:
+        levels = sol.realified.eigenvalues
+        split = f", pair split {np.abs(levels[0::2] - levels[1::2]).max():.1e}"
+    print(f"[k davidson] k={k} realified={realified} nk={op.nk} eigenvalue error {err:.2e} Ha{split}, "
+          f"record={sol.record}", flush=True)
     assert sol.converged.all()
     assert err < 1e-10
     x = sol.eigenvectors.cpu().numpy()
     assert np.abs(x.conj() @ x.T - np.eye(8)).max() < 1e-10
     r = h @ x.T - x.T * sol.eigenvalues[None, :]
     assert np.linalg.norm(r, axis=0).max() < 1e-5
+
+
+def _k_oracle_pair(ecut, k, bands, precond, its, seed):
+    """The GPU complex Davidson and the oracle's, `its` iterations from the
+    same complex start at the same potential."""
+    import torch
+    from baandi_amd import ref_davidson, lib as L
+    from baandi_amd.kpoint import KPointHamiltonian, solve
+    from oracle import davidson as OD
+    from oracle.kpoint import apply_k, static_diagonal
+    system, ks = _pair(ecut, k)
+    op = KPointHamiltonian(system, k, capacity=bands)
+    pot = _potential(ks, seed)
+    op.set_owned_potential(pot.reshape(-1))
+    rng = np.random.default_rng(seed)
+    x0 = rng.standard_normal((bands, op.nk)) + 1j * rng.standard_normal((bands, op.nk))
+    x0 /= np.linalg.norm(x0, axis=1, keepdims=True)
+    code = L.BD_PRECOND_QE if precond == "qe" else L.BD_PRECOND_TPA
+    diag = torch.from_numpy(op.get("diagonal_static") + pot.mean()).cuda()
+    sol = solve(op, torch.from_numpy(x0).cuda(), diag, ref_davidson.Options(
+        bands=bands, threshold=1e-13, occupied=bands, max_iterations=its, preconditioner=code))
+    ref = OD.Davidson(lambda x: apply_k(ks, x, pot), static_diagonal(ks) + pot.mean(), bands, 1e-13, bands,
+                      max_iterations=its, preconditioner=precond, kinetic=ks.kinetic).solve(x0.T.copy())
+    return op, ks, pot, sol, ref
+
+
+@pytest.mark.parametrize("precond", ["qe", "tpa"])
+def test_k_complex_davidson_follows_oracle(precond):
+    """Iteration by iteration the GPU complex Davidson is the oracle's: the
+    same H columns, restarts and ideal resets, the same Ritz values to
+    rounding (capacity 2 bands, so it contracts from the second iteration)."""
+    for its in (1, 2, 3, 5, 8):
+        op, ks, pot, sol, ref = _k_oracle_pair(5.0, K_GENERIC, 8, precond, its, 7)
+        err = np.abs(sol.eigenvalues - ref["eigenvalues"]).max()
+        rec = sol.record
+        print(f"[k complex vs oracle] {precond} its={its}: H columns {rec['h_columns']}/{ref['h_columns']}, "
+              f"restarts {rec['restarts']}/{ref['restarts']}, ideal {rec['ideal_resets']}/{ref['ideal_resets']}, "
+              f"max |dE| {err:.2e}", flush=True)
+        assert rec["iterations"] == ref["iterations"] == its
+        assert rec["h_columns"] == ref["h_columns"]
+        assert rec["restarts"] == ref["restarts"] and rec["ideal_resets"] == ref["ideal_resets"]
+        assert err < 1e-10
+
+
+def test_k_complex_davidson_rank_repair():
+    """More bands than half the plane waves (nk = 25, 15 bands): the capacity
+    is the dimension, the first corrections do not fit and the rank repair
+    runs, as in the oracle; the solve then reaches the dense spectrum."""
+    import torch
+    from baandi_amd import ref_davidson, lib as L
+    from baandi_amd.kpoint import solve
+    from oracle.kpoint import dense_k
+    k = (0.25, -0.125, 0.375)
+    op, ks, pot, sol, ref = _k_oracle_pair(1.5, k, 15, "qe", 1, 2)
+    assert op.nk == 25
+    print(f"[k complex repair] first iteration: repairs {sol.record['repairs']}/{ref['repairs']}, "
+          f"H columns {sol.record['h_columns']}/{ref['h_columns']}", flush=True)
+    assert sol.record["repairs"] == ref["repairs"] >= 1
+    assert sol.record["h_columns"] == ref["h_columns"]
+    assert np.abs(sol.eigenvalues - ref["eigenvalues"]).max() < 1e-10
+    w = np.linalg.eigvalsh(dense_k(ks, pot))
+    x0 = np.random.default_rng(2).standard_normal((15, 25)) + 0j
+    x0 /= np.linalg.norm(x0, axis=1, keepdims=True)
+    diag = torch.from_numpy(op.get("diagonal_static") + pot.mean()).cuda()
+    s2 = solve(op, torch.from_numpy(x0).cuda(), diag, ref_davidson.Options(
+        bands=15, threshold=1e-12, occupied=15, max_iterations=100, preconditioner=L.BD_PRECOND_QE))
+    print(f"[k complex repair] converged: record={s2.record}", flush=True)
+    assert s2.converged.all()
+    assert np.abs(s2.eigenvalues - w[:15]).max() < 1e-10
 
 
 def test_k_zero_matches_gamma():
@@ -152,10 +227,11 @@ def test_k_density_and_nonlocal_match_oracle():
     assert err < 1e-13 and nerr < 1e-13
 
 
-def test_k_scf_matches_oracle(kat1):
+@pytest.mark.parametrize("columns", ["complex", "realified"])
+def test_k_scf_matches_oracle(kat1, columns):
     """KAT-1 cell at 5 Ha on a 2x2x2 Monkhorst-Pack mesh (4 points after time
     reversal), 8 bands, Fermi-Dirac 0.01 Ha: the Mermin free energy of the
-    oracle's dense-solve k SCF."""
+    oracle's dense-solve k SCF, with the complex and the realified Davidson."""
     from oracle import physics as P
     from oracle import kpoint as OK
     from baandi_amd.hamiltonian import GammaSystem
@@ -164,13 +240,13 @@ def test_k_scf_matches_oracle(kat1):
     kp = monkhorst_pack((2, 2, 2))
     assert [p for p, _ in kp] == [p for p, _ in OK.monkhorst_pack((2, 2, 2))]
     system = GammaSystem(rows, ATOMS, 4.0, kat1["ecut_ha"])
-    r = run_scf_k(system, kp, KScfOptions(bands=8, theta=0.01, threshold_ry=1e-11),
+    r = run_scf_k(system, kp, KScfOptions(bands=8, theta=0.01, threshold_ry=1e-11, columns=columns),
                   log=lambda s: print(s, flush=True))
     ref = OK.scf_k(rows, ATOMS, P.synthetic_pseudo(4.0, 1.0, 801), kat1["ecut_ha"], (2, 2, 2), 8, 0.01,
                    threshold_ry=1e-11)
     de = abs(r["energy"]["total"] - ref["energy"]["total"])
     deig = np.abs(r["eigenvalues"] - ref["eigenvalues"]).max()
-    print(f"[k scf] E={r['energy']['total']:.12f} oracle={ref['energy']['total']:.12f} dE={de:.2e} "
+    print(f"[k scf] {columns} E={r['energy']['total']:.12f} oracle={ref['energy']['total']:.12f} dE={de:.2e} "
           f"TS={r['energy']['entropy_term']:.3e} mu={r['mu']:.8f}/{ref['mu']:.8f} eig err {deig:.2e} "
           f"maps={r['iterations']} wall={r['wall_s']:.2f}s eigen retries={r['totals']['eigen_retries']}",
           flush=True)
@@ -447,7 +523,7 @@ def test_k_scf_block_solver_matches_davidson_and_oracle(kat1):
     kp = monkhorst_pack((2, 2, 2))
     base = dict(bands=wl["bands"], theta=wl["theta"], threshold_ry=1e-10, preconditioning="thomas_fermi")
     rd = run_scf_k(system, kp, KScfOptions(**base))
-    rb = run_scf_k(system, kp, KScfOptions(solver="block", **base))
+    rb = run_scf_k(system, kp, KScfOptions(solver="block", columns="realified", **base))
     de = abs(rd["energy"]["total"] - rb["energy"]["total"])
     deig = np.abs(rd["eigenvalues"] - rb["eigenvalues"]).max()
     print(f"[k scf block] Al 2x2x2: davidson {rd['energy']['total']:.12f} ({rd['wall_s']:.2f}s) block "
@@ -456,7 +532,8 @@ def test_k_scf_block_solver_matches_davidson_and_oracle(kat1):
     assert deig < 1e-6
     rows = fcc_rows(kat1["cell_a_bohr"])
     s2 = GammaSystem(rows, ATOMS, 4.0, kat1["ecut_ha"])
-    r = run_scf_k(s2, kp, KScfOptions(bands=8, theta=0.01, threshold_ry=1e-11, solver="block"))
+    r = run_scf_k(s2, kp, KScfOptions(bands=8, theta=0.01, threshold_ry=1e-11, solver="block",
+                                      columns="realified"))
     ref = OK.scf_k(rows, ATOMS, P.synthetic_pseudo(4.0, 1.0, 801), kat1["ecut_ha"], (2, 2, 2), 8, 0.01,
                    threshold_ry=1e-11)
     de = abs(r["energy"]["total"] - ref["energy"]["total"])

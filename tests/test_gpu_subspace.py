@@ -255,3 +255,61 @@ def test_eigen_cooperative_abandon_is_reported():
     finally:
         _coop(old)
     _check_pairs(A, 20, "coop after abandon")
+
+
+def _herm_dev(A, want):
+    import torch
+    from baandi_amd import lib as L
+    lib = L.load()
+    m = A.shape[0]
+    Af = np.asfortranarray(A).ravel(order="F")
+    Ad = torch.from_numpy(np.stack([Af.real, Af.imag], axis=1).ravel().copy()).cuda()
+    vals = torch.zeros(want, dtype=torch.float64, device="cuda")
+    Y = torch.zeros(2 * m * want, dtype=torch.float64, device="cuda")
+    work = torch.zeros(lib.bd_sym_eigen_workspace(m, want), dtype=torch.float64, device="cuda")
+    L.check(lib.bd_herm_eigen_dev(m, want, L.ptr(Ad), m, L.ptr(vals), L.ptr(Y), m, L.ptr(work)))
+    y = Y.cpu().numpy().reshape(want, m, 2)
+    return vals.cpu().numpy(), (y[..., 0] + 1j * y[..., 1]).T
+
+
+def _check_herm(A, want, label):
+    w, Y = _herm_dev(A, want)
+    ref = np.linalg.eigvalsh(A)[:want]
+    m = A.shape[0]
+    scale = max(np.abs(A).sum(axis=0).max(), 1e-300)
+    verr = np.abs(w - ref).max() / scale
+    res = np.abs(A @ Y - Y * w[None, :]).max() / scale
+    orth = np.abs(Y.conj().T @ Y - np.eye(want)).max()
+    print(f"[herm] {label} m={m} want={want}: value {verr:.2e}, residual {res:.2e}, orthogonality {orth:.2e}")
+    assert verr < 1e-13 * max(m, 10)
+    assert res < 1e-12 * max(m, 10)
+    assert orth < 1e-12 * max(m, 10)
+
+
+@pytest.mark.parametrize("m,want", [(1, 1), (2, 2), (3, 2), (16, 16), (32, 16), (64, 40), (160, 80), (300, 120)])
+def test_herm_eigen_random(m, want):
+    """The complex Hermitian eigensolver (complex Householder reduction to a
+    real tridiagonal, the real bisection and inverse iteration, the complex
+    back-transform) against LAPACK's zheevd through numpy."""
+    rng = np.random.default_rng(1000 + m)
+    B = rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m))
+    _check_herm(0.5 * (B + B.conj().T), want, "random")
+
+
+@pytest.mark.parametrize("m", [12, 48, 96])
+def test_herm_eigen_degenerate_and_real(m):
+    """Exact multiplicities and a 1e-13 cluster under a random unitary, a
+    real symmetric input (zero imaginary parts: every reflector real) and a
+    diagonal one (every tau zero)."""
+    rng = np.random.default_rng(m + 5)
+    vals = np.sort(rng.uniform(-1.0, 3.0, m))
+    vals[3:6] = vals[3]
+    vals[8:10] = vals[8]
+    vals[-4:-1] = vals[-4] + np.array([0.0, 1e-13, 2e-13])
+    vals = np.sort(vals)
+    Q, _ = np.linalg.qr(rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m)))
+    A = (Q * vals[None, :]) @ Q.conj().T
+    _check_herm(0.5 * (A + A.conj().T), m // 2 + 1, "degenerate")
+    B = rng.standard_normal((m, m))
+    _check_herm(0.5 * (B + B.T).astype(complex), m // 2, "real")
+    _check_herm(np.diag(vals).astype(complex), m // 2, "diagonal")
