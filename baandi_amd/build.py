@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "libbaandi_amd.so")
 ARCH = "gfx950"
 
 SOURCES = ["errors.cpp", "setup.cpp", "gemm.hip", "gemm_bf16x3.hip", "op.hip", "kbatch.hip", "eigen.hip", "subspace.hip",
-           "mlp.hip", "davidson.hip", "field.hip", "setup_dev.hip"]
+           "mlp.hip", "davidson.hip", "kdavidson.hip", "field.hip", "setup_dev.hip"]
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-result",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-march=x86-64-v3", "-fopenmp", "-ffp-contract=off"]

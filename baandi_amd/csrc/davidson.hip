@@ -816,6 +816,7 @@ struct ComplexWork {
     fw.gwork = fgwork.p;
     fw.flag = info + 1;
     fw.min_ratio = fmin.p;
+    fw.growth_guard = 0;
   }
   ~ComplexWork() {
     if (idx) (void)hipFree(idx);

@@ -1636,8 +1636,10 @@ void allow_full_lds(const void* fn, const char* name) {
   static std::set<const void*> done;
   std::lock_guard<std::mutex> lock(mu);
   if (done.count(fn)) return;
-  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess)
+  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess) {
+    (void)hipGetLastError();  // not sticky for the caller's later calls
     fail(BD_ERR_DEVICE, std::string(name) + " LDS attribute");
+  }
   done.insert(fn);
 }
 

@@ -37,6 +37,10 @@ struct FactorWork {
   double* gwork = nullptr;
   int* flag = nullptr;          // 0 ok, else rejected
   double* min_ratio = nullptr;  // smallest pivot ratio of the last call
+  // Row growth guard of the real factor (lean_davidson.rs); off for the
+  // complex factor, whose extension checks only the pivots and finiteness
+  // (crates/dft-numerics/src/linalg.rs:277-340).
+  int growth_guard = 1;
 };
 
 // Extend T = L^{-1} of S (both cap x cap, ld cap, lower triangular T with a

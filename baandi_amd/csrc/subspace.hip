@@ -134,7 +134,7 @@ __global__ __launch_bounds__(kCT) void k_chol_inv_lds(int k, const double* __res
 
 // One wave per new row i of T: finiteness and the growth guard.
 __global__ __launch_bounds__(256) void k_growth_guard(const double* __restrict__ S, const double* __restrict__ T,
-                                                      int cap, int from, int to, int* __restrict__ flag) {
+                                                      int cap, int from, int to, int guard, int* __restrict__ flag) {
   const int lane = threadIdx.x & 63;
   const int i = from + blockIdx.x * 4 + (threadIdx.x >> 6);
   if (i >= to) return;
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(256) void k_growth_guard(const double* __restrict__
     g += fabs(v) * sqrt(S[j + int64_t(j) * cap]);
   }
   g = wave_sum(g);
-  if (lane == 0 && (bad || !isfinite(g) || DBL_EPSILON * g * g > sqrt(DBL_EPSILON))) flag[0] = 1;
+  if (lane == 0 && guard && (bad || !isfinite(g) || DBL_EPSILON * g * g > sqrt(DBL_EPSILON))) flag[0] = 1;
   if (bad) flag[0] = 1;
 }
 
@@ -221,7 +221,8 @@ void extend_block(hipStream_t s, const double* S, double* T, int cap, int from, 
     gemm_tn(s, k, from, from, 1.0, fw.W, from, T, cap, 0.0, fw.U, k, fw.gwork);         // U = W^T T11
     gemm_nn(s, k, from, k, -1.0, T22, cap, fw.U, k, 0.0, T + from, cap);                  // T21 = -T22 U
   }
-  hipLaunchKernelGGL(k_growth_guard, dim3((k + 3) / 4), dim3(256), 0, s, S, T, cap, from, to, fw.flag);
+  hipLaunchKernelGGL(k_growth_guard, dim3((k + 3) / 4), dim3(256), 0, s, S, T, cap, from, to, fw.growth_guard,
+                     fw.flag);
 }
 }  // namespace
 

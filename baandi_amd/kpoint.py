@@ -230,6 +230,71 @@ class KPointBatch:
             self.handle = None
 
 
+class KDavidson:
+    """The complex Davidson of every k point in lock step (bd_k_davidson_*):
+    each k point keeps its own Davidson (ref_davidson_complex.rs:641-1113),
+    every stage of an iteration is one launch over all k points and H of all
+    new columns one fused launch. Blocks are CUDA float64 tensors (bands,
+    2 nk_i), complex columns as interleaved (re, im) rows."""
+
+    def __init__(self, ops, bands, capacity=0):
+        self.ops, self.bands, self.capacity = list(ops), int(bands), int(capacity)
+        arr = (C.c_void_p * len(self.ops))(*[op.handle.value for op in self.ops])
+        h = C.c_void_p()
+        L.check(L.load().bd_k_davidson_create(arr, len(self.ops), self.bands, self.capacity, C.byref(h)))
+        self.handle = h
+
+    def solve(self, starts, diagonals, options: ref_davidson.Options, outs, initial_hx=None):
+        """Solve every k point from ``starts`` into ``outs`` ((x, hx or None)
+        per k point); returns one ref_davidson.Solution per k point."""
+        import torch
+        count = len(self.ops)
+        assert len(starts) == count and len(diagonals) == count and len(outs) == count
+        assert options.bands == self.bands
+        arr = lambda vals: (C.c_void_p * count)(*vals)
+        for x, op in zip(starts, self.ops):
+            assert x.is_contiguous() and x.shape == (self.bands, op.global_len())
+        x0 = arr([x.data_ptr() for x in starts])
+        hx0 = arr([h.data_ptr() for h in initial_hx]) if initial_hx is not None else None
+        dg = arr([d.contiguous().data_ptr() for d in diagonals])
+        xo = arr([o[0].data_ptr() for o in outs])
+        hxo = arr([(o[1].data_ptr() if o[1] is not None else None) for o in outs]) if options.return_hx else None
+        ld0 = np.array([x.shape[1] for x in starts], dtype=np.int64)
+        ldx = np.array([o[0].shape[1] for o in outs], dtype=np.int64)
+        opts = L.DavidsonOptions(bands=self.bands, threshold=float(options.threshold), occupied=int(options.occupied),
+                                 first_step=int(bool(options.first_step)), capacity=int(options.capacity),
+                                 max_iterations=int(options.max_iterations), return_hx=int(bool(options.return_hx)),
+                                 preconditioner=int(options.preconditioner), complex_columns=1)
+        eig = np.zeros((count, self.bands))
+        conv = np.zeros((count, self.bands), dtype=np.int32)
+        recs = (L.DavidsonRecord * count)()
+        statuses = np.zeros(count, dtype=np.int32)
+        for op in self.ops:
+            op.enter()
+        torch.cuda.current_stream().synchronize()
+        st = L.load().bd_k_davidson_solve_dev(self.handle, x0, L.ptr(ld0), hx0, dg, C.byref(opts), L.ptr(eig), xo,
+                                              hxo, L.ptr(ldx), L.ptr(conv), recs, L.ptr(statuses))
+        L.check(st, allow=(L.BD_NOT_CONVERGED,))
+        sols = []
+        for k in range(count):
+            record = {f: getattr(recs[k], f) for f, _ in L.DavidsonRecord._fields_}
+            sols.append(ref_davidson.Solution(eigenvalues=eig[k].copy(), eigenvectors=outs[k][0], hx=outs[k][1],
+                                              converged=conv[k].astype(bool),
+                                              band_thresholds_ha=[options.band_threshold(b)
+                                                                  for b in range(self.bands)],
+                                              record=record))
+        return sols
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            try:
+                L.load().bd_k_davidson_destroy(h)
+            except (TypeError, AttributeError):
+                pass
+            self.handle = None
+
+
 class KBlocks:
     """The block iterations of all k points of a map stepped together
     (bd_kblocks_*): per-k work on each op's stream, one fused H·W launch over
@@ -393,6 +458,10 @@ class KScfOptions:
     # (ref_davidson_complex.rs); "realified": the real Davidson on 2 bands
     # realified columns (the pairs (x, i x)).
     columns: str = "complex"
+    # Complex columns: the Davidsons of all k points in lock step (one launch
+    # per stage for all k points, KDavidson) where the ops allow it (fused
+    # path, bands <= 24); False: one solve per k point, `threads` at a time.
+    lockstep: bool = True
 
 
 def _complex_start(nk, bands, seed):
@@ -448,6 +517,13 @@ def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
                 pass
     diag_static = [torch.from_numpy(op.get("diagonal_static")).to(dev) for op in ops]
     blocks = KBlocks(ops, 2 * b, options.preconditioner) if options.solver == "block" else None
+    lockstep = None
+    if cplx and options.lockstep and b <= 24 and all(op.fused_shape()[0] > 0 for op in ops) \
+            and 2 * b <= min(op.nk for op in ops):
+        try:
+            lockstep = KDavidson(ops, b)
+        except L.BaandiError:
+            lockstep = None  # no common fused shape: one solve per k point
     kinetic = [torch.from_numpy(op.get("kinetic")).to(dev) for op in ops]
     ewald = float(system.get("ewald")[0])
     x = torch.from_numpy(gaussian_start(system)).to(dev)
@@ -487,8 +563,12 @@ def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
                                         preconditioner=options.preconditioner)
             diags = [d + vmean for d in diag_static]
             if blocks is None:
-                sols = ref_davidson.solve_many(ops, orbitals, diags, [opts] * len(ops), [(o, None) for o in outs],
-                                               threads=options.threads, complex_columns=cplx)
+                if lockstep is not None:
+                    sols = lockstep.solve(orbitals, diags, opts, [(o, None) for o in outs])
+                else:
+                    sols = ref_davidson.solve_many(ops, orbitals, diags, [opts] * len(ops),
+                                                   [(o, None) for o in outs], threads=options.threads,
+                                                   complex_columns=cplx)
                 for sol in sols:
                     totals["h_columns"] += sol.record["h_columns"]
                     totals["eigen_retries"] += sol.record["eigen_retries"]

@@ -307,6 +307,38 @@ bd_status bd_davidson_solve_many_dev(int32_t count, bd_gamma_op* const* ops, con
                                      int32_t* const* converged, bd_davidson_record* records,
                                      bd_status* statuses, int32_t threads);
 
+/* The complex Davidson of every k point of a map in lock step
+ * (ref_davidson_complex::solve_with_initial_hx,
+ * crates/dft-solvers/src/ref_davidson_complex.rs:641-1113, for the k points
+ * that crates/dft-engine/src/refscf/k_map.rs:181-208 solves one after the
+ * other). Each k point keeps its own basis, active bands, convergence,
+ * contractions and ideal resets; each stage of an iteration is one launch
+ * over all k points, H of every k point's new columns one launch of the fused
+ * k-point kernel, and the subspace step of a k point (bordered complex
+ * Cholesky, L^-1 H L^-H, a complex Jacobi eigensolve, C = L^-H Y) one
+ * workgroup with its matrices in LDS. Two status read-backs per iteration
+ * cover all k points. A k point whose corrections need the rank repair
+ * continues in bd_davidson_solve_dev (complex columns) from its Ritz block
+ * (counted in record.repairs). The ops: k-point ops on the fused path of one
+ * system and precision, each at most once; bands <= 24, capacity (0: 2 bands)
+ * <= 48 complex columns and <= n_k. */
+typedef struct bd_k_davidson bd_k_davidson;
+bd_status bd_k_davidson_create(bd_gamma_op* const* ops, int32_t count, int32_t bands, int32_t capacity,
+                               bd_k_davidson** out);
+void bd_k_davidson_destroy(bd_k_davidson* solver);
+/* Per k point: x0 (n x bands complex columns, ld ldx0 even), optional hx0
+ * (NULL, or an array of H·x0 blocks with the same ld), diag (n rows); one
+ * options struct for all (bands as created, complex_columns ignored, QE or TPA,
+ * no Ritz histories); eigenvalues and converged: count x bands (host); x and
+ * hx (hx when return_hx): n x bands blocks, ld ldx; records and statuses: one
+ * per k point. Returns the first failed k point's status (its message as the
+ * last error), else BD_NOT_CONVERGED if any k point hit the limit, else OK. */
+bd_status bd_k_davidson_solve_dev(bd_k_davidson* solver, const double* const* x0_dev, const int64_t* ldx0,
+                                  const double* const* hx0_dev, const double* const* diag_dev,
+                                  const bd_davidson_options* options, double* eigenvalues,
+                                  double* const* x_dev, double* const* hx_dev, const int64_t* ldx,
+                                  int32_t* converged, bd_davidson_record* records, bd_status* statuses);
+
 /* ------------------------------------------------------------------------
  * Fixed-shape block iteration (no reference counterpart; the iteration of
  * ref_davidson.rs:117-366 with every band active and a restart to the Ritz

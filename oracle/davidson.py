@@ -131,6 +131,10 @@ class Davidson:
             return False, T
         Tn = np.linalg.inv(L)
         Tn = np.tril(Tn)
+        if np.iscomplexobj(Sm):
+            # The complex factor checks the pivots and finiteness only
+            # (crates/dft-numerics/src/linalg.rs:277-340): no row growth guard.
+            return bool(np.all(np.isfinite(Tn))), Tn
         growth = np.abs(Tn[m_from:m_to, :]) @ np.sqrt(np.diag(Sm).real)
         if np.any(~np.isfinite(growth)) or np.any(EPS * growth * growth > math.sqrt(EPS)):
             return False, T

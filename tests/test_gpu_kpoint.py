@@ -227,8 +227,8 @@ def test_k_density_and_nonlocal_match_oracle():
     assert err < 1e-13 and nerr < 1e-13
 
 
-@pytest.mark.parametrize("columns", ["complex", "realified"])
-def test_k_scf_matches_oracle(kat1, columns):
+@pytest.mark.parametrize("columns,lockstep", [("complex", True), ("complex", False), ("realified", False)])
+def test_k_scf_matches_oracle(kat1, columns, lockstep):
     """KAT-1 cell at 5 Ha on a 2x2x2 Monkhorst-Pack mesh (4 points after time
     reversal), 8 bands, Fermi-Dirac 0.01 Ha: the Mermin free energy of the
     oracle's dense-solve k SCF, with the complex and the realified Davidson."""
@@ -240,13 +240,14 @@ def test_k_scf_matches_oracle(kat1, columns):
     kp = monkhorst_pack((2, 2, 2))
     assert [p for p, _ in kp] == [p for p, _ in OK.monkhorst_pack((2, 2, 2))]
     system = GammaSystem(rows, ATOMS, 4.0, kat1["ecut_ha"])
-    r = run_scf_k(system, kp, KScfOptions(bands=8, theta=0.01, threshold_ry=1e-11, columns=columns),
+    r = run_scf_k(system, kp, KScfOptions(bands=8, theta=0.01, threshold_ry=1e-11, columns=columns,
+                                          lockstep=lockstep),
                   log=lambda s: print(s, flush=True))
     ref = OK.scf_k(rows, ATOMS, P.synthetic_pseudo(4.0, 1.0, 801), kat1["ecut_ha"], (2, 2, 2), 8, 0.01,
                    threshold_ry=1e-11)
     de = abs(r["energy"]["total"] - ref["energy"]["total"])
     deig = np.abs(r["eigenvalues"] - ref["eigenvalues"]).max()
-    print(f"[k scf] {columns} E={r['energy']['total']:.12f} oracle={ref['energy']['total']:.12f} dE={de:.2e} "
+    print(f"[k scf] {columns} lockstep={lockstep} E={r['energy']['total']:.12f} oracle={ref['energy']['total']:.12f} dE={de:.2e} "
           f"TS={r['energy']['entropy_term']:.3e} mu={r['mu']:.8f}/{ref['mu']:.8f} eig err {deig:.2e} "
           f"maps={r['iterations']} wall={r['wall_s']:.2f}s eigen retries={r['totals']['eigen_retries']}",
           flush=True)
@@ -541,3 +542,94 @@ def test_k_scf_block_solver_matches_davidson_and_oracle(kat1):
           f"dE={de:.2e}", flush=True)
     assert de < 1e-8 * len(ATOMS)
     assert np.abs(r["eigenvalues"] - ref["eigenvalues"]).max() < 1e-6
+
+
+def _al_starts(ops, bands, seed):
+    import torch
+    from baandi_amd.kpoint import as_real
+    rng = np.random.default_rng(seed)
+    starts, diags = [], []
+    for op in ops:
+        z = rng.standard_normal((bands, op.nk)) + 1j * rng.standard_normal((bands, op.nk))
+        z /= np.linalg.norm(z, axis=1, keepdims=True)
+        starts.append(as_real(torch.from_numpy(z).cuda()).contiguous())
+    return starts
+
+
+@pytest.mark.parametrize("precond", ["qe", "tpa"])
+def test_lockstep_k_davidson_follows_per_k_solves(precond):
+    """The Davidsons of all Al 2x2x2 k points in lock step (one launch per
+    stage for every k point, the subspace step in one workgroup per k point)
+    take each k point along the path of its own complex Davidson: the same
+    iterations, H columns, restarts and ideal resets, the same Ritz values to
+    rounding, after 1, 2, 3, 5 and 8 iterations and at convergence."""
+    import torch
+    from baandi_amd import ref_davidson, lib as L
+    from baandi_amd.kpoint import KDavidson, solve, as_complex
+    bands = 8
+    system, ops = _al_ops(capacity=bands)
+    code = L.BD_PRECOND_QE if precond == "qe" else L.BD_PRECOND_TPA
+    starts = _al_starts(ops, bands, 17)
+    diags = [torch.from_numpy(op.get("diagonal_static") - 0.4).cuda() for op in ops]
+    kd = KDavidson(ops, bands)
+    for its in (1, 2, 3, 5, 8, 200):
+        thr = 1e-13 if its < 200 else 1e-11
+        o = ref_davidson.Options(bands=bands, threshold=thr, occupied=bands, max_iterations=its,
+                                 preconditioner=code)
+        outs = [(torch.empty_like(x), None) for x in starts]
+        sols = kd.solve(starts, diags, o, outs)
+        for k, op in enumerate(ops):
+            ref = solve(op, as_complex(starts[k]), diags[k], o)
+            err = np.abs(sols[k].eigenvalues - ref.eigenvalues).max()
+            a, b = sols[k].record, ref.record
+            print(f"[lockstep] {precond} its={its} k={k}: iterations {a['iterations']}/{b['iterations']}, "
+                  f"H columns {a['h_columns']}/{b['h_columns']}, restarts {a['restarts']}/{b['restarts']}, "
+                  f"ideal {a['ideal_resets']}/{b['ideal_resets']}, max |dE| {err:.2e}", flush=True)
+            assert a["iterations"] == b["iterations"] and a["h_columns"] == b["h_columns"]
+            assert a["restarts"] == b["restarts"] and a["ideal_resets"] == b["ideal_resets"]
+            assert a["repairs"] == 0
+            assert err < 1e-10
+            if its == 200:
+                assert sols[k].converged.all()
+                x = as_complex(outs[k][0])
+                gram = (x.conj() @ x.T).cpu().numpy()
+                assert np.abs(gram - np.eye(bands)).max() < 1e-10
+
+
+def test_lockstep_k_davidson_matches_dense_spectrum():
+    """Lock step on the Si2 cell at two k points (one generic, one on the zone
+    boundary) converges to the dense spectra of H_k. Near convergence a
+    border turns dependent: that k point leaves the lock step for the
+    single-k solver's rank repair (record.repairs), as the test shows."""
+    import torch
+    from baandi_amd import ref_davidson, lib as L
+    from baandi_amd.kpoint import KPointHamiltonian, KDavidson, as_real, as_complex
+    from oracle.kpoint import dense_k
+    from oracle import physics as P
+    from oracle import kpoint as OK
+    ks_list, ops = [], []
+    system, ks0 = _pair(6.0, K_GENERIC)
+    pot = _potential(ks0, 5)
+    for k in (K_GENERIC, (0.5, 0.5, 0.5)):
+        ks = OK.build_k_system(ROWS, ATOMS, P.synthetic_pseudo(4.0, 1.0, 801), 6.0, k)
+        op = KPointHamiltonian(system, k, capacity=8)
+        op.set_owned_potential(pot.reshape(-1))
+        ks_list.append(ks)
+        ops.append(op)
+    starts = _al_starts(ops, 8, 3)
+    diags = [torch.from_numpy(op.get("diagonal_static") + pot.mean()).cuda() for op in ops]
+    kd = KDavidson(ops, 8)
+    outs = [(torch.empty_like(x), torch.empty_like(x)) for x in starts]
+    sols = kd.solve(starts, diags, ref_davidson.Options(bands=8, threshold=1e-12, occupied=8, max_iterations=300,
+                                                        preconditioner=L.BD_PRECOND_TPA, return_hx=True), outs)
+    for k, (ks, op) in enumerate(zip(ks_list, ops)):
+        h = dense_k(ks, pot)
+        w = np.linalg.eigvalsh(h)
+        x = as_complex(outs[k][0]).cpu().numpy()
+        hx = as_complex(outs[k][1]).cpu().numpy()
+        err = np.abs(sols[k].eigenvalues - w[:8]).max()
+        print(f"[lockstep dense] k={k} nk={op.nk} error {err:.2e} record={sols[k].record}", flush=True)
+        assert sols[k].converged.all() and err < 1e-10
+        assert np.abs(x.conj() @ x.T - np.eye(8)).max() < 1e-10
+        assert np.abs(hx - (h @ x.T).T).max() < 1e-9
+    assert any(s.record["repairs"] >= 1 for s in sols)
