@@ -38,6 +38,7 @@ namespace {
 constexpr int kKB = 24;    // bands per k point
 constexpr int kKCap = 48;  // complex basis columns per k point
 constexpr int kKT = 256;
+constexpr int kEigT = 256;  // k_kd_eigen workgroup (64 measured twice as slow: the rounds are work-bound)
 constexpr double kEps = std::numeric_limits<double>::epsilon();
 constexpr double kPivotMin = kEps / 1e-12;  // PIVOT_RATIO_MIN, lean_davidson.rs:64
 constexpr double kOrthoLimit = 1e-12;       // ORTHOGONALITY_LIMIT, ref_davidson/ideal_reset.rs:7
@@ -71,10 +72,56 @@ struct KDesc {
 };
 
 struct KStat {
-  int kept, nonfinite_e, factor_bad, eig_bad, nonfinite_h, refresh, pad0, pad1;
-  int contractions, disabled, ideal_resets, periodic, guard_fallbacks, pad2;
+  int kept, nonfinite_e, factor_bad, eig_bad, nonfinite_h, refresh, refresh_bad, repair;
+  int contractions, disabled, ideal_resets, periodic, guard_fallbacks, repair_bad, kept2, pad;
   double values[kKB];
 };
+
+// The border / factor range [from, to) of one k point for a stage, or false
+// when it takes no part. 0: the new corrections [m, m + kept) of live k
+// points; 1: the restarted block [0, nb) of contracting k points that measure
+// it; 2: [0, nb) of every live k point (the start); 3: the whitened basis
+// [0, m) of k points in rank repair; 4: their repaired corrections
+// [m, m + kept2); 5: the columns H was applied to, [m, m + kept) or
+// [m, m + kept2) after a repair.
+__device__ __forceinline__ bool kd_range(const KDesc& d, const KStat& st, int mode, int nb, int& from, int& to) {
+  switch (mode) {
+    case 0:
+      if (!d.live || st.kept <= 0) return false;
+      from = d.m;
+      to = d.m + st.kept;
+      return true;
+    case 1:
+      if (!d.contract || !st.refresh) return false;
+      from = 0;
+      to = nb;
+      return true;
+    case 2:
+      if (!d.live) return false;
+      from = 0;
+      to = nb;
+      return true;
+    case 3:
+      if (!d.stage2 || !st.repair) return false;
+      from = 0;
+      to = d.m;
+      return true;
+    case 4:
+      if (!d.stage2 || !st.repair || st.kept2 <= 0) return false;
+      from = d.m;
+      to = d.m + st.kept2;
+      return true;
+    case 5: {
+      if (!d.stage2) return false;
+      const int kk = st.repair ? st.kept2 : st.kept;
+      if (kk <= 0) return false;
+      from = d.m;
+      to = d.m + kk;
+      return true;
+    }
+  }
+  return false;
+}
 
 __device__ __forceinline__ double2 zmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -114,20 +161,31 @@ __global__ void k_kd_copy_in(const KDesc* __restrict__ D, int nb) {
 
 // Y[:, j] = V[:, :m] C[:, col_j], HY likewise from HV. mode 0: the active
 // bands into X / HX (live k points); 1: every band into X / HX (contracting k
-// points); 2: every band into the caller's outputs (live k points).
-__global__ __launch_bounds__(kKT) void k_kd_rotate(const KDesc* __restrict__ D, int mode, int nb, int cap) {
+// points); 2: every band into the caller's outputs (live k points); 3: the
+// whitening V L^-H (coefficients in S) into X / HX (k points in rank repair);
+// 4: V[:, m + j] -= V[:, :m] G[:, j] for the repair's projection (G in E,
+// ld m).
+__global__ __launch_bounds__(kKT) void k_kd_rotate(const KDesc* __restrict__ D, const KStat* __restrict__ St,
+                                                   int mode, int nb, int cap) {
   __shared__ double2 cs[kKCap][8];
   const KDesc& d = D[blockIdx.z];
-  const bool on = mode == 1 ? d.contract : d.live;
+  const KStat& st = St[blockIdx.z];
+  const bool on = mode == 1 ? d.contract : mode >= 3 ? (d.stage2 && st.repair) : d.live;
   if (!on) return;
-  const int ncols = mode == 0 ? d.nact : nb;
+  const int ncols = mode == 0 ? d.nact : mode == 3 ? d.m : mode == 4 ? st.kept : nb;
   const int j0 = blockIdx.y * 8;
   if (j0 >= ncols) return;
   const int m = d.m;
+  const double2* G = reinterpret_cast<const double2*>(d.E);
   for (int t = threadIdx.x; t < m * 8; t += blockDim.x) {
     const int l = t >> 3, jj = t & 7, j = j0 + jj;
-    const int col = j < ncols ? (mode == 0 ? d.act[j] : j) : 0;
-    cs[l][jj] = j < ncols ? d.C[l + int64_t(col) * cap] : make_double2(0.0, 0.0);
+    double2 c = make_double2(0.0, 0.0);
+    if (j < ncols) {
+      if (mode == 3) c = d.S[l + int64_t(j) * cap];
+      else if (mode == 4) c = G[l + int64_t(j) * m];
+      else c = d.C[l + int64_t(mode == 0 ? d.act[j] : j) * cap];
+    }
+    cs[l][jj] = c;
   }
   __syncthreads();
   const int64_t nc = d.n / 2;
@@ -146,6 +204,13 @@ __global__ __launch_bounds__(kKT) void k_kd_rotate(const KDesc* __restrict__ D, 
       ah[jj] = zadd(ah[jj], zmul(hv, cs[l][jj]));
     }
   }
+  if (mode == 4) {
+    double2* E = reinterpret_cast<double2*>(d.V) + int64_t(m) * nc;
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj)
+      if (j0 + jj < ncols) E[r + (j0 + jj) * nc] = zsub(E[r + (j0 + jj) * nc], ax[jj]);
+    return;
+  }
   double2* Y = reinterpret_cast<double2*>(mode == 2 ? d.out_x : d.X);
   double2* HY = reinterpret_cast<double2*>(mode == 2 ? d.out_hx : d.HX);
   const int64_t ldy = mode == 2 ? d.ldo / 2 : nc;
@@ -158,11 +223,13 @@ __global__ __launch_bounds__(kKT) void k_kd_rotate(const KDesc* __restrict__ D, 
   }
 }
 
-// V[:, :nb] = X[:, :nb], HV[:, :nb] = HX[:, :nb] for contracting k points.
-__global__ void k_kd_take(const KDesc* __restrict__ D, int nb) {
+// V[:, :c] = X[:, :c], HV[:, :c] = HX[:, :c]: c = nb for contracting k points
+// (mode 0), c = m for k points in rank repair (mode 1).
+__global__ void k_kd_take(const KDesc* __restrict__ D, const KStat* __restrict__ St, int mode, int nb) {
   const KDesc& d = D[blockIdx.y];
-  if (!d.contract) return;
-  const int64_t total = d.n * nb;
+  const KStat& st = St[blockIdx.y];
+  if (mode == 0 ? !d.contract : !(d.stage2 && st.repair)) return;
+  const int64_t total = d.n * (mode == 0 ? nb : d.m);
   for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += int64_t(gridDim.x) * blockDim.x) {
     d.V[t] = d.X[t];
     d.HV[t] = d.HX[t];
@@ -263,11 +330,11 @@ __global__ __launch_bounds__(kKT) void k_kd_normalize(const KDesc* __restrict__ 
 
 // Hermitian Gram border (ref_davidson_complex.rs:235-276): G = V[:, :to]^H
 // B[:, from:to] with B = V (which 0: S) or HV (which 1: H), written to both
-// triangles of M (exactly Hermitian, real diagonal). mode 0: the border of
-// stage-2 k points ([m, m + kk)); mode 1: the full block [0, nb) of
-// contracting k points that measure it; mode 2: [0, nb) of every live k
-// point (the start). A 16 x 16 tile of G per workgroup, 64 rows at a time
-// through LDS; for which 1 a non-finite H·V entry sets stat.nonfinite_h.
+// triangles of M (exactly Hermitian, real diagonal), over the range of
+// kd_range(mode). which 2: the rank repair's projection G = V[:, :m]^H
+// V[:, m:m + kept] into E (ld m, k points in repair). A 16 x 16 tile of G per
+// workgroup, 64 rows at a time through LDS; for which 1 a non-finite H·V
+// entry sets stat.nonfinite_h.
 __global__ __launch_bounds__(kKT) void k_kd_border(const KDesc* __restrict__ D, KStat* __restrict__ St, int which,
                                                    int mode, int nb, int cap) {
   constexpr int R = 64;
@@ -275,26 +342,22 @@ __global__ __launch_bounds__(kKT) void k_kd_border(const KDesc* __restrict__ D, 
   __shared__ double2 bs[16][R + 1];
   const KDesc& d = D[blockIdx.z];
   KStat& st = St[blockIdx.z];
-  int from, to;
-  if (mode == 0) {
-    if (!d.stage2) return;
+  int from, to, rows;
+  if (which == 2) {
+    if (!d.stage2 || !st.repair || st.kept <= 0) return;
     from = d.m;
-    to = d.m + d.kk;
-  } else if (mode == 1) {
-    if (!d.contract || !st.refresh) return;
-    from = 0;
-    to = nb;
+    to = d.m + st.kept;
+    rows = d.m;
   } else {
-    if (!d.live) return;
-    from = 0;
-    to = nb;
+    if (!kd_range(d, st, mode, nb, from, to)) return;
+    rows = to;
   }
   const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16;
-  if (i0 >= to || j0 >= to - from) return;
+  if (i0 >= rows || j0 >= to - from) return;
   const int ti = threadIdx.x & 15, tj = threadIdx.x >> 4;
   const int64_t nc = d.n / 2;
   const double2* V = reinterpret_cast<const double2*>(d.V);
-  const double2* B = reinterpret_cast<const double2*>(which ? d.HV : d.V);
+  const double2* B = reinterpret_cast<const double2*>(which == 1 ? d.HV : d.V);
   double2 acc = make_double2(0.0, 0.0);
   int bad = 0;
   for (int64_t r0 = 0; r0 < nc; r0 += R) {
@@ -302,7 +365,7 @@ __global__ __launch_bounds__(kKT) void k_kd_border(const KDesc* __restrict__ D, 
       const int c = t / R, rr = t % R;
       const int64_t r = r0 + rr;
       const int iv = i0 + c, jb = from + j0 + c;
-      vs[c][rr] = (iv < to && r < nc) ? V[r + iv * nc] : make_double2(0.0, 0.0);
+      vs[c][rr] = (iv < rows && r < nc) ? V[r + iv * nc] : make_double2(0.0, 0.0);
       double2 b = (jb < to && r < nc) ? B[r + jb * nc] : make_double2(0.0, 0.0);
       bad |= !(isfinite(b.x) && isfinite(b.y));
       bs[c][rr] = b;
@@ -314,8 +377,12 @@ __global__ __launch_bounds__(kKT) void k_kd_border(const KDesc* __restrict__ D, 
   }
   if (which == 1 && bad) st.nonfinite_h = 1;
   const int i = i0 + ti, jc = from + j0 + tj;
-  if (i >= to || jc >= to) return;
-  double2* M = which ? d.H : d.S;
+  if (i >= rows || jc >= to) return;
+  if (which == 2) {
+    reinterpret_cast<double2*>(d.E)[i + int64_t(jc - from) * rows] = acc;
+    return;
+  }
+  double2* M = which == 1 ? d.H : d.S;
   if (i < jc) {
     M[i + int64_t(jc) * cap] = acc;
     M[jc + int64_t(i) * cap] = zconj(acc);
@@ -337,19 +404,7 @@ __global__ __launch_bounds__(kKT) void k_kd_factor(const KDesc* __restrict__ D, 
   const KDesc& d = D[blockIdx.x];
   KStat& st = St[blockIdx.x];
   int from, to;
-  if (mode == 0) {
-    if (!d.stage2) return;
-    from = d.m;
-    to = d.m + d.kk;
-  } else if (mode == 1) {
-    if (!d.contract || !st.refresh) return;
-    from = 0;
-    to = nb;
-  } else {
-    if (!d.live) return;
-    from = 0;
-    to = nb;
-  }
+  if (!kd_range(d, st, mode, nb, from, to)) return;
   const int k = to - from;
   double2* Y = fsh;            // from x k: L11^-1 S12, column-major (ld from)
   double2* Z = Y + from * k;   // k x k: the Schur complement, then L22 (ld k)
@@ -410,7 +465,11 @@ __global__ __launch_bounds__(kKT) void k_kd_factor(const KDesc* __restrict__ D, 
   }
   if (nf) bad = 1;
   __syncthreads();
-  if (threadIdx.x == 0) st.factor_bad = bad;
+  if (threadIdx.x == 0) {
+    if (mode == 1) st.refresh_bad = bad;
+    else if (mode >= 3) st.repair_bad |= bad;
+    else st.factor_bad = bad;
+  }
 }
 
 // The reduced problem of one k point per workgroup
@@ -431,54 +490,58 @@ __global__ __launch_bounds__(kKT) void k_kd_eigen(const KDesc* __restrict__ D, K
   KStat& st = St[blockIdx.x];
   int m;
   if (mode == 0) {
-    if (!d.stage2) return;
-    m = d.m + d.kk;
+    int from;
+    if (!kd_range(d, st, 5, nb, from, m)) return;  // nothing new (a repair that kept no correction)
   } else {
     if (!d.live) return;
     m = nb;
   }
-  if (st.factor_bad || st.nonfinite_h) return;
-  double2* Ls = esh;        // m x m (ld m)
-  double2* A = Ls + m * m;  // m x m
-  double2* W = A + m * m;   // m x m: L^-1 H, then the eigenvectors Q
+  if (st.factor_bad || st.nonfinite_h || st.repair_bad) return;
+  // Leading dimension m + 1 (odd): a row of a matrix (stride la) spreads over
+  // the LDS banks.
+  const int la = m + 1;
+  double2* Ls = esh;          // m x m
+  double2* A = Ls + m * la;   // m x m
+  double2* W = A + m * la;    // m x m: L^-1 H, then the eigenvectors Q
   for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
     const int i = t % m, j = t / m;
-    Ls[t] = d.L[i + int64_t(j) * cap];
-    W[t] = d.H[i + int64_t(j) * cap];
+    Ls[i + j * la] = d.L[i + int64_t(j) * cap];
+    W[i + j * la] = d.H[i + int64_t(j) * cap];
   }
   __syncthreads();
   // W = L^-1 H (forward substitution per column), then A = L^-1 W^H.
   for (int j = threadIdx.x; j < m; j += blockDim.x)
     for (int i = 0; i < m; ++i) {
-      double2 z = W[i + j * m];
-      for (int p = 0; p < i; ++p) z = zsub(z, zmul(Ls[i + p * m], W[p + j * m]));
-      W[i + j * m] = zscale(1.0 / Ls[i + i * m].x, z);
+      double2 z = W[i + j * la];
+      for (int p = 0; p < i; ++p) z = zsub(z, zmul(Ls[i + p * la], W[p + j * la]));
+      W[i + j * la] = zscale(1.0 / Ls[i + i * la].x, z);
     }
   __syncthreads();
   for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
     const int i = t % m, j = t / m;
-    A[i + j * m] = zconj(W[j + i * m]);
+    A[i + j * la] = zconj(W[j + i * la]);
   }
   __syncthreads();
   for (int j = threadIdx.x; j < m; j += blockDim.x)
     for (int i = 0; i < m; ++i) {
-      double2 z = A[i + j * m];
-      for (int p = 0; p < i; ++p) z = zsub(z, zmul(Ls[i + p * m], A[p + j * m]));
-      A[i + j * m] = zscale(1.0 / Ls[i + i * m].x, z);
+      double2 z = A[i + j * la];
+      for (int p = 0; p < i; ++p) z = zsub(z, zmul(Ls[i + p * la], A[p + j * la]));
+      A[i + j * la] = zscale(1.0 / Ls[i + i * la].x, z);
     }
   __syncthreads();
   // Hermitise (in W), then back into A; Q = I in W.
   for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
     const int i = t % m, j = t / m;
-    const double2 a = A[i + j * m], b = zconj(A[j + i * m]);
-    W[t] = i == j ? make_double2(0.5 * (a.x + b.x), 0.0) : zscale(0.5, zadd(a, b));
+    const double2 a = A[i + j * la], b = zconj(A[j + i * la]);
+    W[i + j * la] = i == j ? make_double2(0.5 * (a.x + b.x), 0.0) : zscale(0.5, zadd(a, b));
   }
   __syncthreads();
   double fro = 0.0;
   for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
-    A[t] = W[t];
-    W[t] = (t % m == t / m) ? make_double2(1.0, 0.0) : make_double2(0.0, 0.0);
-    fro += zabs2(A[t]);
+    const int i = t % m, j = t / m;
+    A[i + j * la] = W[i + j * la];
+    W[i + j * la] = i == j ? make_double2(1.0, 0.0) : make_double2(0.0, 0.0);
+    fro += zabs2(A[i + j * la]);
   }
   const double thr = kEps * sqrt(kd_sum(fro, red));
   const int mp = m + (m & 1);  // players; an odd m has one idle player per round
@@ -494,16 +557,17 @@ __global__ __launch_bounds__(kKT) void k_kd_eigen(const KDesc* __restrict__ D, K
         int p = min(a, b), q = max(a, b);
         double c = 1.0, s = 0.0, er = 1.0, ei = 0.0;
         if (q < m) {
-          const double2 apq = A[p + q * m];
+          const double2 apq = A[p + q * la];
           const double r = sqrt(zabs2(apq));
           if (r > thr) {
-            const double app = A[p + p * m].x, aqq = A[q + q * m].x;
-            const double th = (aqq - app) / (2.0 * r);
+            const double app = A[p + p * la].x, aqq = A[q + q * la].x;
+            const double ir = 1.0 / r;
+            const double th = 0.5 * (aqq - app) * ir;
             const double tt = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
             c = 1.0 / sqrt(tt * tt + 1.0);
             s = tt * c;
-            er = apq.x / r;
-            ei = apq.y / r;
+            er = apq.x * ir;
+            ei = apq.y * ir;
             atomicAdd(&nrot, 1);
           } else {
             q = -1;
@@ -526,12 +590,12 @@ __global__ __launch_bounds__(kKT) void k_kd_eigen(const KDesc* __restrict__ D, K
         if (q < 0) continue;
         const double c = rot[t][0], s = rot[t][1];
         const double2 e = make_double2(rot[t][2], rot[t][3]);
-        const double2 aip = A[i + p * m], aiq = A[i + q * m];
-        A[i + p * m] = zsub(zscale(c, aip), zscale(s, zmul(zconj(e), aiq)));
-        A[i + q * m] = zadd(zscale(s, zmul(e, aip)), zscale(c, aiq));
-        const double2 qip = W[i + p * m], qiq = W[i + q * m];
-        W[i + p * m] = zsub(zscale(c, qip), zscale(s, zmul(zconj(e), qiq)));
-        W[i + q * m] = zadd(zscale(s, zmul(e, qip)), zscale(c, qiq));
+        const double2 aip = A[i + p * la], aiq = A[i + q * la];
+        A[i + p * la] = zsub(zscale(c, aip), zscale(s, zmul(zconj(e), aiq)));
+        A[i + q * la] = zadd(zscale(s, zmul(e, aip)), zscale(c, aiq));
+        const double2 qip = W[i + p * la], qiq = W[i + q * la];
+        W[i + p * la] = zsub(zscale(c, qip), zscale(s, zmul(zconj(e), qiq)));
+        W[i + q * la] = zadd(zscale(s, zmul(e, qip)), zscale(c, qiq));
       }
       __syncthreads();
       // Rows: A <- U^H A; the pivot entries are set to their exact values.
@@ -541,7 +605,7 @@ __global__ __launch_bounds__(kKT) void k_kd_eigen(const KDesc* __restrict__ D, K
         if (q < 0) continue;
         const double c = rot[t][0], s = rot[t][1];
         const double2 e = make_double2(rot[t][2], rot[t][3]);
-        const double2 apj = A[p + j * m], aqj = A[q + j * m];
+        const double2 apj = A[p + j * la], aqj = A[q + j * la];
         double2 np = zsub(zscale(c, apj), zscale(s, zmul(e, aqj)));
         double2 nq = zadd(zscale(s, zmul(zconj(e), apj)), zscale(c, aqj));
         if (j == p) {
@@ -551,8 +615,8 @@ __global__ __launch_bounds__(kKT) void k_kd_eigen(const KDesc* __restrict__ D, K
           np = make_double2(0.0, 0.0);
           nq.y = 0.0;
         }
-        A[p + j * m] = np;
-        A[q + j * m] = nq;
+        A[p + j * la] = np;
+        A[q + j * la] = nq;
       }
       __syncthreads();
     }
@@ -561,10 +625,10 @@ __global__ __launch_bounds__(kKT) void k_kd_eigen(const KDesc* __restrict__ D, K
   }
   // Ascending order (ties by index), the lowest nb.
   for (int i = threadIdx.x; i < m; i += blockDim.x) {
-    const double di = A[i + i * m].x;
+    const double di = A[i + i * la].x;
     int rank = 0;
     for (int j = 0; j < m; ++j) {
-      const double dj = A[j + j * m].x;
+      const double dj = A[j + j * la].x;
       rank += (dj < di || (dj == di && j < i)) ? 1 : 0;
     }
     perm[rank] = i;
@@ -572,7 +636,7 @@ __global__ __launch_bounds__(kKT) void k_kd_eigen(const KDesc* __restrict__ D, K
   __syncthreads();
   int nf = 0;
   for (int r = threadIdx.x; r < nb; r += blockDim.x) {
-    const double v = A[perm[r] + perm[r] * m].x;
+    const double v = A[perm[r] + perm[r] * la].x;
     nf |= !isfinite(v);
     st.values[r] = v;
   }
@@ -581,9 +645,9 @@ __global__ __launch_bounds__(kKT) void k_kd_eigen(const KDesc* __restrict__ D, K
     const int src = perm[r];
     double2* c = d.C + int64_t(r) * cap;
     for (int i = m - 1; i >= 0; --i) {
-      double2 z = W[i + src * m];
-      for (int p = i + 1; p < m; ++p) z = zsub(z, zcmul(Ls[p + i * m], c[p]));
-      c[i] = zscale(1.0 / Ls[i + i * m].x, z);
+      double2 z = W[i + src * la];
+      for (int p = i + 1; p < m; ++p) z = zsub(z, zcmul(Ls[p + i * la], c[p]));
+      c[i] = zscale(1.0 / Ls[i + i * la].x, z);
       nf |= !(isfinite(c[i].x) && isfinite(c[i].y));
     }
   }
@@ -672,15 +736,144 @@ __global__ __launch_bounds__(kKT) void k_kd_defect(const KDesc* __restrict__ D, 
 __global__ void k_kd_clear_flags(KStat* __restrict__ St, int count) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < count) {
+    St[k].kept = 0;
     St[k].factor_bad = 0;
     St[k].eig_bad = 0;
     St[k].nonfinite_h = 0;
     St[k].nonfinite_e = 0;
+    St[k].refresh_bad = 0;
+    St[k].repair = 0;
+    St[k].repair_bad = 0;
+    St[k].kept2 = 0;
   }
 }
 
+// Rank repair, first step (ref_davidson_complex.rs:826-850 with
+// ComplexOverlapFactor::basis_transform_into, linalg.rs:342-360), one
+// workgroup per k point whose border the factor rejected: the Ritz block's
+// coordinates in the whitened basis V L^-H, C <- L^H C, and the whitening
+// R = L^-H into S (the overlap is measured again after the rotation).
+__global__ __launch_bounds__(kKT) void k_kd_repair_prep(const KDesc* __restrict__ D, KStat* __restrict__ St,
+                                                        int nb, int cap) {
+  extern __shared__ double2 rsh[];
+  const KDesc& d = D[blockIdx.x];
+  KStat& st = St[blockIdx.x];
+  if (!d.stage2 || !st.factor_bad) return;
+  const int m = d.m;
+  double2* Ls = rsh;        // m x m
+  double2* Cs = Ls + m * m;  // m x nb
+  for (int t = threadIdx.x; t < m * m; t += blockDim.x) Ls[t] = d.L[t % m + int64_t(t / m) * cap];
+  for (int t = threadIdx.x; t < m * nb; t += blockDim.x) Cs[t] = d.C[t % m + int64_t(t / m) * cap];
+  __syncthreads();
+  for (int t = threadIdx.x; t < m * nb; t += blockDim.x) {
+    const int i = t % m, j = t / m;
+    double2 z = make_double2(0.0, 0.0);
+    for (int p = i; p < m; ++p) z = zadd(z, zcmul(Ls[p + i * m], Cs[p + j * m]));
+    d.C[i + int64_t(j) * cap] = z;
+  }
+  // R = L^-H, upper triangular: one thread per column, back substitution.
+  for (int j = threadIdx.x; j < m; j += blockDim.x) {
+    double2* R = d.S + int64_t(j) * cap;
+    for (int i = m - 1; i > j; --i) R[i] = make_double2(0.0, 0.0);
+    R[j] = make_double2(1.0 / Ls[j + j * m].x, 0.0);
+    for (int i = j - 1; i >= 0; --i) {
+      double2 z = make_double2(0.0, 0.0);
+      for (int p = i + 1; p <= j; ++p) z = zadd(z, zcmul(Ls[p + i * m], R[p]));
+      R[i] = zscale(-1.0 / Ls[i + i * m].x, z);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    st.repair = 1;
+    st.factor_bad = 0;
+  }
+}
+
+// Rank repair, last step (ref_davidson_complex.rs:862-888), one workgroup per
+// k point in repair: the projected corrections V[:, m:m + kept] in order,
+// each made orthogonal to the ones kept before it (two classical
+// Gram-Schmidt passes), dropped at norm <= 1e-12 or when the basis is full,
+// else normalised into the next free column. stat.kept2 = the count kept.
+__global__ __launch_bounds__(kKT) void k_kd_repair_gs(const KDesc* __restrict__ D, KStat* __restrict__ St,
+                                                      int cap) {
+  __shared__ double red[kKT / 64][2 * kKB];
+  __shared__ double2 zs[kKB];
+  __shared__ double nrm2;
+  const KDesc& d = D[blockIdx.x];
+  KStat& st = St[blockIdx.x];
+  if (!d.stage2 || !st.repair) return;
+  const int m = d.m, kin = st.kept;
+  const int64_t nc = d.n / 2;
+  double2* Q = reinterpret_cast<double2*>(d.V) + int64_t(m) * nc;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int kept = 0;
+  for (int j = 0; j < kin; ++j) {
+    double2* e = Q + int64_t(j) * nc;
+    for (int pass = 0; pass < 2 && kept > 0; ++pass) {
+      double2 acc[kKB];
+#pragma unroll
+      for (int a = 0; a < kKB; ++a) acc[a] = make_double2(0.0, 0.0);
+      for (int64_t r = threadIdx.x; r < nc; r += blockDim.x) {
+        const double2 er = e[r];
+#pragma unroll
+        for (int a = 0; a < kKB; ++a)
+          if (a < kept) acc[a] = zadd(acc[a], zcmul(Q[r + a * nc], er));
+      }
+#pragma unroll
+      for (int a = 0; a < kKB; ++a) {
+        if (a < kept) {
+          double x = acc[a].x, y = acc[a].y;
+          for (int off = 32; off > 0; off >>= 1) {
+            x += __shfl_down(x, off, 64);
+            y += __shfl_down(y, off, 64);
+          }
+          if (lane == 0) {
+            red[wave][2 * a] = x;
+            red[wave][2 * a + 1] = y;
+          }
+        }
+      }
+      __syncthreads();
+      if (threadIdx.x < kept) {
+        double x = 0.0, y = 0.0;
+        for (int w = 0; w < kKT / 64; ++w) {
+          x += red[w][2 * threadIdx.x];
+          y += red[w][2 * threadIdx.x + 1];
+        }
+        zs[threadIdx.x] = make_double2(x, y);
+      }
+      __syncthreads();
+      for (int64_t r = threadIdx.x; r < nc; r += blockDim.x) {
+        double2 v = e[r];
+        for (int a = 0; a < kept; ++a) v = zsub(v, zmul(zs[a], Q[r + a * nc]));
+        e[r] = v;
+      }
+      __syncthreads();
+    }
+    double s = 0.0;
+    for (int64_t r = threadIdx.x; r < nc; r += blockDim.x) s += zabs2(e[r]);
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) red[wave][0] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int w = 0; w < kKT / 64; ++w) t += red[w][0];
+      nrm2 = t;
+    }
+    __syncthreads();
+    const double nrm = sqrt(nrm2);
+    if (!(nrm > 1e-12) || m + kept == cap) continue;  // DEPENDENCE_TOLERANCE, linalg.rs:12
+    const double inv = 1.0 / nrm;
+    double2* q = Q + int64_t(kept) * nc;
+    for (int64_t r = threadIdx.x; r < nc; r += blockDim.x) q[r] = zscale(inv, e[r]);
+    __syncthreads();
+    ++kept;
+  }
+  if (threadIdx.x == 0) st.kept2 = kept;
+}
+
 size_t factor_lds(int cap) { return size_t(cap) * cap * 2 * sizeof(double2); }
-size_t eigen_lds(int cap) { return size_t(3) * cap * cap * sizeof(double2); }
+size_t eigen_lds(int cap) { return size_t(3) * cap * (cap + 1) * sizeof(double2); }
 
 // The two LDS-resident kernels may take their dynamic LDS at the largest
 // capacity (their static LDS comes on top, within the 160 KB of a workgroup).
@@ -692,6 +885,9 @@ void set_lds_limits() {
                               int(eigen_lds(kKCap)));
     if (err == hipSuccess)
       err = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_kd_factor),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(factor_lds(kKCap)));
+    if (err == hipSuccess)
+      err = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_kd_repair_prep),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(factor_lds(kKCap)));
     if (err != hipSuccess) (void)hipGetLastError();
   });
@@ -844,14 +1040,14 @@ struct KDavidson {
   }
 
   void border(int which, int mode) {
-    hipLaunchKernelGGL(k_kd_border, dim3((cap + 15) / 16, (nb + 15) / 16, count), dim3(kKT), 0, main, desc, stat,
+    hipLaunchKernelGGL(k_kd_border, dim3((cap + 15) / 16, (cap + 15) / 16, count), dim3(kKT), 0, main, desc, stat,
                        which, mode, nb, cap);
   }
   void factor(int mode) {
     hipLaunchKernelGGL(k_kd_factor, dim3(count), dim3(kKT), factor_lds(cap), main, desc, stat, mode, nb, cap);
   }
   void eigen(int mode) {
-    hipLaunchKernelGGL(k_kd_eigen, dim3(count), dim3(kKT), eigen_lds(cap), main, desc, stat, mode, nb, cap);
+    hipLaunchKernelGGL(k_kd_eigen, dim3(count), dim3(kEigT), eigen_lds(cap), main, desc, stat, mode, nb, cap);
   }
 
   double band_threshold(const bd_davidson_options& o, int i) const {
@@ -926,6 +1122,7 @@ struct KDavidson {
         KDesc& d = hdesc[k];
         d.live = !p.done && !p.ejected;
         d.stage2 = 0;
+        d.kk = 0;
         d.m = p.m;
         d.contract = d.live && p.contract;
         if (d.contract) p.rec.restarts += 1;
@@ -942,8 +1139,9 @@ struct KDavidson {
       // H and the factor.
       if (contracting) {
         push_desc();
-        hipLaunchKernelGGL(k_kd_rotate, dim3(row_blocks(), (nb + 7) / 8, count), dim3(kKT), 0, main, desc, 1, nb, cap);
-        hipLaunchKernelGGL(k_kd_take, dim3(64, count), dim3(256), 0, main, desc, nb);
+        hipLaunchKernelGGL(k_kd_rotate, dim3(row_blocks(), (nb + 7) / 8, count), dim3(kKT), 0, main, desc, stat, 1,
+                           nb, cap);
+        hipLaunchKernelGGL(k_kd_take, dim3(64, count), dim3(256), 0, main, desc, stat, 0, nb);
         hipLaunchKernelGGL(k_kd_restart, dim3(count), dim3(kKT), 0, main, desc, stat, nb, cap);
         border(0, 1);
         border(1, 1);
@@ -955,18 +1153,24 @@ struct KDavidson {
             kp[k].m = nb;
           }
       }
+      // The Ritz columns of the active bands, their corrections (dropped and
+      // normalised on the device, appended at m), the border of S and the
+      // bordered factor; then one read-back for every k point.
       push_desc();
-      hipLaunchKernelGGL(k_kd_rotate, dim3(row_blocks(), (nb + 7) / 8, count), dim3(kKT), 0, main, desc, 0, nb, cap);
+      hipLaunchKernelGGL(k_kd_rotate, dim3(row_blocks(), (nb + 7) / 8, count), dim3(kKT), 0, main, desc, stat, 0, nb,
+                         cap);
       hipLaunchKernelGGL(k_kd_corrections, dim3(nb, count), dim3(kKT), 0, main, desc, stat, mode, norms.p);
       hipLaunchKernelGGL(k_kd_normalize, dim3(count), dim3(kKT), 0, main, desc, stat, norms.p, cap);
+      border(0, 0);
+      factor(0);
       pull_stat();
-      int stage2 = 0;
+      int stage2 = 0, repairs = 0;
       for (int k = 0; k < count; ++k) {
         KDesc& d = hdesc[k];
         if (!d.live) continue;
         const KStat& st = hstat[k];
         KPoint& p = kp[k];
-        if (d.contract && (st.factor_bad || st.nonfinite_h)) {
+        if (d.contract && (st.refresh_bad || st.nonfinite_h)) {
           *msg = "complex restart metric is dependent";
           statuses[k] = BD_ERR_APPLICATION;
           return k;
@@ -978,23 +1182,45 @@ struct KDavidson {
         }
         d.kk = st.kept;
         if (d.kk > 0 && p.m + d.kk > cap) {
-          // Not staged: the rank repair of the reference; this k point
-          // continues in the single-k complex Davidson from its Ritz block.
+          // Not staged (the capacity rule keeps m + active <= cap, so this
+          // does not happen): this k point continues in the single-k
+          // complex Davidson from its Ritz block.
           p.ejected = true;
           ejected.push_back(k);
+          d.kk = 0;
           continue;
         }
-        d.m = p.m;
         d.stage2 = d.kk > 0;
         stage2 += d.stage2;
+        repairs += d.stage2 && st.factor_bad;
       }
       auto td = std::chrono::steady_clock::now();
       if (stage2) {
         push_desc();
-        border(0, 0);
-        factor(0);
+        if (repairs) {
+          // Rank repair of the k points whose border the factor rejected
+          // (ref_davidson_complex.rs:822-900): whiten the old basis (V, HV
+          // <- V L^-H, HV L^-H, C <- L^H C), measure S and H of it and
+          // factor, project the corrections out twice, orthonormalise them
+          // among themselves, border S and extend the factor.
+          hipLaunchKernelGGL(k_kd_repair_prep, dim3(count), dim3(kKT), factor_lds(cap), main, desc, stat, nb, cap);
+          hipLaunchKernelGGL(k_kd_rotate, dim3(row_blocks(), (cap + 7) / 8, count), dim3(kKT), 0, main, desc, stat, 3,
+                             nb, cap);
+          hipLaunchKernelGGL(k_kd_take, dim3(64, count), dim3(256), 0, main, desc, stat, 1, nb);
+          border(0, 3);
+          border(1, 3);
+          factor(3);
+          for (int pass = 0; pass < 2; ++pass) {
+            border(2, 0);
+            hipLaunchKernelGGL(k_kd_rotate, dim3(row_blocks(), (nb + 7) / 8, count), dim3(kKT), 0, main, desc, stat,
+                               4, nb, cap);
+          }
+          hipLaunchKernelGGL(k_kd_repair_gs, dim3(count), dim3(kKT), 0, main, desc, stat, cap);
+          border(0, 4);
+          factor(4);
+        }
         apply(false);
-        border(1, 0);
+        border(1, 5);
         eigen(0);
       }
       pull_stat();
@@ -1011,20 +1237,25 @@ struct KDavidson {
             statuses[k] = BD_ERR_APPLICATION;
             return k;
           }
-          if (st.factor_bad) {
-            // A dependent border: the reference repairs it in place; this k
-            // point continues in the single-k solver from its Ritz block
-            // (C still belongs to the basis [0, m)).
-            p.ejected = true;
-            ejected.push_back(k);
-            continue;
+          if (st.repair_bad) {
+            *msg = "complex overlap remains dependent after repair";
+            statuses[k] = BD_ERR_APPLICATION;
+            return k;
           }
           if (st.eig_bad) {
             *msg = "complex reduced eigensolve failed";
             statuses[k] = BD_ERR_APPLICATION;
             return k;
           }
-          p.m += d.kk;
+          int added = d.kk;
+          if (st.repair) {
+            // H was applied to every correction; the reference applies it to
+            // the kept ones only, and so counts them.
+            added = st.kept2;
+            p.rec.repairs += 1;
+            p.rec.h_columns -= d.kk - added;
+          }
+          p.m += added;
           std::copy(st.values, st.values + nb, p.values.begin());
         }
         p.active.clear();
@@ -1049,7 +1280,8 @@ struct KDavidson {
       d.ldo = ldx[k];
     }
     push_desc();
-    hipLaunchKernelGGL(k_kd_rotate, dim3(row_blocks(), (nb + 7) / 8, count), dim3(kKT), 0, main, desc, 2, nb, cap);
+    hipLaunchKernelGGL(k_kd_rotate, dim3(row_blocks(), (nb + 7) / 8, count), dim3(kKT), 0, main, desc, stat, 2, nb,
+                       cap);
     pull_stat();
     const double total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     int failed = -1;
@@ -1068,8 +1300,8 @@ struct KDavidson {
         for (int q = 0; q < count; ++q)
           if (q != k) hdesc[q].live = 0;
         push_desc();
-        hipLaunchKernelGGL(k_kd_rotate, dim3(row_blocks(), (nb + 7) / 8, count), dim3(kKT), 0, main, desc, 2, nb,
-                           cap);
+        hipLaunchKernelGGL(k_kd_rotate, dim3(row_blocks(), (nb + 7) / 8, count), dim3(kKT), 0, main, desc, stat, 2,
+                           nb, cap);
         sync();
         bd_davidson_options o2 = o;
         o2.complex_columns = 1;

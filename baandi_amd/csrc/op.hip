@@ -13,10 +13,13 @@
 //   projections      Q = P^T X (split-K GEMM), W = D Q (per-atom blocks), Y += P W.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <set>
 #include <string>
@@ -453,12 +456,14 @@ __global__ void k_to_float(const double* __restrict__ src, int64_t n, float* __r
 }
 
 // rho = sum over groups of the partial densities, in group order.
-__global__ void k_sum_groups(const double* __restrict__ part, int64_t N, int groups, double* __restrict__ rho) {
+// rho = sum of the group partials (accumulate: rho += that sum).
+__global__ void k_sum_groups(const double* __restrict__ part, int64_t N, int groups, double* __restrict__ rho,
+                             int accumulate) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= N) return;
   double s = 0.0;
   for (int g = 0; g < groups; ++g) s += part[int64_t(g) * N + i];
-  rho[i] = s;
+  rho[i] = accumulate ? rho[i] + s : s;
 }
 
 // W = D Q per atom block (distributed_real_gamma.rs:1183-1195).
@@ -934,7 +939,7 @@ void GammaOp::local_apply(const double* x, int64_t ldx, int m, double* y, int64_
 }
 
 template <typename F>
-void GammaOp::density(const double* x, int64_t ldx, int m, const double* w, double* rho) {
+void GammaOp::density(const double* x, int64_t ldx, int m, const double* w, double* rho, bool accumulate) {
   const StickPlan& p = plan;
   const int pairs = slots(m);
   const size_t e = sizeof(c2<F>);
@@ -964,7 +969,7 @@ void GammaOp::density(const double* x, int64_t ldx, int m, const double* w, doub
                        p.n1, p.n2, p.Spad, p.Z, p.chunk, d_stick_yz.p, d_zc.p, d_roots.p, p.ry, p.rz, nullptr,
                        pairs, w, 1.0 / sys->volume, d_rho_part.p, keep);
   hipLaunchKernelGGL(k_sum_groups, dim3(unsigned((N + 255) / 256)), dim3(256), 0, stream, d_rho_part.p, N,
-                     groups, rho);
+                     groups, rho, accumulate ? 1 : 0);
 }
 
 // Keep (on = true) the real-space pair fields of every later band density,
@@ -1074,12 +1079,12 @@ void GammaOp::apply(const double* x, int64_t ldx, int m, double* y, int64_t ldy)
   BD_HIP(hipGetLastError());
 }
 
-void GammaOp::band_density(const double* x, int64_t ldx, int m, const double* w, double* rho) {
+void GammaOp::band_density(const double* x, int64_t ldx, int m, const double* w, double* rho, bool accumulate) {
   if (m <= 0 || m > capacity) fail(BD_ERR_INVALID_PARAMETER, "column count");
   if (ldx < n) fail(BD_ERR_INVALID_PARAMETER, "leading dimension smaller than rows");
   if (kb && !complex_aligned(x, ldx)) fail(BD_ERR_INVALID_PARAMETER, "k-point X must be 16-byte aligned, even ldx");
-  if (precision == BD_MIXED) density<float>(x, ldx, m, w, rho);
-  else density<double>(x, ldx, m, w, rho);
+  if (precision == BD_MIXED) density<float>(x, ldx, m, w, rho, accumulate);
+  else density<double>(x, ldx, m, w, rho, accumulate);
   BD_HIP(hipGetLastError());
 }
 
@@ -1212,6 +1217,131 @@ extern "C" bd_status bd_gamma_op_band_density_dev(bd_gamma_op* op, const double*
     o.d_weights.upload(hw.data(), hw.size(), o.stream);
     o.band_density(x, ldx, m, o.d_weights.p, rho);
     BD_HIP(hipStreamSynchronize(o.stream));
+  });
+}
+
+namespace bd {
+namespace {
+// rho[i] = sum over k in order of part[k N + i].
+__global__ void k_sum_k(const double* __restrict__ part, int64_t N, int count, double* __restrict__ rho) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double s = part[i];
+  for (int k = 1; k < count; ++k) s += part[int64_t(k) * N + i];
+  rho[i] = s;
+}
+
+// Device scratch of the k-point batch calls, grown on demand (one per process;
+// the calls are synchronous and not meant to overlap).
+struct KScratch {
+  std::mutex mu;
+  double* p = nullptr;
+  size_t n = 0;
+  double* get(size_t count) {
+    if (count > n) {
+      if (p) BD_HIP(hipFree(p));
+      p = nullptr;
+      n = 0;
+      BD_HIP(hipMalloc(&p, count * sizeof(double)));
+      n = count;
+    }
+    return p;
+  }
+};
+KScratch& k_scratch() {
+  static KScratch s;
+  return s;
+}
+
+// Order every op's stream after `caller` (null: no caller stream to wait for).
+void k_after(bd_gamma_op* const* ops, int count, hipStream_t caller, hipEvent_t ev) {
+  if (!caller) return;
+  BD_HIP(hipEventRecord(ev, caller));
+  for (int k = 0; k < count; ++k) BD_HIP(hipStreamWaitEvent(ops[k]->op.stream, ev, 0));
+}
+}  // namespace
+}  // namespace bd
+
+// The band densities of every k point of a map summed into one grid, and the
+// potential of every k point set from one grid: one call each instead of one
+// per k point. The k points' densities run side by side on their ops'
+// streams into separate grids, summed in k order at the end (the bits of the
+// per-k densities added in k order).
+extern "C" bd_status bd_k_band_density_dev(bd_gamma_op* const* ops, int32_t count, const double* const* x,
+                                           const int64_t* ldx, const int32_t* m, const double* const* weights,
+                                           double* rho, void* stream) {
+  BD_GUARD({
+    if (!ops || !x || !ldx || !m || !weights || !rho || count <= 0) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    int mmax = 0;
+    for (int k = 0; k < count; ++k) {
+      if (!ops[k] || !x[k] || !weights[k]) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+      const auto& o = ops[k]->op;
+      if (!o.kb) bd::fail(BD_ERR_INVALID_PARAMETER, "k-point ops only");
+      if (o.N != ops[0]->op.N) bd::fail(BD_ERR_INVALID_PARAMETER, "k-point ops must share the grid");
+      if (m[k] <= 0 || m[k] > o.capacity) bd::fail(BD_ERR_INVALID_PARAMETER, "column count");
+      mmax = std::max(mmax, int(m[k]));
+    }
+    const int64_t N = ops[0]->op.N;
+    bd::KScratch& ks = bd::k_scratch();
+    std::lock_guard<std::mutex> lock(ks.mu);
+    // Per k: the 2 m weights (each complex column's |psi|^2 = Re^2 + Im^2 takes
+    // its weight twice), then count density grids.
+    const size_t wstride = size_t(2 * mmax);
+    double* dev = ks.get(wstride * count + size_t(N) * count);
+    double* wdev = dev;
+    double* part = dev + wstride * count;
+    std::vector<double> hw(wstride * count, 0.0);
+    for (int k = 0; k < count; ++k)
+      for (int j = 0; j < m[k]; ++j) hw[k * wstride + 2 * j] = hw[k * wstride + 2 * j + 1] = weights[k][j];
+    hipStream_t s0 = ops[0]->op.stream;
+    hipEvent_t ev = nullptr;
+    BD_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    try {
+      bd::k_after(ops, count, static_cast<hipStream_t>(stream), ev);
+      BD_HIP(hipMemcpyAsync(wdev, hw.data(), hw.size() * sizeof(double), hipMemcpyHostToDevice, s0));
+      BD_HIP(hipEventRecord(ev, s0));
+      for (int k = 0; k < count; ++k) {
+        auto& o = ops[k]->op;
+        if (k > 0) BD_HIP(hipStreamWaitEvent(o.stream, ev, 0));
+        o.band_density(x[k], ldx[k], m[k], wdev + k * wstride, part + int64_t(k) * N, false);
+      }
+      for (int k = 1; k < count; ++k) {
+        BD_HIP(hipEventRecord(ev, ops[k]->op.stream));
+        BD_HIP(hipStreamWaitEvent(s0, ev, 0));
+      }
+      hipLaunchKernelGGL(bd::k_sum_k, dim3(unsigned((N + 255) / 256)), dim3(256), 0, s0, part, N, count, rho);
+      BD_HIP(hipGetLastError());
+      BD_HIP(hipStreamSynchronize(s0));
+    } catch (...) {
+      (void)hipEventDestroy(ev);
+      throw;
+    }
+    BD_HIP(hipEventDestroy(ev));
+  });
+}
+
+extern "C" bd_status bd_k_set_potential_dev(bd_gamma_op* const* ops, int32_t count, const double* v,
+                                            uint64_t epoch, void* stream) {
+  BD_GUARD({
+    if (!ops || !v || count <= 0) bd::fail(BD_ERR_INVALID_PARAMETER, "null argument");
+    for (int k = 0; k < count; ++k)
+      if (!ops[k]) bd::fail(BD_ERR_INVALID_PARAMETER, "null op");
+    hipEvent_t ev = nullptr;
+    BD_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    try {
+      bd::k_after(ops, count, static_cast<hipStream_t>(stream), ev);
+      for (int k = 0; k < count; ++k) ops[k]->op.set_potential_dev(v, epoch);
+      // The caller may overwrite v once every copy has been made.
+      if (stream)
+        for (int k = 0; k < count; ++k) {
+          BD_HIP(hipEventRecord(ev, ops[k]->op.stream));
+          BD_HIP(hipStreamWaitEvent(static_cast<hipStream_t>(stream), ev, 0));
+        }
+    } catch (...) {
+      (void)hipEventDestroy(ev);
+      throw;
+    }
+    BD_HIP(hipEventDestroy(ev));
   });
 }
 

@@ -160,14 +160,14 @@ struct GammaOp {
   ~GammaOp();
   void set_potential_dev(const double* v, uint64_t epoch);
   void apply(const double* x, int64_t ldx, int m, double* y, int64_t ldy);
-  void band_density(const double* x, int64_t ldx, int m, const double* w, double* rho);
+  void band_density(const double* x, int64_t ldx, int m, const double* w, double* rho, bool accumulate = false);
   void projections(const double* x, int64_t ldx, int m);  // d_q = P^T X
   void refresh_potential32();
   template <typename F>
   void local_apply(const double* x, int64_t ldx, int m, double* y, int64_t ldy);
   size_t plane_lds_bytes(int chunk) const;
   template <typename F>
-  void density(const double* x, int64_t ldx, int m, const double* w, double* rho);
+  void density(const double* x, int64_t ldx, int m, const double* w, double* rho, bool accumulate = false);
 };
 
 // Launch the fused H·X for `total` work items over `count` k points (works on

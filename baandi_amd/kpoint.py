@@ -230,6 +230,35 @@ class KPointBatch:
             self.handle = None
 
 
+def set_potential_all(ops, potential):
+    """The same V(r) (CUDA float64 grid) into every k-point op, one call,
+    ordered after torch's current stream (which then waits for the copies)."""
+    import torch
+    v = potential.reshape(-1).contiguous()
+    epoch = max(op.epoch for op in ops) + 1
+    arr = (C.c_void_p * len(ops))(*[op.handle.value for op in ops])
+    L.check(L.load().bd_k_set_potential_dev(arr, len(ops), L.ptr(v), epoch,
+                                            C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    for op in ops:
+        op.epoch = epoch
+
+
+def band_density_all(ops, xs, weights, out):
+    """out(grid) = sum over k of the band density of xs[k] ((m, 2 nk) CUDA
+    float64, complex columns) with weights[k], summed in k order (one call,
+    ordered after torch's current stream; synchronous)."""
+    import torch
+    count = len(ops)
+    arr = lambda vals: (C.c_void_p * count)(*vals)
+    ws = [np.ascontiguousarray(np.asarray(w, dtype=np.float64)) for w in weights]
+    ld = np.array([x.shape[1] for x in xs], dtype=np.int64)
+    m = np.array([x.shape[0] for x in xs], dtype=np.int32)
+    L.check(L.load().bd_k_band_density_dev(arr([op.handle.value for op in ops]), count,
+                                           arr([x.data_ptr() for x in xs]), L.ptr(ld), L.ptr(m),
+                                           arr([w.ctypes.data for w in ws]), L.ptr(out),
+                                           C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+
 class KDavidson:
     """The complex Davidson of every k point in lock step (bd_k_davidson_*):
     each k point keeps its own Davidson (ref_davidson_complex.rs:641-1113),
@@ -269,8 +298,8 @@ class KDavidson:
         conv = np.zeros((count, self.bands), dtype=np.int32)
         recs = (L.DavidsonRecord * count)()
         statuses = np.zeros(count, dtype=np.int32)
-        for op in self.ops:
-            op.enter()
+        # The inputs are complete once torch's stream is; the solver orders
+        # itself after the ops' own streams.
         torch.cuda.current_stream().synchronize()
         st = L.load().bd_k_davidson_solve_dev(self.handle, x0, L.ptr(ld0), hx0, dg, C.byref(opts), L.ptr(eig), xo,
                                               hxo, L.ptr(ldx), L.ptr(conv), recs, L.ptr(statuses))
@@ -447,6 +476,7 @@ class KScfOptions:
     seed: int = 1
     threads: int = 8                # k points solved concurrently (host threads, one stream each)
     autotune: bool = False          # tune the fused kernel's launch shape at set-up (bit-neutral)
+    fused_shape: tuple = None       # (threads, xlines, zrows, xplanes): fix the fused shape instead
     preconditioning: str = "plain"  # density mixing: "plain" or "thomas_fermi"
     # "davidson": the reference Davidson per k point, `threads` at a time;
     # "block": the block iterations of all k points stepped together (one
@@ -505,7 +535,10 @@ def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
     nb = b if cplx else 2 * b          # Davidson columns per k point
     level_capacity = 2.0 if cplx else 1.0
     ops = [KPointHamiltonian(system, k, capacity=nb, precision=options.precision) for k, _ in kpoints]
-    if options.autotune and ops[0].fused_shape()[0] > 0:
+    if options.fused_shape is not None:
+        for op in ops:
+            op.set_fused_shape(tuple(options.fused_shape))
+    elif options.autotune and ops[0].fused_shape()[0] > 0:
         # Tuned on the first k point with the ionic potential, then shared by
         # every k point that accepts it (bases differ by a few coefficients).
         ops[0].set_owned_potential(vion.reshape(-1).contiguous())
@@ -515,7 +548,13 @@ def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
                 op.set_fused_shape(shape)
             except L.BaandiError:
                 pass
-    diag_static = [torch.from_numpy(op.get("diagonal_static")).to(dev) for op in ops]
+    # The static diagonals of all k points in one tensor (one add of <V> per map).
+    diag_flat = torch.from_numpy(np.concatenate([op.get("diagonal_static") for op in ops])).to(dev)
+    diag_now = torch.empty_like(diag_flat)
+    diags, at = [], 0
+    for op in ops:
+        diags.append(diag_now[at:at + 2 * op.nk])
+        at += 2 * op.nk
     blocks = KBlocks(ops, 2 * b, options.preconditioner) if options.solver == "block" else None
     lockstep = None
     if cplx and options.lockstep and b <= 24 and all(op.fused_shape()[0] > 0 for op in ops) \
@@ -532,7 +571,6 @@ def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
     orbitals = [start(_complex_start(op.nk, b, options.seed + i).to(dev)).contiguous() for i, op in enumerate(ops)]
     outs = [torch.empty_like(o) for o in orbitals]
     rho_dev = torch.empty(field.size, dtype=torch.float64, device=dev)
-    rho_k = torch.empty_like(rho_dev)
     potential = torch.empty_like(rho_dev)
     fres = torch.empty_like(x)
     torch.cuda.synchronize()
@@ -553,15 +591,14 @@ def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
         while True:
             tf = time.perf_counter()
             vmean = field.potential(x, vion, potential)  # V_ion + V_H + v_xc of the input density
-            for op in ops:
-                op.set_owned_potential(potential)
+            set_potential_all(ops, potential)
             totals["fields_s"] += time.perf_counter() - tf
             td = time.perf_counter()
             opts = ref_davidson.Options(bands=nb, threshold=ethr / 2.0, occupied=nb,
                                         first_step=(iteration == 1 and not redo),
                                         max_iterations=options.davidson_max_iterations,
                                         preconditioner=options.preconditioner)
-            diags = [d + vmean for d in diag_static]
+            torch.add(diag_flat, vmean, out=diag_now)
             if blocks is None:
                 if lockstep is not None:
                     sols = lockstep.solve(orbitals, diags, opts, [(o, None) for o in outs])
@@ -594,10 +631,7 @@ def run_scf_k(system: GammaSystem, kpoints, options: KScfOptions, log=None):
             totals["davidson_s"] += time.perf_counter() - td
             tf = time.perf_counter()
             occ, mu, ts = fermi_dirac(levels, weights, electrons, options.theta, capacity=level_capacity)
-            rho_dev.zero_()
-            for i, op in enumerate(ops):
-                op.band_density_sum(outs[i], weights[i] * occ[i], rho_k)
-                rho_dev += rho_k
+            band_density_all(ops, outs, [weights[i] * occ[i] for i in range(len(ops))], rho_dev)
             rho_out = rho_dev.view(field.dims)
             dr2 = field.residual(rho_out, x, fres)
             totals["fields_s"] += time.perf_counter() - tf

@@ -22,7 +22,7 @@ EXPORTS = [
     "bd_gamma_op_set_potential_dev", "bd_gamma_op_apply", "bd_gamma_op_apply_dev",
     "bd_gamma_op_band_density_dev", "bd_gamma_op_nonlocal_energies_dev", "bd_gamma_op_stream",
     "bd_gamma_op_set_profiling", "bd_gamma_op_stage_times", "bd_gamma_op_plan_info", "bd_davidson_solve_dev",
-    "bd_davidson_solve_many_dev", "bd_k_davidson_create", "bd_k_davidson_destroy", "bd_k_davidson_solve_dev", "bd_kblocks_create", "bd_kblocks_destroy", "bd_kblocks_set_diagonals_dev", "bd_kblocks_start_dev", "bd_kblocks_step", "bd_kblocks_vectors_dev", "bd_block_create", "bd_block_destroy", "bd_block_set_diagonal_dev",
+    "bd_davidson_solve_many_dev", "bd_k_band_density_dev", "bd_k_set_potential_dev", "bd_k_davidson_create", "bd_k_davidson_destroy", "bd_k_davidson_solve_dev", "bd_kblocks_create", "bd_kblocks_destroy", "bd_kblocks_set_diagonals_dev", "bd_kblocks_start_dev", "bd_kblocks_step", "bd_kblocks_vectors_dev", "bd_block_create", "bd_block_destroy", "bd_block_set_diagonal_dev",
     "bd_block_start_dev", "bd_block_step", "bd_block_step_async", "bd_block_finish", "bd_block_vectors_dev",
     "bd_k_op_create", "bd_k_op_info", "bd_k_op_get", "bd_k_batch_create", "bd_k_batch_destroy",
     "bd_k_batch_stream", "bd_k_batch_apply_dev", "bd_gamma_op_autotune", "bd_gamma_op_fused_shape",
@@ -146,6 +146,8 @@ def load():
                                   C.POINTER(DavidsonRecord)],
         "bd_davidson_solve_many_dev": [I32, P, P, P, P, C.POINTER(DavidsonOptions), P, P, P, P, P,
                                        C.POINTER(DavidsonRecord), P, I32],
+        "bd_k_band_density_dev": [P, I32, P, P, P, P, P, P],
+        "bd_k_set_potential_dev": [P, I32, P, U64, P],
         "bd_k_davidson_create": [P, I32, I32, I32, C.POINTER(P)],
         "bd_k_davidson_solve_dev": [P, P, P, P, P, C.POINTER(DavidsonOptions), P, P, P, P, P,
                                     C.POINTER(DavidsonRecord), P],

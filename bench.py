@@ -148,7 +148,7 @@ def gemm_leg(reps=10):
     return out
 
 
-def kpoint_leg(steps, threads=16, variants=True):
+def kpoint_leg(steps, threads=16, variants=True, fused_shape=None):
     """Al fcc 4-atom cell, 6x6x6 Monkhorst-Pack (108 k points), 25 Ry, 16
     bands, Fermi-Dirac 0.01 Ha: k-point SCF wall time with the complex
     Davidsons of all k points in lock step (beside it: the realified Davidson
@@ -162,7 +162,7 @@ def kpoint_leg(steps, threads=16, variants=True):
     system = workloads.build(wl)
     kp = K.monkhorst_pack(wl["mesh"])
     opts = K.KScfOptions(bands=wl["bands"], theta=wl["theta"], threshold_ry=1e-8, threads=threads,
-                         preconditioning="thomas_fermi", autotune=True)
+                         preconditioning="thomas_fermi", autotune=fused_shape is None, fused_shape=fused_shape)
     walls, r = [], None
     for k in range(steps):
         r = K.run_scf_k(system, kp, opts)
@@ -174,9 +174,10 @@ def kpoint_leg(steps, threads=16, variants=True):
     # (one fused H·W launch per step over every column of every k point, the
     # step replayed from a captured HIP graph; realified columns).
     from dataclasses import replace
+    split = {key: round(r["totals"][key], 4) for key in ("davidson_s", "fields_s")}
     if not variants:
         return {"wall_s": round(sum(walls) / len(walls), 4), "scf_iterations": r["iterations"],
-                "h_columns": r["totals"]["h_columns"], "energy_ha": r["energy"]["total"]}
+                "h_columns": r["totals"]["h_columns"], "energy_ha": r["energy"]["total"], "split_s": split}
     rr = K.run_scf_k(system, kp, replace(opts, columns="realified"))
     print(f"[bench] al4 k-SCF realified Davidson: {rr['wall_s']:.3f}s maps={rr['iterations']} "
           f"E={rr['energy']['total']:.12f}", file=sys.stderr, flush=True)
@@ -215,7 +216,7 @@ def kpoint_leg(steps, threads=16, variants=True):
         "workload": "Al fcc 4-atom cubic a=7.65 Bohr, 6x6x6 MP (108 k), 25 Ry, 16 bands, FD 0.01 Ha, "
                     "Thomas-Fermi mixing, SCF to dr2<1e-8 Ry",
         "k_points": len(kp), "nk_range": [min(o.nk for o in ops), max(o.nk for o in ops)],
-        "wall_s": round(sum(walls) / len(walls), 4), "scf_iterations": r["iterations"],
+        "wall_s": round(sum(walls) / len(walls), 4), "scf_iterations": r["iterations"], "split_s": split,
         "davidson": "complex Hermitian, 16 complex columns per k point, all 108 k points in lock step",
         "realified_davidson": realified,
         "block_solver": block,
@@ -406,6 +407,9 @@ def main():
                     help="only the Si512 SCF legs (FP64 and mixed; --mixed-steps 0: FP64 alone), JSON printed")
     ap.add_argument("--si512-iterations", type=int, default=2,
                     help="FP64 Davidson iterations of the Si512 leg (0 skips the leg)")
+    ap.add_argument("--fused-shape", default=None,
+                    help="THREADS,XLINES,ZROWS,XPLANES: fix the k-point fused kernel's shape instead of autotuning "
+                         "it (profile runs of --kpoint-only)")
     ap.add_argument("--plane-shape", default=None,
                     help="THREADS,ROWS: fix the plane kernel's shape instead of autotuning it (profile runs: "
                          "the kernel's launches are then the SCF's only)")
@@ -421,7 +425,8 @@ def main():
     from baandi_amd import workloads
     from baandi_amd.scf import run_scf, ScfOptions
     if args.kpoint_only:
-        print(json.dumps(kpoint_leg(max(args.kpoint_steps, 1), variants=args.mixed_steps > 0)))
+        fshape = tuple(int(v) for v in args.fused_shape.split(",")) if args.fused_shape else None
+        print(json.dumps(kpoint_leg(max(args.kpoint_steps, 1), variants=args.mixed_steps > 0, fused_shape=fshape)))
         return
     if args.si128_only:
         print(json.dumps(si128_leg(args.mlp, variants=args.mixed_steps > 0)))

@@ -132,6 +132,20 @@ bd_status bd_gamma_op_apply_dev(bd_gamma_op* op, const double* x_dev, int64_t ld
  * grid (band_density_sum, distributed_real_gamma.rs:741-769). */
 bd_status bd_gamma_op_band_density_dev(bd_gamma_op* op, const double* x_dev, int64_t ldx,
                                        int32_t m, const double* weights, double* rho_dev);
+/* The k points of a map (the k map's density sum, crates/dft-engine/src/refscf/
+ * k_map.rs, one call instead of one per k point): rho = sum over k of the
+ * band density of k-point op k (m[k] complex columns at x_dev[k], host
+ * weights[k]). The k points run side by side on their ops' streams, each
+ * into its own grid, summed in k order at the end: the same bits as the
+ * per-k calls added in k order. k-point ops of one grid. `stream` (a HIP
+ * stream, or NULL): the work is ordered after what it holds. Synchronises. */
+bd_status bd_k_band_density_dev(bd_gamma_op* const* ops, int32_t count, const double* const* x_dev,
+                                const int64_t* ldx, const int32_t* m, const double* const* weights,
+                                double* rho_dev, void* stream);
+/* Set every op's potential from one grid (bd_gamma_op_set_potential_dev per
+ * op), ordered after `stream` (or NULL), which then waits for every copy. */
+bd_status bd_k_set_potential_dev(bd_gamma_op* const* ops, int32_t count, const double* v_grid_dev,
+                                 uint64_t epoch, void* stream);
 /* Warm start of an SCF map (KeptHx: crates/dft-engine/src/refscf/kept_hx.rs,
  * apply_local_change of distributed_real_gamma.rs:1370-1438). With
  * keep_fields on, every band density also keeps its columns' real-space pair
