@@ -1350,6 +1350,13 @@ __device__ void invit_vector(int m, int stride, int lane, int j, const double* d
   }
   if (fabs(a) < tiny) a = a >= 0.0 ? tiny : -tiny;
   g.U0i[at(m - 1)] = 1.0 / a;
+  // The last row has no superdiagonals: the back substitution multiplies
+  // these by zero, so they must not be left as whatever the memory held (an
+  // Inf or NaN there would poison the vector).
+  g.U1[at(m - 1)] = 0.0;
+  g.U2[at(m - 1)] = 0.0;
+  g.Lm[at(m - 1)] = 0.0;
+  g.piv[at(m - 1)] = 0;
   for (int i = 0; i < m; ++i) z[at(i)] = hash_unit(i, j, 0);
   // Two solves (the second from the unscaled first: growth per solve is at
   // most ~1/(eps ||T||), far from overflow), then unit norm.
