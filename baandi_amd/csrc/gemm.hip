@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <string>
 
 #include "common.hpp"
 #include "gemm.hpp"
@@ -311,6 +312,47 @@ extern "C" bd_status bd_gemm_dev(int32_t mode, int32_t trans_a, int64_t M, int64
     else g.nn(nullptr, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
     if (hipDeviceSynchronize() != hipSuccess || hipGetLastError() != hipSuccess)
       bd::fail(BD_ERR_DEVICE, "gemm launch failed");
+  });
+}
+
+// The same product `reps` times back to back on one stream between two HIP
+// events (no host round trip in between): the average milliseconds per
+// product in *ms. For measuring the kernels apart from the call overhead.
+#define GEMM_HIP(call)                                                               \
+  do {                                                                               \
+    const hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess) bd::fail(BD_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+extern "C" bd_status bd_gemm_time_dev(int32_t mode, int32_t trans_a, int64_t M, int64_t N, int64_t K,
+                                      const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
+                                      int64_t ldc, double* work, int32_t reps, double* ms) {
+  BD_GUARD({
+    if (M < 0 || N < 0 || K < 0 || (mode != 0 && mode != 1) || reps <= 0 || !ms)
+      bd::fail(BD_ERR_INVALID_PARAMETER, "gemm timing shape/mode");
+    if (trans_a && !work && M * N > 0) bd::fail(BD_ERR_INVALID_PARAMETER, "gemm workspace");
+    bd::Gemm g;
+    g.bf16x3 = mode == 1;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    GEMM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    GEMM_HIP(hipEventCreate(&e0));
+    GEMM_HIP(hipEventCreate(&e1));
+    auto run = [&] {
+      if (trans_a) g.tn(s, M, N, K, 1.0, A, lda, B, ldb, 0.0, C, ldc, work);
+      else g.nn(s, M, N, K, 1.0, A, lda, B, ldb, 0.0, C, ldc);
+    };
+    run();
+    GEMM_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) run();
+    GEMM_HIP(hipEventRecord(e1, s));
+    GEMM_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    GEMM_HIP(hipEventElapsedTime(&t, e0, e1));
+    *ms = double(t) / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
   });
 }
 

@@ -30,7 +30,7 @@ EXPORTS = [
     "bd_sym_eigen_multi_from", "bd_herm_eigen_dev", "bd_sym_eigen_coop", "bd_field_create", "bd_field_destroy", "bd_field_stream", "bd_field_to_grid_dev",
     "bd_field_from_grid_dev", "bd_field_potential_dev", "bd_field_residual_dev", "bd_field_gram_dev",
     "bd_field_mix_dev", "bd_field_energies_dev",
-    "bd_gemm_workspace", "bd_gemm_dev", "bd_mfma_probe_dev", "bd_mfma_f64_probe_dev",
+    "bd_gemm_workspace", "bd_gemm_dev", "bd_gemm_time_dev", "bd_mfma_probe_dev", "bd_mfma_f64_probe_dev",
     "bd_sym_eigen_workspace", "bd_sym_eigen_dev", "bd_inverse_cholesky_dev", "bd_inverse_cholesky_static_dev",
     "bd_mlp_param_count", "bd_mlp_blob_floats", "bd_mlp_quantize_dev", "bd_mlp_forward_dev",
     "bd_mlp_train_workspace", "bd_mlp_loss_grad_dev", "bd_mlp_adam_dev", "bd_mlp_train_dev",
@@ -164,6 +164,7 @@ def load():
         "bd_block_finish": [P, P],
         "bd_block_vectors_dev": [P, P, P, I64],
         "bd_gemm_dev": [I32, I32, I64, I64, I64, D, P, I64, P, I64, D, P, I64, P],
+        "bd_gemm_time_dev": [I32, I32, I64, I64, I64, P, I64, P, I64, P, I64, P, I32, P],
         "bd_mfma_probe_dev": [P, P, P],
         "bd_mfma_f64_probe_dev": [P, P, P],
         "bd_sym_eigen_dev": [I32, I32, P, I64, P, P, I64, P],

@@ -111,8 +111,10 @@ def hx_profile(system, wl, count):
 def gemm_leg(reps=10):
     """The Si64 subspace products on the matrix cores: the border Gram
     V^T E (320 x 160 x 23,847, split-K) and the rotation V C (23,847 x 160 x
-    320), FP64 MFMA and BF16x3 MFMA. Rates: 2MNK / t ("effective") and, for
-    BF16x3, the 3 bf16 products per term the MFMA units execute."""
+    320), FP64 MFMA and BF16x3 MFMA, each timed over `reps` back-to-back
+    launches between HIP events (bd_gemm_time_dev). Rates: 2MNK / t
+    ("effective") and, for BF16x3, the 3 bf16 products per term the MFMA
+    units execute."""
     import numpy as np
     import torch
     from baandi_amd import lib as L
@@ -127,16 +129,10 @@ def gemm_leg(reps=10):
         for mode, label in ((0, "fp64"), (1, "bf16x3")):
             work = torch.empty(max(lib.bd_gemm_workspace(mode, trans, M, N, K), 1), dtype=torch.float64,
                                device="cuda")
-            call = lambda: L.check(lib.bd_gemm_dev(mode, trans, M, N, K, 1.0, L.ptr(A), lda, L.ptr(B), K, 0.0,
-                                                   L.ptr(C), M, L.ptr(work)))
-            call()
-            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a.record()
-            for _ in range(reps):
-                call()
-            b.record()
-            b.synchronize()
-            ms = a.elapsed_time(b) / reps
+            t = np.zeros(1)
+            L.check(lib.bd_gemm_time_dev(mode, trans, M, N, K, L.ptr(A), lda, L.ptr(B), K, L.ptr(C), M,
+                                         L.ptr(work), reps, L.ptr(t)))
+            ms = float(t[0])
             tf = 2.0 * M * N * K / (ms * 1e-3) / 1e12
             rec = {"shape": [M, N, K], "ms": round(ms, 4), "effective_TFLOPs": round(tf, 2)}
             if mode == 0:

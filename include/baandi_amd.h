@@ -417,6 +417,12 @@ int64_t bd_gemm_workspace(int32_t mode, int32_t trans_a, int64_t M, int64_t N, i
 bd_status bd_gemm_dev(int32_t mode, int32_t trans_a, int64_t M, int64_t N, int64_t K, double alpha,
                       const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
                       int64_t ldc, double* work);
+/* The same product reps times back to back on one stream between two HIP
+ * events; *ms = the average per product (kernel time without the call's
+ * host round trip). No reference counterpart (measurement). */
+bd_status bd_gemm_time_dev(int32_t mode, int32_t trans_a, int64_t m, int64_t n, int64_t k, const double* a,
+                           int64_t lda, const double* b, int64_t ldb, double* c, int64_t ldc, double* work,
+                           int32_t reps, double* ms);
 /* One v_mfma_f32_16x16x16_bf16 of row-major 16x16 bf16 bit patterns (a self
  * check of the MFMA operand layout the kernels assume). */
 /* Lane-layout probe of v_mfma_f64_16x16x4f64 (the FP64 GEMMs' product):

@@ -99,5 +99,14 @@ def test_gemm_workspace_covers_smaller_shapes(mode):
                 for n in list(range(1, nmax + 1, 5)) + [nmax]:
                     w = lib.bd_gemm_workspace(mode, 1, m, n, K)
                     assert m * n <= w <= top, (mode, K, mmax, nmax, m, n, w, top)
+                    if mode == 1:
+                        # The BF16x3 TN kernel: 128 x 128 tiles, about 512
+                        # workgroups, chunks of >= 256 rows (multiples of 32),
+                        # FP32 partials two to a double.
+                        tiles = -(-m // 128) * -(-n // 128)
+                        splits = max(1, min(-(-512 // tiles), max(1, K // 256), 256))
+                        chunk = -(-(-(-K // splits)) // 32) * 32
+                        used = -(-K // chunk)
+                        assert 2 * top >= used * m * n
                     assert w >= prev
                     prev = w
