@@ -147,18 +147,27 @@ __global__ __launch_bounds__(256) void k_gemm_nn(int64_t M, int64_t N, int64_t K
 }
 
 // TN partial: part[s](MxN) = sum over k in chunk s of A[k,i] B[k,j].
+// Workgroups are dealt to the 8 XCDs round robin (id % 8); the logical
+// (tile, chunk) index puts the tiles of one K chunk on one XCD, one after the
+// other, where they share its L2 for the chunk's rows of A and B.
+constexpr int kXcds = 8;
+
 template <int BN>
-__global__ __launch_bounds__(256) void k_gemm_tn_part(int64_t M, int64_t N, int64_t K, int64_t chunk,
-                                                      const double* __restrict__ A, int64_t lda,
-                                                      const double* __restrict__ B, int64_t ldb,
+__global__ __launch_bounds__(256) void k_gemm_tn_part(int64_t M, int64_t N, int64_t K, int64_t chunk, int tiles_m,
+                                                      int tiles, int total, const double* __restrict__ A,
+                                                      int64_t lda, const double* __restrict__ B, int64_t ldb,
                                                       double* __restrict__ part) {
   constexpr int NB = BN / 32;
   __shared__ double As[TK][TM + 1];
   __shared__ double Bs[TK][BN + 1];
+  const int per_xcd = (total + kXcds - 1) / kXcds;
+  const int logical = (int(blockIdx.x) % kXcds) * per_xcd + int(blockIdx.x) / kXcds;
+  if (logical >= total) return;
+  const int tile = logical % tiles, kc = logical / tiles;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6, wr = wave & 1, wc = wave >> 1;
-  const int64_t row0 = int64_t(blockIdx.x) * TM, col0 = int64_t(blockIdx.y) * BN;
-  const int64_t kbeg = int64_t(blockIdx.z) * chunk, kend = std::min(K, kbeg + chunk);
+  const int64_t row0 = int64_t(tile % tiles_m) * TM, col0 = int64_t(tile / tiles_m) * BN;
+  const int64_t kbeg = int64_t(kc) * chunk, kend = std::min(K, kbeg + chunk);
   f64x4 acc[2][NB];
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi)
@@ -197,7 +206,7 @@ __global__ __launch_bounds__(256) void k_gemm_tn_part(int64_t M, int64_t N, int6
       __syncthreads();
     }
   }
-  double* out = part + int64_t(blockIdx.z) * M * N;
+  double* out = part + int64_t(kc) * M * N;
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
@@ -229,8 +238,17 @@ __global__ void k_gemm_tn_reduce(int64_t M, int64_t N, int splits, double alpha,
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= M * N) return;
   const int64_t i = e % M, j = e / M;
+  // In chunk order; eight loads in flight at a time.
   double s = 0.0;
-  for (int p = 0; p < splits; ++p) s += part[int64_t(p) * M * N + e];
+  int p = 0;
+  for (; p + 8 <= splits; p += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = part[int64_t(p + u) * M * N + e];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += v[u];
+  }
+  for (; p < splits; ++p) s += part[int64_t(p) * M * N + e];
   double* c = C + i + j * ldc;
   *c = beta == 0.0 ? alpha * s : alpha * s + beta * *c;
 }
@@ -280,11 +298,15 @@ void gemm_tn(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const
   chunk = ((chunk + TK - 1) / TK) * TK;
   const int used = int((K + chunk - 1) / std::max<int64_t>(chunk, 1));
   const int bn = col_tile(N);
-  dim3 grid(unsigned((M + TM - 1) / TM), unsigned((N + bn - 1) / bn), unsigned(std::max(used, 1)));
+  const int tiles_m = int((M + TM - 1) / TM), tiles = tiles_m * int((N + bn - 1) / bn);
+  const int wgs = tiles * std::max(used, 1);
+  const dim3 grid(unsigned((wgs + kXcds - 1) / kXcds * kXcds));
   if (bn == 128)
-    hipLaunchKernelGGL(k_gemm_tn_part<128>, grid, dim3(256), 0, s, M, N, K, chunk, A, lda, B, ldb, work);
+    hipLaunchKernelGGL(k_gemm_tn_part<128>, grid, dim3(256), 0, s, M, N, K, chunk, tiles_m, tiles, wgs, A, lda, B,
+                       ldb, work);
   else
-    hipLaunchKernelGGL(k_gemm_tn_part<64>, grid, dim3(256), 0, s, M, N, K, chunk, A, lda, B, ldb, work);
+    hipLaunchKernelGGL(k_gemm_tn_part<64>, grid, dim3(256), 0, s, M, N, K, chunk, tiles_m, tiles, wgs, A, lda, B,
+                       ldb, work);
   const int64_t total = M * N;
   hipLaunchKernelGGL(k_gemm_tn_reduce, dim3(unsigned((total + 255) / 256)), dim3(256), 0, s, M, N,
                      std::max(used, 1), alpha, work, beta, C, ldc);
