@@ -331,9 +331,9 @@ bd_status bd_davidson_solve_many_dev(int32_t count, bd_gamma_op* const* ops, con
  * k-point kernel, and the subspace step of a k point (bordered complex
  * Cholesky, L^-1 H L^-H, a complex Jacobi eigensolve, C = L^-H Y) one
  * workgroup with its matrices in LDS. Two status read-backs per iteration
- * cover all k points. A k point whose corrections need the rank repair
- * continues in bd_davidson_solve_dev (complex columns) from its Ritz block
- * (counted in record.repairs). The ops: k-point ops on the fused path of one
+ * cover all k points. The rank repair of a k point whose border the factor
+ * rejects runs inside the lock step (counted in record.repairs). The ops:
+ * k-point ops on the fused path of one
  * system and precision, each at most once; bands <= 24, capacity (0: 2 bands)
  * <= 48 complex columns and <= n_k. */
 typedef struct bd_k_davidson bd_k_davidson;
