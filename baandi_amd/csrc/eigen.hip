@@ -1579,6 +1579,94 @@ __global__ __launch_bounds__(256) void k_back_transform(int m, int want, const d
     for (int i = lane; i < m; i += 64) yg[i] = y[i];
 }
 
+// The same for m <= 64 YP with the vector in registers (YP rows per lane):
+// per reflector the chain is the lane's dot, one wave sum and the update,
+// with the chunk's reflector rows read from LDS ahead of it. The next chunk
+// of 8 reflectors is read from memory into registers while the current one
+// is applied (two LDS buffers).
+constexpr int kBtRegCh = 8;
+size_t bt_reg_lds(int m) { return size_t(2 * kBtRegCh * m + 2 * kBtRegCh) * 8; }
+
+template <int YP>
+__global__ __launch_bounds__(256) void k_back_transform_reg(int m, int want, const double* __restrict__ A,
+                                                            int64_t lda, const double* __restrict__ tau,
+                                                            double* __restrict__ Y, int64_t ldy) {
+  constexpr int CH = kBtRegCh;
+  constexpr int NPT = (CH * 64 * YP + 255) / 256;  // staged values per thread (m <= 64 YP)
+  extern __shared__ double sh[];
+  double* vb[2] = {sh, sh + CH * m};
+  double* tb[2] = {sh + 2 * CH * m, sh + 2 * CH * m + CH};
+  const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
+  const int col = blockIdx.x * (blockDim.x >> 6) + wl;
+  const bool active = col < want;
+  double* yg = Y + int64_t(active ? col : 0) * ldy;
+  double y[YP];
+#pragma unroll
+  for (int r = 0; r < YP; ++r) {
+    const int i = lane + 64 * r;
+    y[r] = (active && i < m) ? yg[i] : 0.0;
+  }
+  // Chunk starting at reflector k1 (k1, k1 - 1, ...): values into registers.
+  double nv[NPT];
+  double nt = 0.0;
+  auto fetch = [&](int k1) {
+    const int cnt = k1 + 1 < CH ? k1 + 1 : CH;
+#pragma unroll
+    for (int u = 0; u < NPT; ++u) {
+      const int e = u * 256 + int(threadIdx.x);
+      const int c = e / m, i = e - c * m, k = k1 - c;
+      nv[u] = (c < cnt && i > k) ? A[i + int64_t(k) * lda] : 0.0;
+    }
+    nt = int(threadIdx.x) < cnt ? tau[k1 - int(threadIdx.x)] : 0.0;
+  };
+  auto store = [&](int b) {
+#pragma unroll
+    for (int u = 0; u < NPT; ++u) {
+      const int e = u * 256 + int(threadIdx.x);
+      if (e < CH * m) vb[b][e] = nv[u];
+    }
+    if (int(threadIdx.x) < CH) tb[b][threadIdx.x] = nt;
+  };
+  int k1 = m - 3;
+  if (k1 >= 0) {
+    fetch(k1);
+    store(0);
+  }
+  __syncthreads();
+  for (int b = 0; k1 >= 0; k1 -= CH, b ^= 1) {
+    if (k1 - CH >= 0) fetch(k1 - CH);
+    if (active) {
+      const double* vs = vb[b];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const double t = tb[b][c];  // 0 for reflectors past the chunk's end
+        double v[YP];
+#pragma unroll
+        for (int r = 0; r < YP; ++r) {
+          const int i = lane + 64 * r;
+          v[r] = i < m ? vs[c * m + i] : 0.0;
+        }
+        double acc = 0.0;
+#pragma unroll
+        for (int r = 0; r < YP; ++r) acc = fma(v[r], y[r], acc);
+        const double sc = t * wave_sum(acc);
+#pragma unroll
+        for (int r = 0; r < YP; ++r) y[r] = fma(-sc, v[r], y[r]);
+      }
+    }
+    __syncthreads();  // every wave is done with buffer b ^ 1 (read a chunk ago)
+    if (k1 - CH >= 0) store(b ^ 1);
+    __syncthreads();
+  }
+  if (active) {
+#pragma unroll
+    for (int r = 0; r < YP; ++r) {
+      const int i = lane + 64 * r;
+      if (i < m) yg[i] = y[r];
+    }
+  }
+}
+
 __global__ void k_copy_cols(int m, const double* __restrict__ src, int64_t lds, double* __restrict__ dst,
                             int64_t ldd) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
@@ -1750,7 +1838,15 @@ void sym_eigen_lowest(hipStream_t s, int m, int want, double* A, int64_t lda, do
     tridiag_unblocked(s, m, A, lda, L);
   }
   tridiag_vectors(s, m, want, L, values, Y, ldy, info);
-  if (bt_ch == 8) {
+  if (m <= 320) {
+    hipLaunchKernelGGL(k_back_transform_reg<5>, dim3((want + 3) / 4), dim3(256), bt_reg_lds(m), s, m, want, A, lda,
+                       L.tau, L.Ytmp, int64_t(m));
+  } else if (m <= 640) {
+    if (bt_reg_lds(m) > 64 * 1024)
+      allow_full_lds(reinterpret_cast<const void*>(&k_back_transform_reg<10>), "k_back_transform_reg");
+    hipLaunchKernelGGL(k_back_transform_reg<10>, dim3((want + 3) / 4), dim3(256), bt_reg_lds(m), s, m, want, A, lda,
+                       L.tau, L.Ytmp, int64_t(m));
+  } else if (bt_ch == 8) {
     if (bt_lds(8) > 64 * 1024) allow_full_lds(reinterpret_cast<const void*>(&k_back_transform<8>), "k_back_transform");
     hipLaunchKernelGGL(k_back_transform<8>, dim3((want + 3) / 4), dim3(256), bt_lds(8), s, m, want, A, lda, L.tau,
                        L.Ytmp, int64_t(m));
