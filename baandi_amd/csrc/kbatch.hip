@@ -206,7 +206,7 @@ void launch_fused_t(const GammaOp& op, const FusedShape& f, size_t lds, const KW
 }  // namespace
 
 size_t GammaOp::fused_lds(const FusedShape& f) const {
-  const size_t e = precision == BD_MIXED ? sizeof(c2<float>) : sizeof(c2<double>);
+  const size_t e = fft32() ? sizeof(c2<float>) : sizeof(c2<double>);
   const StickPlan& p = plan;
   return e * size_t(p.n0 + p.n1 + p.n2 + p.S * p.n0 + size_t(f.xplanes) * (p.Z * p.n1 + f.zrows * p.n2));
 }
@@ -235,7 +235,7 @@ KWork GammaOp::kwork(const double* x, int64_t ldx, int m, double* y, int64_t ldy
   w.yz = d_stick_yz.p;
   w.z_of_zc = d_zc.p;
   w.kinetic = d_kinetic.p;
-  w.potential = precision == BD_MIXED ? static_cast<const void*>(d_potential32.p)
+  w.potential = fft32() ? static_cast<const void*>(d_potential32.p)
                                       : static_cast<const void*>(d_potential.p);
   w.proj2 = d_proj.p;
   w.blockd = d_blockd.p;
@@ -248,7 +248,7 @@ void fused_launch(const GammaOp& op, const FusedShape& f, size_t lds, const KWor
                   int total, hipStream_t s) {
   if (total <= 0) return;
   if (lds > kFusedDynMax) fail(BD_ERR_INVALID_PARAMETER, "fused kernel LDS exceeds the workgroup limit");
-  if (op.precision == BD_MIXED) {
+  if (op.fft32()) {
     if (f.threads == 512) launch_fused_t<float, 512>(op, f, lds, works, one, count, total, s);
     else launch_fused_t<float, 256>(op, f, lds, works, one, count, total, s);
   } else {

@@ -664,11 +664,11 @@ StickPlan build_k_stick_plan(const System& s, const KBasis& kb) {
 // Operator
 // ---------------------------------------------------------------------------
 size_t GammaOp::plane_lds_bytes(int chunk) const {
-  const size_t e = precision == BD_MIXED ? sizeof(c2<float>) : sizeof(c2<double>);
+  const size_t e = fft32() ? sizeof(c2<float>) : sizeof(c2<double>);
   return size_t(plan.Z * plan.n1 + chunk * plan.n2 + plan.n1 + plan.n2) * e;
 }
 
-size_t GammaOp::elem_bytes() const { return precision == BD_MIXED ? sizeof(c2<float>) : sizeof(c2<double>); }
+size_t GammaOp::elem_bytes() const { return fft32() ? sizeof(c2<float>) : sizeof(c2<double>); }
 
 // A (threads, chunk) shape of the plane pass: the fused k_plane holds the
 // whole Z x n1 plane; the split pass holds zb z-lines (y transforms) or a
@@ -756,7 +756,8 @@ static void allow_lds(const StickPlan& plan) {
 }
 
 GammaOp::GammaOp(const System& s, const KBasis* k, int cap, int prec) : sys(&s), kb(k) {
-  if (prec != BD_FP64 && prec != BD_MIXED) fail(BD_ERR_INVALID_PARAMETER, "unknown precision mode");
+  if (prec != BD_FP64 && prec != BD_MIXED && prec != BD_FP32_HX)
+    fail(BD_ERR_INVALID_PARAMETER, "unknown precision mode");
   if (cap <= 0) fail(BD_ERR_INVALID_PARAMETER, "capacity must be positive");
   int dev = 0, count = 0;
   BD_HIP(hipGetDeviceCount(&count));
@@ -776,9 +777,9 @@ GammaOp::GammaOp(const System& s, const KBasis* k, int cap, int prec) : sys(&s),
   nproj = kb ? 2 * s.nproj : s.nproj;
   plan = kb ? build_k_stick_plan(s, *kb) : build_stick_plan(s);
   size_t lds_max = size_t(prop.sharedMemPerBlock);
-  const size_t elem = precision == BD_MIXED ? sizeof(c2<float>) : sizeof(c2<double>);
+  const size_t elem = fft32() ? sizeof(c2<float>) : sizeof(c2<double>);
   if (stick_lds(plan, elem) > lds_max) fail(BD_ERR_INVALID_PARAMETER, "x FFT tile exceeds the device LDS");
-  if (precision == BD_MIXED) allow_lds<float>(plan);
+  if (fft32()) allow_lds<float>(plan);
   else allow_lds<double>(plan);
   BD_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   // Roots e^{-2 pi i k/n} per axis, in long double.
@@ -801,7 +802,7 @@ GammaOp::GammaOp(const System& s, const KBasis* k, int cap, int prec) : sys(&s),
   d_kinetic.alloc(n);
   d_kinetic.upload(kb ? kb->kinetic2.data() : s.kinetic_orc.data(), n, stream);
   d_potential.alloc(N);
-  if (precision == BD_MIXED) d_potential32.alloc(N);
+  if (fft32()) d_potential32.alloc(N);
   if (nproj > 0) {
     d_proj.alloc(size_t(n) * nproj);
     d_proj.upload(kb ? kb->proj2.data() : s.projectors_orc.data(), size_t(n) * nproj, stream);
@@ -862,7 +863,7 @@ void GammaOp::set_potential_dev(const double* v, uint64_t ep) {
 }
 
 void GammaOp::refresh_potential32() {
-  if (precision != BD_MIXED) return;
+  if (!fft32()) return;
   hipLaunchKernelGGL(k_to_float, dim3(unsigned((N + 255) / 256)), dim3(256), 0, stream, d_potential.p, N,
                      d_potential32.p);
   BD_HIP(hipGetLastError());
@@ -1017,7 +1018,7 @@ void GammaOp::apply_kept(const double* x, int64_t ldx, int m, const double* dv, 
     fail(BD_ERR_INVALID_PARAMETER, "no kept fields of these columns (band density of the same X first)");
   if (ldy < n) fail(BD_ERR_INVALID_PARAMETER, "leading dimension smaller than rows");
   if (m == 0) return;
-  if (precision == BD_MIXED) kept_apply<float>(x, ldx, m, dv, y, ldy);
+  if (fft32()) kept_apply<float>(x, ldx, m, dv, y, ldy);
   else kept_apply<double>(x, ldx, m, dv, y, ldy);
   BD_HIP(hipGetLastError());
 }
@@ -1051,7 +1052,7 @@ void GammaOp::apply(const double* x, int64_t ldx, int m, double* y, int64_t ldy)
     BD_HIP(hipGetLastError());
     return;
   }
-  if (precision == BD_MIXED) local_apply<float>(x, ldx, m, y, ldy);
+  if (fft32()) local_apply<float>(x, ldx, m, y, ldy);
   else local_apply<double>(x, ldx, m, y, ldy);
   if (nproj > 0) {
     projections(x, ldx, m);
@@ -1083,7 +1084,7 @@ void GammaOp::band_density(const double* x, int64_t ldx, int m, const double* w,
   if (m <= 0 || m > capacity) fail(BD_ERR_INVALID_PARAMETER, "column count");
   if (ldx < n) fail(BD_ERR_INVALID_PARAMETER, "leading dimension smaller than rows");
   if (kb && !complex_aligned(x, ldx)) fail(BD_ERR_INVALID_PARAMETER, "k-point X must be 16-byte aligned, even ldx");
-  if (precision == BD_MIXED) density<float>(x, ldx, m, w, rho, accumulate);
+  if (fft32()) density<float>(x, ldx, m, w, rho, accumulate);
   else density<double>(x, ldx, m, w, rho, accumulate);
   BD_HIP(hipGetLastError());
 }

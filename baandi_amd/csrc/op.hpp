@@ -99,6 +99,9 @@ struct GammaOp {
   int dims[3];
   int capacity;
   int precision;
+  // The FFT stages (sticks, planes, the potential) run in FP32 in every mode
+  // but FP64; the GEMMs are BF16x3 only in BD_MIXED.
+  bool fft32() const { return precision != BD_FP64; }
   int64_t nproj;
   hipStream_t stream = nullptr;
   StickPlan plan;

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libbaandi_amd.so")
 
 BD_OK, BD_ERR_LINALG, BD_ERR_INVALID_PARAMETER, BD_ERR_APPLICATION, BD_NOT_CONVERGED, BD_ERR_DEVICE = range(6)
-BD_FP64, BD_MIXED = 0, 1
+BD_FP64, BD_MIXED, BD_FP32_HX = 0, 1, 2
 BD_PRECOND_QE, BD_PRECOND_TPA, BD_PRECOND_MLP = 0, 1, 2
 
 # Every function include/baandi_amd.h declares.

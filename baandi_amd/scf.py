@@ -45,9 +45,11 @@ class ScfOptions:
     # BD_FP64: every map in FP64. BD_MIXED: maps whose Davidson threshold is at
     # least mixed_floor_ha run on the mixed-tier operator (FP32 FFT H·X, BF16x3
     # GEMMs, FP64 factor/eigensolve); the later maps, and the final answer, run
-    # in FP64, so the fixed point is the FP64 one.
+    # in FP64, so the fixed point is the FP64 one. BD_FP32_HX: the same with
+    # FP64 GEMMs (FP32 FFT H·X only). The floor defaults to the tier's noise:
+    # 5e-5 Ha for the BF16x3 Gram, 1e-6 Ha for FP32 H·X (~3e-7 Ha Ritz error).
     precision: int = L.BD_FP64
-    mixed_floor_ha: float = 5e-5
+    mixed_floor_ha: float = None
     mlp: object = None  # a baandi_amd.learned.Network when preconditioner == BD_PRECOND_MLP
     allow_unconverged: bool = False  # return the last map instead of raising at max_iterations
     preconditioning: str = "plain"   # density mixing: "plain" or "thomas_fermi" (refscf/profile.rs:4-7)
@@ -109,8 +111,13 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
     vion_g = torch.from_numpy(system.get("ionic_density_re") + 1j * system.get("ionic_density_im")).to(dev)
     vion = field.to_grid(vion_g).real.contiguous()
     op = DistributedRealHamiltonian(system, capacity=bands, precision=L.BD_FP64)
-    op_mixed = (DistributedRealHamiltonian(system, capacity=bands, precision=L.BD_MIXED)
-                if options.precision == L.BD_MIXED else None)
+    if options.precision not in (L.BD_FP64, L.BD_MIXED, L.BD_FP32_HX):
+        raise L.BaandiError(L.BD_ERR_INVALID_PARAMETER, "unknown precision mode")
+    op_mixed = (DistributedRealHamiltonian(system, capacity=bands, precision=options.precision)
+                if options.precision != L.BD_FP64 else None)
+    mixed_floor = options.mixed_floor_ha
+    if mixed_floor is None:
+        mixed_floor = 5e-5 if options.precision == L.BD_MIXED else 1e-6
     if options.plane_shape is not None:
         for o in (op, op_mixed):
             if o is not None:
@@ -193,7 +200,7 @@ def run_scf(system: GammaSystem, options: ScfOptions, log=None, start=None, prof
         while True:
             tf = time.perf_counter()
             vmean = field.potential(x, vion, pot)  # V_ion + V_H + v_xc of the input density
-            use_mixed = op_mixed is not None and ethr / 2.0 >= options.mixed_floor_ha
+            use_mixed = op_mixed is not None and ethr / 2.0 >= mixed_floor
             cur = op_mixed if use_mixed else op
             initial_hx = None
             if warm is cur:

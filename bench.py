@@ -480,7 +480,13 @@ def main():
                                       mlp=net, preconditioning=mix, autotune=True),
                 "mixed_learned": ScfOptions(bands=wl["bands"], threshold_ry=1e-8, precision=L.BD_MIXED,
                                             preconditioner=L.BD_PRECOND_MLP, mlp=net, preconditioning=mix,
-                                            autotune=True)}
+                                            autotune=True),
+                # FP32 H·X with FP64 GEMMs: no BF16x3 Gram noise, so more maps run reduced
+                "fp32_hx": ScfOptions(bands=wl["bands"], threshold_ry=1e-8, precision=L.BD_FP32_HX,
+                                      preconditioning=mix, autotune=True),
+                "fp32_hx_learned": ScfOptions(bands=wl["bands"], threshold_ry=1e-8, precision=L.BD_FP32_HX,
+                                              preconditioner=L.BD_PRECOND_MLP, mlp=net, preconditioning=mix,
+                                              autotune=True)}
         mixed = {}
         for name, mopts in legs.items():
             mwalls, mres = [], None
@@ -524,7 +530,9 @@ def main():
                    "mixed_s": None if mixed is None else mixed["mixed"]["wall_s"],
                    "mixed_learned_s": None if mixed is None else mixed["mixed_learned"]["wall_s"],
                    "speedup_mixed_learned_vs_fp64": None if mixed is None else
-                   round(ms_per_step / 1e3 / mixed["mixed_learned"]["wall_s"], 3)},
+                   round(ms_per_step / 1e3 / mixed["mixed_learned"]["wall_s"], 3),
+                   "fp32_hx_s": None if mixed is None else mixed["fp32_hx"]["wall_s"],
+                   "fp32_hx_learned_s": None if mixed is None else mixed["fp32_hx_learned"]["wall_s"]},
         "variants": mixed,
         "energy_ha": last["energy"]["total"],
         "breakdown_s": {k: round(v, 4) for k, v in last["totals"].items() if k.endswith("_s")},

@@ -31,8 +31,12 @@ typedef enum {
 } bd_status;
 
 /* Arithmetic of the hot path. FP64: every stage in double (the reference
- * path). MIXED is reserved for the FP32-FFT / BF16x3-GEMM tier. */
-typedef enum { BD_FP64 = 0, BD_MIXED = 1 } bd_precision;
+ * path). MIXED: the FFT stages of H.X (sticks, planes, V(r)) in FP32 and the
+ * subspace GEMMs as BF16x3 split products. FP32_HX: the FFT stages in FP32,
+ * the GEMMs in FP64 (no BF16x3 Gram noise, about 3e-7 Ha of Ritz error from
+ * H.X). Factorisation, eigensolve and convergence control are FP64 in every
+ * mode. */
+typedef enum { BD_FP64 = 0, BD_MIXED = 1, BD_FP32_HX = 2 } bd_precision;
 
 /* Last error message of the calling thread ("" when none). */
 const char* bd_last_error(void);
