@@ -775,29 +775,38 @@ __global__ __launch_bounds__(256) void k_td_update(int m, double* __restrict__ A
 // 1d. Tridiagonalisation by cooperating workgroups ("workers"), for subspaces
 // whose matrix fits the workers' LDS together (m up to ~720). Column c lives
 // in the LDS of worker c mod nw, so the products and updates of every step
-// read LDS instead of streaming the trailing block through one CU. Step k:
-//  B. every worker reads reflector k (v and tau) and multiplies its columns
-//     c > k by v: its part of y = A22 v, which it publishes;
-//  C. every worker reads all of y, forms w = tau y - (tau^2/2 y.v) v (the
-//     same code on the same data: the same bits everywhere) and updates its
-//     columns by -(v w^T + w v^T), each product rounded on its own so the
-//     matrix stays exactly symmetric. The owner of column k + 1 updates that
-//     column first, forms reflector k + 1 from it (d, e, tau) and publishes
-//     it before its other columns, so the next step starts at once.
+// read LDS instead of streaming the trailing block through one CU. Every
+// worker also keeps copies of columns k + 1 and k + 2 and forms each
+// reflector itself, so a step needs one exchange between workers, not two
+// (the exchange, a trip through memory, is what bounds a step). Step k, with
+// v_k, tau_k and the copy of column k + 1 (as of step k - 1) in every worker:
+//  A. each worker multiplies its columns c > k by v_k, its part of
+//     y = A22 v_k, and publishes it; the owner of column k + 2 also
+//     publishes that column (as of step k - 1);
+//  B. every worker reads all of y and column k + 2, forms
+//     w = tau y - (tau^2/2 y.v) v and applies -(v w^T + w v^T) to its
+//     columns and to its copies of columns k + 1 and k + 2, each product
+//     rounded on its own: the copies get the owners' bits, and the matrix
+//     stays exactly symmetric;
+//  C. every worker forms reflector k + 1 from its copy of column k + 1 (the
+//     same code on the same data: the same bits everywhere); worker 0
+//     records d, e and tau, the owner of column k + 1 keeps the reflector.
 // Values travel as two 8-byte words, each holding 32 bits of the double and
 // the 32-bit tag k + 1 of their step (written and read as single atomic
 // words at agent scope): a reader polls the words until both carry the tag,
-// with no separate flag and no cache flush. Reflectors and parts alternate
+// with no separate flag and no cache flush. Parts and columns alternate
 // between two slots by the parity of k (a worker reaches step k + 2 only
 // after every worker has read step k), and the slots are cleared before the
 // launch, so a tag is never stale. The workers are blocks 0 .. nw-1, spread
 // over the XCDs by the round-robin dispatch. Every wait is bounded: a worker
 // that polls a word more than `spins` times (at once when spins < 0), or
 // sees another worker's abort, sets *info = -1 and every worker leaves (the
-// caller redoes the solve with the one-workgroup kernels). The reflectors end
-// in A[k+1:, k] as k_tridiag leaves them.
+// caller redoes the solve with the one-workgroup kernels). A is only read
+// until the loop ends; the reflectors are then written to A[k+1:, k] as
+// k_tridiag leaves them.
 // ---------------------------------------------------------------------------
 constexpr int kCT = 256;
+constexpr int kCoopRd = 5;  // words of each slot a thread reads per step (m <= 5 kCT)
 
 __device__ __forceinline__ void ll_put(uint64_t* p, double v, uint32_t tag) {
   const uint64_t b = uint64_t(__double_as_longlong(v));
@@ -837,17 +846,19 @@ __device__ double coop_sum(double v, double* red) {
   return t;
 }
 
-size_t coop_lds(int m, int nw) { return size_t((int64_t((m + nw - 1) / nw) * m + 2 * m + kCT / 64 + 2) * 8); }
+size_t coop_lds(int m, int nw) { return size_t((int64_t((m + nw - 1) / nw) * m + 4 * m + kCT / 64 + 2) * 8); }
 // (the last double of the reduction slots holds the workgroup's go flag)
 
-// Words of the exchange slots: two slots of (m + 1) reflector values and two
-// of m product values, two words each.
+// Words of the exchange slots: two slots of m parts of y and two of m column
+// values, two words each (8 m of the 8 m + 4 reserved).
 int64_t coop_words(int m) { return 4 * int64_t(m + 1) + 4 * int64_t(m); }
 
-// Reflector k from column k (updated through step k - 1) into d, e, tau and
-// the exchange slot: the owner's whole workgroup.
-__device__ void coop_reflector(int m, int k, double* col, double* red, double* d, double* e, double* tau,
-                               uint64_t* vb) {
+// Reflector k from column `col` (rows k .. m-1 current; every worker): v
+// into vv[0 .. m-k-2] (vv[i] for row k + 1 + i), tau into *tkp, and
+// x0 - alpha (or x0) into *v1 for the owner's copy. Worker `record` writes
+// d, e, tau. Ends with a barrier.
+__device__ void coop_reflector(int m, int k, const double* col, double* vv, double* red, double* tkp, double* v1,
+                               bool record, double* d, double* e, double* tau) {
   const int tid = threadIdx.x;
   double ss = 0.0;
   for (int r = k + 1 + tid; r < m; r += kCT) ss += col[r] * col[r];
@@ -859,16 +870,17 @@ __device__ void coop_reflector(int m, int k, double* col, double* red, double* d
     alpha = x0 >= 0.0 ? -sigma : sigma;
     tk = 1.0 / (sigma * (sigma + fabs(x0)));
   }
-  const uint32_t tag = uint32_t(k + 1);
-  for (int r = k + 1 + tid; r < m; r += kCT) ll_put(vb + 2 * r, r == k + 1 && sigma > 0.0 ? x0 - alpha : col[r], tag);
+  const double first = sigma > 0.0 ? x0 - alpha : x0;
+  for (int r = k + 1 + tid; r < m; r += kCT) vv[r - (k + 1)] = r == k + 1 ? first : col[r];
   if (tid == 0) {
-    d[k] = col[k];
-    e[k] = alpha;
-    tau[k] = tk;
-    ll_put(vb + 2 * m, tk, tag);
+    *tkp = tk;
+    *v1 = first;
+    if (record) {
+      d[k] = col[k];
+      e[k] = alpha;
+      tau[k] = tk;
+    }
   }
-  __syncthreads();
-  if (tid == 0) col[k + 1] = sigma > 0.0 ? x0 - alpha : x0;
   __syncthreads();
 }
 
@@ -882,47 +894,99 @@ __global__ __launch_bounds__(kCT) void k_td_coop(int m, int nw, double* __restri
   double* Al = sh;                         // ncl columns of m rows
   double* vv = Al + int64_t((m + nw - 1) / nw) * m;
   double* yw = vv + m;
-  double* red = yw + m;
+  double* c1 = yw + m;  // copy of column k + 1
+  double* c2 = c1 + m;  // copy of column k + 2
+  double* red = c2 + m;
   constexpr int NWV = kCT / 64;
   int& go = *reinterpret_cast<int*>(red + NWV + 1);
+  double& v1 = yw[m - 1];  // scratch for the reflector's first entry (yw is idle then)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  uint64_t* vslot[2] = {words, words + 2 * int64_t(m + 1)};
-  uint64_t* yslot[2] = {words + 4 * int64_t(m + 1), words + 4 * int64_t(m + 1) + 2 * int64_t(m)};
+  uint64_t* yslot[2] = {words, words + 2 * int64_t(m)};
+  uint64_t* cslot[2] = {words + 4 * int64_t(m), words + 6 * int64_t(m)};
   if (tid == 0) go = 1;
   for (int lc = 0; lc < ncl; ++lc) {
     const int c = me + lc * nw;
     for (int r = tid; r < m; r += kCT)
       Al[int64_t(lc) * m + r] = r >= c ? A[r + int64_t(c) * lda] : A[c + int64_t(r) * lda];
   }
+  for (int r = tid; r < m; r += kCT) {
+    c2[r] = A[r];  // column 0
+    c1[r] = r >= 1 ? A[r + lda] : A[1];
+  }
   __syncthreads();
-  if (me == 0 && m > 2) coop_reflector(m, 0, Al, red, d, e, tau, vslot[0]);
+  if (m > 2) {
+    coop_reflector(m, 0, c2, vv, red, &red[NWV], &v1, me == 0, d, e, tau);
+    if (me == 0 && tid == 0) Al[1] = v1;  // column 0's copy keeps its reflector
+    __syncthreads();
+  }
   for (int k = 0; k + 2 < m; ++k) {
     const int L = m - k - 1;
     const uint32_t tag = uint32_t(k + 1);
-    uint64_t* vb = vslot[k & 1];
     uint64_t* yb = yslot[k & 1];
-    // B. reflector k, and this worker's part of y = A22 v.
-    bool ok = true;
-    for (int i = tid; i < L && ok; i += kCT) ok = ll_get(vb + 2 * (k + 1 + i), tag, &vv[i], spins, info);
-    if (tid == 0 && ok) ok = ll_get(vb + 2 * m, tag, &red[NWV], spins, info);
-    if (!ok) go = 0;
-    __syncthreads();
-    if (!go) break;
+    uint64_t* cb = cslot[k & 1];
     const double tk = red[NWV];
-    const int lc0 = k + 1 <= me ? 0 : (k + 1 - me + nw - 1) / nw;
-    for (int lc = lc0 + wave; lc < ncl; lc += NWV) {
+    const int lc0 = k + 1 <= me ? 0 : (k + 1 - me + nw - 1) / nw;  // first own column > k
+    // A. This worker's parts of y (a wave's columns three at a time, so
+    // their wave sums overlap); column k + 2 from its owner.
+    for (int lc = lc0 + wave; lc < ncl; lc += 3 * NWV) {
       const double* col = Al + int64_t(lc) * m + k + 1;
-      double acc = 0.0;
-      for (int i = lane; i < L; i += 64) acc += col[i] * vv[i];
-      acc = wave_sum(acc);
-      if (lane == 0) ll_put(yb + 2 * (me + lc * nw), acc, tag);
+      const bool has1 = lc + NWV < ncl, has2 = lc + 2 * NWV < ncl;
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+      for (int i = lane; i < L; i += 64) {
+        const double v = vv[i];
+        a0 += col[i] * v;
+        if (has1) a1 += col[i + NWV * int64_t(m)] * v;
+        if (has2) a2 += col[i + 2 * NWV * int64_t(m)] * v;
+      }
+      a0 = wave_sum(a0);
+      a1 = wave_sum(a1);
+      a2 = wave_sum(a2);
+      if (lane == 0) {
+        ll_put(yb + 2 * (me + lc * nw), a0, tag);
+        if (has1) ll_put(yb + 2 * (me + (lc + NWV) * nw), a1, tag);
+        if (has2) ll_put(yb + 2 * (me + (lc + 2 * NWV) * nw), a2, tag);
+      }
     }
-    // C. all of y, then w, then the update (column k + 1 and its reflector
-    // first on the owner of k + 1).
+    if ((k + 2) % nw == me) {
+      const double* col = Al + int64_t((k + 2 - me) / nw) * m;
+      for (int r = k + 1 + tid; r < m; r += kCT) ll_put(cb + 2 * r, col[r], tag);
+    }
+    // B. All of y and column k + 2, then w and the updates. Every load of
+    // this thread's words is issued before any is waited on (one trip
+    // through memory when the values are there); late words are polled one
+    // at a time (polling them all again each time loads the memory system
+    // enough to slow the writers: measured slower).
+    bool ok = true;
     double s = 0.0;
-    for (int i = tid; i < L && ok; i += kCT) {
-      ok = ll_get(yb + 2 * (k + 1 + i), tag, &yw[i], spins, info);
-      s += yw[i] * vv[i];
+    {
+      uint64_t w4[kCoopRd][4];
+#pragma unroll
+      for (int j = 0; j < kCoopRd; ++j) {
+        const int i = tid + j * kCT;
+        if (i < L) {
+          const uint64_t* py = yb + 2 * (k + 1 + i);
+          const uint64_t* pc = cb + 2 * (k + 1 + i);
+          w4[j][0] = __hip_atomic_load(py, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          w4[j][1] = __hip_atomic_load(py + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          w4[j][2] = __hip_atomic_load(pc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          w4[j][3] = __hip_atomic_load(pc + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kCoopRd; ++j) {
+        const int i = tid + j * kCT;
+        if (i < L && ok) {
+          if (spins >= 0 && uint32_t(w4[j][0] >> 32) == tag && uint32_t(w4[j][1] >> 32) == tag)
+            yw[i] = __longlong_as_double((long long)((w4[j][1] << 32) | (w4[j][0] & 0xffffffffull)));
+          else
+            ok = ll_get(yb + 2 * (k + 1 + i), tag, &yw[i], spins, info);
+          if (ok && uint32_t(w4[j][2] >> 32) == tag && uint32_t(w4[j][3] >> 32) == tag)
+            c2[k + 1 + i] = __longlong_as_double((long long)((w4[j][3] << 32) | (w4[j][2] & 0xffffffffull)));
+          else if (ok)
+            ok = ll_get(cb + 2 * (k + 1 + i), tag, &c2[k + 1 + i], spins, info);
+          s += yw[i] * vv[i];
+        }
+      }
     }
     if (!ok) go = 0;
     s = coop_sum(s, red);
@@ -930,23 +994,44 @@ __global__ __launch_bounds__(kCT) void k_td_coop(int m, int nw, double* __restri
     const double K = 0.5 * tk * tk * s;
     for (int i = tid; i < L; i += kCT) yw[i] = tk * yw[i] - K * vv[i];
     __syncthreads();
-    int first = lc0;
-    if (k + 1 < m - 2 && (k + 1) % nw == me) {
-      double* col = Al + int64_t(lc0) * m;  // column k + 1
-      for (int i = tid; i < L; i += kCT)
-        col[k + 1 + i] = col[k + 1 + i] - __dadd_rn(__dmul_rn(vv[i], yw[0]), __dmul_rn(yw[i], vv[0]));
-      __syncthreads();
-      coop_reflector(m, k + 1, col, red, d, e, tau, vslot[(k + 1) & 1]);
-      first = lc0 + 1;
+    const int ncols = ncl - lc0;
+    {
+      // (column, row) of flat index t = tid, tid + kCT, ... advanced without
+      // a division per element
+      const int dq = kCT / L, dr = kCT - dq * L;
+      int lc = lc0 + tid / L, i = tid - (tid / L) * L;
+      for (int t = tid; t < ncols * L; t += kCT) {
+        const int j = me + lc * nw - (k + 1);
+        double* a = Al + int64_t(lc) * m + k + 1 + i;
+        *a = *a - __dadd_rn(__dmul_rn(vv[i], yw[j]), __dmul_rn(yw[i], vv[j]));
+        lc += dq;
+        i += dr;
+        if (i >= L) {
+          i -= L;
+          ++lc;
+        }
+      }
     }
-    const int ncols = ncl - first;
-    for (int t = tid; t < ncols * L; t += kCT) {
-      const int lc = first + t / L, i = t - (t / L) * L;
-      const int j = me + lc * nw - (k + 1);
-      double* a = Al + int64_t(lc) * m + k + 1 + i;
-      *a = *a - __dadd_rn(__dmul_rn(vv[i], yw[j]), __dmul_rn(yw[i], vv[j]));
+    for (int i = tid; i < L; i += kCT) {
+      c1[k + 1 + i] = c1[k + 1 + i] - __dadd_rn(__dmul_rn(vv[i], yw[0]), __dmul_rn(yw[i], vv[0]));
+      c2[k + 1 + i] = c2[k + 1 + i] - __dadd_rn(__dmul_rn(vv[i], yw[1]), __dmul_rn(yw[i], vv[1]));
     }
     __syncthreads();
+    // C. Reflector k + 1 (or, after the last step, the trailing 2 x 2).
+    if (k + 3 < m) {
+      coop_reflector(m, k + 1, c1, vv, red, &red[NWV], &v1, me == 0, d, e, tau);
+      if ((k + 1) % nw == me && tid == 0) Al[int64_t((k + 1 - me) / nw) * m + k + 2] = v1;
+      __syncthreads();
+    } else if (me == 0 && tid == 0) {
+      d[m - 2] = c1[m - 2];
+      e[m - 2] = c1[m - 1];
+      tau[m - 2] = 0.0;
+      d[m - 1] = c2[m - 1];
+      tau[m - 1] = 0.0;
+    }
+    double* t = c1;
+    c1 = c2;
+    c2 = t;
   }
   if (!go && m > 2) {
     // Abandoned: the failure is in *info; a zero tridiagonal keeps the
@@ -959,20 +1044,12 @@ __global__ __launch_bounds__(kCT) void k_td_coop(int m, int nw, double* __restri
     }
     return;
   }
-  // The trailing 2 x 2 and the reflectors into A.
+  // The reflectors into A.
   for (int lc = 0; lc < ncl; ++lc) {
     const int c = me + lc * nw;
+    if (c >= m - 2) continue;
     const double* col = Al + int64_t(lc) * m;
-    if (c == m - 2 && tid == 0) {
-      d[m - 2] = col[m - 2];
-      e[m - 2] = col[m - 1];
-      tau[m - 2] = 0.0;
-    } else if (c == m - 1 && tid == 0) {
-      d[m - 1] = col[m - 1];
-      tau[m - 1] = 0.0;
-    } else if (c < m - 2) {
-      for (int r = c + 1 + tid; r < m; r += kCT) A[r + int64_t(c) * lda] = col[r];
-    }
+    for (int r = c + 1 + tid; r < m; r += kCT) A[r + int64_t(c) * lda] = col[r];
   }
 }
 
@@ -1756,7 +1833,7 @@ std::atomic<int> coop_spins{1 << 21};
 int coop_worker_count(int m) {
   int w = coop_workers.load();
   if (w < 0) w = std::min(32, std::max(8, m / 10));
-  if (w == 0 || m < 3 || m < coop_from.load() || m > coop_to.load()) return 0;
+  if (w == 0 || m < 3 || m < coop_from.load() || m > coop_to.load() || m > kCoopRd * kCT) return 0;
   for (int nw = w; nw <= 32; ++nw)
     if (coop_lds(m, nw) <= 160 * 1024) return nw;
   return 0;
