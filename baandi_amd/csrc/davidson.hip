@@ -93,8 +93,29 @@ __global__ __launch_bounds__(kT) void k_corrections(const double* __restrict__ X
     __syncthreads();
     scale = bcast;
   }
+  // Four rows per thread and pass, their loads issued before any is used
+  // (one workgroup per column: the loads' latency, not the arithmetic,
+  // bounds the pass); the sum keeps the row order i, i + T, i + 2T, ...
   double acc = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+  const int64_t T = blockDim.x;
+  int64_t i = threadIdx.x;
+  for (; i + 3 * T < n; i += 4 * T) {
+    double xv[4], hv[4], wv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      xv[u] = x[i + u * T];
+      hv[u] = hx[i + u * T];
+      wv[u] = mode == 1 ? kinetic[i + u * T] : diag[i + u * T];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double r = hv[u] - th * xv[u];
+      const double v = mode == 1 ? -r * tpa_weight(wv[u], scale) : r / sqrt_denominator(wv[u], th);
+      e[i + u * T] = v;
+      acc += v * v;
+    }
+  }
+  for (; i < n; i += T) {
     const double r = hx[i] - th * x[i];
     const double v = mode == 1 ? -r * tpa_weight(kinetic[i], scale) : r / sqrt_denominator(diag[i], th);
     e[i] = v;
@@ -129,7 +150,16 @@ __global__ __launch_bounds__(kT) void k_norm2_cols(const double* __restrict__ V,
   __shared__ double red[16];
   const double* v = V + int64_t(blockIdx.x) * n;
   double acc = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) acc += v[i] * v[i];
+  const int64_t T = blockDim.x;
+  int64_t i = threadIdx.x;
+  for (; i + 3 * T < n; i += 4 * T) {  // loads first, sum in row order
+    double a[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a[u] = v[i + u * T];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc += a[u] * a[u];
+  }
+  for (; i < n; i += T) acc += v[i] * v[i];
   const double t = block_sum(acc, red);
   if (threadIdx.x == 0) out[blockIdx.x] = t;
 }
