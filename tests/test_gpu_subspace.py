@@ -38,10 +38,12 @@ def _check_pairs(A, want, label):
 # The blocked tridiagonalisation holds m <= 523 in LDS (523 and 524 straddle
 # that limit); the LDS inverse iteration holds 8 vectors per workgroup up to
 # m = 476 and 4 up to 910. Above those the one-step and global-memory kernels
-# run, with a back-transform above 64 KB of LDS from m = 683.
+# run. The back-transform keeps the vector in registers, 5 rows a lane up to
+# m = 320 and 10 up to 640 (320/321 and 640/641 straddle those), and stages it
+# in LDS above, past 64 KB of LDS from m = 683.
 @pytest.mark.parametrize("m,want", [(1, 1), (2, 2), (3, 3), (5, 2), (64, 64), (160, 160), (320, 160), (321, 100),
-                                    (523, 60), (524, 100), (576, 288), (659, 40), (660, 30), (700, 60),
-                                    (910, 40), (911, 40), (974, 30), (1025, 30)])
+                                    (523, 60), (524, 100), (576, 288), (640, 64), (641, 64), (659, 40), (660, 30),
+                                    (700, 60), (910, 40), (911, 40), (974, 30), (1025, 30)])
 def test_eigen_random_symmetric(m, want):
     rng = np.random.default_rng(m * 7 + want)
     B = rng.standard_normal((m, m))
