@@ -243,7 +243,8 @@ def si128_leg(mlp_path=None, theta=0.01, variants=True):
     legs = {"fp64": ScfOptions(**base),
             "mixed": ScfOptions(precision=L.BD_MIXED, **base),
             "learned": ScfOptions(preconditioner=L.BD_PRECOND_MLP, mlp=net, **base),
-            "mixed_learned": ScfOptions(precision=L.BD_MIXED, preconditioner=L.BD_PRECOND_MLP, mlp=net, **base)}
+            "mixed_learned": ScfOptions(precision=L.BD_MIXED, preconditioner=L.BD_PRECOND_MLP, mlp=net, **base),
+            "fp32_hx": ScfOptions(precision=L.BD_FP32_HX, **base)}
     if not variants:
         legs = {"fp64": legs["fp64"]}
     out, ref = {"workload": f"{wl['name']}: n={system.n}, grid {list(system.dims)}, {system.nproj} projectors, "
