@@ -146,6 +146,83 @@ __global__ __launch_bounds__(256) void k_gemm_nn(int64_t M, int64_t N, int64_t K
     }
 }
 
+// NN with 32 x 32 output tiles, for products whose 64-wide tiles would fill
+// only a fraction of the CUs (the subspace step's m x m and m x bands
+// products): four times the workgroups, each wave one 16 x 16 block, so the
+// K loop's MFMA chain runs on four times the SIMDs. Same operand layouts and
+// K slices as k_gemm_nn.
+constexpr int TS = 32;
+template <bool TRANS_A>
+__global__ __launch_bounds__(256) void k_gemm_s32(int64_t M, int64_t N, int64_t K, double alpha,
+                                                  const double* __restrict__ A, int64_t lda,
+                                                  const double* __restrict__ B, int64_t ldb, double beta,
+                                                  double* __restrict__ C, int64_t ldc) {
+  __shared__ double As[TK][TS + 1];
+  __shared__ double Bs[TK][TS + 1];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6, wr = wave & 1, wc = wave >> 1;
+  const int li = lane & 15, lk = lane >> 4;
+  const int64_t row0 = int64_t(blockIdx.x) * TS, col0 = int64_t(blockIdx.y) * TS;
+  f64x4 acc = f64x4{0.0, 0.0, 0.0, 0.0};
+  // A slice (32 rows x TK k): thread t holds rows 2 (t & 15) + u of k = t >> 4.
+  // B slice (TK k x 32 columns): thread t holds k = 2 (t & 7) + u of column t >> 3.
+  const int ka = tid >> 4, ia = (tid & 15) * 2;
+  const int jb = tid >> 3, kb = (tid & 7) * 2;
+  double ra[2], rb[2];
+  // (TRANS_A: A is K x M, k contiguous; its slice is loaded like B's.)
+  auto fetch = [&](int64_t k0) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      if (TRANS_A) {
+        const int64_t gi = row0 + jb, gk = k0 + kb + u;
+        ra[u] = (gi < M && gk < K) ? A[gk + gi * lda] : 0.0;
+      } else {
+        const int64_t gi = row0 + ia + u, gk = k0 + ka;
+        ra[u] = (gi < M && gk < K) ? A[gi + gk * lda] : 0.0;
+      }
+      const int64_t gj = col0 + jb, gk2 = k0 + kb + u;
+      rb[u] = (gj < N && gk2 < K) ? B[gk2 + gj * ldb] : 0.0;
+    }
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      if (TRANS_A)
+        As[kb + u][jb] = ra[u];
+      else
+        As[ka][ia + u] = ra[u];
+      Bs[kb + u][jb] = rb[u];
+    }
+  };
+  if (K > 0) {
+    fetch(0);
+    stash();
+    __syncthreads();
+  }
+  for (int64_t k0 = 0; k0 < K; k0 += TK) {
+    const bool more = k0 + TK < K;
+    if (more) fetch(k0 + TK);
+#pragma unroll
+    for (int kk = 0; kk < TK; kk += 4)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(As[kk + lk][wr * 16 + li], Bs[kk + lk][wc * 16 + li], acc, 0, 0,
+                                                 0);
+    __syncthreads();
+    if (more) {
+      stash();
+      __syncthreads();
+    }
+  }
+  const int64_t gj = col0 + wc * 16 + li;
+  if (gj >= N) return;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int64_t gi = row0 + wr * 16 + lk + 4 * v;
+    if (gi >= M) continue;
+    double* p = C + gi + gj * ldc;
+    *p = beta == 0.0 ? alpha * acc[v] : alpha * acc[v] + beta * *p;
+  }
+}
+
 // TN partial: part[s](MxN) = sum over k in chunk s of A[k,i] B[k,j].
 // Workgroups are dealt to the 8 XCDs round robin (id % 8); the logical
 // (tile, chunk) index puts the tiles of one K chunk on one XCD, one after the
@@ -273,7 +350,10 @@ void gemm_nn(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const
   if (M <= 0 || N <= 0) return;
   const int bn = col_tile(N);
   dim3 grid(unsigned((M + TM - 1) / TM), unsigned((N + bn - 1) / bn));
-  if (bn == 128)
+  if (int64_t(grid.x) * grid.y < 128 && K >= TK) {  // under half the CUs busy: quarter-size tiles
+    dim3 g32(unsigned((M + TS - 1) / TS), unsigned((N + TS - 1) / TS));
+    hipLaunchKernelGGL(k_gemm_s32<false>, g32, dim3(256), 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+  } else if (bn == 128)
     hipLaunchKernelGGL(k_gemm_nn<128>, grid, dim3(256), 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
   else
     hipLaunchKernelGGL(k_gemm_nn<64>, grid, dim3(256), 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
@@ -299,6 +379,13 @@ void gemm_tn(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const
   const int used = int((K + chunk - 1) / std::max<int64_t>(chunk, 1));
   const int bn = col_tile(N);
   const int tiles_m = int((M + TM - 1) / TM), tiles = tiles_m * int((N + bn - 1) / bn);
+  if (used <= 1 && tiles < 128 && K >= TK) {
+    // one K chunk and under half the CUs busy: quarter-size tiles, C written
+    // directly (no partial sums)
+    dim3 g32(unsigned((M + TS - 1) / TS), unsigned((N + TS - 1) / TS));
+    hipLaunchKernelGGL(k_gemm_s32<true>, g32, dim3(256), 0, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+    return;
+  }
   const int wgs = tiles * std::max(used, 1);
   const dim3 grid(unsigned((wgs + kXcds - 1) / kXcds * kXcds));
   if (bn == 128)

@@ -47,8 +47,14 @@ def test_mfma_f64_operand_layout():
     assert np.abs(d.cpu().numpy().reshape(16, 16) - a @ b).max() < 1e-14
 
 
+# Products that leave most CUs idle with 64-wide tiles run on 32 x 32 tiles
+# (NN 300 x 47, 320 x 320, 33 x 65, 64 x 16 x 16; TN with one K chunk: 65 x
+# 64 x 17, 320 x 160 x 320, 33 x 7 x 100); 1 x 1 x 1 (K below a slice), the
+# split-K TN (70 x 33 x 5000) and the tall products keep the 64-wide kernels.
 @pytest.mark.parametrize("trans_a,M,N,K", [(False, 300, 47, 130), (True, 70, 33, 5000), (False, 1, 1, 1),
-                                           (True, 65, 64, 17)])
+                                           (True, 65, 64, 17), (False, 320, 320, 320), (False, 33, 65, 17),
+                                           (False, 64, 16, 16), (False, 4000, 160, 300), (True, 320, 160, 320),
+                                           (True, 33, 7, 100)])
 def test_fp64_gemm_matches_numpy(trans_a, M, N, K):
     """The FP64 MFMA GEMMs (NN, and TN with split-K) against NumPy: 1e-14 of
     sum |a||b| (edge tiles, beta = 0.5 included)."""
