@@ -47,11 +47,7 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 }
 
 // QE smooth denominator in Ha (ref_davidson/stream.rs:16-22).
-__device__ __forceinline__ double sqrt_denominator(double h, double e) {
-  const double x = 2.0 * (h - e);
-  const double z = fabs(x - 1.0);
-  return 0.25 * (2.0 * fmax(x, 1.0) + 1.0 / (sqrt(1.0 + z * z) + z));
-}
+__device__ __forceinline__ double sqrt_denominator(double h, double e) { return mlp::qe_denominator(h, e); }
 
 // Teter-Payne-Allan K(T/<T>) (crates/dft-solvers/src/lib.rs:197-207).
 __device__ __forceinline__ double tpa_weight(double kinetic, double scale) {
@@ -476,7 +472,7 @@ struct Solver {
     }
     std::vector<double> nrm(k);
     if (o.preconditioner == BD_PRECOND_MLP && !mlp_off) {
-      mlp::correct(s, n, k, X, HX, w.theta.p, op.d_kinetic.p, o.mlp_vbar, o.mlp_blob, w.E.p, w.norm2.p,
+      mlp::correct(s, n, k, X, HX, w.theta.p, op.d_kinetic.p, diag, o.mlp_vbar, o.mlp_blob, w.E.p, w.norm2.p,
                    w.rnorm2.p, w.mlpw.p);
       std::vector<double> rn(k);
       BD_HIP(hipMemcpyAsync(rn.data(), w.rnorm2.p, size_t(k) * 8, hipMemcpyDeviceToHost, s));
@@ -484,8 +480,8 @@ struct Solver {
       bool finite = true;
       for (double v : nrm) finite = finite && std::isfinite(v);
       if (!finite || mlp_regressed(active, rn)) {
-        // TPA fallback for the rest of this solve; a non-finite network
-        // correction is replaced in this iteration already.
+        // QE fallback for the rest of this solve (the network's K = 1); a
+        // non-finite network correction is replaced in this iteration already.
         mlp_off = true;
         rec.mlp_fallback = step_now;
         if (finite) return nrm;
@@ -493,7 +489,7 @@ struct Solver {
         return nrm;
       }
     }
-    const int mode = (o.preconditioner == BD_PRECOND_QE) ? 0 : 1;
+    const int mode = (o.preconditioner == BD_PRECOND_TPA) ? 1 : 0;
     hipLaunchKernelGGL(k_corrections, dim3(k), dim3(kT), 0, s, X, HX, n, w.theta.p, diag, op.d_kinetic.p, mode,
                        w.E.p, w.norm2.p);
     down(nrm.data(), w.norm2.p, k);

@@ -97,8 +97,11 @@ __device__ __forceinline__ void forward_group(const float* sb, float u0, float u
   G.o = part + sb[hb3];
 }
 
+// The first `count` floats of the blob into LDS: inference reads W1, the
+// biases, w3 and W2 (kBlobW2T floats); training also reads W2^T.
+template <int count = kBlobFloats>
 __device__ __forceinline__ void load_blob(float* sb, const float* blob) {
-  for (int i = threadIdx.x; i < kBlobFloats; i += blockDim.x) sb[i] = blob[i];
+  for (int i = threadIdx.x; i < count; i += blockDim.x) sb[i] = blob[i];
   __syncthreads();
 }
 
@@ -159,12 +162,13 @@ __global__ __launch_bounds__(256) void k_kinetic_scale(const double* __restrict_
 
 __global__ __launch_bounds__(256) void k_correct(int64_t n, const double* __restrict__ X, const double* __restrict__ HX,
                                                  const double* __restrict__ theta_ritz, const double* __restrict__ Tb,
-                                                 const double* __restrict__ kinetic, double vbar,
+                                                 const double* __restrict__ kinetic,
+                                                 const double* __restrict__ diag, double vbar,
                                                  const float* __restrict__ blob, double* __restrict__ E,
                                                  double* __restrict__ part) {
-  __shared__ __attribute__((aligned(16))) float sb[kBlobFloats];
+  __shared__ __attribute__((aligned(16))) float sb[kBlobW2T];
   __shared__ double red[2][4];
-  load_blob(sb, blob);
+  load_blob<kBlobW2T>(sb, blob);
   const int j = blockIdx.y, lane = threadIdx.x & 63, wave = threadIdx.x >> 6, q = lane >> 4, e = lane & 15;
   const double th = theta_ritz[j];
   const float T = float(Tb[j]);
@@ -181,7 +185,7 @@ __global__ __launch_bounds__(256) void k_correct(int64_t n, const double* __rest
     forward_group(sb, log1pf(g / T), u1, u2, u3, G);
     if (ok && q == 0) {
       const double r = hx[row] - th * x[row];
-      const double v = -double(expf(G.o)) * r;
+      const double v = -double(expf(G.o)) * r / qe_denominator(diag[row], th);
       ej[row] = v;
       se += v * v;
       sr += r * r;
@@ -216,8 +220,8 @@ __global__ void k_reduce_norms(int k, int nblk, const double* __restrict__ part,
 
 __global__ __launch_bounds__(256) void k_forward_k(int64_t n, const float* __restrict__ g, float T, float ritz,
                                                    float vbar, const float* __restrict__ blob, float* __restrict__ K) {
-  __shared__ __attribute__((aligned(16))) float sb[kBlobFloats];
-  load_blob(sb, blob);
+  __shared__ __attribute__((aligned(16))) float sb[kBlobW2T];
+  load_blob<kBlobW2T>(sb, blob);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, q = lane >> 4, e = lane & 15;
   const float u1 = band_u1(T), u2 = (ritz - vbar) / T, u3 = vbar;
   for (int64_t base = int64_t(blockIdx.x) * 64; base < n; base += int64_t(gridDim.x) * 64) {
@@ -478,15 +482,18 @@ int64_t correct_work(int64_t n, int k) {
 }
 
 void correct(hipStream_t s, int64_t n, int k, const double* X, const double* HX, const double* theta_ritz,
-             const double* kinetic, double vbar, const float* blob, double* E, double* norm2_e, double* norm2_r,
-             double* work) {
+             const double* kinetic, const double* diag, double vbar, const float* blob, double* E,
+             double* norm2_e, double* norm2_r, double* work) {
   if (k <= 0) return;
-  const int nblk = int(std::min<int64_t>((n + 63) / 64, kCorrectBlocks));
+  // About 4096 workgroups in all (each loads the network once), at most
+  // kCorrectBlocks per column and at least 16.
+  const int per_column = std::min(kCorrectBlocks, std::max(16, 4096 / k));
+  const int nblk = int(std::min<int64_t>((n + 63) / 64, per_column));
   double* Tb = work;
   double* part = work + k;
   hipLaunchKernelGGL(k_kinetic_scale, dim3(k), dim3(256), 0, s, X, n, kinetic, Tb);
-  hipLaunchKernelGGL(k_correct, dim3(nblk, k), dim3(256), 0, s, n, X, HX, theta_ritz, Tb, kinetic, vbar, blob, E,
-                     part);
+  hipLaunchKernelGGL(k_correct, dim3(nblk, k), dim3(256), 0, s, n, X, HX, theta_ritz, Tb, kinetic, diag, vbar, blob,
+                     E, part);
   hipLaunchKernelGGL(k_reduce_norms, dim3((k + 63) / 64), dim3(64), 0, s, k, nblk, part, norm2_e, norm2_r);
 }
 

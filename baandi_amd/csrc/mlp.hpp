@@ -1,7 +1,8 @@
 // Learned Davidson preconditioner: a 4 -> 64 -> 64 -> 1 tanh network giving
-// the diagonal K of the correction e = -K r (no reference counterpart; it
-// takes the place of the Teter-Payne-Allan weight of
-// crates/dft-solvers/src/lib.rs:176-280).
+// a row factor K that scales the QE correction, e = -K r / d_QE(h, theta)
+// (no reference counterpart; the QE denominator is ref_davidson/stream.rs:16-22
+// and the preconditioner it joins is crates/dft-solvers/src/lib.rs:176-280).
+// K = 1 is the QE correction itself, so an untrained output layer starts at QE.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -27,17 +28,26 @@ constexpr int kBlobW2T = kBlobW2 + kHidden * kHidden / 2;
 constexpr int kBlobFloats = kBlobW2T + kHidden * kHidden / 2;
 constexpr float kKineticFloor = 0.1f;  // kinetic_scale floor, lib.rs:178-193
 
+// QE smooth denominator in Ha for the Davidson diagonal h and Ritz value e
+// (ref_davidson/stream.rs:16-22).
+__device__ __forceinline__ double qe_denominator(double h, double e) {
+  const double x = 2.0 * (h - e);
+  const double z = fabs(x - 1.0);
+  return 0.25 * (2.0 * fmax(x, 1.0) + 1.0 / (sqrt(1.0 + z * z) + z));
+}
+
 // Blob from FP32 master parameters.
 void quantize(hipStream_t s, const float* theta, float* blob);
 
 // Davidson corrections with the network: for each active column j,
-// r = HX_j - theta_j X_j, e = -K(g, T_j, theta_j, vbar) r, where T_j is the
-// column's kinetic scale. Writes E (n x k) and |e|^2, |r|^2 per column.
+// r = HX_j - theta_j X_j, e = -K(g, T_j, theta_j, vbar) r / d_QE(diag, theta_j),
+// where T_j is the column's kinetic scale and diag the Davidson diagonal.
+// Writes E (n x k) and |e|^2, |r|^2 per column.
 // `work` holds mlp_correct_work(n, k) doubles.
 int64_t correct_work(int64_t n, int k);
 void correct(hipStream_t s, int64_t n, int k, const double* X, const double* HX, const double* theta_ritz,
-             const double* kinetic, double vbar, const float* blob, double* E, double* norm2_e, double* norm2_r,
-             double* work);
+             const double* kinetic, const double* diag, double vbar, const float* blob, double* E,
+             double* norm2_e, double* norm2_r, double* work);
 
 // Training on a set of bands, each a residual r and target t (float, n each)
 // with scalars (T, theta, vbar), sharing the row energies g (float, n).

@@ -1,21 +1,22 @@
 """Learned Davidson preconditioner: the network, its training pairs and its
 training, all on the GPU through the bd_mlp_* kernels of libbaandi_amd.so.
 
-The network replaces the Teter-Payne-Allan weight of the correction
-e = -K r (crates/dft-solvers/src/lib.rs:176-280 has the TPA form; there is no
-learned counterpart in the reference). K = exp(MLP(u)) with
+The network scales the QE correction row by row: e = -K r / d_QE(h, theta)
+(ref_davidson/stream.rs:16-22 has the QE denominator and
+crates/dft-solvers/src/lib.rs:176-280 the preconditioners it sits beside;
+there is no learned counterpart in the reference). K = exp(MLP(u)) with
 u = [log(1 + g/T), log T, (theta - vbar)/T, vbar] per row: g = |G|^2/2 of the
 row, T the column's kinetic scale, theta its Ritz value, vbar the mean local
-potential. Davidson falls back to TPA when the network's residual reduction
-regresses (bd_davidson_solve_dev, BD_PRECOND_MLP).
+potential. K = 1 is QE itself, and Davidson falls back to QE when the
+network's residual reduction regresses (bd_davidson_solve_dev, BD_PRECOND_MLP).
 
-Training pairs come from the FP64 GPU Davidson itself: from a seeded start,
-the Ritz block after k iterations gives residuals r = HX - theta X; the
-converged eigenvectors give each column's ideal correction t (its closest
-exact eigenvector, within a degenerate cluster, minus the column, with the
-current Ritz block projected out). The loss per column,
-1 - <K r, t>^2 / (|K r|^2 |t|^2), is blind to the correction's scale, as
-Davidson is.
+Training pairs come from the FP64 GPU Davidson itself (QE corrections): from
+a seeded start, the Ritz block after k iterations gives residuals
+r = HX - theta X, stored QE-scaled as r / d_QE; the converged eigenvectors
+give each column's ideal correction t (its closest exact eigenvector, within a
+degenerate cluster, minus the column, with the current Ritz block projected
+out). The loss per column, 1 - <K r, t>^2 / (|K r|^2 |t|^2), is blind to the
+correction's scale, as Davidson is.
 """
 from __future__ import annotations
 
@@ -50,14 +51,15 @@ class Network:
         self.refresh()
 
     @classmethod
-    def initial(cls, seed=0):
-        """Glorot-uniform weights, zero biases, small output layer."""
+    def initial(cls, seed=0, output_scale=0.1):
+        """Glorot-uniform weights, zero biases, output layer U(-s, s) with
+        s = output_scale (0: K = 1 everywhere, the QE correction)."""
         rng = np.random.default_rng(seed)
         l1 = math.sqrt(6.0 / (FEATURES + HIDDEN))
         l2 = math.sqrt(6.0 / (2 * HIDDEN))
         parts = [rng.uniform(-l1, l1, HIDDEN * FEATURES), np.zeros(HIDDEN),
                  rng.uniform(-l2, l2, HIDDEN * HIDDEN), np.zeros(HIDDEN),
-                 rng.uniform(-0.1, 0.1, HIDDEN), np.zeros(1)]
+                 rng.uniform(-output_scale, output_scale, HIDDEN), np.zeros(1)]
         return cls(np.concatenate(parts))
 
     @classmethod
@@ -114,6 +116,15 @@ def kinetic_scale(x, g):
     return ((w * g[None, :]).sum(dim=1) / w.sum(dim=1)).clamp_min(0.1)
 
 
+def qe_denominator(h, e):
+    """The QE smooth denominator d_QE(h, e) in Ha (ref_davidson/stream.rs:16-22),
+    elementwise on tensors."""
+    import torch
+    x = 2.0 * (h - e)
+    z = (x - 1.0).abs()
+    return 0.25 * (2.0 * torch.clamp_min(x, 1.0) + 1.0 / (torch.sqrt(1.0 + z * z) + z))
+
+
 def generate_pairs(operator, diag, vbar, bands, occupied, seed,
                    iterations=(1, 2, 3, 4, 5, 6, 8, 10, 12, 15, 19, 24, 30, 37),
                    extra=8, cluster_ha=1e-6, min_overlap=0.3):
@@ -136,10 +147,10 @@ def generate_pairs(operator, diag, vbar, bands, occupied, seed,
     for k in iterations:
         sol = solve(operator, start, diag, Options(bands=bands, threshold=1e-14, occupied=bands, first_step=True,
                                                    max_iterations=int(k), return_hx=True,
-                                                   preconditioner=L.BD_PRECOND_TPA))
+                                                   preconditioner=L.BD_PRECOND_QE))
         X, HX = sol.eigenvectors, sol.hx
         th = torch.from_numpy(sol.eigenvalues).cuda()
-        R = HX - th[:, None] * X
+        R = (HX - th[:, None] * X) / qe_denominator(diag[None, :], th[:, None])
         ov = X @ Psi.T                                    # (bands, nref)
         best = ov.abs().argmax(dim=1)
         keep = []
@@ -198,8 +209,18 @@ def train(net, pairs, steps, batch=16, lr=2e-3, seed=0):
     return losses
 
 
+def qe_loss(pairs):
+    """The same loss at K = 1: for pairs whose R holds the QE-scaled residual,
+    the loss of the QE correction (the baseline the network must beat)."""
+    import torch
+    a = pairs.R.double()
+    t = pairs.Tg.double()
+    c = (a * t).sum(dim=1)
+    return float((1.0 - c * c / ((a * a).sum(dim=1) * (t * t).sum(dim=1))).mean())
+
+
 def tpa_loss(pairs):
-    """The same loss for the TPA weight (the baseline the network must beat)."""
+    """The same loss for the TPA weight applied to R as stored."""
     import torch
     g = pairs.g[None, :].double()
     T = pairs.scal[:, 0:1].double()

@@ -383,10 +383,10 @@ def si512_scf_leg(theta=0.01, variants=True):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mixed-steps", type=int, default=1,
+    ap.add_argument("--mixed-steps", type=int, default=5,
                     help="SCFs timed with the mixed tier (loose maps in FP32/BF16x3, final maps in FP64)")
     ap.add_argument("--kpoint-steps", type=int, default=1,
                     help="Al 6x6x6 k-point SCFs timed (0 skips the k-point leg)")
@@ -523,7 +523,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded N(0,1) start, synthetic Gaussian pseudopotential); learned weights "
-                "trained on displaced Si8 cells by baandi_amd.train_preconditioner",
+                "trained on 32 displaced Si64 cells (seeds 101-132) by baandi_amd.train_preconditioner",
         "config": {"workload": "Si64 cubic 2x2x2, a=20.52 Bohr, Gamma, 30 Ry, 160 bands (128 occ), "
                                "SCF to dr2<1e-8 Ry (Thomas-Fermi mixing), plain FP64 path",
                    "n_pw": int(system.n), "fft_grid": list(system.dims), "bands": wl["bands"],

@@ -249,8 +249,9 @@ bd_status bd_gamma_op_plane_split(bd_gamma_op* op, int32_t* inout, int32_t set);
  * (lean_davidson.rs:167-331), Ritz-change convergence, contraction at
  * capacity, rank repair. Options mirror ref_davidson::Options (:38-78).
  * ---------------------------------------------------------------------- */
-/* BD_PRECOND_MLP: the learned diagonal (no reference counterpart; see the
- * bd_mlp_* entry points) with the TPA weight as the fallback. */
+/* BD_PRECOND_MLP: the learned row factor K on the QE correction,
+ * e = -K r / d_QE (no reference counterpart; see the bd_mlp_* entry points),
+ * with the QE correction (K = 1) as the fallback. */
 typedef enum { BD_PRECOND_QE = 0, BD_PRECOND_TPA = 1, BD_PRECOND_MLP = 2 } bd_preconditioner;
 
 typedef struct {
@@ -291,7 +292,7 @@ typedef struct {
   double seconds_total;
   double seconds_apply;
   double seconds_dense;  /* subspace step: factor + reduced eigensolve (GPU, synchronised) */
-  int32_t mlp_fallback;  /* BD_PRECOND_MLP: iteration that fell back to TPA, -1 if none */
+  int32_t mlp_fallback;  /* BD_PRECOND_MLP: iteration that fell back to QE, -1 if none */
   int32_t eigen_retries; /* reduced eigensolves that lost a vector of a tight cluster and were
                             redone with the one-step tridiagonalisation */
   /* Restarts by kind (ResetStats, ref_davidson/ideal_reset.rs): ideal resets
@@ -532,7 +533,9 @@ bd_status bd_field_energies_dev(bd_field* field, const double* rho, const double
  * floats) holds bf16 weights for the MFMA kernels.
  * Training data: `bands` residuals R and ideal corrections Tg (float, bands x
  * n, row-major per band), per-band (T, theta, vbar) in scal (bands x 3), the
- * row energies g (n). Loss per band: 1 - <K r, t>^2 / (|K r|^2 |t|^2).
+ * row energies g (n). Loss per band: 1 - <K r, t>^2 / (|K r|^2 |t|^2). The
+ * Davidson applies K to the QE-scaled residual r / d_QE, so training pairs
+ * for it carry that residual in R.
  * ---------------------------------------------------------------------- */
 int32_t bd_mlp_param_count(void);
 int32_t bd_mlp_blob_floats(void);

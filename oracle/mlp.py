@@ -5,9 +5,11 @@ bench.py's cpu_baseline leg as the checker of the HIP kernels; never shipped
 or called by the product path.
 
 The reference has no learned preconditioner: it applies the Teter-Payne-Allan
-diagonal (crates/dft-solvers/src/lib.rs:176-280) as e = -r K(T_G/<T>). The
-learned form keeps e = -K r but takes K from a 4 -> 64 -> 64 -> 1 tanh network
-of per-plane-wave features:
+diagonal (crates/dft-solvers/src/lib.rs:176-280) as e = -r K(T_G/<T>), or the
+QE correction e = r / d_QE(h, theta) (ref_davidson/stream.rs:16-22). The
+learned form scales the QE correction by a row factor, e = -K r / d_QE(h,
+theta) (the sign is immaterial to Davidson), with K from a 4 -> 64 -> 64 -> 1
+tanh network of per-plane-wave features:
 
     u = [log(1 + g/T), log(T), (theta - vbar)/T, vbar]
     h1 = tanh(W1 u + b1), h2 = tanh(W2 h1 + b2), o = w3 . h2 + b3, K = exp(o)
@@ -21,7 +23,8 @@ in high precision. Gradients treat that rounding as the identity.
 Training loss per band (scale invariant, because Davidson normalises every
 correction): L = 1 - <a, t>^2 / (|a|^2 |t|^2), a = K r, t = the ideal
 correction (the converged eigenvector's component outside the current Ritz
-block). The batch loss is the mean over bands.
+block), r the QE-scaled residual r / d_QE. The batch loss is the mean over
+bands. K = 1 (a zero output layer) is the QE correction.
 
 Parity: the backward pass is pinned by central finite differences of the
 float64 forward (tests/test_oracle_mlp.py); the GPU kernels are checked
@@ -154,14 +157,16 @@ def adam_step(theta, grad, m, v, step, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
     return theta - lr * mh / (np.sqrt(vh) + eps), m, v
 
 
-def correction(theta, r, x, g, ritz, vbar, bf16=True):
-    """e = -K r for one band (the inference the Davidson step applies)."""
+def correction(theta, r, x, g, ritz, vbar, diag, bf16=True):
+    """e = -K r / d_QE(diag, ritz) for one band (the inference the Davidson
+    step applies: the network's row factor on the QE correction)."""
+    from .davidson import sqrt_denominator
     T = kinetic_scale(x, g)
-    return -forward(theta, features(g, T, ritz, vbar), bf16)["K"] * r
+    return -forward(theta, features(g, T, ritz, vbar), bf16)["K"] * r / sqrt_denominator(diag, ritz)
 
 
 def fallback_after(ratios, limit=1.0, strikes=2):
-    """The TPA fallback rule: the first iteration index at which the
+    """The fallback rule (back to the QE correction): the first iteration index at which the
     residual-reduction ratio has been >= limit for `strikes` consecutive
     iterations (None if never). A non-finite ratio counts as a strike."""
     run = 0

@@ -1,6 +1,7 @@
 """Learned preconditioner on MI355X: the MFMA forward pass and the training
 kernels against the oracle (oracle/mlp.py, bf16 emulation), training
-progress, and Davidson with the network (same fixed point, TPA fallback)."""
+progress, and Davidson with the network (same fixed point, QE trajectory at
+K = 1, QE fallback, gain over QE on a held-out cell)."""
 import numpy as np
 import pytest
 
@@ -123,7 +124,39 @@ def test_davidson_with_network_keeps_fixed_point():
     assert sol.converged[:4].all() and err < 1e-10
 
 
-def test_davidson_falls_back_to_tpa_on_nonfinite_network():
+def _unit_theta():
+    """Parameters whose network output is exactly 0 (K = 1): the QE correction."""
+    from oracle import mlp as M
+    p = M.unpack(_theta(3))
+    p["w3"] = np.zeros(M.HIDDEN)
+    p["b3"] = np.zeros(1)
+    return M.pack(p)
+
+
+@pytest.mark.parametrize("iterations", [1, 2, 3])
+def test_unit_network_follows_the_qe_trajectory(iterations):
+    """K = 1 makes the network's correction -r / d_QE, the QE correction up to
+    its sign and the order of one sum, so the Ritz values after 1-3 iterations
+    match the QE solve's to rounding; a wrong diagonal, Ritz value or
+    denominator in the network's correction moves them by far more."""
+    from baandi_amd import ref_davidson, lib as L
+    from baandi_amd.learned import Network
+    op, x0, diag, w, vbar = _si2()
+    net = Network(_unit_theta())
+    common = dict(bands=8, threshold=1e-14, occupied=8, max_iterations=iterations, first_step=True)
+    mlp = ref_davidson.solve(op, x0, diag, ref_davidson.Options(preconditioner=L.BD_PRECOND_MLP, mlp=net,
+                                                                mlp_vbar=vbar, **common))
+    qe = ref_davidson.solve(op, x0, diag, ref_davidson.Options(preconditioner=L.BD_PRECOND_QE, **common))
+    tpa = ref_davidson.solve(op, x0, diag, ref_davidson.Options(preconditioner=L.BD_PRECOND_TPA, **common))
+    d = np.abs(mlp.eigenvalues - qe.eigenvalues).max()
+    d_tpa = np.abs(tpa.eigenvalues - qe.eigenvalues).max()
+    print(f"[mlp unit] {iterations} iterations: |mlp - qe| {d:.2e} Ha, |tpa - qe| {d_tpa:.2e} Ha")
+    assert mlp.record["mlp_fallback"] == -1
+    assert d < 1e-11
+    assert d_tpa > 1e3 * max(d, 1e-14)
+
+
+def test_davidson_falls_back_to_qe_on_nonfinite_network():
     from baandi_amd import ref_davidson, lib as L
     from baandi_amd.learned import Network
     op, x0, diag, w, vbar = _si2()
@@ -133,20 +166,22 @@ def test_davidson_falls_back_to_tpa_on_nonfinite_network():
     sol = ref_davidson.solve(op, x0, diag, ref_davidson.Options(
         bands=8, threshold=1e-12, occupied=4, max_iterations=200, preconditioner=L.BD_PRECOND_MLP, mlp=net,
         mlp_vbar=vbar))
-    tpa = ref_davidson.solve(op, x0, diag, ref_davidson.Options(
-        bands=8, threshold=1e-12, occupied=4, max_iterations=200, preconditioner=L.BD_PRECOND_TPA))
+    qe = ref_davidson.solve(op, x0, diag, ref_davidson.Options(
+        bands=8, threshold=1e-12, occupied=4, max_iterations=200, preconditioner=L.BD_PRECOND_QE))
     print(f"[mlp fallback] record={sol.record}")
     assert sol.record["mlp_fallback"] == 0
-    # From the first step on this is the TPA solve, bit for bit.
-    assert np.array_equal(sol.eigenvalues, tpa.eigenvalues)
-    assert sol.record["h_columns"] == tpa.record["h_columns"]
+    # From the first step on this is the QE solve, bit for bit.
+    assert np.array_equal(sol.eigenvalues, qe.eigenvalues)
+    assert sol.record["h_columns"] == qe.record["h_columns"]
     assert np.abs(sol.eigenvalues[:4] - w[:4]).max() < 1e-10
 
 
-def test_trained_network_beats_tpa_on_held_out_cell():
-    """The committed network (trained on displaced Si8 cells 1000-1007) on a
-    held-out displaced cell: fewer H columns than TPA from the same start,
-    and the same converged eigenvalues (the fixed point does not move)."""
+def test_trained_network_beats_qe_on_held_out_cell():
+    """The committed network (trained on displaced Si64 cells, seeds 101-132)
+    on a held-out displaced Si64 cell (seed 2024) with a partly screened
+    potential: fewer H columns than the QE correction it scales, from the
+    same start, and the same converged eigenvalues (the fixed point does not
+    move)."""
     import os
     import torch
     from baandi_amd import ref_davidson, workloads, lib as L
@@ -156,7 +191,7 @@ def test_trained_network_beats_tpa_on_held_out_cell():
     from baandi_amd.train_preconditioner import displaced, DEFAULT_OUT
     assert os.path.exists(DEFAULT_OUT)
     net = Network.load(DEFAULT_OUT)
-    wl = displaced(workloads.si8(), 0.05, 2024)
+    wl = displaced(workloads.si64(), 0.05, 2024)
     system = workloads.build(wl)
     potential = run_scf(system, ScfOptions(bands=wl["bands"], threshold_ry=1e-3, max_iterations=12,
                                            allow_unconverged=True))["potential"]
@@ -176,9 +211,10 @@ def test_trained_network_beats_tpa_on_held_out_cell():
         out[name] = sol
         print(f"[mlp gain] {name}: iterations {sol.record['iterations']}, H columns {sol.record['h_columns']}, "
               f"fallback {sol.record['mlp_fallback']}, converged {bool(sol.converged.all())}")
-    tpa, mlp = out["tpa"], out["mlp"]
-    assert tpa.converged[:occ].all() and mlp.converged[:occ].all()
-    assert np.abs(tpa.eigenvalues[:occ] - mlp.eigenvalues[:occ]).max() < 1e-9
-    gain = 1.0 - mlp.record["h_columns"] / tpa.record["h_columns"]
-    print(f"[mlp gain] H columns saved against TPA: {100 * gain:.1f}%")
+    qe, mlp = out["qe"], out["mlp"]
+    assert qe.converged[:occ].all() and mlp.converged[:occ].all()
+    assert np.abs(qe.eigenvalues[:occ] - mlp.eigenvalues[:occ]).max() < 1e-9
+    gain = 1.0 - mlp.record["h_columns"] / qe.record["h_columns"]
+    print(f"[mlp gain] H columns saved against QE: {100 * gain:.1f}%, "
+          f"iterations {mlp.record['iterations']} against {qe.record['iterations']}")
     assert gain > 0.0

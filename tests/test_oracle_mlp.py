@@ -1,5 +1,7 @@
 """The oracle of the learned preconditioner: backward pass against central
 finite differences, loss invariants, bf16 rounding and the fallback rule."""
+import math
+
 import numpy as np
 import pytest
 
@@ -91,3 +93,25 @@ def test_adam_moves_against_gradient():
         L, g = M.loss_and_grad(th, batch)
         th, m, v = M.adam_step(th, g, m, v, step, lr=3e-3)
     assert M.loss_and_grad(th, batch)[0] < L0
+
+
+def test_correction_scales_the_qe_correction():
+    """A zero output layer (K = 1) gives -r / d_QE(diag, theta) exactly; a
+    nonzero b3 alone scales every row by e^b3."""
+    from oracle import davidson as D
+    rng = np.random.default_rng(11)
+    n = 300
+    g = np.sort(rng.uniform(0.0, 15.0, n))
+    diag = g + rng.uniform(-1.0, 2.0, n)
+    r, x = rng.standard_normal(n), rng.standard_normal(n)
+    ritz, vbar = -0.2, -0.4
+    p = M.unpack(M.init_params(2))
+    p["w3"] = np.zeros(M.HIDDEN)
+    p["b3"] = np.zeros(1)
+    e = M.correction(M.pack(p), r, x, g, ritz, vbar, diag, bf16=False)
+    xq = 2.0 * (diag - ritz)
+    d_ry = (1.0 + xq + np.sqrt(1.0 + (xq - 1.0) ** 2)) / 2.0  # ref_davidson.rs:246-255, Ry form
+    np.testing.assert_allclose(e, -r / (d_ry / 2.0), rtol=1e-13)
+    p["b3"] = np.array([0.7])
+    e2 = M.correction(M.pack(p), r, x, g, ritz, vbar, diag, bf16=False)
+    np.testing.assert_allclose(e2, math.exp(0.7) * e, rtol=1e-13)
