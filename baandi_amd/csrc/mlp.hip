@@ -43,8 +43,11 @@ __device__ __forceinline__ unsigned short bf16_bits(float f) {
 }
 __device__ __forceinline__ float bf16_value(float f) { return __uint_as_float(unsigned(bf16_bits(f)) << 16); }
 
-// tanh(x) = 1 - 2 / (1 + e^{2x}); saturates to +-1 for large |x|.
-__device__ __forceinline__ float tanh_f(float x) { return 1.0f - 2.0f / (1.0f + __expf(2.0f * x)); }
+// tanh(x) = 1 - 2 / (1 + e^{2x}) with the hardware reciprocal (1 ulp, no
+// division sequence); saturates to +-1 for large |x| (rcp(inf) = 0).
+__device__ __forceinline__ float tanh_f(float x) {
+  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + __expf(2.0f * x));
+}
 
 __device__ __forceinline__ double wave_sum_d(double v) {
 #pragma unroll
@@ -140,7 +143,25 @@ __global__ __launch_bounds__(256) void k_kinetic_scale(const double* __restrict_
   const int j = blockIdx.x;
   const double* x = X + int64_t(j) * n;
   double a = 0.0, b = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+  const int64_t S = blockDim.x;
+  int64_t i = threadIdx.x;
+  // Four rows a pass with their loads issued first; the sums run in row
+  // order, as in the one-row loop below.
+  for (; i + 3 * S < n; i += 4 * S) {
+    double xv[4], kv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      xv[u] = x[i + u * S];
+      kv[u] = kinetic[i + u * S];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double w = xv[u] * xv[u];
+      a += w * kv[u];
+      b += w;
+    }
+  }
+  for (; i < n; i += S) {
     const double w = x[i] * x[i];
     a += w * kinetic[i];
     b += w;
@@ -177,15 +198,27 @@ __global__ __launch_bounds__(256) void k_correct(int64_t n, const double* __rest
   const double* hx = HX + int64_t(j) * n;
   double* ej = E + int64_t(j) * n;
   double se = 0.0, sr = 0.0;
-  for (int64_t base = int64_t(blockIdx.x) * 64; base < n; base += int64_t(gridDim.x) * 64) {
-    const int64_t row = base + 16 * wave + e;
+  const int64_t stride = int64_t(gridDim.x) * 64;
+  int64_t row = int64_t(blockIdx.x) * 64 + 16 * wave + e;
+  float g_next = row < n ? float(kinetic[row]) : 0.0f;
+  for (int64_t base = int64_t(blockIdx.x) * 64; base < n; base += stride, row += stride) {
     const bool ok = row < n;
-    const float g = ok ? float(kinetic[row]) : 0.0f;
+    const float g = g_next;
+    // This pass's operands and the next pass's row energy are loaded before
+    // the network runs, so their latency hides behind it.
+    double xr = 0.0, hr = 0.0, dr = 0.0;
+    if (ok && q == 0) {
+      xr = x[row];
+      hr = hx[row];
+      dr = diag[row];
+    }
+    const int64_t next = row + stride;
+    g_next = next < n ? float(kinetic[next]) : 0.0f;
     Group G;
     forward_group(sb, log1pf(g / T), u1, u2, u3, G);
     if (ok && q == 0) {
-      const double r = hx[row] - th * x[row];
-      const double v = -double(expf(G.o)) * r / qe_denominator(diag[row], th);
+      const double r = hr - th * xr;
+      const double v = -double(expf(G.o)) * r / qe_denominator(dr, th);
       ej[row] = v;
       se += v * v;
       sr += r * r;
